@@ -1,0 +1,466 @@
+// SPDX-License-Identifier: Apache-2.0
+//
+// extern "C" boundary of libchysx_amd.so (declared in include/chysx_cloth.h).
+#include "../../include/chysx_cloth.h"
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "engine.hpp"
+
+using namespace chysx_amd;
+
+struct chysx_cloth {
+    ClothSim sim;
+};
+struct chysx_batch {
+    ClothBatch batch;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return CHYSX_OK;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return CHYSX_EINVAL;
+    } catch (const std::out_of_range& e) {
+        g_last_error = e.what();
+        return CHYSX_ERANGE;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return CHYSX_ERUNTIME;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return CHYSX_ERUNTIME;
+    }
+}
+
+void need(const void* p, const char* what) {
+    if (p == nullptr) throw std::invalid_argument(std::string(what) + " must be non-null");
+}
+
+Material to_mat(const chysx_cloth_material* m) {
+    Material r;
+    r.lame_mu = m->lame_mu;
+    r.lame_lambda = m->lame_lambda;
+    r.bending = m->bending;
+    r.density = m->density;
+    r.damping = m->damping;
+    r.gx = m->gx;
+    r.gy = m->gy;
+    r.gz = m->gz;
+    return r;
+}
+
+__global__ void force_running_kernel(InstState* st, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        st[i].status = kRunning;
+        st[i].sigma = 1.0;
+        st[i].rho = 1.0;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char* chysx_last_error(void) { return g_last_error.c_str(); }
+const char* chysx_version(void) { return "0.1.0"; }
+
+int chysx_cloth_create(chysx_cloth** out) {
+    return guard([&] {
+        need(out, "out");
+        *out = new chysx_cloth();
+    });
+}
+int chysx_cloth_destroy(chysx_cloth* sim) {
+    return guard([&] { delete sim; });
+}
+int chysx_cloth_set_material(chysx_cloth* sim, const chysx_cloth_material* m) {
+    return guard([&] {
+        need(sim, "sim");
+        need(m, "material");
+        sim->sim.set_material(to_mat(m));
+    });
+}
+int chysx_cloth_get_material(const chysx_cloth* sim, chysx_cloth_material* out) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        const Material& m = sim->sim.material();
+        *out = chysx_cloth_material{m.lame_mu, m.lame_lambda, m.bending, m.density, m.damping, m.gx, m.gy, m.gz};
+    });
+}
+int chysx_cloth_set_external_buffers(chysx_cloth* sim, uintptr_t pos, uintptr_t vel, int n, uintptr_t inv_mass,
+                                     uintptr_t force) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.set_external_buffers(pos, vel, n, inv_mass, force);
+    });
+}
+int chysx_cloth_set_mesh(chysx_cloth* sim, const int32_t* tris, int ntri) {
+    return guard([&] {
+        need(sim, "sim");
+        if (ntri > 0) need(tris, "triangles");
+        sim->sim.set_mesh(tris, ntri);
+    });
+}
+int chysx_cloth_build_springs(chysx_cloth* sim, float k, uintptr_t stream) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.build_springs(k, stream);
+    });
+}
+int chysx_cloth_num_springs(const chysx_cloth* sim, int* out) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        *out = sim->sim.num_springs();
+    });
+}
+int chysx_cloth_build_fem_stretch(chysx_cloth* sim, float k, uintptr_t stream) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.build_fem_stretch(k, stream);
+    });
+}
+int chysx_cloth_num_fem_stretch(const chysx_cloth* sim, int* out) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        *out = sim->sim.num_fem_stretch();
+    });
+}
+int chysx_cloth_set_pins(chysx_cloth* sim, const int32_t* idx, const float* targets, int n, float k) {
+    return guard([&] {
+        need(sim, "sim");
+        if (n > 0) {
+            need(idx, "indices");
+            need(targets, "targets");
+        }
+        sim->sim.set_pins(idx, targets, n, k);
+    });
+}
+int chysx_cloth_clear_pins(chysx_cloth* sim) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.clear_pins();
+    });
+}
+int chysx_cloth_num_pins(const chysx_cloth* sim, int* out) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        *out = sim->sim.num_pins();
+    });
+}
+int chysx_cloth_update_pin_targets(chysx_cloth* sim, const float* targets, int n, uintptr_t stream) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.update_pin_targets(targets, n, stream);
+    });
+}
+int chysx_cloth_redistribute_mass_area_weighted(chysx_cloth* sim, float density, uintptr_t inv_mass, int n,
+                                                uintptr_t stream) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.redistribute_mass_area_weighted(density, inv_mass, n, stream);
+    });
+}
+int chysx_cloth_set_pcg_iterations(chysx_cloth* sim, int max_iter) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.set_pcg_iterations(max_iter);
+    });
+}
+int chysx_cloth_pcg_iterations(const chysx_cloth* sim, int* out) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        *out = sim->sim.pcg_iterations();
+    });
+}
+int chysx_cloth_step(chysx_cloth* sim, float dt, uintptr_t stream) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.step(dt, stream);
+    });
+}
+int chysx_cloth_num_particles(const chysx_cloth* sim, int* out) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        *out = sim->sim.num_particles();
+    });
+}
+int chysx_cloth_num_off_diag_blocks(const chysx_cloth* sim, int* out) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        *out = sim->sim.num_off_diag_blocks();
+    });
+}
+int chysx_cloth_debug_copy_hessian_diag(const chysx_cloth* sim, float* out) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.debug_copy_hessian_diag(out);
+    });
+}
+int chysx_cloth_debug_copy_hessian_csr(const chysx_cloth* sim, int32_t* ro, int32_t* cols, float* vals) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.debug_copy_hessian_csr(ro, cols, vals);
+    });
+}
+int chysx_cloth_debug_copy_last_rhs(const chysx_cloth* sim, float* out) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.debug_copy_last_rhs(out);
+    });
+}
+int chysx_cloth_debug_copy_last_dx(const chysx_cloth* sim, float* out) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.debug_copy_last_dx(out);
+    });
+}
+int chysx_cloth_set_pcg_tolerance(chysx_cloth* sim, double tol) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.set_pcg_tolerance(tol);
+    });
+}
+int chysx_cloth_set_springs_active(chysx_cloth* sim, int on) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.set_springs_active(on != 0);
+    });
+}
+int chysx_cloth_set_external_force_active(chysx_cloth* sim, int on) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.set_external_force_active(on != 0);
+    });
+}
+int chysx_cloth_set_spring_stiffnesses(chysx_cloth* sim, const float* k, int n) {
+    return guard([&] {
+        need(sim, "sim");
+        need(k, "k");
+        sim->sim.set_spring_stiffnesses(k, n);
+    });
+}
+int chysx_cloth_set_fem_stretch_stiffnesses(chysx_cloth* sim, const float* k, int n) {
+    return guard([&] {
+        need(sim, "sim");
+        need(k, "k");
+        sim->sim.set_fem_stretch_stiffnesses(k, n);
+    });
+}
+int chysx_cloth_last_solve_stats(const chysx_cloth* sim, int* iters, int* restarts, int* status, double* relres) {
+    return guard([&] {
+        need(sim, "sim");
+        const SolveStats s = sim->sim.last_stats();
+        if (iters) *iters = s.iters;
+        if (restarts) *restarts = s.restarts;
+        if (status) *status = s.status;
+        if (relres) *relres = s.relres;
+    });
+}
+int chysx_cloth_set_warm_start(chysx_cloth* sim, const double* dx, int n) {
+    return guard([&] {
+        need(sim, "sim");
+        need(dx, "dx");
+        sim->sim.ensure_topology();
+        if (n != sim->sim.instance().n) throw std::invalid_argument("set_warm_start: n does not match particle count");
+        sim->sim.set_warm_start(dx);
+    });
+}
+int chysx_cloth_zero_warm_start(chysx_cloth* sim) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.zero_warm_start();
+    });
+}
+int chysx_cloth_get_dx64(const chysx_cloth* sim, double* out, int n) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        if (n != sim->sim.instance().n) throw std::invalid_argument("get_dx64: n does not match particle count");
+        const_cast<chysx_cloth*>(sim)->sim.problem().copy_dx64(0, out);
+    });
+}
+int chysx_cloth_num_edges(const chysx_cloth* sim, int* out) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        *out = sim->sim.num_edges();
+    });
+}
+int chysx_cloth_get_edges(const chysx_cloth* sim, int32_t* out) {
+    return guard([&] {
+        need(sim, "sim");
+        const auto& e = sim->sim.edges();
+        if (!e.empty()) need(out, "out");
+        std::memcpy(out, e.data(), e.size() * sizeof(int));
+    });
+}
+
+int chysx_topology_query(int n, const int32_t* tris, int ntri, int include_springs, const int32_t* pins, int npin,
+                         int* nnz, int* nedges, int32_t* row_offsets, int32_t* col_indices, int32_t* edges,
+                         int32_t* tri_slots, int32_t* spring_slots, int32_t* pin_slots) {
+    return guard([&] {
+        if (n < 0 || ntri < 0 || npin < 0) throw std::invalid_argument("topology_query: negative size");
+        if (ntri > 0) need(tris, "triangles");
+        if (npin > 0) need(pins, "pins");
+        const std::vector<int> e = build_edges(tris, ntri);
+        const int E = int(e.size() / 2);
+        const int* spr = (include_springs && E) ? e.data() : nullptr;
+        const int Es = spr ? E : 0;
+        const Pattern p = build_pattern(n, tris, ntri, spr, Es);
+        std::vector<int> ts, ss, ps;
+        build_slot_maps(p, tris, ntri, spr, Es, pins, npin, ts, ss, ps);
+        if (nnz) *nnz = int(p.cols.size());
+        if (nedges) *nedges = E;
+        if (row_offsets) std::memcpy(row_offsets, p.row_offsets.data(), p.row_offsets.size() * sizeof(int));
+        if (col_indices && !p.cols.empty()) std::memcpy(col_indices, p.cols.data(), p.cols.size() * sizeof(int));
+        if (edges && !e.empty()) std::memcpy(edges, e.data(), e.size() * sizeof(int));
+        if (tri_slots && !ts.empty()) std::memcpy(tri_slots, ts.data(), ts.size() * sizeof(int));
+        if (spring_slots && !ss.empty()) std::memcpy(spring_slots, ss.data(), ss.size() * sizeof(int));
+        if (pin_slots && !ps.empty()) std::memcpy(pin_slots, ps.data(), ps.size() * sizeof(int));
+    });
+}
+
+int chysx_batch_create(chysx_batch** out) {
+    return guard([&] {
+        need(out, "out");
+        *out = new chysx_batch();
+    });
+}
+int chysx_batch_destroy(chysx_batch* b) {
+    return guard([&] { delete b; });
+}
+int chysx_batch_add_cloth(chysx_batch* b, const int32_t* tris, int ntri, int n, const float* spring_k,
+                          float spring_k_scalar, const float* stretch_k, float stretch_k_scalar, const int32_t* pins,
+                          const float* pin_targets, int npin, float pin_k, int* index) {
+    return guard([&] {
+        need(b, "batch");
+        if (ntri > 0) need(tris, "triangles");
+        if (npin > 0) {
+            need(pins, "pins");
+            need(pin_targets, "pin_targets");
+        }
+        const int k = b->batch.add_cloth(tris, ntri, n, spring_k, spring_k_scalar, stretch_k, stretch_k_scalar, pins,
+                                         pin_targets, npin, pin_k);
+        if (index) *index = k;
+    });
+}
+int chysx_batch_set_external_buffers(chysx_batch* b, uintptr_t pos, uintptr_t vel, int total, uintptr_t inv_mass,
+                                     uintptr_t force) {
+    return guard([&] {
+        need(b, "batch");
+        b->batch.set_external_buffers(pos, vel, total, inv_mass, force);
+    });
+}
+int chysx_batch_set_material(chysx_batch* b, int index, const chysx_cloth_material* m) {
+    return guard([&] {
+        need(b, "batch");
+        need(m, "material");
+        if (index < 0 || index >= b->batch.count()) throw std::out_of_range("batch: cloth index out of range");
+        b->batch.set_material(index, to_mat(m));
+    });
+}
+int chysx_batch_set_options(chysx_batch* b, double tol, int max_iter, int springs_active, int force_active) {
+    return guard([&] {
+        need(b, "batch");
+        if (!(tol >= 0.0)) throw std::invalid_argument("batch: tolerance must be >= 0");
+        b->batch.set_options(tol, max_iter, springs_active != 0, force_active != 0);
+    });
+}
+int chysx_batch_step(chysx_batch* b, float dt, uintptr_t stream) {
+    return guard([&] {
+        need(b, "batch");
+        b->batch.step(dt, stream);
+    });
+}
+int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts, int* status, double* relres) {
+    return guard([&] {
+        need(b, "batch");
+        if (index < 0 || index >= b->batch.count()) throw std::out_of_range("batch: cloth index out of range");
+        const SolveStats s = b->batch.stats(index);
+        if (iters) *iters = s.iters;
+        if (restarts) *restarts = s.restarts;
+        if (status) *status = s.status;
+        if (relres) *relres = s.relres;
+    });
+}
+
+int chysx_cloth_spmv_bytes(const chysx_cloth* sim, double* bytes) {
+    return guard([&] {
+        need(sim, "sim");
+        need(bytes, "bytes");
+        *bytes = const_cast<chysx_cloth*>(sim)->sim.problem().spmv_bytes();
+    });
+}
+
+int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uintptr_t stream, double* ms) {
+    return guard([&] {
+        need(sim, "sim");
+        need(ms, "ms");
+        if (reps <= 0) throw std::invalid_argument("time_kernel: reps must be positive");
+        if (which < 0 || which > 4) throw std::invalid_argument("time_kernel: unknown kernel id");
+        ClothSim& S = sim->sim;
+        if (!S.problem().built()) throw std::runtime_error("time_kernel: step() once before timing");
+        Problem& P = S.problem();
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        // Keep every PCG kernel live: no tolerance test, no iteration cap.
+        Instance tmp = S.instance();
+        tmp.tol = 0.0;
+        tmp.max_iter = INT_MAX;
+        P.upload_params({&tmp}, s);
+        const DevExt ext = S.ext();
+        const DevMatrix A = P.matrix();
+        const DevVectors v = P.vectors();
+        const LaunchGeom g = P.geom();
+        hipLaunchKernelGGL(force_running_kernel, dim3((P.ninst() + 63) / 64), dim3(64), 0, s, P.states(), P.ninst());
+        check_hip(hipGetLastError(), "force_running_kernel");
+        hipEvent_t e0, e1;
+        check_hip(hipEventCreate(&e0), "hipEventCreate");
+        check_hip(hipEventCreate(&e1), "hipEventCreate");
+        auto run = [&](int n) {
+            for (int i = 0; i < n; ++i) {
+                switch (which) {
+                    case 0: launch_spmv(g, A, v.p0, v.ap, s); break;
+                    case 1: launch_pcg_spmv(g, A, v, P.states(), s); break;
+                    case 2: launch_pcg_update(g, A, v, P.states(), P.params(), s); break;
+                    case 3: launch_assemble(g, P.mesh(), A, v, ext, P.params(), dt, s); break;
+                    case 4:
+                        launch_pcg_spmv(g, A, v, P.states(), s);
+                        launch_pcg_update(g, A, v, P.states(), P.params(), s);
+                        break;
+                }
+            }
+        };
+        run(3);
+        check_hip(hipEventRecord(e0, s), "hipEventRecord");
+        run(reps);
+        check_hip(hipEventRecord(e1, s), "hipEventRecord");
+        check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+        float t = 0.0f;
+        check_hip(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = double(t) / reps;
+        P.upload_params({&S.instance()}, s);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,677 @@
+// SPDX-License-Identifier: Apache-2.0
+//
+// gfx950 kernels of the implicit cloth step.
+//
+// Work decomposition: a 256-lane workgroup owns one Tile (up to 256
+// consecutive rows of one instance); each 64-lane wavefront owns one 64-row
+// slice of the wave-sliced ELL matrix, one lane per block row.  Workgroup ids
+// are remapped so that each of the 8 XCDs walks a contiguous eighth of the
+// tiles, keeping neighbour gathers (rows +-1, +-W) inside one XCD's L2.
+//
+// Cross-workgroup reductions (inner products) use per-tile FP64 partials and
+// an arrival counter; the last workgroup of an instance sums the partials in
+// tile order, so every reduction is deterministic.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "cloth_kernels.hpp"
+
+namespace chysx_amd {
+
+namespace {
+
+constexpr int kTileRows = 256;
+constexpr float kMinLength = 1.0e-6f;  // spring_constraint.cu:35
+constexpr float kMinNorm = 1.0e-6f;    // triangle_stretch_constraint.cu:39
+constexpr double kGuard = 1.0e-37;     // pcg_solver.cu:421-423,451
+
+__device__ __forceinline__ int logical_tile(int ntiles) {
+    const int per = (ntiles + 7) >> 3;
+    const int b = blockIdx.x;
+    return (b & 7) * per + (b >> 3);
+}
+inline int grid_for_tiles(int ntiles) { return 8 * ((ntiles + 7) / 8); }
+
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 ld3(const float* p, int i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ __forceinline__ void st3(float* p, int i, F3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+
+// Symmetric 3x3 (xx, xy, xz, yy, yz, zz).
+struct S6 {
+    float m[6];
+};
+
+// Sum of a value over the 256-lane workgroup, fixed order.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double (&out)[N]) {
+    __shared__ double sh[4][N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        double a = v[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        v[j] = a;
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) sh[wave][j] = v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+    __syncthreads();
+}
+
+// Publish this tile's partial sums; the last tile of the instance to arrive
+// returns true with the instance totals in `tot` (valid in thread 0).
+template <int N>
+__device__ bool reduce_across_tiles(double (&v)[N], double* partials, int tile, InstState* S, double (&tot)[N]) {
+    __shared__ int is_last;
+    double bs[N];
+    block_sum<N>(v, bs);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            __hip_atomic_store(&partials[4 * tile + j], bs[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned prev = atomicAdd(&S->cnt, 1u);
+        is_last = (prev == unsigned(S->ntiles - 1)) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    const int t0 = S->tile0, nt = S->ntiles;
+    double acc[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) acc[j] = 0.0;
+    for (int t = threadIdx.x; t < nt; t += kTileRows) {
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            acc[j] += __hip_atomic_load(&partials[4 * (t0 + t) + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<N>(acc, tot);
+    if (threadIdx.x == 0) __hip_atomic_store(&S->cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Rest data (set-up).
+// ---------------------------------------------------------------------------
+
+// spring_rest_length_kernel (spring_constraint.cu:41-52).
+__global__ void rest_length_kernel(const int* __restrict__ spr_v, const float* __restrict__ pos,
+                                   float* __restrict__ rest, int n) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const F3 d = ld3(pos, spr_v[2 * e]) - ld3(pos, spr_v[2 * e + 1]);
+    rest[e] = sqrtf(dot(d, d));
+}
+
+// triangle_rest_shape_kernel (triangle_stretch_constraint.cu:50-110): 2-D rest
+// frame, Dm^-1 (identity when det <= 1e-6), rest area, times R(theta)^-1.
+__global__ void rest_shape_kernel(const int* __restrict__ tri_v, const float* __restrict__ pos,
+                                  float4* __restrict__ dminv, float* __restrict__ area, int n, float ct, float st) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const F3 a = ld3(pos, tri_v[3 * t]), b = ld3(pos, tri_v[3 * t + 1]), c = ld3(pos, tri_v[3 * t + 2]);
+    const F3 ab = b - a, ac = c - a;
+    const float lab = sqrtf(dot(ab, ab));
+    const float lac = sqrtf(dot(ac, ac));
+    const F3 abh = (lab > kMinNorm) ? ab * (1.0f / lab) : f3(1.0f, 0.0f, 0.0f);
+    const float uc = dot(abh, ac);
+    const float vc2 = lac * lac - uc * uc;
+    const float vc = (vc2 > 0.0f) ? sqrtf(vc2) : 0.0f;
+    area[t] = 0.5f * lab * vc;
+    const float det = lab * vc;
+    float d00 = 1.0f, d01 = 0.0f, d10 = 0.0f, d11 = 1.0f;
+    if (det > kMinNorm) {
+        const float inv_det = 1.0f / (lab * vc - uc * 0.0f);
+        d00 = vc * inv_det;
+        d01 = -uc * inv_det;
+        d10 = -0.0f * inv_det;
+        d11 = lab * inv_det;
+    }
+    // Dm_inv * R^-1 with R^-1 = [[c, s], [-s, c]].
+    dminv[t] = make_float4(d00 * ct + d01 * (-st), d00 * st + d01 * ct, d10 * ct + d11 * (-st), d10 * st + d11 * ct);
+}
+
+// Area-lumped mass: m_v = sum over incident triangles of density/3 * area,
+// inv_mass = 1/m (0 when m <= 1e-12).  Restates scatter_triangle_mass_kernel +
+// mass_to_inv_mass_kernel (cloth_simulator.cu:151-190) as a per-vertex gather.
+__global__ void area_mass_kernel(const int* __restrict__ vt_off, const int* __restrict__ vt_ids,
+                                 const int* __restrict__ tri_v, const float* __restrict__ pos, float d3,
+                                 float* __restrict__ inv_mass, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float m = 0.0f;
+    for (int q = vt_off[i]; q < vt_off[i + 1]; ++q) {
+        const int t = vt_ids[q];
+        const F3 x0 = ld3(pos, tri_v[3 * t]), x1 = ld3(pos, tri_v[3 * t + 1]), x2 = ld3(pos, tri_v[3 * t + 2]);
+        const F3 e1 = x1 - x0, e2 = x2 - x0;
+        const F3 cr = f3(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
+        m += d3 * (0.5f * sqrtf(dot(cr, cr)));
+    }
+    inv_mass[i] = (m > 1.0e-12f) ? (1.0f / m) : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Assembly: one lane per block row.  The lane computes the row's right-hand
+// side and every Hessian block of its row from the elements incident to it
+// (pins, springs, stretch triangles, inertia), writes the diagonal and its
+// symmetric inverse, and accumulates off-diagonal blocks into its own ELL
+// entries.  No atomics; the incidence order is fixed, so the result is
+// deterministic.
+//
+// Element formulas restate:
+//   pins      pin_constraint.cu:61-92
+//   springs   spring_constraint.cu:76-151
+//   stretch   triangle_stretch_constraint.cu:153-311
+//   inertia   cloth_simulator.cu:199-210
+//   rhs       cloth_simulator.cu:83-142, written as m*(v/dt + g + f/m) - grad,
+//             the same quantity as (m/dt^2)(x_tilde - x_n) - grad without the
+//             FP32 cancellation of x_tilde - x_n.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void add_block(float* __restrict__ vrow, int k, const float (&H)[9], float sgn) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) vrow[(k * 9 + c) * 64] += sgn * H[c];
+}
+
+__global__ __launch_bounds__(256) void assemble_kernel(LaunchGeom g, DevMesh m, DevMatrix A, DevVectors v, DevExt e,
+                                                       const InstParams* __restrict__ params, float dt) {
+    const int t = logical_tile(g.ntiles);
+    if (t >= g.ntiles) return;
+    const Tile tile = g.tiles[t];
+    const int tid = threadIdx.x;
+    if (tid >= tile.nrows) return;
+    const int row = tile.row0 + tid;
+    const int p = tile.part0 + tid;
+    const InstParams pr = params[tile.inst];
+    const int s = row >> 6, lane = row & 63;
+    const int sb = A.slice_ptr[s];
+    const int deg = A.row_deg[row];
+    float* __restrict__ vrow = A.val + (size_t)sb * 9 * 64 + lane;
+    for (int k = 0; k < deg; ++k)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) vrow[(k * 9 + c) * 64] = 0.0f;
+
+    const F3 xi = ld3(e.pos, p);
+    const F3 vi = ld3(e.vel, p);
+    // mass_from_inv_mass_kernel (cloth_simulator.cu:66-77)
+    float mass = 1.0f;
+    if (e.inv_mass != nullptr) {
+        const float w = e.inv_mass[p];
+        mass = (w > 1.0e-12f) ? (1.0f / w) : 1.0e8f;
+    }
+    F3 acc = f3(pr.gx, pr.gy, pr.gz);
+    if (pr.force_active && e.force != nullptr) {
+        const float im = (mass > 0.0f) ? (1.0f / mass) : 0.0f;
+        acc = acc + ld3(e.force, p) * im;
+    }
+    const float inv_dt = 1.0f / dt;
+    const F3 b = (vi * inv_dt + acc) * mass;
+
+    float D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    F3 grad = f3(0.0f, 0.0f, 0.0f);
+
+    // pins
+    for (int q = m.pin_inc_off[row]; q < m.pin_inc_off[row + 1]; ++q) {
+        const int c = m.pin_inc[q];
+        const F3 tg = ld3(m.pin_tgt, c);
+        grad = grad + (xi - tg) * pr.pin_k;
+        D[0] += pr.pin_k;
+        D[4] += pr.pin_k;
+        D[8] += pr.pin_k;
+    }
+    // springs
+    if (pr.springs_active) {
+        for (int q = m.spr_inc_off[row]; q < m.spr_inc_off[row + 1]; ++q) {
+            const int2 ent = m.spr_inc[q];
+            const int ei = ent.x >> 1, side = ent.x & 1;
+            const int va = m.spr_v[2 * ei], vb = m.spr_v[2 * ei + 1];
+            const F3 d = ld3(e.pos, va) - ld3(e.pos, vb);
+            float l = sqrtf(dot(d, d));
+            if (l < kMinLength) l = kMinLength;
+            const float inv_l = 1.0f / l;
+            const F3 dh = d * inv_l;
+            const float ke = m.spr_k[ei], L = m.spr_rest[ei];
+            const F3 ge = dh * (ke * (l - L));
+            grad = side ? grad - ge : grad + ge;
+            const float a = ke * L * inv_l;
+            const float iso = (l < L) ? 0.0f : (ke - a);
+            float H[9];
+            H[0] = a * (dh.x * dh.x) + iso;
+            H[1] = a * (dh.x * dh.y);
+            H[2] = a * (dh.x * dh.z);
+            H[3] = H[1];
+            H[4] = a * (dh.y * dh.y) + iso;
+            H[5] = a * (dh.y * dh.z);
+            H[6] = H[2];
+            H[7] = H[5];
+            H[8] = a * (dh.z * dh.z) + iso;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) D[c] += H[c];
+            const int ko = ent.y;
+            if (ko == 1023) {
+#pragma unroll
+                for (int c = 0; c < 9; ++c) D[c] -= H[c];
+            } else {
+                add_block(vrow, ko, H, -1.0f);
+            }
+        }
+    }
+    // stretch triangles
+    for (int q = m.tri_inc_off[row]; q < m.tri_inc_off[row + 1]; ++q) {
+        const int2 ent = m.tri_inc[q];
+        const int ti = ent.x >> 2, a = ent.x & 3;
+        const F3 x0 = ld3(e.pos, m.tri_v[3 * ti]);
+        const F3 x1 = ld3(e.pos, m.tri_v[3 * ti + 1]);
+        const F3 x2 = ld3(e.pos, m.tri_v[3 * ti + 2]);
+        const float4 Dm = m.tri_dminv[ti];
+        const F3 ea = x1 - x0, eb = x2 - x0;
+        const F3 wu = ea * Dm.x + eb * Dm.z;
+        const F3 wv = ea * Dm.y + eb * Dm.w;
+        float lu = sqrtf(dot(wu, wu));
+        float lv = sqrtf(dot(wv, wv));
+        if (lu < kMinNorm) lu = kMinNorm;
+        if (lv < kMinNorm) lv = kMinNorm;
+        const float inv_lu = 1.0f / lu, inv_lv = 1.0f / lv;
+        const float cu = lu - 1.0f, cv = lv - 1.0f;
+        const F3 uh = wu * inv_lu, vh = wv * inv_lv;
+        const float dwu[3] = {-Dm.x - Dm.z, Dm.x, Dm.z};
+        const float dwv[3] = {-Dm.y - Dm.w, Dm.y, Dm.w};
+        const float keff = m.tri_area[ti] * m.tri_k[ti];
+        grad = grad + (uh * (cu * dwu[a]) + vh * (cv * dwv[a])) * keff;
+        // H_ab = dwu_a dwu_b Mu + dwv_a dwv_b Mv with
+        // Mu = keff uh uh^T + [cu > 0] keff cu / lu (I - uh uh^T), Mv likewise.
+        const float su = (cu > 0.0f) ? keff * cu * inv_lu : 0.0f;
+        const float sv = (cv > 0.0f) ? keff * cv * inv_lv : 0.0f;
+        const float qu = keff - su, qv = keff - sv;
+        float Mu[6], Mv[6];
+        Mu[0] = qu * uh.x * uh.x + su;
+        Mu[1] = qu * uh.x * uh.y;
+        Mu[2] = qu * uh.x * uh.z;
+        Mu[3] = qu * uh.y * uh.y + su;
+        Mu[4] = qu * uh.y * uh.z;
+        Mu[5] = qu * uh.z * uh.z + su;
+        Mv[0] = qv * vh.x * vh.x + sv;
+        Mv[1] = qv * vh.x * vh.y;
+        Mv[2] = qv * vh.x * vh.z;
+        Mv[3] = qv * vh.y * vh.y + sv;
+        Mv[4] = qv * vh.y * vh.z;
+        Mv[5] = qv * vh.z * vh.z + sv;
+        const int packed = ent.y;
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) {
+            const float cu_ab = dwu[a] * dwu[bb], cv_ab = dwv[a] * dwv[bb];
+            float H[9];
+            H[0] = cu_ab * Mu[0] + cv_ab * Mv[0];
+            H[1] = cu_ab * Mu[1] + cv_ab * Mv[1];
+            H[2] = cu_ab * Mu[2] + cv_ab * Mv[2];
+            H[4] = cu_ab * Mu[3] + cv_ab * Mv[3];
+            H[5] = cu_ab * Mu[4] + cv_ab * Mv[4];
+            H[8] = cu_ab * Mu[5] + cv_ab * Mv[5];
+            H[3] = H[1];
+            H[6] = H[2];
+            H[7] = H[5];
+            const int kb = (packed >> (10 * bb)) & 1023;
+            if (bb == a || kb == 1023) {
+#pragma unroll
+                for (int c = 0; c < 9; ++c) D[c] += H[c];
+            } else {
+                add_block(vrow, kb, H, 1.0f);
+            }
+        }
+    }
+    // inertia (add_inertia_diag_kernel, cloth_simulator.cu:199-210)
+    const float mi = mass * (1.0f / (dt * dt));
+    D[0] += mi;
+    D[4] += mi;
+    D[8] += mi;
+
+    st3(v.rhs, row, b - grad);
+    float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) dg[c * 64] = D[c];
+    // Inverse of the symmetric diagonal block by adjugate / determinant, no
+    // guard (math::inverse, matrix.cuh:219-235; residual_m_inv_kernel,
+    // pcg_solver.cu:29-51).
+    const float a00 = D[4] * D[8] - D[5] * D[7];
+    const float a01 = -D[1] * D[8] + D[2] * D[7];
+    const float a02 = D[1] * D[5] - D[2] * D[4];
+    const float a11 = D[0] * D[8] - D[2] * D[6];
+    const float a12 = -D[0] * D[5] + D[2] * D[3];
+    const float a22 = D[0] * D[4] - D[1] * D[3];
+    const float a10 = -D[3] * D[8] + D[5] * D[6];
+    const float a20 = D[3] * D[7] - D[4] * D[6];
+    const float det = D[0] * a00 + D[1] * a10 + D[2] * a20;
+    const float id = 1.0f / det;
+    float* mv = A.minv + (size_t)s * 6 * 64 + lane;
+    mv[0 * 64] = a00 * id;
+    mv[1 * 64] = a01 * id;
+    mv[2 * 64] = a02 * id;
+    mv[3 * 64] = a11 * id;
+    mv[4 * 64] = a12 * id;
+    mv[5 * 64] = a22 * id;
+}
+
+__device__ __forceinline__ F3 minv_mul(const DevMatrix& A, int s, int lane, F3 r) {
+    const float* mv = A.minv + (size_t)s * 6 * 64 + lane;
+    const float m0 = mv[0], m1 = mv[64], m2 = mv[128], m3 = mv[192], m4 = mv[256], m5 = mv[320];
+    return f3(m0 * r.x + m1 * r.y + m2 * r.z, m1 * r.x + m3 * r.y + m4 * r.z, m2 * r.x + m4 * r.y + m5 * r.z);
+}
+
+// ---------------------------------------------------------------------------
+// PCG residual: b - A x with FP64 products and sums (x is FP64).
+//   mode 0: start a solve -- r, z = M^-1 r, p = z, rho, |b|^2, test.
+//   mode 1: runs only for instances in kCheck -- confirm convergence on the
+//           true residual, or restart the recursion from it.
+//   mode 2: runs only for kStopped instances -- record the true residual.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pcg_residual_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+                                                           InstState* __restrict__ st,
+                                                           const InstParams* __restrict__ params, int mode) {
+    const int t = logical_tile(g.ntiles);
+    if (t >= g.ntiles) return;
+    const Tile tile = g.tiles[t];
+    InstState* S = st + tile.inst;
+    const int status = S->status;
+    if (mode == 1 && status != kCheck) return;
+    if (mode == 2 && status != kStopped) return;
+    const int parity = S->parity;
+    float* pcur = parity ? v.p1 : v.p0;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};  // rr, bb, rz
+    const int tid = threadIdx.x;
+    if (tid < tile.nrows) {
+        const int row = tile.row0 + tid;
+        const int s = row >> 6, lane = row & 63;
+        const double* x = v.x;
+        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+        const double x0 = x[3 * row], x1 = x[3 * row + 1], x2 = x[3 * row + 2];
+        double a0 = double(dg[0]) * x0 + double(dg[64]) * x1 + double(dg[128]) * x2;
+        double a1 = double(dg[192]) * x0 + double(dg[256]) * x1 + double(dg[320]) * x2;
+        double a2 = double(dg[384]) * x0 + double(dg[448]) * x1 + double(dg[512]) * x2;
+        const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
+        const int* colp = A.col + (size_t)sb * 64 + lane;
+        const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+        for (int k = 0; k < w; ++k) {
+            const int c = colp[k * 64];
+            if (c >= 0) {
+                const float* bk = vp + k * 9 * 64;
+                const double y0 = x[3 * c], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
+                a0 += double(bk[0]) * y0 + double(bk[64]) * y1 + double(bk[128]) * y2;
+                a1 += double(bk[192]) * y0 + double(bk[256]) * y1 + double(bk[320]) * y2;
+                a2 += double(bk[384]) * y0 + double(bk[448]) * y1 + double(bk[512]) * y2;
+            }
+        }
+        const F3 bf = ld3(v.rhs, row);
+        const double r0 = double(bf.x) - a0, r1 = double(bf.y) - a1, r2 = double(bf.z) - a2;
+        part[0] = r0 * r0 + r1 * r1 + r2 * r2;
+        part[1] = double(bf.x) * bf.x + double(bf.y) * bf.y + double(bf.z) * bf.z;
+        if (mode != 2) {
+            const F3 rf = f3(float(r0), float(r1), float(r2));
+            const F3 zf = minv_mul(A, s, lane, rf);
+            st3(v.r, row, rf);
+            st3(v.z, row, zf);
+            st3(pcur, row, zf);
+            part[2] = double(rf.x) * zf.x + double(rf.y) * zf.y + double(rf.z) * zf.z;
+        }
+    }
+    double tot[4];
+    if (!reduce_across_tiles<4>(part, v.partials, t, S, tot)) return;
+    if (threadIdx.x != 0) return;
+    if (mode == 0) S->bb = fmax(tot[1], 1.0e-30);
+    const double bb = S->bb;
+    S->rr = tot[0];
+    const double relres = sqrt(tot[0] / bb);
+    if (mode == 2) {
+        S->relres = relres;
+        return;
+    }
+    if (mode == 0) {
+        S->iters = 0;
+        S->restarts = 0;
+    } else {
+        S->restarts += 1;
+    }
+    S->relres = relres;
+    const InstParams& pr = params[tile.inst];
+    if (pr.tol2 > 0.0 && tot[0] <= pr.tol2 * bb) {
+        S->status = kConverged;
+        return;
+    }
+    S->rho = tot[2];
+    S->beta = 0.0;
+    S->status = (S->iters >= pr.max_iter) ? kStopped : kRunning;
+}
+
+// ---------------------------------------------------------------------------
+// PCG direction + SpMV: p = z + beta p_prev (beta = 0 right after a
+// (re)start), Ap = A p, sigma = p . Ap.  The direction update of the
+// reference's update_p_kernel (pcg_solver.cu:456-464) is fused here: every
+// gathered column recomputes z + beta p_prev, and the row's own p is written
+// to the other p buffer.  SpMV restates spmv_kernel (block_csr.cu:54-83) on
+// the wave-sliced ELL layout; dot_partial_kernel (pcg_solver.cu:340-354)
+// becomes an FP64 partial.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+                                                       InstState* __restrict__ st) {
+    const int t = logical_tile(g.ntiles);
+    if (t >= g.ntiles) return;
+    const Tile tile = g.tiles[t];
+    InstState* S = st + tile.inst;
+    if (S->status != kRunning) return;
+    const float beta = float(S->beta);
+    const int parity = S->parity;
+    const float* __restrict__ pold = parity ? v.p1 : v.p0;
+    float* __restrict__ pnew = parity ? v.p0 : v.p1;
+    const float* __restrict__ z = v.z;
+    double part[1] = {0.0};
+    const int tid = threadIdx.x;
+    if (tid < tile.nrows) {
+        const int row = tile.row0 + tid;
+        const int s = row >> 6, lane = row & 63;
+        const F3 pi = ld3(z, row) + ld3(pold, row) * beta;
+        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+        float a0 = dg[0] * pi.x + dg[64] * pi.y + dg[128] * pi.z;
+        float a1 = dg[192] * pi.x + dg[256] * pi.y + dg[320] * pi.z;
+        float a2 = dg[384] * pi.x + dg[448] * pi.y + dg[512] * pi.z;
+        const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
+        const int* colp = A.col + (size_t)sb * 64 + lane;
+        const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+        for (int k = 0; k < w; ++k) {
+            const int c = colp[k * 64];
+            if (c >= 0) {
+                const float* bk = vp + k * 9 * 64;
+                const F3 pj = ld3(z, c) + ld3(pold, c) * beta;
+                a0 += bk[0] * pj.x + bk[64] * pj.y + bk[128] * pj.z;
+                a1 += bk[192] * pj.x + bk[256] * pj.y + bk[320] * pj.z;
+                a2 += bk[384] * pj.x + bk[448] * pj.y + bk[512] * pj.z;
+            }
+        }
+        st3(v.ap, row, f3(a0, a1, a2));
+        st3(pnew, row, pi);
+        part[0] = double(pi.x) * a0 + double(pi.y) * a1 + double(pi.z) * a2;
+    }
+    double tot[1];
+    if (!reduce_across_tiles<1>(part, v.partials, t, S, tot)) return;
+    if (threadIdx.x == 0) S->sigma = tot[0];
+}
+
+// ---------------------------------------------------------------------------
+// PCG update: alpha = rho / sigma (guarded), x += alpha p (FP64), r -= alpha
+// Ap, z = M^-1 r, rho' = r . z, rr = r . r; the last workgroup forms beta,
+// counts the iteration and applies the stopping rule.  Restates
+// alpha_jacobi_dot_partial_kernel + reduce_rho_beta_kernel
+// (pcg_solver.cu:406-454) with FP64 scalars.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pcg_update_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+                                                         InstState* __restrict__ st,
+                                                         const InstParams* __restrict__ params) {
+    const int t = logical_tile(g.ntiles);
+    if (t >= g.ntiles) return;
+    const Tile tile = g.tiles[t];
+    InstState* S = st + tile.inst;
+    if (S->status != kRunning) return;
+    const double sig = S->sigma, rho = S->rho;
+    const double alpha = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;
+    const float af = float(alpha);
+    const float* __restrict__ p = S->parity ? v.p0 : v.p1;  // written by the SpMV
+    double part[2] = {0.0, 0.0};
+    const int tid = threadIdx.x;
+    if (tid < tile.nrows) {
+        const int row = tile.row0 + tid;
+        const int s = row >> 6, lane = row & 63;
+        const F3 pi = ld3(p, row);
+        double* x = v.x + 3 * row;
+        x[0] += alpha * double(pi.x);
+        x[1] += alpha * double(pi.y);
+        x[2] += alpha * double(pi.z);
+        const F3 ri = ld3(v.r, row) - ld3(v.ap, row) * af;
+        const F3 zi = minv_mul(A, s, lane, ri);
+        st3(v.r, row, ri);
+        st3(v.z, row, zi);
+        part[0] = double(ri.x) * zi.x + double(ri.y) * zi.y + double(ri.z) * zi.z;
+        part[1] = double(ri.x) * ri.x + double(ri.y) * ri.y + double(ri.z) * ri.z;
+    }
+    double tot[2];
+    if (!reduce_across_tiles<2>(part, v.partials, t, S, tot)) return;
+    if (threadIdx.x != 0) return;
+    const double rho_new = tot[0];
+    S->beta = (rho > kGuard || rho < -kGuard) ? rho_new / rho : 0.0;
+    S->rho = rho_new;
+    S->rr = tot[1];
+    S->parity ^= 1;
+    S->iters += 1;
+    const InstParams& pr = params[tile.inst];
+    if (pr.tol2 > 0.0 && tot[1] <= pr.tol2 * S->bb) S->status = kCheck;
+    else if (S->iters >= pr.max_iter) S->status = kStopped;
+}
+
+// finalize_step_kernel (cloth_simulator.cu:222-244): x = x_n + dx,
+// v = dx / dt (times exp(-damping dt) when damping > 0).
+__global__ __launch_bounds__(256) void finalize_kernel(LaunchGeom g, DevVectors v, DevExt e,
+                                                       const InstParams* __restrict__ params, float dt) {
+    const int t = logical_tile(g.ntiles);
+    if (t >= g.ntiles) return;
+    const Tile tile = g.tiles[t];
+    const int tid = threadIdx.x;
+    if (tid >= tile.nrows) return;
+    const int row = tile.row0 + tid, p = tile.part0 + tid;
+    const InstParams pr = params[tile.inst];
+    const F3 d = f3(float(v.x[3 * row]), float(v.x[3 * row + 1]), float(v.x[3 * row + 2]));
+    const F3 xn = ld3(e.pos, p);
+    const float inv_dt = 1.0f / dt;
+    F3 vn = d * inv_dt;
+    if (pr.damping > 0.0f) vn = vn * expf(-pr.damping * dt);
+    st3(e.pos, p, xn + d);
+    st3(e.vel, p, vn);
+}
+
+// Plain y = A x on the ELL layout (spmv_kernel, block_csr.cu:54-83, alpha=1, beta=0).
+__global__ __launch_bounds__(256) void spmv_kernel(LaunchGeom g, DevMatrix A, const float* __restrict__ x,
+                                                   float* __restrict__ y) {
+    const int t = logical_tile(g.ntiles);
+    if (t >= g.ntiles) return;
+    const Tile tile = g.tiles[t];
+    const int tid = threadIdx.x;
+    if (tid >= tile.nrows) return;
+    const int row = tile.row0 + tid;
+    const int s = row >> 6, lane = row & 63;
+    const F3 xi = ld3(x, row);
+    const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+    float a0 = dg[0] * xi.x + dg[64] * xi.y + dg[128] * xi.z;
+    float a1 = dg[192] * xi.x + dg[256] * xi.y + dg[320] * xi.z;
+    float a2 = dg[384] * xi.x + dg[448] * xi.y + dg[512] * xi.z;
+    const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
+    const int* colp = A.col + (size_t)sb * 64 + lane;
+    const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+    for (int k = 0; k < w; ++k) {
+        const int c = colp[k * 64];
+        if (c >= 0) {
+            const float* bk = vp + k * 9 * 64;
+            const F3 xj = ld3(x, c);
+            a0 += bk[0] * xj.x + bk[64] * xj.y + bk[128] * xj.z;
+            a1 += bk[192] * xj.x + bk[256] * xj.y + bk[320] * xj.z;
+            a2 += bk[384] * xj.x + bk[448] * xj.y + bk[512] * xj.z;
+        }
+    }
+    st3(y, row, f3(a0, a1, a2));
+}
+
+}  // namespace
+
+#define CHYSX_LAUNCH_CHECK()                                                            \
+    do {                                                                                \
+        const hipError_t err_ = hipGetLastError();                                      \
+        if (err_ != hipSuccess) throw_hip(err_, __func__);                              \
+    } while (0)
+
+[[noreturn]] void throw_hip(hipError_t err, const char* what);
+
+void launch_rest_lengths(const int* spr_v, const float* pos, float* rest, int n, hipStream_t s) {
+    if (n <= 0) return;
+    rest_length_kernel<<<(n + 255) / 256, 256, 0, s>>>(spr_v, pos, rest, n);
+    CHYSX_LAUNCH_CHECK();
+}
+void launch_rest_shapes(const int* tri_v, const float* pos, float4* dminv, float* area, int n, float ct, float st,
+                        hipStream_t s) {
+    if (n <= 0) return;
+    rest_shape_kernel<<<(n + 255) / 256, 256, 0, s>>>(tri_v, pos, dminv, area, n, ct, st);
+    CHYSX_LAUNCH_CHECK();
+}
+void launch_area_mass(const int* vt_off, const int* vt_ids, const int* tri_v, const float* pos, float d3,
+                      float* inv_mass, int n, hipStream_t s) {
+    if (n <= 0) return;
+    area_mass_kernel<<<(n + 255) / 256, 256, 0, s>>>(vt_off, vt_ids, tri_v, pos, d3, inv_mass, n);
+    CHYSX_LAUNCH_CHECK();
+}
+void launch_assemble(const LaunchGeom& g, const DevMesh& m, const DevMatrix& A, const DevVectors& v,
+                     const DevExt& e, const InstParams* params, float dt, hipStream_t s) {
+    if (g.ntiles <= 0) return;
+    assemble_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, m, A, v, e, params, dt);
+    CHYSX_LAUNCH_CHECK();
+}
+void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
+                         const InstParams* params, int mode, hipStream_t s) {
+    if (g.ntiles <= 0) return;
+    pcg_residual_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, v, st, params, mode);
+    CHYSX_LAUNCH_CHECK();
+}
+void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st, hipStream_t s) {
+    if (g.ntiles <= 0) return;
+    pcg_spmv_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, v, st);
+    CHYSX_LAUNCH_CHECK();
+}
+void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
+                       const InstParams* params, hipStream_t s) {
+    if (g.ntiles <= 0) return;
+    pcg_update_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, v, st, params);
+    CHYSX_LAUNCH_CHECK();
+}
+void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
+                     hipStream_t s) {
+    if (g.ntiles <= 0) return;
+    finalize_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, v, e, params, dt);
+    CHYSX_LAUNCH_CHECK();
+}
+void launch_spmv(const LaunchGeom& g, const DevMatrix& A, const float* x, float* y, hipStream_t s) {
+    if (g.ntiles <= 0) return;
+    spmv_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, x, y);
+    CHYSX_LAUNCH_CHECK();
+}
+
+}  // namespace chysx_amd

@@ -1,0 +1,89 @@
+// SPDX-License-Identifier: Apache-2.0
+//
+// Device views and launch wrappers for the gfx950 cloth kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cloth_device.hpp"
+
+namespace chysx_amd {
+
+// Element data (device pointers).  Vertex indices are particle indices.
+struct DevMesh {
+    const int* tri_v;         // 3 per triangle
+    const float4* tri_dminv;  // (D00, D01, D10, D11) per triangle
+    const float* tri_area;
+    const float* tri_k;
+    const int* spr_v;         // 2 per spring
+    const float* spr_rest;
+    const float* spr_k;
+    const int* pin_v;
+    const float* pin_tgt;     // 3 per pin
+    // row-owned incidence (row space)
+    const int* tri_inc_off;
+    const int2* tri_inc;
+    const int* spr_inc_off;
+    const int2* spr_inc;
+    const int* pin_inc_off;
+    const int* pin_inc;
+};
+
+// Wave-sliced ELL matrix: entry (s, k, lane) of component c lives at
+// val[((slice_ptr[s] + k) * 9 + c) * 64 + lane]; diag[(s * 9 + c) * 64 + lane];
+// symmetric inverse diagonal minv[(s * 6 + c) * 64 + lane] with c over
+// (xx, xy, xz, yy, yz, zz).
+struct DevMatrix {
+    const int* slice_ptr;
+    const int* col;
+    const int* row_deg;
+    float* val;
+    float* diag;
+    float* minv;
+};
+
+// Solver vectors in padded row space, 3 components per row (AoS).
+struct DevVectors {
+    float* rhs;
+    float* r;
+    float* z;
+    float* ap;
+    float* p0;
+    float* p1;
+    double* x;         // the iterate dx (FP64)
+    double* partials;  // 4 per tile
+};
+
+// Caller-owned buffers (particle space).
+struct DevExt {
+    float* pos;
+    float* vel;
+    const float* inv_mass;  // may be null (mass 1)
+    const float* force;     // may be null
+};
+
+struct LaunchGeom {
+    const Tile* tiles;
+    int ntiles;
+};
+
+void launch_rest_lengths(const int* spr_v, const float* pos, float* rest, int nspring, hipStream_t s);
+void launch_rest_shapes(const int* tri_v, const float* pos, float4* dminv, float* area, int ntri, float cos_t,
+                        float sin_t, hipStream_t s);
+void launch_assemble(const LaunchGeom& g, const DevMesh& m, const DevMatrix& A, const DevVectors& v,
+                     const DevExt& e, const InstParams* params, float dt, hipStream_t s);
+// mode 0: initial residual (starts a solve); 1: true-residual check after a
+// recursive-residual pass; 2: report the true residual of a stopped solve.
+void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
+                         const InstParams* params, int mode, hipStream_t s);
+void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st, hipStream_t s);
+void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
+                       const InstParams* params, hipStream_t s);
+void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
+                     hipStream_t s);
+void launch_area_mass(const int* vt_off, const int* vt_ids, const int* tri_v, const float* pos, float density_over_3,
+                      float* inv_mass_out, int n, hipStream_t s);
+// Plain SpMV y = A x (FP32 in, FP32 out) over the tiles, for benchmarks and tests.
+void launch_spmv(const LaunchGeom& g, const DevMatrix& A, const float* x, float* y, hipStream_t s);
+
+}  // namespace chysx_amd

@@ -1,0 +1,779 @@
+// SPDX-License-Identifier: Apache-2.0
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace chysx_amd {
+
+[[noreturn]] void throw_hip(hipError_t err, const char* what) {
+    throw std::runtime_error(std::string("chysx: ") + what + " failed: " + hipGetErrorString(err));
+}
+
+void check_hip(hipError_t err, const char* what) {
+    if (err != hipSuccess) throw_hip(err, what);
+}
+
+// ---------------------------------------------------------------------------
+// DeviceArray
+// ---------------------------------------------------------------------------
+template <class T>
+void DeviceArray<T>::resize(size_t n) {
+    if (n == n_ && ptr_) return;
+    release();
+    if (n == 0) return;
+    check_hip(hipMalloc(reinterpret_cast<void**>(&ptr_), n * sizeof(T)), "hipMalloc");
+    n_ = n;
+}
+template <class T>
+void DeviceArray<T>::assign(const T* host, size_t n, hipStream_t s) {
+    resize(n);
+    if (n == 0) return;
+    check_hip(hipMemcpyAsync(ptr_, host, n * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync(H2D)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+template <class T>
+void DeviceArray<T>::zero(hipStream_t s) {
+    if (n_ == 0) return;
+    check_hip(hipMemsetAsync(ptr_, 0, n_ * sizeof(T), s), "hipMemsetAsync");
+}
+template <class T>
+void DeviceArray<T>::release() {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    n_ = 0;
+}
+template class DeviceArray<int>;
+template class DeviceArray<int2>;
+template class DeviceArray<float>;
+template class DeviceArray<float4>;
+template class DeviceArray<double>;
+template class DeviceArray<Tile>;
+template class DeviceArray<InstState>;
+template class DeviceArray<InstParams>;
+
+// ---------------------------------------------------------------------------
+// Rest data
+// ---------------------------------------------------------------------------
+void compute_rest_springs(Instance& inst, const float* pos, const float* k_per_edge, float k, hipStream_t s) {
+    const int E = int(inst.edges.size() / 2);
+    inst.spr_rest.assign(size_t(E), 0.0f);
+    inst.spr_k.assign(size_t(E), k);
+    if (k_per_edge) std::copy(k_per_edge, k_per_edge + E, inst.spr_k.begin());
+    inst.has_springs = true;
+    if (E == 0) return;
+    std::vector<int> v(inst.edges);
+    for (int& i : v) i += inst.part0;
+    DeviceArray<int> dv;
+    DeviceArray<float> rest;
+    dv.assign(v.data(), v.size(), s);
+    rest.resize(size_t(E));
+    launch_rest_lengths(dv.data(), pos, rest.data(), E, s);
+    check_hip(hipMemcpyAsync(inst.spr_rest.data(), rest.data(), sizeof(float) * E, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(rest lengths)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(rest lengths)");
+}
+
+void compute_rest_stretch(Instance& inst, const float* pos, const float* k_per_tri, float k, hipStream_t s) {
+    const int T = int(inst.tris.size() / 3);
+    inst.tri_dminv.assign(size_t(4) * T, 0.0f);
+    inst.tri_area.assign(size_t(T), 0.0f);
+    inst.tri_k.assign(size_t(T), k);
+    if (k_per_tri) std::copy(k_per_tri, k_per_tri + T, inst.tri_k.begin());
+    inst.has_stretch = true;
+    if (T == 0) return;
+    std::vector<int> v(inst.tris);
+    for (int& i : v) i += inst.part0;
+    DeviceArray<int> dv;
+    DeviceArray<float4> dm;
+    DeviceArray<float> ar;
+    dv.assign(v.data(), v.size(), s);
+    dm.resize(size_t(T));
+    ar.resize(size_t(T));
+    launch_rest_shapes(dv.data(), pos, dm.data(), ar.data(), T, 1.0f, 0.0f, s);
+    check_hip(hipMemcpyAsync(inst.tri_dminv.data(), dm.data(), sizeof(float4) * T, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(rest shapes)");
+    check_hip(hipMemcpyAsync(inst.tri_area.data(), ar.data(), sizeof(float) * T, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(rest areas)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(rest shapes)");
+}
+
+// ---------------------------------------------------------------------------
+// Problem
+// ---------------------------------------------------------------------------
+Problem::Problem() = default;
+
+Problem::~Problem() {
+    destroy_graph();
+    if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+    if (host_states_) (void)hipHostFree(host_states_);
+}
+
+void Problem::destroy_graph() {
+    if (chunk_exec_) (void)hipGraphExecDestroy(chunk_exec_);
+    chunk_exec_ = nullptr;
+}
+
+void Problem::build(const std::vector<const Instance*>& insts) {
+    destroy_graph();
+    built_ = false;
+    ninst_ = int(insts.size());
+    pats_.assign(size_t(ninst_), Pattern());
+    tri_slots_.assign(size_t(ninst_), {});
+    spr_slots_.assign(size_t(ninst_), {});
+    pin_slots_.assign(size_t(ninst_), {});
+    inst_row0_.assign(size_t(ninst_), 0);
+    inst_n_.assign(size_t(ninst_), 0);
+    inst_slice0_.assign(size_t(ninst_), 0);
+    inst_nslices_.assign(size_t(ninst_), 0);
+    pin_offset_.assign(size_t(ninst_) + 1, 0);
+
+    std::vector<DeviceLayout> lay(static_cast<size_t>(ninst_));
+    int row0 = 0, slice0 = 0;
+    n_rows_ = 0;
+    nnz_total_ = 0;
+    size_t ntri_tot = 0, nspr_tot = 0;
+    for (int k = 0; k < ninst_; ++k) {
+        const Instance& I = *insts[size_t(k)];
+        const int T = I.has_stretch ? int(I.tris.size() / 3) : 0;
+        const int E = (I.has_springs && I.springs_active) ? int(I.edges.size() / 2) : 0;
+        const int P = int(I.pin_idx.size());
+        const int* tris = T ? I.tris.data() : nullptr;
+        const int* spr = E ? I.edges.data() : nullptr;
+        pats_[size_t(k)] = build_pattern(I.n, tris, T, spr, E);
+        build_slot_maps(pats_[size_t(k)], tris, T, spr, E, P ? I.pin_idx.data() : nullptr, P,
+                        tri_slots_[size_t(k)], spr_slots_[size_t(k)], pin_slots_[size_t(k)]);
+        lay[size_t(k)] = build_device_layout(pats_[size_t(k)], tris, T, tri_slots_[size_t(k)], spr, E,
+                                             spr_slots_[size_t(k)], P ? I.pin_idx.data() : nullptr, P);
+        inst_row0_[size_t(k)] = row0;
+        inst_n_[size_t(k)] = I.n;
+        inst_slice0_[size_t(k)] = slice0;
+        inst_nslices_[size_t(k)] = lay[size_t(k)].nslices;
+        row0 += lay[size_t(k)].n_pad;
+        slice0 += lay[size_t(k)].nslices;
+        n_rows_ += I.n;
+        nnz_total_ += (long long)pats_[size_t(k)].cols.size();
+        ntri_tot += size_t(T);
+        nspr_tot += size_t(E);
+        pin_offset_[size_t(k) + 1] = pin_offset_[size_t(k)] + P;
+    }
+    const int old_pad = n_pad_;
+    n_pad_ = row0;
+    const int nslices = slice0;
+
+    // ---- concatenate layouts ---------------------------------------------
+    slice_ptr_h_.assign(size_t(nslices) + 1, 0);
+    row_deg_h_.assign(size_t(n_pad_), 0);
+    std::vector<int> tri_v, spr_v, pin_v, tri_inc_off(size_t(n_pad_) + 1, 0), spr_inc_off(size_t(n_pad_) + 1, 0),
+        pin_inc_off(size_t(n_pad_) + 1, 0), pin_inc;
+    std::vector<int2> tri_inc, spr_inc;
+    std::vector<float> tri_dminv, tri_area, tri_k, spr_rest, spr_k, pin_tgt;
+    tri_v.reserve(3 * ntri_tot);
+    spr_v.reserve(2 * nspr_tot);
+    int entries = 0;
+    int tri_base = 0, spr_base = 0;
+    for (int k = 0; k < ninst_; ++k) {
+        const Instance& I = *insts[size_t(k)];
+        const DeviceLayout& L = lay[size_t(k)];
+        const int r0 = inst_row0_[size_t(k)], s0 = inst_slice0_[size_t(k)];
+        for (int s = 0; s < L.nslices; ++s) slice_ptr_h_[size_t(s0 + s) + 1] = entries + L.slice_ptr[size_t(s) + 1];
+        entries += L.slice_ptr[size_t(L.nslices)];
+        for (int r = 0; r < L.n_pad; ++r) row_deg_h_[size_t(r0 + r)] = L.row_deg[size_t(r)];
+        const int T = I.has_stretch ? int(I.tris.size() / 3) : 0;
+        const int E = (I.has_springs && I.springs_active) ? int(I.edges.size() / 2) : 0;
+        const int P = int(I.pin_idx.size());
+        for (int q = 0; q < 3 * T; ++q) tri_v.push_back(I.tris[size_t(q)] + I.part0);
+        tri_dminv.insert(tri_dminv.end(), I.tri_dminv.begin(), I.tri_dminv.begin() + 4 * T);
+        tri_area.insert(tri_area.end(), I.tri_area.begin(), I.tri_area.begin() + T);
+        tri_k.insert(tri_k.end(), I.tri_k.begin(), I.tri_k.begin() + T);
+        for (int q = 0; q < 2 * E; ++q) spr_v.push_back(I.edges[size_t(q)] + I.part0);
+        if (E) {
+            spr_rest.insert(spr_rest.end(), I.spr_rest.begin(), I.spr_rest.begin() + E);
+            spr_k.insert(spr_k.end(), I.spr_k.begin(), I.spr_k.begin() + E);
+        }
+        for (int c = 0; c < P; ++c) {
+            pin_v.push_back(I.pin_idx[size_t(c)] + I.part0);
+            for (int d = 0; d < 3; ++d) pin_tgt.push_back(I.pin_tgt[size_t(3 * c + d)]);
+        }
+        // incidence, rows offset by r0, element ids offset by bases
+        const int tb = int(tri_inc.size()), sbs = int(spr_inc.size()), pb = int(pin_inc.size());
+        for (int r = 0; r < L.n_pad; ++r) {
+            tri_inc_off[size_t(r0 + r) + 1] = tb + L.tri_inc_off[size_t(r) + 1];
+            spr_inc_off[size_t(r0 + r) + 1] = sbs + L.spr_inc_off[size_t(r) + 1];
+            pin_inc_off[size_t(r0 + r) + 1] = pb + L.pin_inc_off[size_t(r) + 1];
+        }
+        for (size_t q = 0; q < L.tri_inc.size() / 2; ++q) {
+            const int id = L.tri_inc[2 * q];
+            tri_inc.push_back(make_int2(((id >> 2) + tri_base) * 4 + (id & 3), L.tri_inc[2 * q + 1]));
+        }
+        for (size_t q = 0; q < L.spr_inc.size() / 2; ++q) {
+            const int id = L.spr_inc[2 * q];
+            spr_inc.push_back(make_int2(((id >> 1) + spr_base) * 2 + (id & 1), L.spr_inc[2 * q + 1]));
+        }
+        for (int c : L.pin_inc) pin_inc.push_back(c + pin_offset_[size_t(k)]);
+        tri_base += T;
+        spr_base += E;
+        if (tri_base > (1 << 29)) throw std::invalid_argument("too many triangles");
+    }
+    // fill leading offsets of rows with no later entry (monotone already by construction)
+    std::vector<int> sell_col(size_t(entries) * kSlice, -1);
+    for (int k = 0; k < ninst_; ++k) {
+        const DeviceLayout& L = lay[size_t(k)];
+        const int r0 = inst_row0_[size_t(k)], s0 = inst_slice0_[size_t(k)];
+        const size_t base = size_t(slice_ptr_h_[size_t(s0)]) * kSlice;
+        for (size_t q = 0; q < L.sell_col.size(); ++q) {
+            const int c = L.sell_col[q];
+            sell_col[base + q] = (c >= 0) ? c + r0 : -1;
+        }
+    }
+
+    // ---- tiles -------------------------------------------------------------
+    tiles_h_.clear();
+    std::vector<InstState> st(static_cast<size_t>(ninst_));
+    std::memset(st.data(), 0, sizeof(InstState) * st.size());
+    for (int k = 0; k < ninst_; ++k) {
+        const Instance& I = *insts[size_t(k)];
+        st[size_t(k)].tile0 = int(tiles_h_.size());
+        for (int r = 0; r < I.n; r += 256)
+            tiles_h_.push_back(Tile{k, inst_row0_[size_t(k)] + r, std::min(256, I.n - r), I.part0 + r});
+        st[size_t(k)].ntiles = int(tiles_h_.size()) - st[size_t(k)].tile0;
+        st[size_t(k)].status = kStopped;
+    }
+
+    // ---- upload --------------------------------------------------------------
+    hipStream_t s = nullptr;
+    tiles_.assign(tiles_h_.data(), tiles_h_.size(), s);
+    state_.assign(st.data(), st.size(), s);
+    tri_v_.assign(tri_v.data(), tri_v.size(), s);
+    tri_dminv_.assign(reinterpret_cast<const float4*>(tri_dminv.data()), tri_dminv.size() / 4, s);
+    tri_area_.assign(tri_area.data(), tri_area.size(), s);
+    tri_k_.assign(tri_k.data(), tri_k.size(), s);
+    spr_v_.assign(spr_v.data(), spr_v.size(), s);
+    spr_rest_.assign(spr_rest.data(), spr_rest.size(), s);
+    spr_k_.assign(spr_k.data(), spr_k.size(), s);
+    pin_v_.assign(pin_v.data(), pin_v.size(), s);
+    pin_tgt_.assign(pin_tgt.data(), pin_tgt.size(), s);
+    tri_inc_off_.assign(tri_inc_off.data(), tri_inc_off.size(), s);
+    tri_inc_.assign(tri_inc.data(), tri_inc.size(), s);
+    spr_inc_off_.assign(spr_inc_off.data(), spr_inc_off.size(), s);
+    spr_inc_.assign(spr_inc.data(), spr_inc.size(), s);
+    pin_inc_off_.assign(pin_inc_off.data(), pin_inc_off.size(), s);
+    pin_inc_.assign(pin_inc.data(), pin_inc.size(), s);
+    slice_ptr_.assign(slice_ptr_h_.data(), slice_ptr_h_.size(), s);
+    col_.assign(sell_col.data(), sell_col.size(), s);
+    row_deg_.assign(row_deg_h_.data(), row_deg_h_.size(), s);
+    val_.resize(size_t(entries) * kSlice * 9);
+    val_.zero(s);
+    diag_.resize(size_t(nslices) * kSlice * 9);
+    diag_.zero(s);
+    minv_.resize(size_t(nslices) * kSlice * 6);
+    minv_.zero(s);
+    const size_t v3 = size_t(n_pad_) * 3;
+    for (DeviceArray<float>* a : {&rhs_, &r_, &z_, &ap_, &p0_, &p1_}) {
+        a->resize(v3);
+        a->zero(s);
+    }
+    if (old_pad != n_pad_ || x_.size() != v3) {
+        x_.resize(v3);
+        x_.zero(s);
+    }
+    partials_.resize(std::max<size_t>(size_t(4) * tiles_h_.size(), 4));
+    partials_.zero(s);
+    params_.resize(size_t(std::max(ninst_, 1)));
+    if (host_states_) (void)hipHostFree(host_states_);
+    host_states_ = nullptr;
+    check_hip(hipHostMalloc(reinterpret_cast<void**>(&host_states_), sizeof(InstState) * std::max(ninst_, 1)),
+              "hipHostMalloc(states)");
+    std::memcpy(host_states_, st.data(), sizeof(InstState) * st.size());
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(build)");
+    stats_.assign(size_t(ninst_), SolveStats());
+    last_iters_ = 0;
+    built_ = true;
+}
+
+void Problem::upload_params(const std::vector<const Instance*>& insts, hipStream_t s) {
+    std::vector<InstParams> p(insts.size());
+    for (size_t k = 0; k < insts.size(); ++k) {
+        const Instance& I = *insts[k];
+        InstParams& q = p[k];
+        q.tol2 = I.tol > 0.0 ? I.tol * I.tol : 0.0;
+        q.max_iter = I.max_iter;
+        q.gx = I.material.gx;
+        q.gy = I.material.gy;
+        q.gz = I.material.gz;
+        q.damping = I.material.damping;
+        q.pin_k = I.pin_k;
+        q.force_active = I.force_active ? 1 : 0;
+        q.springs_active = (I.springs_active && I.has_springs) ? 1 : 0;
+    }
+    if (p.empty()) return;
+    check_hip(hipMemcpyAsync(params_.data(), p.data(), sizeof(InstParams) * p.size(), hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(params)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(params)");
+}
+
+void Problem::update_pin_targets(const Instance& inst, int k, hipStream_t s) {
+    const int P = int(inst.pin_idx.size());
+    if (P == 0) return;
+    check_hip(hipMemcpyAsync(pin_tgt_.data() + 3 * pin_offset_[size_t(k)], inst.pin_tgt.data(), sizeof(float) * 3 * P,
+                             hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(pin targets)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(pin targets)");
+}
+
+DevMatrix Problem::matrix() const {
+    return DevMatrix{slice_ptr_.data(), col_.data(), row_deg_.data(), val_.data(), diag_.data(), minv_.data()};
+}
+DevVectors Problem::vectors() const {
+    return DevVectors{rhs_.data(), r_.data(), z_.data(), ap_.data(), p0_.data(), p1_.data(), x_.data(),
+                      partials_.data()};
+}
+DevMesh Problem::mesh() const {
+    return DevMesh{tri_v_.data(),       tri_dminv_.data(),   tri_area_.data(), tri_k_.data(),       spr_v_.data(),
+                   spr_rest_.data(),    spr_k_.data(),       pin_v_.data(),    pin_tgt_.data(),     tri_inc_off_.data(),
+                   tri_inc_.data(),     spr_inc_off_.data(), spr_inc_.data(),  pin_inc_off_.data(), pin_inc_.data()};
+}
+
+double Problem::spmv_bytes() const { return 40.0 * double(nnz_total_) + 64.0 * double(n_rows_) + 4.0; }
+
+void Problem::launch_assemble(float dt, const DevExt& ext, hipStream_t s) {
+    chysx_amd::launch_assemble(geom(), mesh(), matrix(), vectors(), ext, params_.data(), dt, s);
+}
+void Problem::launch_pcg_start(hipStream_t s) {
+    launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), params_.data(), 0, s);
+}
+void Problem::capture_chunk_graph() {
+    if (!cap_stream_) check_hip(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hipGraph_t graph = nullptr;
+    check_hip(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        for (int i = 0; i < kChunkIters; ++i) {
+            launch_pcg_spmv(geom(), matrix(), vectors(), state_.data(), cap_stream_);
+            launch_pcg_update(geom(), matrix(), vectors(), state_.data(), params_.data(), cap_stream_);
+            launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), params_.data(), 1, cap_stream_);
+        }
+    } catch (...) {
+        (void)hipStreamEndCapture(cap_stream_, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    check_hip(hipStreamEndCapture(cap_stream_, &graph), "hipStreamEndCapture");
+    hipError_t e = hipGraphInstantiate(&chunk_exec_, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    check_hip(e, "hipGraphInstantiate");
+}
+void Problem::launch_pcg_chunk(hipStream_t s) {
+    if (!chunk_exec_) capture_chunk_graph();
+    check_hip(hipGraphLaunch(chunk_exec_, s), "hipGraphLaunch(pcg chunk)");
+}
+void Problem::launch_finalize(float dt, const DevExt& ext, hipStream_t s) {
+    chysx_amd::launch_finalize(geom(), vectors(), ext, params_.data(), dt, s);
+}
+void Problem::read_states(hipStream_t s) {
+    check_hip(hipMemcpyAsync(host_states_, state_.data(), sizeof(InstState) * ninst_, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(states)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(states)");
+}
+bool Problem::all_done() const {
+    for (int k = 0; k < ninst_; ++k) {
+        const int st = host_states_[k].status;
+        if (st != kConverged && st != kStopped) return false;
+    }
+    return true;
+}
+
+void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
+    if (!built_ || ninst_ == 0) return;
+    launch_assemble(dt, ext, s);
+    launch_pcg_start(s);
+    // Launch about as many iterations as the previous step needed before the
+    // first poll; chunks after convergence exit at once.
+    int chunks = std::max(1, (last_iters_ + 1) / kChunkIters);
+    int guard = 0;
+    for (;;) {
+        for (int c = 0; c < chunks; ++c) launch_pcg_chunk(s);
+        read_states(s);
+        if (all_done()) break;
+        chunks = 1;
+        if (++guard > (1 << 22)) throw std::runtime_error("PCG did not terminate");
+    }
+    bool any_stopped = false;
+    int max_it = 0;
+    for (int k = 0; k < ninst_; ++k) {
+        any_stopped |= host_states_[k].status == kStopped;
+        max_it = std::max(max_it, host_states_[k].iters);
+    }
+    if (any_stopped) {
+        launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), params_.data(), 2, s);
+        read_states(s);
+    }
+    launch_finalize(dt, ext, s);
+    for (int k = 0; k < ninst_; ++k) {
+        SolveStats& q = stats_[size_t(k)];
+        q.iters = host_states_[k].iters;
+        q.restarts = host_states_[k].restarts;
+        q.status = host_states_[k].status;
+        q.relres = host_states_[k].relres;
+    }
+    last_iters_ = max_it;
+}
+
+// ---- reference-layout copies --------------------------------------------
+namespace {
+template <class T>
+std::vector<T> download(const T* dev, size_t n) {
+    std::vector<T> h(n);
+    if (n) {
+        check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize(debug)");
+        check_hip(hipMemcpy(h.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy(debug)");
+    }
+    return h;
+}
+}  // namespace
+
+void Problem::copy_diag(int k, float* out) const {
+    const std::vector<float> d = download(diag_.data(), diag_.size());
+    const int r0 = inst_row0_[size_t(k)];
+    for (int i = 0; i < inst_n_[size_t(k)]; ++i) {
+        const int row = r0 + i, s = row / kSlice, lane = row % kSlice;
+        for (int c = 0; c < 9; ++c) out[9 * i + c] = d[(size_t(s) * 9 + c) * kSlice + lane];
+    }
+}
+void Problem::copy_values(int k, float* out) const {
+    const std::vector<float> v = download(val_.data(), val_.size());
+    const Pattern& P = pats_[size_t(k)];
+    const int r0 = inst_row0_[size_t(k)];
+    for (int i = 0; i < P.n; ++i) {
+        const int row = r0 + i, s = row / kSlice, lane = row % kSlice;
+        const int sb = slice_ptr_h_[size_t(s)];
+        for (int q = P.row_offsets[size_t(i)]; q < P.row_offsets[size_t(i) + 1]; ++q) {
+            const int kk = q - P.row_offsets[size_t(i)];
+            for (int c = 0; c < 9; ++c) out[9 * size_t(q) + c] = v[((size_t(sb) + kk) * 9 + c) * kSlice + lane];
+        }
+    }
+}
+void Problem::copy_rhs(int k, float* out) const {
+    const std::vector<float> v = download(rhs_.data(), rhs_.size());
+    const int r0 = inst_row0_[size_t(k)];
+    std::copy(v.begin() + 3 * size_t(r0), v.begin() + 3 * size_t(r0 + inst_n_[size_t(k)]), out);
+}
+void Problem::copy_dx64(int k, double* out) const {
+    const std::vector<double> v = download(x_.data(), x_.size());
+    const int r0 = inst_row0_[size_t(k)];
+    std::copy(v.begin() + 3 * size_t(r0), v.begin() + 3 * size_t(r0 + inst_n_[size_t(k)]), out);
+}
+void Problem::copy_dx(int k, float* out) const {
+    std::vector<double> d(static_cast<size_t>(3 * inst_n_[size_t(k)]));
+    copy_dx64(k, d.data());
+    for (size_t i = 0; i < d.size(); ++i) out[i] = float(d[i]);
+}
+void Problem::set_dx64(int k, const double* in, hipStream_t s) {
+    const int r0 = inst_row0_[size_t(k)];
+    check_hip(hipMemcpyAsync(x_.data() + 3 * size_t(r0), in, sizeof(double) * 3 * inst_n_[size_t(k)],
+                             hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync(dx)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(dx)");
+}
+void Problem::zero_dx(hipStream_t s) { x_.zero(s); }
+
+// ---------------------------------------------------------------------------
+// ClothSim
+// ---------------------------------------------------------------------------
+ClothSim::ClothSim() = default;
+
+void ClothSim::set_external_buffers(std::uintptr_t pos, std::uintptr_t vel, int n, std::uintptr_t inv_mass,
+                                    std::uintptr_t force) noexcept {
+    pos_ = pos;
+    vel_ = vel;
+    inv_mass_ = inv_mass;
+    force_ = force;
+    n_ = n;
+}
+
+void ClothSim::set_mesh(const int* tris, int ntri) {
+    if (ntri < 0) throw std::invalid_argument("ClothSimulator::set_mesh: negative triangle count");
+    inst_.tris.assign(tris, tris + 3 * size_t(ntri));
+    inst_.edges = build_edges(tris, ntri);
+    topo_dirty_ = true;
+}
+
+namespace {
+// Kernels read positions at these indices: every index must lie in the
+// caller's buffer of n particles.
+void check_indices(const std::vector<int>& idx, int n, const char* what) {
+    for (int v : idx)
+        if (v < 0 || v >= n) throw std::out_of_range(std::string(what) + ": vertex index outside [0, particle_count)");
+}
+}  // namespace
+
+void ClothSim::build_springs(float k, std::uintptr_t stream) {
+    const int E = int(inst_.edges.size() / 2);
+    if (E > 0 && pos_ == 0)
+        throw std::runtime_error(
+            "ClothSimulator::build_springs_from_current_positions: external positions must be set first via "
+            "set_external_buffers().");
+    check_indices(inst_.edges, n_, "build_springs_from_current_positions");
+    inst_.part0 = 0;
+    compute_rest_springs(inst_, reinterpret_cast<const float*>(pos_), nullptr, k,
+                         reinterpret_cast<hipStream_t>(stream));
+    topo_dirty_ = true;
+}
+
+void ClothSim::build_fem_stretch(float k, std::uintptr_t stream) {
+    const int T = int(inst_.tris.size() / 3);
+    if (T > 0 && pos_ == 0)
+        throw std::runtime_error(
+            "ClothSimulator::build_fem_stretch_from_current_positions: external positions must be set first via "
+            "set_external_buffers().");
+    check_indices(inst_.tris, n_, "build_fem_stretch_from_current_positions");
+    inst_.part0 = 0;
+    compute_rest_stretch(inst_, reinterpret_cast<const float*>(pos_), nullptr, k,
+                         reinterpret_cast<hipStream_t>(stream));
+    topo_dirty_ = true;
+}
+
+void ClothSim::set_spring_stiffnesses(const float* k, int n) {
+    if (!inst_.has_springs) throw std::runtime_error("set_spring_stiffnesses: build springs first");
+    if (n != int(inst_.spr_k.size())) throw std::invalid_argument("set_spring_stiffnesses: need one value per spring");
+    std::copy(k, k + n, inst_.spr_k.begin());
+    topo_dirty_ = true;
+}
+
+void ClothSim::set_fem_stretch_stiffnesses(const float* k, int n) {
+    if (!inst_.has_stretch) throw std::runtime_error("set_fem_stretch_stiffnesses: build stretch first");
+    if (n != int(inst_.tri_k.size()))
+        throw std::invalid_argument("set_fem_stretch_stiffnesses: need one value per triangle");
+    std::copy(k, k + n, inst_.tri_k.begin());
+    topo_dirty_ = true;
+}
+
+void ClothSim::set_pins(const int* idx, const float* targets, int n, float k) {
+    if (n < 0) throw std::invalid_argument("PinConstraint::set_pins: negative count");
+    inst_.pin_k = k;
+    inst_.pin_idx.assign(idx, idx + n);
+    inst_.pin_tgt.assign(targets, targets + 3 * size_t(n));
+    topo_dirty_ = true;
+}
+
+void ClothSim::clear_pins() {
+    inst_.pin_idx.clear();
+    inst_.pin_tgt.clear();
+    topo_dirty_ = true;
+}
+
+void ClothSim::update_pin_targets(const float* targets, int n, std::uintptr_t stream) {
+    if (targets == nullptr || n <= 0) return;
+    if (n != int(inst_.pin_idx.size()))
+        throw std::invalid_argument(
+            "ClothSimulator::update_pin_targets: n_pins does not match the currently installed pin set; call "
+            "set_pins(...) to change the pin set instead.");
+    std::copy(targets, targets + 3 * size_t(n), inst_.pin_tgt.begin());
+    if (!topo_dirty_ && prob_.built()) prob_.update_pin_targets(inst_, 0, reinterpret_cast<hipStream_t>(stream));
+}
+
+void ClothSim::redistribute_mass_area_weighted(float density, std::uintptr_t inv_mass, int n,
+                                               std::uintptr_t stream) {
+    if (n <= 0) return;
+    if (inv_mass == 0)
+        throw std::invalid_argument("ClothSimulator::redistribute_mass_area_weighted: inv_mass_ptr must be non-null");
+    if (!(density > 0.0f))
+        throw std::invalid_argument("ClothSimulator::redistribute_mass_area_weighted: surface_density must be positive");
+    const int T = int(inst_.tris.size() / 3);
+    if (T == 0)
+        throw std::runtime_error(
+            "ClothSimulator::redistribute_mass_area_weighted: call set_mesh(...) before redistributing mass.");
+    if (pos_ == 0 || n_ < n)
+        throw std::runtime_error(
+            "ClothSimulator::redistribute_mass_area_weighted: external positions must be set first via "
+            "set_external_buffers().");
+    std::vector<int> off, ids;
+    build_vertex_triangles(n, inst_.tris.data(), T, off, ids);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    DeviceArray<int> doff, dids, dtri;
+    doff.assign(off.data(), off.size(), s);
+    dids.assign(ids.data(), ids.size(), s);
+    dtri.assign(inst_.tris.data(), inst_.tris.size(), s);
+    launch_area_mass(doff.data(), dids.data(), dtri.data(), reinterpret_cast<const float*>(pos_), density / 3.0f,
+                     reinterpret_cast<float*>(inv_mass), n, s);
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(area mass)");
+}
+
+void ClothSim::set_pcg_tolerance(double tol) {
+    if (!(tol >= 0.0)) throw std::invalid_argument("set_pcg_tolerance: tolerance must be >= 0");
+    inst_.tol = tol;
+    params_dirty_ = true;
+}
+
+void ClothSim::set_springs_active(bool on) {
+    if (on != inst_.springs_active) topo_dirty_ = true;
+    inst_.springs_active = on;
+    params_dirty_ = true;
+}
+
+void ClothSim::ensure_topology() {
+    if (!topo_dirty_ && prob_.built() && inst_.n == n_) return;
+    inst_.n = n_;
+    inst_.part0 = 0;
+    prob_.build({&inst_});
+    topo_dirty_ = false;
+    params_dirty_ = true;
+}
+
+void ClothSim::step(float dt, std::uintptr_t stream) {
+    if (n_ <= 0) return;
+    if (pos_ == 0 || vel_ == 0)
+        throw std::runtime_error(
+            "chysx::cloth::ClothSimulator::step: pos / vel spans must be set via set_external_buffers() before "
+            "stepping.");
+    if (!(dt > 0.0f)) throw std::invalid_argument("chysx::cloth::ClothSimulator::step: dt must be positive");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    ensure_topology();
+    prob_.upload_params({&inst_}, s);
+    params_dirty_ = false;
+    DevExt e{reinterpret_cast<float*>(pos_), reinterpret_cast<float*>(vel_),
+             reinterpret_cast<const float*>(inv_mass_), reinterpret_cast<const float*>(force_)};
+    prob_.step(dt, e, s);
+    dumped_n_ = n_;
+}
+
+int ClothSim::num_off_diag_blocks() const {
+    if (!prob_.built() || prob_.ninst() == 0) return 0;
+    return int(prob_.pattern(0).cols.size());
+}
+
+SolveStats ClothSim::last_stats() const {
+    if (!prob_.built() || prob_.ninst() == 0) return SolveStats();
+    return prob_.stats()[0];
+}
+
+void ClothSim::set_warm_start(const double* dx) {
+    ensure_topology();
+    prob_.set_dx64(0, dx, nullptr);
+}
+
+void ClothSim::zero_warm_start() {
+    if (prob_.built()) prob_.zero_dx(nullptr);
+}
+
+void ClothSim::debug_copy_hessian_diag(float* out) const {
+    if (dumped_n_ == 0) return;
+    if (!out) throw std::invalid_argument("ClothSimulator::debug_copy_hessian_diag: out must be non-null");
+    prob_.copy_diag(0, out);
+}
+void ClothSim::debug_copy_hessian_csr(int* ro, int* cols, float* vals) const {
+    if (dumped_n_ == 0) return;
+    if (!ro) throw std::invalid_argument("ClothSimulator::debug_copy_hessian_csr: out_row_offsets null");
+    const Pattern& P = prob_.pattern(0);
+    std::copy(P.row_offsets.begin(), P.row_offsets.end(), ro);
+    if (!P.cols.empty()) {
+        if (!cols || !vals)
+            throw std::invalid_argument(
+                "ClothSimulator::debug_copy_hessian_csr: col_indices/values null but nnz_off > 0");
+        std::copy(P.cols.begin(), P.cols.end(), cols);
+        prob_.copy_values(0, vals);
+    }
+}
+void ClothSim::debug_copy_last_rhs(float* out) const {
+    if (dumped_n_ == 0) return;
+    if (!out) throw std::invalid_argument("ClothSimulator::debug_copy_last_rhs: out must be non-null");
+    prob_.copy_rhs(0, out);
+}
+void ClothSim::debug_copy_last_dx(float* out) const {
+    if (dumped_n_ == 0) return;
+    if (!out) throw std::invalid_argument("ClothSimulator::debug_copy_last_dx: out must be non-null");
+    prob_.copy_dx(0, out);
+}
+
+// ---------------------------------------------------------------------------
+// ClothBatch
+// ---------------------------------------------------------------------------
+int ClothBatch::add_cloth(const int* tris, int ntri, int n, const float* spring_k, float spring_k_scalar,
+                          const float* stretch_k, float stretch_k_scalar, const int* pins, const float* pin_targets,
+                          int npin, float pin_k) {
+    if (ntri < 0 || n <= 0 || npin < 0) throw std::invalid_argument("ClothBatch::add_cloth: bad sizes");
+    auto I = std::make_unique<Instance>();
+    I->n = n;
+    I->part0 = total_;
+    I->tris.assign(tris, tris + 3 * size_t(ntri));
+    for (int v : I->tris)
+        if (v < 0 || v >= n) throw std::out_of_range("ClothBatch::add_cloth: triangle index out of range");
+    I->edges = build_edges(tris, ntri);
+    const int E = int(I->edges.size() / 2);
+    pending_spring_k_.emplace_back(spring_k ? std::vector<float>(spring_k, spring_k + E)
+                                            : std::vector<float>(size_t(E), spring_k_scalar));
+    pending_stretch_k_.emplace_back(stretch_k ? std::vector<float>(stretch_k, stretch_k + ntri)
+                                              : std::vector<float>(size_t(ntri), stretch_k_scalar));
+    I->pin_idx.assign(pins, pins + npin);
+    for (int v : I->pin_idx)
+        if (v < 0 || v >= n) throw std::out_of_range("ClothBatch::add_cloth: pin index out of range");
+    I->pin_tgt.assign(pin_targets, pin_targets + 3 * size_t(npin));
+    I->pin_k = pin_k;
+    I->springs_active = true;
+    total_ += n;
+    insts_.push_back(std::move(I));
+    rest_done_.push_back(0);
+    dirty_ = true;
+    return int(insts_.size()) - 1;
+}
+
+void ClothBatch::set_external_buffers(std::uintptr_t pos, std::uintptr_t vel, int total, std::uintptr_t inv_mass,
+                                      std::uintptr_t force) noexcept {
+    pos_ = pos;
+    vel_ = vel;
+    ext_n_ = total;
+    inv_mass_ = inv_mass;
+    force_ = force;
+}
+
+void ClothBatch::set_material(int k, const Material& m) {
+    insts_.at(size_t(k))->material = m;
+    params_dirty_ = true;
+}
+
+void ClothBatch::set_options(double tol, int max_iter, bool springs_active, bool force_active) {
+    for (auto& I : insts_) {
+        if (I->springs_active != springs_active) dirty_ = true;
+        I->tol = tol;
+        I->max_iter = max_iter > 0 ? max_iter : 1;
+        I->springs_active = springs_active;
+        I->force_active = force_active;
+    }
+    params_dirty_ = true;
+}
+
+void ClothBatch::step(float dt, std::uintptr_t stream) {
+    if (insts_.empty()) return;
+    if (pos_ == 0 || vel_ == 0) throw std::runtime_error("ClothBatch::step: set_external_buffers() first");
+    if (ext_n_ < total_) throw std::invalid_argument("ClothBatch::step: external buffers hold too few particles");
+    if (!(dt > 0.0f)) throw std::invalid_argument("ClothBatch::step: dt must be positive");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    for (size_t k = 0; k < insts_.size(); ++k) {
+        if (rest_done_[k]) continue;
+        Instance& I = *insts_[k];
+        compute_rest_springs(I, reinterpret_cast<const float*>(pos_), pending_spring_k_[k].data(), 0.0f, s);
+        compute_rest_stretch(I, reinterpret_cast<const float*>(pos_), pending_stretch_k_[k].data(), 0.0f, s);
+        rest_done_[k] = 1;
+        dirty_ = true;
+    }
+    std::vector<const Instance*> v;
+    for (auto& I : insts_) v.push_back(I.get());
+    if (dirty_ || !prob_.built()) {
+        prob_.build(v);
+        dirty_ = false;
+    }
+    prob_.upload_params(v, s);
+    params_dirty_ = false;
+    DevExt e{reinterpret_cast<float*>(pos_), reinterpret_cast<float*>(vel_),
+             reinterpret_cast<const float*>(inv_mass_), reinterpret_cast<const float*>(force_)};
+    prob_.step(dt, e, s);
+}
+
+SolveStats ClothBatch::stats(int k) const {
+    if (!prob_.built()) return SolveStats();
+    return prob_.stats().at(size_t(k));
+}
+
+}  // namespace chysx_amd

@@ -1,0 +1,290 @@
+// SPDX-License-Identifier: Apache-2.0
+//
+// Host engine of the implicit cloth step on gfx950.
+//
+// A Problem is one block-diagonal linear system made of one or more cloth
+// instances laid end to end in a padded row space; every kernel launch covers
+// all instances, each with its own FP64 PCG state and convergence status.
+// ClothSim (one cloth, the reference's ClothSimulator) and ClothBatch (many
+// cloths in one launch) are thin owners of Instances plus one Problem.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cloth_device.hpp"
+#include "cloth_kernels.hpp"
+#include "topology.hpp"
+
+namespace chysx_amd {
+
+struct Material {  // same field order as ClothMaterial (cloth_material.h:20-43)
+    float lame_mu = 0.0f;
+    float lame_lambda = 0.0f;
+    float bending = 0.0f;
+    float density = 0.05f;
+    float damping = 0.0f;
+    float gx = 0.0f;
+    float gy = 0.0f;
+    float gz = -9.81f;
+};
+
+template <class T>
+class DeviceArray {
+public:
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    ~DeviceArray() { release(); }
+    void resize(size_t n);          // contents undefined
+    void assign(const T* host, size_t n, hipStream_t s = nullptr);
+    void zero(hipStream_t s = nullptr);
+    void release();
+    T* data() const { return ptr_; }
+    size_t size() const { return n_; }
+
+private:
+    T* ptr_ = nullptr;
+    size_t n_ = 0;
+};
+
+// One cloth: mesh, element rest data, pins and solver options.
+struct Instance {
+    int n = 0;      // particles
+    int part0 = 0;  // first particle in the external buffers
+    std::vector<int> tris;   // 3 per triangle (local indices)
+    std::vector<int> edges;  // 2 per unique edge (local), from tris
+    // springs (one per edge); present once built
+    bool has_springs = false;
+    std::vector<float> spr_rest, spr_k;
+    // stretch triangles (theta = 0), present once built
+    bool has_stretch = false;
+    std::vector<float> tri_dminv;  // 4 per triangle
+    std::vector<float> tri_area, tri_k;
+    // pins
+    std::vector<int> pin_idx;
+    std::vector<float> pin_tgt;
+    float pin_k = 1.0e6f;
+    // options
+    bool springs_active = false;
+    bool force_active = false;
+    double tol = 0.0;
+    int max_iter = 50;
+    Material material;
+};
+
+struct SolveStats {
+    int iters = 0;
+    int restarts = 0;
+    int status = 0;
+    double relres = 0.0;
+};
+
+class Problem {
+public:
+    Problem();
+    ~Problem();
+    Problem(const Problem&) = delete;
+    Problem& operator=(const Problem&) = delete;
+
+    // Build topology, layout and device data for `insts` (all must outlive the
+    // call).  Keeps the FP64 iterate when the padded row count is unchanged.
+    void build(const std::vector<const Instance*>& insts);
+    bool built() const { return built_; }
+    // Upload per-instance parameters / options (cheap; call before a step
+    // when options changed).
+    void upload_params(const std::vector<const Instance*>& insts, hipStream_t s);
+    void update_pin_targets(const Instance& inst, int inst_index, hipStream_t s);
+
+    // One step over all instances.  Blocks until every PCG solve has ended.
+    void step(float dt, const DevExt& ext, hipStream_t s);
+
+    // Per-instance results of the last step.
+    std::vector<SolveStats> stats() const { return stats_; }
+    int n_rows_padded() const { return n_pad_; }
+    int ninst() const { return ninst_; }
+    int ntiles() const { return int(tiles_h_.size()); }
+
+    // Reference-layout views of instance `k` (host copies).
+    const Pattern& pattern(int k) const { return pats_[size_t(k)]; }
+    const std::vector<int>& tri_slots(int k) const { return tri_slots_[size_t(k)]; }
+    const std::vector<int>& spr_slots(int k) const { return spr_slots_[size_t(k)]; }
+    const std::vector<int>& pin_slots(int k) const { return pin_slots_[size_t(k)]; }
+    void copy_diag(int k, float* out) const;        // (n, 3, 3)
+    void copy_values(int k, float* out) const;      // (nnz, 3, 3) in CSR order
+    void copy_rhs(int k, float* out) const;         // (n, 3)
+    void copy_dx(int k, float* out) const;          // (n, 3) rounded to FP32
+    void copy_dx64(int k, double* out) const;       // (n, 3)
+    void set_dx64(int k, const double* in, hipStream_t s);
+    void zero_dx(hipStream_t s);
+
+    // Kernel-level access for benchmarks.
+    LaunchGeom geom() const { return LaunchGeom{tiles_.data(), int(tiles_h_.size())}; }
+    DevMatrix matrix() const;
+    DevVectors vectors() const;
+    DevMesh mesh() const;
+    InstState* states() const { return state_.data(); }
+    const InstParams* params() const { return params_.data(); }
+    // Bytes of the algorithmic SpMV traffic (reference formulation):
+    // 40 per off-diagonal block + 64 per row + 4.
+    double spmv_bytes() const;
+    long long nnz_total() const { return nnz_total_; }
+    int rows_total() const { return n_rows_; }
+
+    // Launch pieces (used by step and by the benchmark).
+    void launch_assemble(float dt, const DevExt& ext, hipStream_t s);
+    void launch_pcg_start(hipStream_t s);
+    void launch_pcg_chunk(hipStream_t s);  // kChunk iterations (graph when possible)
+    void launch_finalize(float dt, const DevExt& ext, hipStream_t s);
+    void read_states(hipStream_t s);        // blocking
+    bool all_done() const;
+
+    static constexpr int kChunkIters = 8;
+
+private:
+    void capture_chunk_graph();
+    void destroy_graph();
+
+    bool built_ = false;
+    int ninst_ = 0;
+    int n_rows_ = 0;  // real rows
+    int n_pad_ = 0;   // padded rows
+    long long nnz_total_ = 0;
+    std::vector<Pattern> pats_;
+    std::vector<std::vector<int>> tri_slots_, spr_slots_, pin_slots_;
+    std::vector<int> inst_row0_, inst_n_, inst_slice0_, inst_nslices_;
+    std::vector<Tile> tiles_h_;
+    std::vector<int> slice_ptr_h_;
+    std::vector<int> row_deg_h_;
+
+    DeviceArray<Tile> tiles_;
+    DeviceArray<InstState> state_;
+    DeviceArray<InstParams> params_;
+    // mesh
+    DeviceArray<int> tri_v_, spr_v_, pin_v_;
+    DeviceArray<float4> tri_dminv_;
+    DeviceArray<float> tri_area_, tri_k_, spr_rest_, spr_k_, pin_tgt_;
+    DeviceArray<int> tri_inc_off_, spr_inc_off_, pin_inc_off_, pin_inc_;
+    DeviceArray<int2> tri_inc_, spr_inc_;
+    std::vector<int> pin_offset_;  // first global pin of each instance
+    // matrix
+    DeviceArray<int> slice_ptr_, col_, row_deg_;
+    DeviceArray<float> val_, diag_, minv_;
+    // vectors
+    DeviceArray<float> rhs_, r_, z_, ap_, p0_, p1_;
+    DeviceArray<double> x_, partials_;
+
+    InstState* host_states_ = nullptr;  // pinned
+    std::vector<SolveStats> stats_;
+    int last_iters_ = 0;
+
+    hipStream_t cap_stream_ = nullptr;
+    hipGraphExec_t chunk_exec_ = nullptr;
+};
+
+// The reference's ClothSimulator behaviour on one instance.
+class ClothSim {
+public:
+    ClothSim();
+    void set_material(const Material& m) { inst_.material = m; params_dirty_ = true; }
+    Material& material() { params_dirty_ = true; return inst_.material; }
+    const Material& material() const { return inst_.material; }
+    void set_external_buffers(std::uintptr_t pos, std::uintptr_t vel, int n, std::uintptr_t inv_mass,
+                              std::uintptr_t force) noexcept;
+    void set_mesh(const int* tris, int ntri);
+    void build_springs(float k, std::uintptr_t stream);
+    void build_fem_stretch(float k, std::uintptr_t stream);
+    void set_spring_stiffnesses(const float* k, int n);
+    void set_fem_stretch_stiffnesses(const float* k, int n);
+    void set_pins(const int* idx, const float* targets, int n, float k);
+    void clear_pins();
+    void update_pin_targets(const float* targets, int n, std::uintptr_t stream);
+    void redistribute_mass_area_weighted(float density, std::uintptr_t inv_mass, int n, std::uintptr_t stream);
+    void set_pcg_iterations(int it) { inst_.max_iter = it > 0 ? it : 1; params_dirty_ = true; }
+    int pcg_iterations() const { return inst_.max_iter; }
+    void set_pcg_tolerance(double tol);
+    double pcg_tolerance() const { return inst_.tol; }
+    void set_springs_active(bool on);
+    void set_external_force_active(bool on) { inst_.force_active = on; params_dirty_ = true; }
+    void step(float dt, std::uintptr_t stream);
+
+    int num_springs() const { return inst_.has_springs ? int(inst_.edges.size() / 2) : 0; }
+    int num_fem_stretch() const { return inst_.has_stretch ? int(inst_.tris.size() / 3) : 0; }
+    int num_pins() const { return int(inst_.pin_idx.size()); }
+    int num_edges() const { return int(inst_.edges.size() / 2); }
+    const std::vector<int>& edges() const { return inst_.edges; }
+    int num_particles() const { return dumped_n_; }
+    int num_off_diag_blocks() const;
+    SolveStats last_stats() const;
+    Problem& problem() { return prob_; }
+    const Instance& instance() const { return inst_; }
+    // Build topology now (normally done lazily by step()).
+    void ensure_topology();
+    void set_warm_start(const double* dx);  // FP64 (n,3)
+    DevExt ext() const {
+        return DevExt{reinterpret_cast<float*>(pos_), reinterpret_cast<float*>(vel_),
+                      reinterpret_cast<const float*>(inv_mass_), reinterpret_cast<const float*>(force_)};
+    }
+    void zero_warm_start();
+
+    void debug_copy_hessian_diag(float* out) const;
+    void debug_copy_hessian_csr(int* ro, int* cols, float* vals) const;
+    void debug_copy_last_rhs(float* out) const;
+    void debug_copy_last_dx(float* out) const;
+
+private:
+    Instance inst_;
+    Problem prob_;
+    std::uintptr_t pos_ = 0, vel_ = 0, inv_mass_ = 0, force_ = 0;
+    int n_ = 0;
+    bool topo_dirty_ = true;
+    bool params_dirty_ = true;
+    int dumped_n_ = 0;
+};
+
+// Many cloths stepped together.  Particles of instance k live at
+// [part0_k, part0_k + n_k) of one caller-owned position / velocity buffer.
+class ClothBatch {
+public:
+    // Adds a cloth of n particles (the next n of the external buffers).
+    // spring_k / stretch_k: one value per edge / triangle, or null with the
+    // scalar.  Rest state is taken from the external positions at the first
+    // step after the cloth is added.
+    int add_cloth(const int* tris, int ntri, int n, const float* spring_k, float spring_k_scalar,
+                  const float* stretch_k, float stretch_k_scalar, const int* pins, const float* pin_targets,
+                  int npin, float pin_k);
+    void set_external_buffers(std::uintptr_t pos, std::uintptr_t vel, int total, std::uintptr_t inv_mass,
+                              std::uintptr_t force) noexcept;
+    void set_material(int k, const Material& m);
+    void set_options(double tol, int max_iter, bool springs_active, bool force_active);
+    void step(float dt, std::uintptr_t stream);
+    int count() const { return int(insts_.size()); }
+    int total_particles() const { return total_; }
+    SolveStats stats(int k) const;
+    Problem& problem() { return prob_; }
+    const Instance& instance(int k) const { return *insts_[size_t(k)]; }
+
+private:
+    std::vector<std::unique_ptr<Instance>> insts_;
+    std::vector<std::vector<float>> pending_spring_k_, pending_stretch_k_;
+    std::vector<char> rest_done_;
+    Problem prob_;
+    std::uintptr_t pos_ = 0, vel_ = 0, inv_mass_ = 0, force_ = 0;
+    int total_ = 0;
+    int ext_n_ = 0;
+    bool dirty_ = true;
+    bool params_dirty_ = true;
+};
+
+void check_hip(hipError_t err, const char* what);
+
+// Rest lengths and rest shapes of `inst` from device positions `pos`
+// (particle space; the instance's particles start at inst.part0).
+void compute_rest_springs(Instance& inst, const float* pos, const float* k_per_edge, float k, hipStream_t s);
+void compute_rest_stretch(Instance& inst, const float* pos, const float* k_per_tri, float k, hipStream_t s);
+
+}  // namespace chysx_amd

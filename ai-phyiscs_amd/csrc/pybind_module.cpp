@@ -1,0 +1,395 @@
+// SPDX-License-Identifier: Apache-2.0
+//
+// pybind11 module _chysx_native: the reference's Python surface for the cloth
+// step (src/bindings.cpp:22-594, 1069-1075), implemented over the C ABI of
+// libchysx_amd.so.  Argument names, defaults and exception types follow the
+// reference bindings.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/chysx_cloth.h"
+
+namespace py = pybind11;
+
+namespace {
+
+void check(int rc) {
+    if (rc == CHYSX_OK) return;
+    const std::string msg = chysx_last_error();
+    if (rc == CHYSX_EINVAL) throw py::value_error(msg);
+    if (rc == CHYSX_ERANGE) throw py::index_error(msg);
+    throw std::runtime_error(msg);
+}
+
+using IntArr = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
+using FltArr = py::array_t<float, py::array::c_style | py::array::forcecast>;
+using DblArr = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+struct Sim;
+
+// ClothMaterial: a free-standing POD, or (as returned by
+// ClothSimulator.material) a live view whose writes go to the simulator.
+struct Material {
+    chysx_cloth_material m{0.0f, 0.0f, 0.0f, 0.05f, 0.0f, 0.0f, 0.0f, -9.81f};
+    std::shared_ptr<chysx_cloth*> owner;  // set for views
+
+    chysx_cloth_material get() const {
+        if (owner && *owner) {
+            chysx_cloth_material r;
+            check(chysx_cloth_get_material(*owner, &r));
+            return r;
+        }
+        return m;
+    }
+    void put(const chysx_cloth_material& v) {
+        if (owner && *owner) check(chysx_cloth_set_material(*owner, &v));
+        else m = v;
+    }
+};
+
+struct Sim {
+    std::shared_ptr<chysx_cloth*> h;
+    Sim() : h(std::make_shared<chysx_cloth*>(nullptr)) { check(chysx_cloth_create(h.get())); }
+    ~Sim() {
+        if (*h) chysx_cloth_destroy(*h);
+        *h = nullptr;
+    }
+    Sim(const Sim&) = delete;
+    Sim& operator=(const Sim&) = delete;
+    chysx_cloth* p() const { return *h; }
+};
+
+struct Batch {
+    chysx_batch* h = nullptr;
+    Batch() { check(chysx_batch_create(&h)); }
+    ~Batch() {
+        if (h) chysx_batch_destroy(h);
+    }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+};
+
+#define MAT_FIELD(name, doc)                                                                  \
+    def_property(                                                                             \
+        #name, [](const Material& s) { return s.get().name; },                                \
+        [](Material& s, float v) {                                                            \
+            chysx_cloth_material c = s.get();                                                 \
+            c.name = v;                                                                       \
+            s.put(c);                                                                         \
+        },                                                                                    \
+        doc)
+
+py::dict stats_dict(int it, int rs, int st, double rr) {
+    py::dict d;
+    d["iters"] = it;
+    d["restarts"] = rs;
+    d["status"] = st;
+    d["converged"] = (st == CHYSX_PCG_CONVERGED);
+    d["relres"] = rr;
+    return d;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_chysx_native, m) {
+    m.doc() = "ChysX cloth step on AMD Instinct MI355X (gfx950)";
+
+    py::class_<Material>(m, "ClothMaterial", "Plain-old-data material parameters for the cloth simulator.")
+        .def(py::init<>())
+        .MAT_FIELD(lame_mu, "Lame mu [Pa] (in-plane elasticity, unused by the step).")
+        .MAT_FIELD(lame_lambda, "Lame lambda [Pa] (unused by the step).")
+        .MAT_FIELD(bending, "Dihedral bending stiffness [N m] (unused by the step).")
+        .MAT_FIELD(density, "Surface mass density [kg/m^2].")
+        .MAT_FIELD(damping, "Velocity damping [1/s] (v *= exp(-damping * dt)).")
+        .MAT_FIELD(gx, "Gravity x-component [m/s^2].")
+        .MAT_FIELD(gy, "Gravity y-component [m/s^2].")
+        .MAT_FIELD(gz, "Gravity z-component [m/s^2].");
+
+    py::class_<Sim>(m, "ClothSimulator", "Implicit cloth step (block-CSR Hessian + block-Jacobi PCG) on gfx950.")
+        .def(py::init<>())
+        .def(
+            "set_material", [](Sim& s, const Material& mat) {
+                const chysx_cloth_material c = mat.get();
+                check(chysx_cloth_set_material(s.p(), &c));
+            },
+            py::arg("material"), "Copy `material` into the simulator.")
+        .def(
+            "set_external_buffers",
+            [](Sim& s, std::uintptr_t pos, std::uintptr_t vel, int n, std::uintptr_t inv_mass, std::uintptr_t force) {
+                check(chysx_cloth_set_external_buffers(s.p(), pos, vel, n, inv_mass, force));
+            },
+            py::arg("pos_ptr"), py::arg("vel_ptr"), py::arg("particle_count"), py::arg("inv_mass_ptr") = 0,
+            py::arg("force_ptr") = 0,
+            "Stash caller-owned device pointers (float3 AoS positions / velocities / forces, float inverse mass).")
+        .def(
+            "step", [](Sim& s, float dt, std::uintptr_t stream) { check(chysx_cloth_step(s.p(), dt, stream)); },
+            py::arg("dt"), py::arg("cuda_stream") = 0,
+            "Advance the simulation by `dt` seconds; pos / vel are updated in place.")
+        .def(
+            "set_pins",
+            [](Sim& s, IntArr indices, FltArr targets, float stiffness) {
+                if (indices.ndim() != 1) throw py::value_error("ClothSimulator.set_pins: indices must be 1-D");
+                if (targets.ndim() != 2 || targets.shape(1) != 3)
+                    throw py::value_error("ClothSimulator.set_pins: targets must have shape (N, 3)");
+                if (indices.shape(0) != targets.shape(0))
+                    throw py::value_error("ClothSimulator.set_pins: indices and targets must have the same length");
+                check(chysx_cloth_set_pins(s.p(), indices.data(), targets.data(), int(indices.shape(0)), stiffness));
+            },
+            py::arg("indices"), py::arg("targets"), py::arg("stiffness") = 1.0e6f, "Install penalty pins.")
+        .def("clear_pins", [](Sim& s) { check(chysx_cloth_clear_pins(s.p())); })
+        .def("num_pins",
+             [](const Sim& s) {
+                 int n = 0;
+                 check(chysx_cloth_num_pins(s.p(), &n));
+                 return n;
+             })
+        .def(
+            "update_pin_targets",
+            [](Sim& s, FltArr targets, std::uintptr_t stream) {
+                if (targets.ndim() != 2 || targets.shape(1) != 3)
+                    throw py::value_error(
+                        "ClothSimulator.update_pin_targets: targets must have shape (N, 3) and dtype float32");
+                check(chysx_cloth_update_pin_targets(s.p(), targets.data(), int(targets.shape(0)), stream));
+            },
+            py::arg("targets"), py::arg("cuda_stream") = 0)
+        .def(
+            "set_mesh",
+            [](Sim& s, IntArr tris) {
+                if (tris.ndim() != 2 || tris.shape(1) != 3)
+                    throw py::value_error("ClothSimulator.set_mesh: triangles must have shape (M, 3) and dtype int32");
+                check(chysx_cloth_set_mesh(s.p(), tris.data(), int(tris.shape(0))));
+            },
+            py::arg("triangles"))
+        .def(
+            "build_springs_from_current_positions",
+            [](Sim& s, float k, std::uintptr_t stream) { check(chysx_cloth_build_springs(s.p(), k, stream)); },
+            py::arg("stiffness"), py::arg("cuda_stream") = 0)
+        .def("num_springs",
+             [](const Sim& s) {
+                 int n = 0;
+                 check(chysx_cloth_num_springs(s.p(), &n));
+                 return n;
+             })
+        .def(
+            "redistribute_mass_area_weighted",
+            [](Sim& s, float density, std::uintptr_t inv_mass, int n, std::uintptr_t stream) {
+                check(chysx_cloth_redistribute_mass_area_weighted(s.p(), density, inv_mass, n, stream));
+            },
+            py::arg("surface_density"), py::arg("inv_mass_ptr"), py::arg("particle_count"), py::arg("cuda_stream") = 0)
+        .def(
+            "build_fem_stretch_from_current_positions",
+            [](Sim& s, float k, std::uintptr_t stream) { check(chysx_cloth_build_fem_stretch(s.p(), k, stream)); },
+            py::arg("stiffness"), py::arg("cuda_stream") = 0)
+        .def("num_fem_stretch_triangles",
+             [](const Sim& s) {
+                 int n = 0;
+                 check(chysx_cloth_num_fem_stretch(s.p(), &n));
+                 return n;
+             })
+        .def(
+            "set_pcg_iterations", [](Sim& s, int it) { check(chysx_cloth_set_pcg_iterations(s.p(), it)); },
+            py::arg("max_iter"), "Set the maximum number of PCG iterations per step.")
+        .def("pcg_iterations",
+             [](const Sim& s) {
+                 int n = 0;
+                 check(chysx_cloth_pcg_iterations(s.p(), &n));
+                 return n;
+             })
+        .def(
+            "debug_dump_last_solve",
+            [](const Sim& s) {
+                int n = 0, nnz = 0;
+                check(chysx_cloth_num_particles(s.p(), &n));
+                check(chysx_cloth_num_off_diag_blocks(s.p(), &nnz));
+                if (n == 0) nnz = 0;
+                py::array_t<float> diag({n, 3, 3});
+                py::array_t<int32_t> ro({n + 1});
+                py::array_t<int32_t> cols({nnz});
+                py::array_t<float> vals({nnz, 3, 3});
+                py::array_t<float> rhs({n, 3});
+                py::array_t<float> dx({n, 3});
+                if (n == 0) return py::make_tuple(diag, ro, cols, vals, rhs, dx);
+                check(chysx_cloth_debug_copy_hessian_diag(s.p(), diag.mutable_data()));
+                check(chysx_cloth_debug_copy_hessian_csr(s.p(), ro.mutable_data(),
+                                                         nnz > 0 ? cols.mutable_data() : nullptr,
+                                                         nnz > 0 ? vals.mutable_data() : nullptr));
+                check(chysx_cloth_debug_copy_last_rhs(s.p(), rhs.mutable_data()));
+                check(chysx_cloth_debug_copy_last_dx(s.p(), dx.mutable_data()));
+                return py::make_tuple(diag, ro, cols, vals, rhs, dx);
+            },
+            "Return (diag, row_offsets, col_indices, values, rhs, dx) of the last solve.")
+        .def_property_readonly(
+            "material",
+            [](Sim& s) {
+                Material v;
+                v.owner = s.h;
+                return v;
+            },
+            "Live view of the simulator's material (writes go to the simulator).")
+        // ---- extensions --------------------------------------------------
+        .def(
+            "set_pcg_tolerance", [](Sim& s, double tol) { check(chysx_cloth_set_pcg_tolerance(s.p(), tol)); },
+            py::arg("tol"),
+            "Stop PCG at true relative residual <= tol (capped at pcg_iterations()); 0 = fixed iteration count.")
+        .def(
+            "set_springs_active", [](Sim& s, bool on) { check(chysx_cloth_set_springs_active(s.p(), on ? 1 : 0)); },
+            py::arg("active"))
+        .def(
+            "set_external_force_active",
+            [](Sim& s, bool on) { check(chysx_cloth_set_external_force_active(s.p(), on ? 1 : 0)); },
+            py::arg("active"))
+        .def(
+            "set_spring_stiffnesses",
+            [](Sim& s, FltArr k) {
+                if (k.ndim() != 1) throw py::value_error("set_spring_stiffnesses: k must be 1-D");
+                check(chysx_cloth_set_spring_stiffnesses(s.p(), k.data(), int(k.shape(0))));
+            },
+            py::arg("k"))
+        .def(
+            "set_fem_stretch_stiffnesses",
+            [](Sim& s, FltArr k) {
+                if (k.ndim() != 1) throw py::value_error("set_fem_stretch_stiffnesses: k must be 1-D");
+                check(chysx_cloth_set_fem_stretch_stiffnesses(s.p(), k.data(), int(k.shape(0))));
+            },
+            py::arg("k"))
+        .def("last_solve_stats",
+             [](const Sim& s) {
+                 int it = 0, rs = 0, st = 0;
+                 double rr = 0.0;
+                 check(chysx_cloth_last_solve_stats(s.p(), &it, &rs, &st, &rr));
+                 return stats_dict(it, rs, st, rr);
+             })
+        .def(
+            "set_warm_start",
+            [](Sim& s, DblArr dx) {
+                if (dx.ndim() != 2 || dx.shape(1) != 3) throw py::value_error("set_warm_start: dx must be (N, 3)");
+                check(chysx_cloth_set_warm_start(s.p(), dx.data(), int(dx.shape(0))));
+            },
+            py::arg("dx"))
+        .def("zero_warm_start", [](Sim& s) { check(chysx_cloth_zero_warm_start(s.p())); })
+        .def("get_dx64",
+             [](const Sim& s) {
+                 int n = 0;
+                 check(chysx_cloth_num_particles(s.p(), &n));
+                 py::array_t<double> out({n, 3});
+                 if (n) check(chysx_cloth_get_dx64(s.p(), out.mutable_data(), n));
+                 return out;
+             })
+        .def("edges",
+             [](const Sim& s) {
+                 int n = 0;
+                 check(chysx_cloth_num_edges(s.p(), &n));
+                 py::array_t<int32_t> out({n, 2});
+                 if (n) check(chysx_cloth_get_edges(s.p(), out.mutable_data()));
+                 return out;
+             })
+        .def(
+            "time_kernel",
+            [](Sim& s, int which, int reps, float dt, std::uintptr_t stream) {
+                double ms = 0.0;
+                check(chysx_cloth_time_kernel(s.p(), which, reps, dt, stream, &ms));
+                return ms;
+            },
+            py::arg("which"), py::arg("reps"), py::arg("dt"), py::arg("cuda_stream") = 0)
+        .def("spmv_bytes", [](const Sim& s) {
+            double b = 0.0;
+            check(chysx_cloth_spmv_bytes(s.p(), &b));
+            return b;
+        });
+
+    py::class_<Batch>(m, "ClothBatch", "Many cloths advanced together in one launch per kernel.")
+        .def(py::init<>())
+        .def(
+            "add_cloth",
+            [](Batch& b, IntArr tris, int n, py::object spring_k, py::object stretch_k, IntArr pins, FltArr targets,
+               float pin_k) {
+                if (tris.ndim() != 2 || tris.shape(1) != 3) throw py::value_error("add_cloth: triangles must be (M, 3)");
+                if (pins.ndim() != 1 || targets.ndim() != 2 || (targets.shape(0) > 0 && targets.shape(1) != 3) ||
+                    targets.shape(0) != pins.shape(0))
+                    throw py::value_error("add_cloth: pins (P,) and targets (P, 3) required");
+                float ks = 0.0f, kt = 0.0f;
+                FltArr ka, kb;
+                const float* kap = nullptr;
+                const float* kbp = nullptr;
+                if (py::isinstance<py::float_>(spring_k) || py::isinstance<py::int_>(spring_k)) ks = spring_k.cast<float>();
+                else {
+                    ka = spring_k.cast<FltArr>();
+                    kap = ka.data();
+                }
+                if (py::isinstance<py::float_>(stretch_k) || py::isinstance<py::int_>(stretch_k))
+                    kt = stretch_k.cast<float>();
+                else {
+                    kb = stretch_k.cast<FltArr>();
+                    kbp = kb.data();
+                }
+                int idx = -1;
+                check(chysx_batch_add_cloth(b.h, tris.data(), int(tris.shape(0)), n, kap, ks, kbp, kt, pins.data(),
+                                            targets.data(), int(pins.shape(0)), pin_k, &idx));
+                return idx;
+            },
+            py::arg("triangles"), py::arg("particle_count"), py::arg("spring_k"), py::arg("stretch_k"),
+            py::arg("pin_indices"), py::arg("pin_targets"), py::arg("pin_stiffness") = 1.0e6f)
+        .def(
+            "set_external_buffers",
+            [](Batch& b, std::uintptr_t pos, std::uintptr_t vel, int total, std::uintptr_t inv_mass,
+               std::uintptr_t force) { check(chysx_batch_set_external_buffers(b.h, pos, vel, total, inv_mass, force)); },
+            py::arg("pos_ptr"), py::arg("vel_ptr"), py::arg("particle_count"), py::arg("inv_mass_ptr") = 0,
+            py::arg("force_ptr") = 0)
+        .def(
+            "set_material",
+            [](Batch& b, int k, const Material& mat) {
+                const chysx_cloth_material c = mat.get();
+                check(chysx_batch_set_material(b.h, k, &c));
+            },
+            py::arg("index"), py::arg("material"))
+        .def(
+            "set_options",
+            [](Batch& b, double tol, int max_iter, bool springs, bool force) {
+                check(chysx_batch_set_options(b.h, tol, max_iter, springs ? 1 : 0, force ? 1 : 0));
+            },
+            py::arg("tol"), py::arg("max_iter"), py::arg("springs_active") = true, py::arg("force_active") = false)
+        .def(
+            "step", [](Batch& b, float dt, std::uintptr_t stream) { check(chysx_batch_step(b.h, dt, stream)); },
+            py::arg("dt"), py::arg("cuda_stream") = 0)
+        .def(
+            "stats",
+            [](const Batch& b, int k) {
+                int it = 0, rs = 0, st = 0;
+                double rr = 0.0;
+                check(chysx_batch_stats(b.h, k, &it, &rs, &st, &rr));
+                return stats_dict(it, rs, st, rr);
+            },
+            py::arg("index"));
+
+    m.def(
+        "topology_query",
+        [](int n, IntArr tris, bool include_springs, IntArr pins) {
+            if (tris.ndim() != 2 || tris.shape(1) != 3) throw py::value_error("topology_query: triangles must be (M, 3)");
+            const int T = int(tris.shape(0));
+            const int P = int(pins.size());
+            int nnz = 0, ne = 0;
+            check(chysx_topology_query(n, tris.data(), T, include_springs ? 1 : 0, pins.data(), P, &nnz, &ne, nullptr,
+                                       nullptr, nullptr, nullptr, nullptr, nullptr));
+            py::array_t<int32_t> ro({n + 1}), cols({nnz}), edges({ne, 2}), ts({9 * T}),
+                ss({include_springs ? 4 * ne : 0}), ps({P});
+            check(chysx_topology_query(n, tris.data(), T, include_springs ? 1 : 0, pins.data(), P, &nnz, &ne,
+                                       ro.mutable_data(), cols.mutable_data(), edges.mutable_data(), ts.mutable_data(),
+                                       ss.mutable_data(), ps.mutable_data()));
+            py::dict d;
+            d["row_offsets"] = ro;
+            d["col_indices"] = cols;
+            d["edges"] = edges;
+            d["tri_slots"] = ts;
+            d["spring_slots"] = ss;
+            d["pin_slots"] = ps;
+            return d;
+        },
+        py::arg("n"), py::arg("triangles"), py::arg("include_springs") = false,
+        py::arg("pins") = IntArr(0), "Host-only pattern and slot maps of a mesh (no GPU needed).");
+    m.attr("__version__") = chysx_version();
+}

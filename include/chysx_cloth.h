@@ -1,0 +1,189 @@
+/* SPDX-License-Identifier: Apache-2.0
+ *
+ * C ABI of the gfx950 implicit cloth step (libchysx_amd.so).
+ *
+ * Plain pointers and sizes only.  Device buffers are passed as uintptr_t
+ * addresses of caller-owned HIP allocations (12-byte float3 AoS for
+ * positions, velocities and forces; float for inverse mass), exactly like the
+ * reference's raw-pointer interface.  Streams are hipStream_t values cast to
+ * uintptr_t; 0 is the null stream.
+ *
+ * Every function returns a status code; on failure chysx_last_error() holds a
+ * thread-local message.  The codes mirror the reference's C++ exception types
+ * (src/bindings.cpp re-raises them as Python exceptions):
+ *   CHYSX_EINVAL  std::invalid_argument -> ValueError
+ *   CHYSX_ERUNTIME std::runtime_error   -> RuntimeError
+ *   CHYSX_ERANGE  std::out_of_range     -> IndexError
+ *
+ * Citations name the reference interface each entry point replaces (paths
+ * relative to the reference repository root).
+ */
+#ifndef CHYSX_CLOTH_H
+#define CHYSX_CLOTH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CHYSX_OK = 0,
+    CHYSX_EINVAL = 1,
+    CHYSX_ERUNTIME = 2,
+    CHYSX_ERANGE = 3
+};
+
+/* PCG status values reported by chysx_cloth_last_solve_stats. */
+enum {
+    CHYSX_PCG_RUNNING = 0,
+    CHYSX_PCG_CHECK = 1,
+    CHYSX_PCG_CONVERGED = 2,
+    CHYSX_PCG_STOPPED = 3
+};
+
+/* Same fields and order as chysx::cloth::ClothMaterial
+ * (src/cloth/cloth_material.h:20-43). */
+typedef struct chysx_cloth_material {
+    float lame_mu;
+    float lame_lambda;
+    float bending;
+    float density;
+    float damping;
+    float gx;
+    float gy;
+    float gz;
+} chysx_cloth_material;
+
+typedef struct chysx_cloth chysx_cloth;
+
+/* Thread-local message of the last failed call. */
+const char* chysx_last_error(void);
+/* Library version string ("0.1.0", chysx/__init__.py:51). */
+const char* chysx_version(void);
+
+/* ClothSimulator() (src/bindings.cpp:62; src/cloth/cloth_simulator.h:178-838). */
+int chysx_cloth_create(chysx_cloth** out);
+int chysx_cloth_destroy(chysx_cloth* sim);
+
+/* set_material / material (src/bindings.cpp:63-65, 1069-1075). */
+int chysx_cloth_set_material(chysx_cloth* sim, const chysx_cloth_material* m);
+int chysx_cloth_get_material(const chysx_cloth* sim, chysx_cloth_material* out);
+
+/* set_external_buffers (src/cloth/cloth_simulator.cu:248-258; bindings.cpp:66-77).
+ * Never fails (the reference is noexcept); pointers are borrowed. */
+int chysx_cloth_set_external_buffers(chysx_cloth* sim, uintptr_t pos, uintptr_t vel, int particle_count,
+                                     uintptr_t inv_mass, uintptr_t force);
+
+/* set_mesh (src/cloth/cloth_simulator.cu:315-377; bindings.cpp:167-192). */
+int chysx_cloth_set_mesh(chysx_cloth* sim, const int32_t* triangles, int ntri);
+
+/* build_springs_from_current_positions (cloth_simulator.cu:379-398; bindings.cpp:193-206). */
+int chysx_cloth_build_springs(chysx_cloth* sim, float stiffness, uintptr_t stream);
+/* num_springs (bindings.cpp:207-212). */
+int chysx_cloth_num_springs(const chysx_cloth* sim, int* out);
+
+/* build_fem_stretch_from_current_positions (cloth_simulator.cu:400-421; bindings.cpp:244-259). */
+int chysx_cloth_build_fem_stretch(chysx_cloth* sim, float stiffness, uintptr_t stream);
+/* num_fem_stretch_triangles (bindings.cpp:260-265). */
+int chysx_cloth_num_fem_stretch(const chysx_cloth* sim, int* out);
+
+/* set_pins / clear_pins / num_pins / update_pin_targets
+ * (cloth_simulator.cu:277-313; bindings.cpp:85-166).  targets: n x 3 floats. */
+int chysx_cloth_set_pins(chysx_cloth* sim, const int32_t* indices, const float* targets, int n, float stiffness);
+int chysx_cloth_clear_pins(chysx_cloth* sim);
+int chysx_cloth_num_pins(const chysx_cloth* sim, int* out);
+int chysx_cloth_update_pin_targets(chysx_cloth* sim, const float* targets, int n, uintptr_t stream);
+
+/* redistribute_mass_area_weighted (cloth_simulator.cu:516-575; bindings.cpp:213-243). */
+int chysx_cloth_redistribute_mass_area_weighted(chysx_cloth* sim, float surface_density, uintptr_t inv_mass,
+                                                int particle_count, uintptr_t stream);
+
+/* set_pcg_iterations / pcg_iterations (cloth_simulator.h:597-600; bindings.cpp:544-549). */
+int chysx_cloth_set_pcg_iterations(chysx_cloth* sim, int max_iter);
+int chysx_cloth_pcg_iterations(const chysx_cloth* sim, int* out);
+
+/* step (src/cloth/cloth_simulator.cu:814-1261; bindings.cpp:78-84).  Blocks
+ * until the PCG solve has ended, then returns with pos / vel updated. */
+int chysx_cloth_step(chysx_cloth* sim, float dt, uintptr_t stream);
+
+/* num_particles / num_off_diag_blocks of the last solve (cloth_simulator.h:632-633). */
+int chysx_cloth_num_particles(const chysx_cloth* sim, int* out);
+int chysx_cloth_num_off_diag_blocks(const chysx_cloth* sim, int* out);
+
+/* debug_copy_* behind debug_dump_last_solve (cloth_simulator.cu:1272-1348;
+ * bindings.cpp:551-594): diag n x 9, row_offsets n+1, col_indices nnz,
+ * values nnz x 9 (row-major 3x3 blocks, CSR order), rhs n x 3, dx n x 3. */
+int chysx_cloth_debug_copy_hessian_diag(const chysx_cloth* sim, float* out);
+int chysx_cloth_debug_copy_hessian_csr(const chysx_cloth* sim, int32_t* row_offsets, int32_t* col_indices,
+                                       float* values);
+int chysx_cloth_debug_copy_last_rhs(const chysx_cloth* sim, float* out);
+int chysx_cloth_debug_copy_last_dx(const chysx_cloth* sim, float* out);
+
+/* ---- extensions (no reference counterpart; additive) ------------------- */
+
+/* Stop PCG at true relative residual sqrt(|b-Ax|^2 / max(|b|^2,1e-30)) <= tol
+ * (the reference's definition, pcg_solver.cu:595-641, 916-923), capped at
+ * pcg_iterations().  tol = 0 (default) runs exactly pcg_iterations()
+ * iterations like the reference (pcg_solver.cu:542-592). */
+int chysx_cloth_set_pcg_tolerance(chysx_cloth* sim, double tol);
+/* Include springs in the step's gradient and Hessian (the reference builds
+ * them but leaves them out of the step, cloth_simulator.cu:921). Default 0. */
+int chysx_cloth_set_springs_active(chysx_cloth* sim, int on);
+/* Feed the external force buffer into the predictor (the reference passes
+ * null, cloth_simulator.cu:892-894).  Default 0. */
+int chysx_cloth_set_external_force_active(chysx_cloth* sim, int on);
+/* Per-element stiffness (one value per spring in edge order / per triangle). */
+int chysx_cloth_set_spring_stiffnesses(chysx_cloth* sim, const float* k, int n);
+int chysx_cloth_set_fem_stretch_stiffnesses(chysx_cloth* sim, const float* k, int n);
+/* Iterations, restarts, status (CHYSX_PCG_*) and true relative residual of the last solve. */
+int chysx_cloth_last_solve_stats(const chysx_cloth* sim, int* iters, int* restarts, int* status, double* relres);
+/* The FP64 warm start dx (n x 3) for the next step; zero resets it. */
+int chysx_cloth_set_warm_start(chysx_cloth* sim, const double* dx, int n);
+int chysx_cloth_zero_warm_start(chysx_cloth* sim);
+int chysx_cloth_get_dx64(const chysx_cloth* sim, double* out, int n);
+/* Edge list (2 ints per edge, lexicographic) of the current mesh. */
+int chysx_cloth_num_edges(const chysx_cloth* sim, int* out);
+int chysx_cloth_get_edges(const chysx_cloth* sim, int32_t* out);
+
+/* Host-only topology query (no GPU needed): pattern and slot maps of a mesh,
+ * bit-identical to BlockCSR3::build_topology / resolve_slots
+ * (src/sparse/block_csr.cu:91-204; constraint_n.h:115-153).
+ * include_springs adds both directions of every edge.  Call once with null
+ * outputs to get sizes. */
+int chysx_topology_query(int n, const int32_t* triangles, int ntri, int include_springs, const int32_t* pins,
+                         int npin, int* nnz, int* nedges, int32_t* row_offsets, int32_t* col_indices,
+                         int32_t* edges, int32_t* tri_slots, int32_t* spring_slots, int32_t* pin_slots);
+
+/* ---- batched cloths (extension) ---------------------------------------- */
+typedef struct chysx_batch chysx_batch;
+int chysx_batch_create(chysx_batch** out);
+int chysx_batch_destroy(chysx_batch* b);
+/* Adds a cloth of n particles occupying the next n particles of the batch's
+ * external buffers.  spring_k / stretch_k: per element or null (then the
+ * scalar applies). */
+int chysx_batch_add_cloth(chysx_batch* b, const int32_t* triangles, int ntri, int n, const float* spring_k,
+                          float spring_k_scalar, const float* stretch_k, float stretch_k_scalar,
+                          const int32_t* pins, const float* pin_targets, int npin, float pin_k, int* index);
+int chysx_batch_set_external_buffers(chysx_batch* b, uintptr_t pos, uintptr_t vel, int total, uintptr_t inv_mass,
+                                     uintptr_t force);
+int chysx_batch_set_material(chysx_batch* b, int index, const chysx_cloth_material* m);
+int chysx_batch_set_options(chysx_batch* b, double tol, int max_iter, int springs_active, int force_active);
+int chysx_batch_step(chysx_batch* b, float dt, uintptr_t stream);
+int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts, int* status, double* relres);
+
+/* ---- kernel-level benchmark hooks -------------------------------------- */
+/* Time `reps` launches of one kernel of the last assembled system on `stream`
+ * with HIP events recorded on that stream; returns the mean ms per launch.
+ * which: 0 = plain SpMV y = A x, 1 = PCG SpMV (direction + A p + p.Ap),
+ * 2 = PCG update, 3 = assembly, 4 = one full PCG iteration (1 + 2).
+ * The solver state is left unusable for warm starts; call before a step. */
+int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uintptr_t stream, double* ms);
+/* Algorithmic bytes of one plain SpMV: 40 per off-diagonal block + 64 per row + 4. */
+int chysx_cloth_spmv_bytes(const chysx_cloth* sim, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHYSX_CLOTH_H */
