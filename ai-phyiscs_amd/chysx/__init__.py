@@ -17,6 +17,15 @@ Importing fails loudly when the native module has not been built.
 
 from __future__ import annotations
 
+try:
+    # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64 /
+    # libhsa-runtime64 under the same sonames.  Loading it first makes this
+    # library bind to that runtime, so device pointers from torch tensors are
+    # valid here; loading /opt/rocm's copy first would start a second runtime.
+    import torch  # noqa: F401
+except ImportError:  # no PyTorch: the system ROCm runtime is used
+    pass
+
 from . import _chysx_native
 
 ClothMaterial = _chysx_native.ClothMaterial

@@ -1,0 +1,180 @@
+"""Cloth-step benchmark on one MI355X.
+
+Workload (default): the 1024x1024 cloth -- 1 m square, top edge pinned,
+gravity plus a seeded constant wind, springs and Baraff-Witkin stretch
+(k = 1e3), m = 0.01 kg, dt = 1/60, every step solved by block-Jacobi PCG to a
+1e-6 true relative residual, warm-started from the previous step's dx.
+Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: steps/s (value), PCG iterations per step, the roofline
+of the dominant kernel (the PCG SpMV: direction update + A p + p.Ap) from HIP
+events on the launch stream, and a CPU baseline (the FP64 oracle stepping the
+same state on the host cores, bounded sample).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "ai-phyiscs_amd")]
+
+METRIC = "cloth steps/s + PCG iters/step to 1e-6 residual, learned vs plain warm start"
+PEAK_HBM_GBS = 8000.0
+
+
+def kernel_bytes(n_rows: int, nnz: int):
+    """Algorithmic HBM bytes per launch (FP32 blocks of 36 B, 4 B columns,
+    12 B float3, 24 B double3)."""
+    return {
+        # plain y = A x: values+cols 40/block; diag 36 + row offset 4 + x 12 + y 12 per row (+4)
+        "spmv": 40.0 * nnz + 64.0 * n_rows + 4.0,
+        # PCG SpMV: values+cols 40/block; diag 36, z 12, p_prev 12 reads, p 12, Ap 12 writes per row
+        "pcg_spmv": 40.0 * nnz + 84.0 * n_rows,
+        # PCG update: p 12, Ap 12, x 24, r 12, M^-1 24 reads; x 24, r 12, z 12 writes per row
+        "pcg_update": 132.0 * n_rows,
+        # reference formulation of one PCG iteration: B_spmv + 180 per row
+        "pcg_iteration_ref": 40.0 * nnz + 244.0 * n_rows + 4.0,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: every box here has one GPU and this build has no "
+              "multi-GPU path (see the sharding notes in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from chysx import scenes
+    from chysx.harness import make_sim
+
+    dev = torch.device("cuda:0")
+    s = scenes.config3(w=args.size, steps=args.warmup + args.steps)
+    t0 = time.perf_counter()
+    g = make_sim(s, tol=args.tol, max_iter=20000)
+    g.sim.step(s.dt)  # builds topology + first solve
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    for _ in range(args.warmup - 1):
+        g.sim.step(s.dt)
+    torch.cuda.synchronize()
+    # CPU-baseline state: the state the timed region starts from
+    cpu_state = None
+    if not args.no_cpu:
+        cpu_state = (g.positions().astype(np.float64), g.velocities().astype(np.float64), g.sim.get_dx64())
+
+    iters, restarts, relres = [], [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.sim.step(s.dt)
+        st = g.sim.last_solve_stats()
+        iters.append(st["iters"])
+        restarts.append(st["restarts"])
+        relres.append(st["relres"])
+        if not st["converged"]:
+            raise RuntimeError(f"PCG did not converge: {st}")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    steps_per_s = args.steps / elapsed
+
+    n = s.n
+    _, ro, _, _, _, _ = g.sim.debug_dump_last_solve()
+    nnz = int(ro[-1])
+    kb = kernel_bytes(n, nnz)
+    # Kernel timings with HIP events on the launch stream (null stream).
+    stream = torch.cuda.current_stream().cuda_stream
+    t_pcg_spmv = g.sim.time_kernel(1, args.kernel_reps, s.dt, stream)
+    t_pcg_update = g.sim.time_kernel(2, args.kernel_reps, s.dt, stream)
+    t_spmv = g.sim.time_kernel(0, args.kernel_reps, s.dt, stream)
+    t_iter = g.sim.time_kernel(4, args.kernel_reps, s.dt, stream)
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9  # noqa: E731
+
+    roofline = {
+        "bound": "hbm",
+        "kernel": "pcg_spmv_kernel (p = z + beta p_prev; Ap = A p; p.Ap)",
+        "achieved": round(gbs(kb["pcg_spmv"], t_pcg_spmv), 1),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(gbs(kb["pcg_spmv"], t_pcg_spmv) / PEAK_HBM_GBS, 4),
+        "traffic": None,
+        "bytes_per_launch": kb["pcg_spmv"],
+        "ms_per_launch": round(t_pcg_spmv, 5),
+    }
+    extra_kernels = {
+        "spmv_plain": {"bytes": kb["spmv"], "ms": round(t_spmv, 5),
+                       "GB/s": round(gbs(kb["spmv"], t_spmv), 1),
+                       "frac": round(gbs(kb["spmv"], t_spmv) / PEAK_HBM_GBS, 4)},
+        "pcg_update": {"bytes": kb["pcg_update"], "ms": round(t_pcg_update, 5),
+                       "GB/s": round(gbs(kb["pcg_update"], t_pcg_update), 1)},
+        "pcg_iteration": {"ref_bytes": kb["pcg_iteration_ref"], "ms": round(t_iter, 5),
+                          "GB/s_ref_formulation": round(gbs(kb["pcg_iteration_ref"], t_iter), 1)},
+    }
+
+    cpu = None
+    if cpu_state is not None:
+        from oracle.oracle import oracle_from_scene
+        threads = min(16, os.cpu_count() or 1)
+        oc, _, _, im, f = oracle_from_scene(s, precision=64, tol=args.tol, max_iter=20000, threads=threads)
+        p0, v0, dx0 = cpu_state
+        oc.build_topology()
+        oc.set_dx(dx0)
+        secs, it = oc.time_steps(p0, v0, im, f, s.dt, args.cpu_steps)
+        cpu = {"value": round(args.cpu_steps / secs, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle (C++ restatement of the reference step, OpenMP) stepping the "
+                         f"{args.size}x{args.size} cloth {args.cpu_steps} step(s) from the state the GPU timed "
+                         f"region starts from; {it} PCG iterations, {secs:.2f} s",
+               "pcg_iters": it}
+
+    out = {
+        "metric": METRIC,
+        "value": round(steps_per_s, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "precision": "f32 assembly/matrix/SpMV/vectors; f64 iterate dx, inner products, alpha/beta, test",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cloth_{args.size}x{args.size}_wind: 1 m grid, top edge pinned, gravity + seeded wind, "
+                        f"springs + stretch k=1e3, m=0.01 kg, dt=1/60, PCG to {args.tol:g} true relative residual",
+            "particles": n,
+            "off_diag_blocks": nnz,
+            "warm_start": "previous-step dx (plain)",
+            "parallelism": "1 GPU",
+        },
+        "pcg_iters_per_step": round(float(np.mean(iters)), 2),
+        "pcg_restarts_per_step": round(float(np.mean(restarts)), 3),
+        "max_relres": float(np.max(relres)),
+        "learned_warm_start": None,
+        "setup_s": round(setup_s, 2),
+        "roofline": roofline,
+        "kernels": extra_kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
