@@ -65,6 +65,8 @@ __global__ void force_running_kernel(InstState* st, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         st[i].status = kRunning;
+        st[i].mode = kModeIter;
+        st[i].skip = 0;
         st[i].sigma = 1.0;
         st[i].rho = 1.0;
     }
@@ -440,12 +442,9 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
                 switch (which) {
                     case 0: launch_spmv(g, A, v.p0, v.ap, s); break;
                     case 1: launch_pcg_spmv(g, A, v, P.states(), s); break;
-                    case 2: launch_pcg_update(g, A, v, P.states(), P.params(), s); break;
+                    case 2: launch_pcg_update(g, A, v, P.states(), s); break;
                     case 3: launch_assemble(g, P.mesh(), A, v, ext, P.params(), dt, s); break;
-                    case 4:
-                        launch_pcg_spmv(g, A, v, P.states(), s);
-                        launch_pcg_update(g, A, v, P.states(), P.params(), s);
-                        break;
+                    case 4: P.launch_pcg_iteration(s); break;
                 }
             }
         };

@@ -19,14 +19,28 @@ struct Tile {
 // PCG status of one instance.
 enum PcgStatus : int {
     kRunning = 0,    // iterating
-    kCheck = 1,      // recursive residual passed; true residual pending
     kConverged = 2,  // true relative residual <= tol
     kStopped = 3,    // max_iter iterations done (normal end when tol == 0)
 };
 
+// What the next SpMV pass of a running instance computes.
+enum PcgMode : int {
+    kModeIter = 0,   // p = z + beta p_prev, A p, p . A p
+    kModeCheck = 1,  // the recursive residual passed the test: true residual b - A x
+};
+
+// Reduction phases of pcg_reduce_kernel.
+enum PcgPhase : int {
+    kPhaseStart = 0,  // after the initial residual: |r|^2, |b|^2, r . z
+    kPhaseSigma = 1,  // after the SpMV pass: p . A p (or the check's |r|^2, r . z)
+    kPhaseRho = 2,    // after the update pass: r . z, |r|^2
+    kPhaseFinal = 3,  // after the report residual of a stopped solve: |r|^2
+};
+
 // Per-instance solver state, FP64 scalars (inner products, alpha, beta and
-// the convergence test).  Lives in device memory; counters implement the
-// "last workgroup finishes the reduction" pattern.
+// the convergence test).  Lives in device memory; written only by
+// pcg_reduce_kernel (one workgroup per instance) and read by the row kernels
+// that follow it, so no kernel needs cross-workgroup synchronisation.
 struct alignas(16) InstState {
     double rho;      // r . z
     double sigma;    // p . A p
@@ -34,14 +48,14 @@ struct alignas(16) InstState {
     double rr;       // r . r (recursive, or true after a check)
     double beta;     // p <- z + beta p, applied inside the next SpMV
     double relres;   // last true relative residual
-    int status;
+    int status;      // PcgStatus
+    int mode;        // PcgMode
+    int skip;        // 1: this round's update pass is skipped (after a restart)
     int iters;
     int restarts;
     int parity;      // which p buffer holds the current direction
     int ntiles;      // tiles of this instance (set at build)
     int tile0;       // first tile of this instance (set at build)
-    unsigned cnt;    // arrival counter for cross-workgroup reductions
-    int pad;
 };
 
 // Per-instance step parameters and solver options.

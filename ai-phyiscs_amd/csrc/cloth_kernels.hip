@@ -76,36 +76,15 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double (&out)[N]) {
     __syncthreads();
 }
 
-// Publish this tile's partial sums; the last tile of the instance to arrive
-// returns true with the instance totals in `tot` (valid in thread 0).
+// Sum over the workgroup, written by thread 0 to this tile's partial slots.
 template <int N>
-__device__ bool reduce_across_tiles(double (&v)[N], double* partials, int tile, InstState* S, double (&tot)[N]) {
-    __shared__ int is_last;
+__device__ __forceinline__ void write_tile_partials(double (&v)[N], double* __restrict__ partials, int tile) {
     double bs[N];
     block_sum<N>(v, bs);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int j = 0; j < N; ++j)
-            __hip_atomic_store(&partials[4 * tile + j], bs[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned prev = atomicAdd(&S->cnt, 1u);
-        is_last = (prev == unsigned(S->ntiles - 1)) ? 1 : 0;
+        for (int j = 0; j < N; ++j) partials[4 * tile + j] = bs[j];
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    const int t0 = S->tile0, nt = S->ntiles;
-    double acc[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) acc[j] = 0.0;
-    for (int t = threadIdx.x; t < nt; t += kTileRows) {
-#pragma unroll
-        for (int j = 0; j < N; ++j)
-            acc[j] += __hip_atomic_load(&partials[4 * (t0 + t) + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<N>(acc, tot);
-    if (threadIdx.x == 0) __hip_atomic_store(&S->cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -375,112 +354,90 @@ __device__ __forceinline__ F3 minv_mul(const DevMatrix& A, int s, int lane, F3 r
 }
 
 // ---------------------------------------------------------------------------
-// PCG residual: b - A x with FP64 products and sums (x is FP64).
-//   mode 0: start a solve -- r, z = M^-1 r, p = z, rho, |b|^2, test.
-//   mode 1: runs only for instances in kCheck -- confirm convergence on the
-//           true residual, or restart the recursion from it.
-//   mode 2: runs only for kStopped instances -- record the true residual.
+// True residual of one row: b - A x with FP64 products and sums (x is FP64).
+// Returns it and, unless `report_only`, stores r, z = M^-1 r and p = z.
+// Partials: [0] |r|^2, [1] |b|^2, [2] r . z.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVectors& v, float* __restrict__ pcur,
+                                             int row, bool report_only, double (&part)[4]) {
+    const int s = row >> 6, lane = row & 63;
+    const double* x = v.x;
+    const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+    const double x0 = x[3 * row], x1 = x[3 * row + 1], x2 = x[3 * row + 2];
+    double a0 = double(dg[0]) * x0 + double(dg[64]) * x1 + double(dg[128]) * x2;
+    double a1 = double(dg[192]) * x0 + double(dg[256]) * x1 + double(dg[320]) * x2;
+    double a2 = double(dg[384]) * x0 + double(dg[448]) * x1 + double(dg[512]) * x2;
+    const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
+    const int* colp = A.col + (size_t)sb * 64 + lane;
+    const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+    for (int k = 0; k < w; ++k) {
+        const int c = colp[k * 64];
+        if (c >= 0) {
+            const float* bk = vp + k * 9 * 64;
+            const double y0 = x[3 * c], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
+            a0 += double(bk[0]) * y0 + double(bk[64]) * y1 + double(bk[128]) * y2;
+            a1 += double(bk[192]) * y0 + double(bk[256]) * y1 + double(bk[320]) * y2;
+            a2 += double(bk[384]) * y0 + double(bk[448]) * y1 + double(bk[512]) * y2;
+        }
+    }
+    const F3 bf = ld3(v.rhs, row);
+    const double r0 = double(bf.x) - a0, r1 = double(bf.y) - a1, r2 = double(bf.z) - a2;
+    part[0] = r0 * r0 + r1 * r1 + r2 * r2;
+    part[1] = double(bf.x) * bf.x + double(bf.y) * bf.y + double(bf.z) * bf.z;
+    if (!report_only) {
+        const F3 rf = f3(float(r0), float(r1), float(r2));
+        const F3 zf = minv_mul(A, s, lane, rf);
+        st3(v.r, row, rf);
+        st3(v.z, row, zf);
+        st3(pcur, row, zf);
+        part[2] = double(rf.x) * zf.x + double(rf.y) * zf.y + double(rf.z) * zf.z;
+    }
+}
+
+// Residual pass.  report == 0: start a solve (every instance); report == 1:
+// record the true residual of stopped instances only.
 __global__ __launch_bounds__(256) void pcg_residual_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                           InstState* __restrict__ st,
-                                                           const InstParams* __restrict__ params, int mode) {
+                                                           const InstState* __restrict__ st, int report) {
     const int t = logical_tile(g.ntiles);
     if (t >= g.ntiles) return;
     const Tile tile = g.tiles[t];
-    InstState* S = st + tile.inst;
-    const int status = S->status;
-    if (mode == 1 && status != kCheck) return;
-    if (mode == 2 && status != kStopped) return;
-    const int parity = S->parity;
-    float* pcur = parity ? v.p1 : v.p0;
-    double part[4] = {0.0, 0.0, 0.0, 0.0};  // rr, bb, rz
-    const int tid = threadIdx.x;
-    if (tid < tile.nrows) {
-        const int row = tile.row0 + tid;
-        const int s = row >> 6, lane = row & 63;
-        const double* x = v.x;
-        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
-        const double x0 = x[3 * row], x1 = x[3 * row + 1], x2 = x[3 * row + 2];
-        double a0 = double(dg[0]) * x0 + double(dg[64]) * x1 + double(dg[128]) * x2;
-        double a1 = double(dg[192]) * x0 + double(dg[256]) * x1 + double(dg[320]) * x2;
-        double a2 = double(dg[384]) * x0 + double(dg[448]) * x1 + double(dg[512]) * x2;
-        const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
-        const int* colp = A.col + (size_t)sb * 64 + lane;
-        const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
-        for (int k = 0; k < w; ++k) {
-            const int c = colp[k * 64];
-            if (c >= 0) {
-                const float* bk = vp + k * 9 * 64;
-                const double y0 = x[3 * c], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
-                a0 += double(bk[0]) * y0 + double(bk[64]) * y1 + double(bk[128]) * y2;
-                a1 += double(bk[192]) * y0 + double(bk[256]) * y1 + double(bk[320]) * y2;
-                a2 += double(bk[384]) * y0 + double(bk[448]) * y1 + double(bk[512]) * y2;
-            }
-        }
-        const F3 bf = ld3(v.rhs, row);
-        const double r0 = double(bf.x) - a0, r1 = double(bf.y) - a1, r2 = double(bf.z) - a2;
-        part[0] = r0 * r0 + r1 * r1 + r2 * r2;
-        part[1] = double(bf.x) * bf.x + double(bf.y) * bf.y + double(bf.z) * bf.z;
-        if (mode != 2) {
-            const F3 rf = f3(float(r0), float(r1), float(r2));
-            const F3 zf = minv_mul(A, s, lane, rf);
-            st3(v.r, row, rf);
-            st3(v.z, row, zf);
-            st3(pcur, row, zf);
-            part[2] = double(rf.x) * zf.x + double(rf.y) * zf.y + double(rf.z) * zf.z;
-        }
-    }
-    double tot[4];
-    if (!reduce_across_tiles<4>(part, v.partials, t, S, tot)) return;
-    if (threadIdx.x != 0) return;
-    if (mode == 0) S->bb = fmax(tot[1], 1.0e-30);
-    const double bb = S->bb;
-    S->rr = tot[0];
-    const double relres = sqrt(tot[0] / bb);
-    if (mode == 2) {
-        S->relres = relres;
-        return;
-    }
-    if (mode == 0) {
-        S->iters = 0;
-        S->restarts = 0;
-    } else {
-        S->restarts += 1;
-    }
-    S->relres = relres;
-    const InstParams& pr = params[tile.inst];
-    if (pr.tol2 > 0.0 && tot[0] <= pr.tol2 * bb) {
-        S->status = kConverged;
-        return;
-    }
-    S->rho = tot[2];
-    S->beta = 0.0;
-    S->status = (S->iters >= pr.max_iter) ? kStopped : kRunning;
+    const InstState& S = st[tile.inst];
+    if (report && S.status != kStopped) return;
+    float* pcur = S.parity ? v.p1 : v.p0;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    if (threadIdx.x < tile.nrows) residual_row(A, v, pcur, tile.row0 + threadIdx.x, report != 0, part);
+    write_tile_partials<4>(part, v.partials, t);
 }
 
 // ---------------------------------------------------------------------------
-// PCG direction + SpMV: p = z + beta p_prev (beta = 0 right after a
-// (re)start), Ap = A p, sigma = p . Ap.  The direction update of the
+// PCG SpMV pass.  Mode kModeIter: p = z + beta p_prev (beta = 0 right after a
+// (re)start), Ap = A p, partial p . Ap.  The direction update of the
 // reference's update_p_kernel (pcg_solver.cu:456-464) is fused here: every
-// gathered column recomputes z + beta p_prev, and the row's own p is written
-// to the other p buffer.  SpMV restates spmv_kernel (block_csr.cu:54-83) on
-// the wave-sliced ELL layout; dot_partial_kernel (pcg_solver.cu:340-354)
-// becomes an FP64 partial.
+// gathered column recomputes z + beta p_prev and the row's own p is written
+// to the other p buffer.  The SpMV restates spmv_kernel (block_csr.cu:54-83)
+// on the wave-sliced ELL layout.  Mode kModeCheck: the recursive residual
+// passed the test, so this pass computes the true residual instead.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                       InstState* __restrict__ st) {
+                                                       const InstState* __restrict__ st) {
     const int t = logical_tile(g.ntiles);
     if (t >= g.ntiles) return;
     const Tile tile = g.tiles[t];
-    InstState* S = st + tile.inst;
-    if (S->status != kRunning) return;
-    const float beta = float(S->beta);
-    const int parity = S->parity;
+    const InstState& S = st[tile.inst];
+    if (S.status != kRunning) return;
+    const int parity = S.parity;
+    const int tid = threadIdx.x;
+    if (S.mode == kModeCheck) {
+        double part[4] = {0.0, 0.0, 0.0, 0.0};
+        if (tid < tile.nrows) residual_row(A, v, parity ? v.p1 : v.p0, tile.row0 + tid, false, part);
+        write_tile_partials<4>(part, v.partials, t);
+        return;
+    }
+    const float beta = float(S.beta);
     const float* __restrict__ pold = parity ? v.p1 : v.p0;
     float* __restrict__ pnew = parity ? v.p0 : v.p1;
     const float* __restrict__ z = v.z;
     double part[1] = {0.0};
-    const int tid = threadIdx.x;
     if (tid < tile.nrows) {
         const int row = tile.row0 + tid;
         const int s = row >> 6, lane = row & 63;
@@ -506,30 +463,25 @@ __global__ __launch_bounds__(256) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A
         st3(pnew, row, pi);
         part[0] = double(pi.x) * a0 + double(pi.y) * a1 + double(pi.z) * a2;
     }
-    double tot[1];
-    if (!reduce_across_tiles<1>(part, v.partials, t, S, tot)) return;
-    if (threadIdx.x == 0) S->sigma = tot[0];
+    write_tile_partials<1>(part, v.partials, t);
 }
 
 // ---------------------------------------------------------------------------
-// PCG update: alpha = rho / sigma (guarded), x += alpha p (FP64), r -= alpha
-// Ap, z = M^-1 r, rho' = r . z, rr = r . r; the last workgroup forms beta,
-// counts the iteration and applies the stopping rule.  Restates
-// alpha_jacobi_dot_partial_kernel + reduce_rho_beta_kernel
-// (pcg_solver.cu:406-454) with FP64 scalars.
+// PCG update pass: alpha = rho / sigma (guarded), x += alpha p (FP64),
+// r -= alpha Ap, z = M^-1 r, partials r . z and r . r.  Restates
+// alpha_jacobi_dot_partial_kernel (pcg_solver.cu:406-434) with an FP64 iterate.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void pcg_update_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                         InstState* __restrict__ st,
-                                                         const InstParams* __restrict__ params) {
+                                                         const InstState* __restrict__ st) {
     const int t = logical_tile(g.ntiles);
     if (t >= g.ntiles) return;
     const Tile tile = g.tiles[t];
-    InstState* S = st + tile.inst;
-    if (S->status != kRunning) return;
-    const double sig = S->sigma, rho = S->rho;
+    const InstState& S = st[tile.inst];
+    if (S.status != kRunning || S.skip) return;
+    const double sig = S.sigma, rho = S.rho;
     const double alpha = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;
     const float af = float(alpha);
-    const float* __restrict__ p = S->parity ? v.p0 : v.p1;  // written by the SpMV
+    const float* __restrict__ p = S.parity ? v.p0 : v.p1;  // written by the SpMV pass
     double part[2] = {0.0, 0.0};
     const int tid = threadIdx.x;
     if (tid < tile.nrows) {
@@ -547,18 +499,90 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(LaunchGeom g, DevMatrix
         part[0] = double(ri.x) * zi.x + double(ri.y) * zi.y + double(ri.z) * zi.z;
         part[1] = double(ri.x) * ri.x + double(ri.y) * ri.y + double(ri.z) * ri.z;
     }
-    double tot[2];
-    if (!reduce_across_tiles<2>(part, v.partials, t, S, tot)) return;
+    write_tile_partials<2>(part, v.partials, t);
+}
+
+// ---------------------------------------------------------------------------
+// Per-instance reduction: one workgroup per instance sums that instance's
+// tile partials in tile order (deterministic) and advances its solver state.
+// Scalar logic restates reduce_partial_kernel / reduce_rho_beta_kernel
+// (pcg_solver.cu:374-384, 436-454) with the 1e-37 guards, plus the stopping
+// rule: true relative residual sqrt(|b-Ax|^2 / max(|b|^2,1e-30)) <= tol
+// (pcg_solver.cu:595-641, 916-923), or exactly max_iter iterations when
+// tol == 0 (pcg_solver.cu:542-592).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pcg_reduce_kernel(InstState* __restrict__ st,
+                                                         const InstParams* __restrict__ params,
+                                                         const double* __restrict__ partials, int phase) {
+    InstState& S = st[blockIdx.x];
+    const InstParams& pr = params[blockIdx.x];
+    if ((phase == kPhaseSigma || phase == kPhaseRho) && S.status != kRunning) return;
+    if (phase == kPhaseFinal && S.status != kStopped) return;
+    if (phase == kPhaseRho && S.skip) {
+        if (threadIdx.x == 0) S.skip = 0;
+        return;
+    }
+    const int t0 = S.tile0, nt = S.ntiles;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < nt; t += kTileRows) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] += partials[4 * (t0 + t) + j];
+    }
+    double tot[4];
+    block_sum<4>(acc, tot);
     if (threadIdx.x != 0) return;
-    const double rho_new = tot[0];
-    S->beta = (rho > kGuard || rho < -kGuard) ? rho_new / rho : 0.0;
-    S->rho = rho_new;
-    S->rr = tot[1];
-    S->parity ^= 1;
-    S->iters += 1;
-    const InstParams& pr = params[tile.inst];
-    if (pr.tol2 > 0.0 && tot[1] <= pr.tol2 * S->bb) S->status = kCheck;
-    else if (S->iters >= pr.max_iter) S->status = kStopped;
+    const double tol2 = pr.tol2;
+    switch (phase) {
+        case kPhaseStart: {
+            S.bb = fmax(tot[1], 1.0e-30);
+            S.rr = tot[0];
+            S.relres = sqrt(tot[0] / S.bb);
+            S.iters = 0;
+            S.restarts = 0;
+            S.mode = kModeIter;
+            S.skip = 0;
+            S.beta = 0.0;
+            S.rho = tot[2];
+            S.status = (tol2 > 0.0 && tot[0] <= tol2 * S.bb) ? kConverged
+                       : (pr.max_iter <= 0 ? kStopped : kRunning);
+            break;
+        }
+        case kPhaseSigma: {
+            if (S.mode == kModeCheck) {
+                S.rr = tot[0];
+                S.relres = sqrt(tot[0] / S.bb);
+                if (tot[0] <= tol2 * S.bb) {
+                    S.status = kConverged;
+                } else {  // residual replacement: restart the recursion from b - A x
+                    S.restarts += 1;
+                    S.rho = tot[2];
+                    S.beta = 0.0;
+                    S.mode = kModeIter;
+                    S.skip = 1;
+                    if (S.iters >= pr.max_iter) S.status = kStopped;
+                }
+            } else {
+                S.sigma = tot[0];
+            }
+            break;
+        }
+        case kPhaseRho: {
+            const double rho = S.rho, rho_new = tot[0];
+            S.beta = (rho > kGuard || rho < -kGuard) ? rho_new / rho : 0.0;
+            S.rho = rho_new;
+            S.rr = tot[1];
+            S.parity ^= 1;
+            S.iters += 1;
+            if (tol2 > 0.0 && tot[1] <= tol2 * S.bb) S.mode = kModeCheck;
+            else if (S.iters >= pr.max_iter) S.status = kStopped;
+            break;
+        }
+        case kPhaseFinal: {
+            S.rr = tot[0];
+            S.relres = sqrt(tot[0] / S.bb);
+            break;
+        }
+    }
 }
 
 // finalize_step_kernel (cloth_simulator.cu:222-244): x = x_n + dx,
@@ -645,21 +669,28 @@ void launch_assemble(const LaunchGeom& g, const DevMesh& m, const DevMatrix& A, 
     assemble_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, m, A, v, e, params, dt);
     CHYSX_LAUNCH_CHECK();
 }
-void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
-                         const InstParams* params, int mode, hipStream_t s) {
+void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                         int report, hipStream_t s) {
     if (g.ntiles <= 0) return;
-    pcg_residual_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, v, st, params, mode);
+    pcg_residual_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, v, st, report);
     CHYSX_LAUNCH_CHECK();
 }
-void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st, hipStream_t s) {
+void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                     hipStream_t s) {
     if (g.ntiles <= 0) return;
     pcg_spmv_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, v, st);
     CHYSX_LAUNCH_CHECK();
 }
-void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
-                       const InstParams* params, hipStream_t s) {
+void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                       hipStream_t s) {
     if (g.ntiles <= 0) return;
-    pcg_update_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, v, st, params);
+    pcg_update_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, v, st);
+    CHYSX_LAUNCH_CHECK();
+}
+void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int phase,
+                       hipStream_t s) {
+    if (ninst <= 0) return;
+    pcg_reduce_kernel<<<ninst, 256, 0, s>>>(st, params, partials, phase);
     CHYSX_LAUNCH_CHECK();
 }
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,

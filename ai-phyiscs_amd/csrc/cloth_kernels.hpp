@@ -72,13 +72,19 @@ void launch_rest_shapes(const int* tri_v, const float* pos, float4* dminv, float
                         float sin_t, hipStream_t s);
 void launch_assemble(const LaunchGeom& g, const DevMesh& m, const DevMatrix& A, const DevVectors& v,
                      const DevExt& e, const InstParams* params, float dt, hipStream_t s);
-// mode 0: initial residual (starts a solve); 1: true-residual check after a
-// recursive-residual pass; 2: report the true residual of a stopped solve.
-void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
-                         const InstParams* params, int mode, hipStream_t s);
-void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st, hipStream_t s);
-void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
-                       const InstParams* params, hipStream_t s);
+// Residual pass: report == 0 starts a solve (true residual, r, z, p = z);
+// report == 1 only measures the true residual of stopped instances.
+void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                         int report, hipStream_t s);
+// SpMV pass (direction update, A p, p . A p; or the true-residual check).
+void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                     hipStream_t s);
+// Update pass (x, r, z; r . z, r . r).
+void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                       hipStream_t s);
+// One workgroup per instance: finish the reductions of `phase` (PcgPhase).
+void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int phase,
+                       hipStream_t s);
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
                      hipStream_t s);
 void launch_area_mass(const int* vt_off, const int* vt_ids, const int* tri_v, const float* pos, float density_over_3,

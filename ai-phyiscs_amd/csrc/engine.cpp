@@ -343,18 +343,21 @@ void Problem::launch_assemble(float dt, const DevExt& ext, hipStream_t s) {
     chysx_amd::launch_assemble(geom(), mesh(), matrix(), vectors(), ext, params_.data(), dt, s);
 }
 void Problem::launch_pcg_start(hipStream_t s) {
-    launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), params_.data(), 0, s);
+    launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 0, s);
+    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseStart, s);
+}
+void Problem::launch_pcg_iteration(hipStream_t s) {
+    launch_pcg_spmv(geom(), matrix(), vectors(), state_.data(), s);
+    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseSigma, s);
+    launch_pcg_update(geom(), matrix(), vectors(), state_.data(), s);
+    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseRho, s);
 }
 void Problem::capture_chunk_graph() {
     if (!cap_stream_) check_hip(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking), "hipStreamCreate");
     hipGraph_t graph = nullptr;
     check_hip(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
-        for (int i = 0; i < kChunkIters; ++i) {
-            launch_pcg_spmv(geom(), matrix(), vectors(), state_.data(), cap_stream_);
-            launch_pcg_update(geom(), matrix(), vectors(), state_.data(), params_.data(), cap_stream_);
-            launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), params_.data(), 1, cap_stream_);
-        }
+        for (int i = 0; i < kChunkIters; ++i) launch_pcg_iteration(cap_stream_);
     } catch (...) {
         (void)hipStreamEndCapture(cap_stream_, &graph);
         if (graph) (void)hipGraphDestroy(graph);
@@ -380,7 +383,7 @@ void Problem::read_states(hipStream_t s) {
 bool Problem::all_done() const {
     for (int k = 0; k < ninst_; ++k) {
         const int st = host_states_[k].status;
-        if (st != kConverged && st != kStopped) return false;
+        if (st == kRunning) return false;
     }
     return true;
 }
@@ -407,7 +410,8 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
         max_it = std::max(max_it, host_states_[k].iters);
     }
     if (any_stopped) {
-        launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), params_.data(), 2, s);
+        launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 1, s);
+        launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseFinal, s);
         read_states(s);
     }
     launch_finalize(dt, ext, s);

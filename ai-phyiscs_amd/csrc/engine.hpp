@@ -138,6 +138,7 @@ public:
     // Launch pieces (used by step and by the benchmark).
     void launch_assemble(float dt, const DevExt& ext, hipStream_t s);
     void launch_pcg_start(hipStream_t s);
+    void launch_pcg_iteration(hipStream_t s);  // SpMV pass, reduce, update pass, reduce
     void launch_pcg_chunk(hipStream_t s);  // kChunk iterations (graph when possible)
     void launch_finalize(float dt, const DevExt& ext, hipStream_t s);
     void read_states(hipStream_t s);        // blocking

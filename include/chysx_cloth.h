@@ -176,7 +176,8 @@ int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts
 /* Time `reps` launches of one kernel of the last assembled system on `stream`
  * with HIP events recorded on that stream; returns the mean ms per launch.
  * which: 0 = plain SpMV y = A x, 1 = PCG SpMV (direction + A p + p.Ap),
- * 2 = PCG update, 3 = assembly, 4 = one full PCG iteration (1 + 2).
+ * 2 = PCG update, 3 = assembly, 4 = one full PCG iteration (SpMV pass,
+ * its reduction, update pass, its reduction).
  * The solver state is left unusable for warm starts; call before a step. */
 int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uintptr_t stream, double* ms);
 /* Algorithmic bytes of one plain SpMV: 40 per off-diagonal block + 64 per row + 4. */
