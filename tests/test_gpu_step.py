@@ -203,3 +203,22 @@ def test_batch_matches_single_simulators_bitwise(cuda):
         assert st["status"] == CONVERGED
         assert st["iters"] == g.sim.last_solve_stats()["iters"]
         np.testing.assert_array_equal(b.positions(k), g.positions())
+
+
+def test_damping_and_uniform_force_closed_form(cuda):
+    s = scenes.freefall(9)
+    s.damping = 0.7
+    f = np.array([0.02, 0.0, -0.01])
+    s.force = np.tile(f, (s.n, 1)).astype(np.float32)
+    g = make_sim(s, tol=1e-9, max_iter=1000)
+    a = np.array(s.gravity) + f * s.inv_mass[0]
+    decay = np.exp(-s.damping * s.dt)
+    x = np.zeros(3)
+    v = np.zeros(3)
+    for _ in range(12):
+        g.sim.step(s.dt)
+        dx = s.dt * v + s.dt ** 2 * a
+        x = x + dx
+        v = dx / s.dt * decay
+    np.testing.assert_allclose(g.positions() - s.pos, np.tile(x, (s.n, 1)), atol=3e-6)
+    np.testing.assert_allclose(g.velocities(), np.tile(v, (s.n, 1)), atol=3e-4)

@@ -262,3 +262,30 @@ def test_thread_count_does_not_change_results():
         oc.step(pos, vel, im, f, s.dt, 3)
         out.append(pos.copy())
     np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_damping_and_uniform_force_closed_form(prec):
+    """A uniform translation keeps every elastic term at rest, so each step
+    is dx = dt v + dt^2 (g + f/m) and v' = (dx/dt) exp(-c dt)
+    (finalize_step_kernel, cloth_simulator.cu:222-244; compute_x_tilde_kernel
+    with a force, :83-106)."""
+    s = scenes.freefall(5)
+    s.damping = 0.7
+    f = np.array([0.02, 0.0, -0.01])
+    s.force = np.tile(f, (s.n, 1)).astype(np.float32)
+    oc, pos, vel, im, ff = oracle_from_scene(s, precision=prec, tol=1e-10, force_active=True)
+    oc.set_params(s.gravity, s.damping, True, True, 1e-10, 10000)
+    m = 1.0 / s.inv_mass[0]
+    a = np.array(s.gravity) + f / m
+    decay = np.exp(-s.damping * s.dt)
+    x = np.zeros(3)
+    v = np.zeros(3)
+    for _ in range(12):
+        dx = s.dt * v + s.dt ** 2 * a
+        x = x + dx
+        v = dx / s.dt * decay
+    oc.step(pos, vel, im, ff, s.dt, 12)
+    tol = 1e-12 if prec == 64 else 3e-6
+    np.testing.assert_allclose(pos - s.pos, np.tile(x, (s.n, 1)), atol=tol)
+    np.testing.assert_allclose(vel, np.tile(v, (s.n, 1)), atol=tol * 100)
