@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstring>
 #include <stdexcept>
@@ -61,16 +62,6 @@ Material to_mat(const chysx_cloth_material* m) {
     return r;
 }
 
-__global__ void force_running_kernel(InstState* st, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        st[i].status = kRunning;
-        st[i].mode = kModeIter;
-        st[i].skip = 0;
-        st[i].sigma = 1.0;
-        st[i].rho = 1.0;
-    }
-}
 }  // namespace
 
 extern "C" {
@@ -432,18 +423,18 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
         const DevMatrix A = P.matrix();
         const DevVectors v = P.vectors();
         const LaunchGeom g = P.geom();
-        hipLaunchKernelGGL(force_running_kernel, dim3((P.ninst() + 63) / 64), dim3(64), 0, s, P.states(), P.ninst());
-        check_hip(hipGetLastError(), "force_running_kernel");
+        const LaunchConfig& c = P.launch_config();
+        P.prime_states(P.states(), s);
         hipEvent_t e0, e1;
         check_hip(hipEventCreate(&e0), "hipEventCreate");
         check_hip(hipEventCreate(&e1), "hipEventCreate");
         auto run = [&](int n) {
             for (int i = 0; i < n; ++i) {
                 switch (which) {
-                    case 0: launch_spmv(g, A, v.p0, v.ap, s); break;
-                    case 1: launch_pcg_spmv(g, A, v, P.states(), s); break;
-                    case 2: launch_pcg_update(g, A, v, P.states(), s); break;
-                    case 3: launch_assemble(g, P.mesh(), A, v, ext, P.params(), dt, s); break;
+                    case 0: launch_spmv(g, A, v.p0, v.ap, c.spmv_waves, s); break;
+                    case 1: launch_pcg_spmv(g, A, v, P.states(), c.spmv_waves, c.spmv_unroll, s); break;
+                    case 2: launch_pcg_update(g, A, v, P.states(), c.update_waves, s); break;
+                    case 3: launch_assemble(g, P.mesh(), A, v, ext, P.params(), dt, c.assemble_waves, s); break;
                     case 4: P.launch_pcg_iteration(s); break;
                 }
             }
@@ -459,6 +450,65 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
         (void)hipEventDestroy(e1);
         *ms = double(t) / reps;
         P.upload_params({&S.instance()}, s);
+    });
+}
+
+int chysx_cloth_set_autotune(chysx_cloth* sim, int on) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.set_autotune(on != 0);
+    });
+}
+
+int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg6) {
+    return guard([&] {
+        need(sim, "sim");
+        need(cfg6, "cfg");
+        const LaunchConfig& c = const_cast<chysx_cloth*>(sim)->sim.problem().launch_config();
+        const int v[6] = {c.assemble_waves, c.residual_waves, c.spmv_waves, c.spmv_unroll, c.update_waves,
+                          c.finalize_waves};
+        std::memcpy(cfg6, v, sizeof(v));
+    });
+}
+
+int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg6) {
+    return guard([&] {
+        need(sim, "sim");
+        need(cfg6, "cfg");
+        sim->sim.ensure_topology();
+        LaunchConfig c;
+        c.assemble_waves = cfg6[0];
+        c.residual_waves = cfg6[1];
+        c.spmv_waves = cfg6[2];
+        c.spmv_unroll = cfg6[3];
+        c.update_waves = cfg6[4];
+        c.finalize_waves = cfg6[5];
+        sim->sim.problem().set_launch_config(c);
+    });
+}
+
+int chysx_cloth_tune_log(const chysx_cloth* sim, int* count, int* kernel_waves_unroll, double* ms) {
+    return guard([&] {
+        need(sim, "sim");
+        need(count, "count");
+        const auto& L = const_cast<chysx_cloth*>(sim)->sim.problem().tune_log();
+        if (kernel_waves_unroll && ms) {
+            const int n = std::min<int>(*count, int(L.size()));
+            for (int i = 0; i < n; ++i) {
+                kernel_waves_unroll[3 * i] = L[size_t(i)].kernel;
+                kernel_waves_unroll[3 * i + 1] = L[size_t(i)].waves;
+                kernel_waves_unroll[3 * i + 2] = L[size_t(i)].unroll;
+                ms[i] = L[size_t(i)].ms;
+            }
+        }
+        *count = int(L.size());
+    });
+}
+
+int chysx_batch_set_autotune(chysx_batch* b, int on) {
+    return guard([&] {
+        need(b, "batch");
+        b->batch.set_autotune(on != 0);
     });
 }
 

@@ -7,13 +7,14 @@
 
 namespace chysx_amd {
 
-// A tile is the unit of work of one 256-lane workgroup: up to 256
-// consecutive rows of one instance, starting on a 64-row slice boundary.
-struct Tile {
-    int inst;   // instance id
-    int row0;   // first row (padded row space, multiple of 64)
-    int nrows;  // rows in this tile (<= 256)
-    int part0;  // particle index of row0 in the external buffers
+// A slice is 64 consecutive rows of one instance (rows 64 s .. 64 s + 63 of
+// the padded row space), processed by one 64-lane wavefront, one lane per
+// block row.  Instances are padded to whole slices.
+struct SliceDesc {
+    int inst;    // instance id
+    int part0;   // particle index (external buffers) of the slice's first row
+    int nvalid;  // rows of the slice that exist (<= 64)
+    int pad;
 };
 
 // PCG status of one instance.
@@ -54,8 +55,8 @@ struct alignas(16) InstState {
     int iters;
     int restarts;
     int parity;      // which p buffer holds the current direction
-    int ntiles;      // tiles of this instance (set at build)
-    int tile0;       // first tile of this instance (set at build)
+    int slice0;      // first slice of this instance (set at build)
+    int nslices;     // slices of this instance (set at build)
 };
 
 // Per-instance step parameters and solver options.

@@ -2,19 +2,23 @@
 //
 // gfx950 kernels of the implicit cloth step.
 //
-// Work decomposition: a 256-lane workgroup owns one Tile (up to 256
-// consecutive rows of one instance); each 64-lane wavefront owns one 64-row
-// slice of the wave-sliced ELL matrix, one lane per block row.  Workgroup ids
-// are remapped so that each of the 8 XCDs walks a contiguous eighth of the
-// tiles, keeping neighbour gathers (rows +-1, +-W) inside one XCD's L2.
+// Work decomposition: one 64-lane wavefront owns one 64-row slice of the
+// wave-sliced ELL matrix, one lane per block row.  A workgroup is a group of
+// WAVES consecutive slices (a launch parameter picked by the autotuner);
+// nothing synchronises inside a workgroup, so slices of different instances
+// may share one.  Workgroup ids are remapped so that each of the 8 XCDs walks
+// a contiguous eighth of the slices, keeping neighbour gathers (rows +-1,
+// +-W) inside one XCD's L2.
 //
-// Cross-workgroup reductions (inner products) use per-tile FP64 partials and
-// an arrival counter; the last workgroup of an instance sums the partials in
-// tile order, so every reduction is deterministic.
+// Inner products: each wave writes its FP64 partial (wave butterfly sum) to
+// its slice's slot; pcg_reduce_kernel sums an instance's slices in slice
+// order.  The reduction order therefore depends neither on the launch shape
+// nor on scheduling, and every result is bitwise reproducible.
 
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <stdexcept>
 
 #include "cloth_kernels.hpp"
 
@@ -22,17 +26,23 @@ namespace chysx_amd {
 
 namespace {
 
-constexpr int kTileRows = 256;
 constexpr float kMinLength = 1.0e-6f;  // spring_constraint.cu:35
 constexpr float kMinNorm = 1.0e-6f;    // triangle_stretch_constraint.cu:39
 constexpr double kGuard = 1.0e-37;     // pcg_solver.cu:421-423,451
 
-__device__ __forceinline__ int logical_tile(int ntiles) {
-    const int per = (ntiles + 7) >> 3;
+// Slice owned by this wavefront (>= nslices: nothing to do).
+template <int WAVES>
+__device__ __forceinline__ int my_slice(int nslices) {
+    const int nwg = (nslices + WAVES - 1) / WAVES;
+    const int per = (nwg + 7) >> 3;
     const int b = blockIdx.x;
-    return (b & 7) * per + (b >> 3);
+    const int wg = (b & 7) * per + (b >> 3);
+    return wg * WAVES + int(threadIdx.x >> 6);
 }
-inline int grid_for_tiles(int ntiles) { return 8 * ((ntiles + 7) / 8); }
+inline int grid_for(int nslices, int waves) {
+    const int nwg = (nslices + waves - 1) / waves;
+    return 8 * ((nwg + 7) / 8);
+}
 
 struct F3 {
     float x, y, z;
@@ -54,10 +64,10 @@ struct S6 {
     float m[6];
 };
 
-// Sum of a value over the 256-lane workgroup, fixed order.
+// Sum over the 64 lanes of a wave (fixed butterfly order); lane 0 writes the
+// slice's partial slots.  Every lane of the wave must call it.
 template <int N>
-__device__ __forceinline__ void block_sum(double (&v)[N], double (&out)[N]) {
-    __shared__ double sh[4][N];
+__device__ __forceinline__ void wave_partials(double (&v)[N], double* __restrict__ partials, int slice) {
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         double a = v[j];
@@ -65,25 +75,9 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double (&out)[N]) {
         for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
         v[j] = a;
     }
-    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) sh[wave][j] = v[j];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < N; ++j) out[j] = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
-    __syncthreads();
-}
-
-// Sum over the workgroup, written by thread 0 to this tile's partial slots.
-template <int N>
-__device__ __forceinline__ void write_tile_partials(double (&v)[N], double* __restrict__ partials, int tile) {
-    double bs[N];
-    block_sum<N>(v, bs);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int j = 0; j < N; ++j) partials[4 * tile + j] = bs[j];
+        for (int j = 0; j < N; ++j) partials[4 * slice + j] = v[j];
     }
 }
 
@@ -169,17 +163,18 @@ __device__ __forceinline__ void add_block(float* __restrict__ vrow, int k, const
     for (int c = 0; c < 9; ++c) vrow[(k * 9 + c) * 64] += sgn * H[c];
 }
 
-__global__ __launch_bounds__(256) void assemble_kernel(LaunchGeom g, DevMesh m, DevMatrix A, DevVectors v, DevExt e,
-                                                       const InstParams* __restrict__ params, float dt) {
-    const int t = logical_tile(g.ntiles);
-    if (t >= g.ntiles) return;
-    const Tile tile = g.tiles[t];
-    const int tid = threadIdx.x;
-    if (tid >= tile.nrows) return;
-    const int row = tile.row0 + tid;
-    const int p = tile.part0 + tid;
-    const InstParams pr = params[tile.inst];
-    const int s = row >> 6, lane = row & 63;
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevMesh m, DevMatrix A, DevVectors v,
+                                                             DevExt e, const InstParams* __restrict__ params,
+                                                             float dt) {
+    const int s = my_slice<WAVES>(g.nslices);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63;
+    if (lane >= sd.nvalid) return;
+    const int row = s * 64 + lane;
+    const int p = sd.part0 + lane;
+    const InstParams pr = params[sd.inst];
     const int sb = A.slice_ptr[s];
     const int deg = A.row_deg[row];
     float* __restrict__ vrow = A.val + (size_t)sb * 9 * 64 + lane;
@@ -396,17 +391,19 @@ __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVector
 
 // Residual pass.  report == 0: start a solve (every instance); report == 1:
 // record the true residual of stopped instances only.
-__global__ __launch_bounds__(256) void pcg_residual_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                           const InstState* __restrict__ st, int report) {
-    const int t = logical_tile(g.ntiles);
-    if (t >= g.ntiles) return;
-    const Tile tile = g.tiles[t];
-    const InstState& S = st[tile.inst];
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void pcg_residual_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+                                                                 const InstState* __restrict__ st, int report) {
+    const int s = my_slice<WAVES>(g.nslices);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const InstState& S = st[sd.inst];
     if (report && S.status != kStopped) return;
     float* pcur = S.parity ? v.p1 : v.p0;
+    const int lane = threadIdx.x & 63;
     double part[4] = {0.0, 0.0, 0.0, 0.0};
-    if (threadIdx.x < tile.nrows) residual_row(A, v, pcur, tile.row0 + threadIdx.x, report != 0, part);
-    write_tile_partials<4>(part, v.partials, t);
+    if (lane < sd.nvalid) residual_row(A, v, pcur, s * 64 + lane, report != 0, part);
+    wave_partials<4>(part, v.partials, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -415,22 +412,26 @@ __global__ __launch_bounds__(256) void pcg_residual_kernel(LaunchGeom g, DevMatr
 // reference's update_p_kernel (pcg_solver.cu:456-464) is fused here: every
 // gathered column recomputes z + beta p_prev and the row's own p is written
 // to the other p buffer.  The SpMV restates spmv_kernel (block_csr.cu:54-83)
-// on the wave-sliced ELL layout.  Mode kModeCheck: the recursive residual
-// passed the test, so this pass computes the true residual instead.
+// on the wave-sliced ELL layout; UNROLL columns are loaded ahead of their
+// gathers.  Mode kModeCheck: the recursive residual passed the test, so this
+// pass computes the true residual instead.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                       const InstState* __restrict__ st) {
-    const int t = logical_tile(g.ntiles);
-    if (t >= g.ntiles) return;
-    const Tile tile = g.tiles[t];
-    const InstState& S = st[tile.inst];
+template <int WAVES, int UNROLL>
+__global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+                                                             const InstState* __restrict__ st) {
+    const int s = my_slice<WAVES>(g.nslices);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const InstState& S = st[sd.inst];
     if (S.status != kRunning) return;
     const int parity = S.parity;
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = lane < sd.nvalid;
+    const int row = s * 64 + lane;
     if (S.mode == kModeCheck) {
         double part[4] = {0.0, 0.0, 0.0, 0.0};
-        if (tid < tile.nrows) residual_row(A, v, parity ? v.p1 : v.p0, tile.row0 + tid, false, part);
-        write_tile_partials<4>(part, v.partials, t);
+        if (valid) residual_row(A, v, parity ? v.p1 : v.p0, row, false, part);
+        wave_partials<4>(part, v.partials, s);
         return;
     }
     const float beta = float(S.beta);
@@ -438,9 +439,7 @@ __global__ __launch_bounds__(256) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A
     float* __restrict__ pnew = parity ? v.p0 : v.p1;
     const float* __restrict__ z = v.z;
     double part[1] = {0.0};
-    if (tid < tile.nrows) {
-        const int row = tile.row0 + tid;
-        const int s = row >> 6, lane = row & 63;
+    if (valid) {
         const F3 pi = ld3(z, row) + ld3(pold, row) * beta;
         const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
         float a0 = dg[0] * pi.x + dg[64] * pi.y + dg[128] * pi.z;
@@ -449,21 +448,26 @@ __global__ __launch_bounds__(256) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
-        for (int k = 0; k < w; ++k) {
-            const int c = colp[k * 64];
-            if (c >= 0) {
-                const float* bk = vp + k * 9 * 64;
-                const F3 pj = ld3(z, c) + ld3(pold, c) * beta;
-                a0 += bk[0] * pj.x + bk[64] * pj.y + bk[128] * pj.z;
-                a1 += bk[192] * pj.x + bk[256] * pj.y + bk[320] * pj.z;
-                a2 += bk[384] * pj.x + bk[448] * pj.y + bk[512] * pj.z;
+        for (int k0 = 0; k0 < w; k0 += UNROLL) {
+            int c[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) c[u] = (k0 + u < w) ? colp[(k0 + u) * 64] : -1;
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                if (c[u] >= 0) {
+                    const float* bk = vp + (k0 + u) * 9 * 64;
+                    const F3 pj = ld3(z, c[u]) + ld3(pold, c[u]) * beta;
+                    a0 += bk[0] * pj.x + bk[64] * pj.y + bk[128] * pj.z;
+                    a1 += bk[192] * pj.x + bk[256] * pj.y + bk[320] * pj.z;
+                    a2 += bk[384] * pj.x + bk[448] * pj.y + bk[512] * pj.z;
+                }
             }
         }
         st3(v.ap, row, f3(a0, a1, a2));
         st3(pnew, row, pi);
         part[0] = double(pi.x) * a0 + double(pi.y) * a1 + double(pi.z) * a2;
     }
-    write_tile_partials<1>(part, v.partials, t);
+    wave_partials<1>(part, v.partials, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -471,22 +475,22 @@ __global__ __launch_bounds__(256) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A
 // r -= alpha Ap, z = M^-1 r, partials r . z and r . r.  Restates
 // alpha_jacobi_dot_partial_kernel (pcg_solver.cu:406-434) with an FP64 iterate.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void pcg_update_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                         const InstState* __restrict__ st) {
-    const int t = logical_tile(g.ntiles);
-    if (t >= g.ntiles) return;
-    const Tile tile = g.tiles[t];
-    const InstState& S = st[tile.inst];
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void pcg_update_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+                                                               const InstState* __restrict__ st) {
+    const int s = my_slice<WAVES>(g.nslices);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const InstState& S = st[sd.inst];
     if (S.status != kRunning || S.skip) return;
     const double sig = S.sigma, rho = S.rho;
     const double alpha = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;
     const float af = float(alpha);
     const float* __restrict__ p = S.parity ? v.p0 : v.p1;  // written by the SpMV pass
+    const int lane = threadIdx.x & 63;
     double part[2] = {0.0, 0.0};
-    const int tid = threadIdx.x;
-    if (tid < tile.nrows) {
-        const int row = tile.row0 + tid;
-        const int s = row >> 6, lane = row & 63;
+    if (lane < sd.nvalid) {
+        const int row = s * 64 + lane;
         const F3 pi = ld3(p, row);
         double* x = v.x + 3 * row;
         x[0] += alpha * double(pi.x);
@@ -499,21 +503,24 @@ __global__ __launch_bounds__(256) void pcg_update_kernel(LaunchGeom g, DevMatrix
         part[0] = double(ri.x) * zi.x + double(ri.y) * zi.y + double(ri.z) * zi.z;
         part[1] = double(ri.x) * ri.x + double(ri.y) * ri.y + double(ri.z) * ri.z;
     }
-    write_tile_partials<2>(part, v.partials, t);
+    wave_partials<2>(part, v.partials, s);
 }
 
 // ---------------------------------------------------------------------------
 // Per-instance reduction: one workgroup per instance sums that instance's
-// tile partials in tile order (deterministic) and advances its solver state.
+// slice partials in slice order (deterministic) and advances its solver state.
 // Scalar logic restates reduce_partial_kernel / reduce_rho_beta_kernel
 // (pcg_solver.cu:374-384, 436-454) with the 1e-37 guards, plus the stopping
 // rule: true relative residual sqrt(|b-Ax|^2 / max(|b|^2,1e-30)) <= tol
 // (pcg_solver.cu:595-641, 916-923), or exactly max_iter iterations when
 // tol == 0 (pcg_solver.cu:542-592).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void pcg_reduce_kernel(InstState* __restrict__ st,
-                                                         const InstParams* __restrict__ params,
-                                                         const double* __restrict__ partials, int phase) {
+constexpr int kReduceThreads = 1024;
+
+__global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* __restrict__ st,
+                                                                    const InstParams* __restrict__ params,
+                                                                    const double* __restrict__ partials,
+                                                                    int phase) {
     InstState& S = st[blockIdx.x];
     const InstParams& pr = params[blockIdx.x];
     if ((phase == kPhaseSigma || phase == kPhaseRho) && S.status != kRunning) return;
@@ -522,15 +529,32 @@ __global__ __launch_bounds__(256) void pcg_reduce_kernel(InstState* __restrict__
         if (threadIdx.x == 0) S.skip = 0;
         return;
     }
-    const int t0 = S.tile0, nt = S.ntiles;
+    const int s0 = S.slice0, ns = S.nslices;
+    const double2* p2 = reinterpret_cast<const double2*>(partials);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int t = threadIdx.x; t < nt; t += kTileRows) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] += partials[4 * (t0 + t) + j];
+    for (int q = threadIdx.x; q < ns; q += kReduceThreads) {
+        const double2 lo = p2[2 * (s0 + q)], hi = p2[2 * (s0 + q) + 1];
+        acc[0] += lo.x;
+        acc[1] += lo.y;
+        acc[2] += hi.x;
+        acc[3] += hi.y;
     }
-    double tot[4];
-    block_sum<4>(acc, tot);
+    __shared__ double sh[kReduceThreads / 64][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sh[threadIdx.x >> 6][j] = acc[j];
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
+    double tot[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int w = 0; w < kReduceThreads / 64; ++w)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tot[j] += sh[w][j];
     const double tol2 = pr.tol2;
     switch (phase) {
         case kPhaseStart: {
@@ -587,15 +611,16 @@ __global__ __launch_bounds__(256) void pcg_reduce_kernel(InstState* __restrict__
 
 // finalize_step_kernel (cloth_simulator.cu:222-244): x = x_n + dx,
 // v = dx / dt (times exp(-damping dt) when damping > 0).
-__global__ __launch_bounds__(256) void finalize_kernel(LaunchGeom g, DevVectors v, DevExt e,
-                                                       const InstParams* __restrict__ params, float dt) {
-    const int t = logical_tile(g.ntiles);
-    if (t >= g.ntiles) return;
-    const Tile tile = g.tiles[t];
-    const int tid = threadIdx.x;
-    if (tid >= tile.nrows) return;
-    const int row = tile.row0 + tid, p = tile.part0 + tid;
-    const InstParams pr = params[tile.inst];
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void finalize_kernel(LaunchGeom g, DevVectors v, DevExt e,
+                                                             const InstParams* __restrict__ params, float dt) {
+    const int s = my_slice<WAVES>(g.nslices);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63;
+    if (lane >= sd.nvalid) return;
+    const int row = s * 64 + lane, p = sd.part0 + lane;
+    const InstParams pr = params[sd.inst];
     const F3 d = f3(float(v.x[3 * row]), float(v.x[3 * row + 1]), float(v.x[3 * row + 2]));
     const F3 xn = ld3(e.pos, p);
     const float inv_dt = 1.0f / dt;
@@ -606,15 +631,15 @@ __global__ __launch_bounds__(256) void finalize_kernel(LaunchGeom g, DevVectors 
 }
 
 // Plain y = A x on the ELL layout (spmv_kernel, block_csr.cu:54-83, alpha=1, beta=0).
-__global__ __launch_bounds__(256) void spmv_kernel(LaunchGeom g, DevMatrix A, const float* __restrict__ x,
-                                                   float* __restrict__ y) {
-    const int t = logical_tile(g.ntiles);
-    if (t >= g.ntiles) return;
-    const Tile tile = g.tiles[t];
-    const int tid = threadIdx.x;
-    if (tid >= tile.nrows) return;
-    const int row = tile.row0 + tid;
-    const int s = row >> 6, lane = row & 63;
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void spmv_kernel(LaunchGeom g, DevMatrix A, const float* __restrict__ x,
+                                                         float* __restrict__ y) {
+    const int s = my_slice<WAVES>(g.nslices);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63;
+    if (lane >= sd.nvalid) return;
+    const int row = s * 64 + lane;
     const F3 xi = ld3(x, row);
     const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
     float a0 = dg[0] * xi.x + dg[64] * xi.y + dg[128] * xi.z;
@@ -646,6 +671,20 @@ __global__ __launch_bounds__(256) void spmv_kernel(LaunchGeom g, DevMatrix A, co
 
 [[noreturn]] void throw_hip(hipError_t err, const char* what);
 
+// Launch helpers: WAVES (1, 2, 4, 8, 16) selects the workgroup size.
+#define CHYSX_WAVES_SWITCH(waves, LAUNCH) \
+    switch (waves) {                      \
+        case 1: LAUNCH(1); break;         \
+        case 2: LAUNCH(2); break;         \
+        case 4: LAUNCH(4); break;         \
+        case 8: LAUNCH(8); break;         \
+        case 16: LAUNCH(16); break;       \
+        default: throw std::invalid_argument("unsupported waves per workgroup"); \
+    }
+
+bool valid_waves(int w) { return w == 1 || w == 2 || w == 4 || w == 8 || w == 16; }
+bool valid_unroll(int u) { return u == 1 || u == 2 || u == 4; }
+
 void launch_rest_lengths(const int* spr_v, const float* pos, float* rest, int n, hipStream_t s) {
     if (n <= 0) return;
     rest_length_kernel<<<(n + 255) / 256, 256, 0, s>>>(spr_v, pos, rest, n);
@@ -664,44 +703,68 @@ void launch_area_mass(const int* vt_off, const int* vt_ids, const int* tri_v, co
     CHYSX_LAUNCH_CHECK();
 }
 void launch_assemble(const LaunchGeom& g, const DevMesh& m, const DevMatrix& A, const DevVectors& v,
-                     const DevExt& e, const InstParams* params, float dt, hipStream_t s) {
-    if (g.ntiles <= 0) return;
-    assemble_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, m, A, v, e, params, dt);
+                     const DevExt& e, const InstParams* params, float dt, int waves, hipStream_t s) {
+    if (g.nslices <= 0) return;
+#define L(W) assemble_kernel<W><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, m, A, v, e, params, dt)
+    CHYSX_WAVES_SWITCH(waves, L)
+#undef L
     CHYSX_LAUNCH_CHECK();
 }
 void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
-                         int report, hipStream_t s) {
-    if (g.ntiles <= 0) return;
-    pcg_residual_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, v, st, report);
+                         int report, int waves, hipStream_t s) {
+    if (g.nslices <= 0) return;
+#define L(W) pcg_residual_kernel<W><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st, report)
+    CHYSX_WAVES_SWITCH(waves, L)
+#undef L
     CHYSX_LAUNCH_CHECK();
 }
-void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
-                     hipStream_t s) {
-    if (g.ntiles <= 0) return;
-    pcg_spmv_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, v, st);
+void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
+                     int unroll, hipStream_t s) {
+    if (g.nslices <= 0) return;
+#define L1(W) pcg_spmv_kernel<W, 1><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define L2(W) pcg_spmv_kernel<W, 2><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define L4(W) pcg_spmv_kernel<W, 4><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+    if (unroll == 1) {
+        CHYSX_WAVES_SWITCH(waves, L1)
+    } else if (unroll == 2) {
+        CHYSX_WAVES_SWITCH(waves, L2)
+    } else if (unroll == 4) {
+        CHYSX_WAVES_SWITCH(waves, L4)
+    } else {
+        throw std::invalid_argument("unsupported unroll");
+    }
+#undef L1
+#undef L2
+#undef L4
     CHYSX_LAUNCH_CHECK();
 }
-void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
                        hipStream_t s) {
-    if (g.ntiles <= 0) return;
-    pcg_update_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, v, st);
+    if (g.nslices <= 0) return;
+#define L(W) pcg_update_kernel<W><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+    CHYSX_WAVES_SWITCH(waves, L)
+#undef L
     CHYSX_LAUNCH_CHECK();
 }
 void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int phase,
                        hipStream_t s) {
     if (ninst <= 0) return;
-    pcg_reduce_kernel<<<ninst, 256, 0, s>>>(st, params, partials, phase);
+    pcg_reduce_kernel<<<ninst, kReduceThreads, 0, s>>>(st, params, partials, phase);
     CHYSX_LAUNCH_CHECK();
 }
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
-                     hipStream_t s) {
-    if (g.ntiles <= 0) return;
-    finalize_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, v, e, params, dt);
+                     int waves, hipStream_t s) {
+    if (g.nslices <= 0) return;
+#define L(W) finalize_kernel<W><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, v, e, params, dt)
+    CHYSX_WAVES_SWITCH(waves, L)
+#undef L
     CHYSX_LAUNCH_CHECK();
 }
-void launch_spmv(const LaunchGeom& g, const DevMatrix& A, const float* x, float* y, hipStream_t s) {
-    if (g.ntiles <= 0) return;
-    spmv_kernel<<<grid_for_tiles(g.ntiles), 256, 0, s>>>(g, A, x, y);
+void launch_spmv(const LaunchGeom& g, const DevMatrix& A, const float* x, float* y, int waves, hipStream_t s) {
+    if (g.nslices <= 0) return;
+#define L(W) spmv_kernel<W><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, x, y)
+    CHYSX_WAVES_SWITCH(waves, L)
+#undef L
     CHYSX_LAUNCH_CHECK();
 }
 

@@ -51,7 +51,7 @@ struct DevVectors {
     float* p0;
     float* p1;
     double* x;         // the iterate dx (FP64)
-    double* partials;  // 4 per tile
+    double* partials;  // 4 per slice
 };
 
 // Caller-owned buffers (particle space).
@@ -63,33 +63,38 @@ struct DevExt {
 };
 
 struct LaunchGeom {
-    const Tile* tiles;
-    int ntiles;
+    const SliceDesc* slices;
+    int nslices;
 };
+
+// Workgroup shapes: `waves` in {1, 2, 4, 8, 16} (64 x waves lanes), SpMV
+// `unroll` in {1, 2, 4}.  They change speed only, never results.
+bool valid_waves(int waves);
+bool valid_unroll(int unroll);
 
 void launch_rest_lengths(const int* spr_v, const float* pos, float* rest, int nspring, hipStream_t s);
 void launch_rest_shapes(const int* tri_v, const float* pos, float4* dminv, float* area, int ntri, float cos_t,
                         float sin_t, hipStream_t s);
 void launch_assemble(const LaunchGeom& g, const DevMesh& m, const DevMatrix& A, const DevVectors& v,
-                     const DevExt& e, const InstParams* params, float dt, hipStream_t s);
+                     const DevExt& e, const InstParams* params, float dt, int waves, hipStream_t s);
 // Residual pass: report == 0 starts a solve (true residual, r, z, p = z);
 // report == 1 only measures the true residual of stopped instances.
 void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
-                         int report, hipStream_t s);
+                         int report, int waves, hipStream_t s);
 // SpMV pass (direction update, A p, p . A p; or the true-residual check).
-void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
-                     hipStream_t s);
+void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
+                     int unroll, hipStream_t s);
 // Update pass (x, r, z; r . z, r . r).
-void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
                        hipStream_t s);
 // One workgroup per instance: finish the reductions of `phase` (PcgPhase).
 void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int phase,
                        hipStream_t s);
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
-                     hipStream_t s);
+                     int waves, hipStream_t s);
 void launch_area_mass(const int* vt_off, const int* vt_ids, const int* tri_v, const float* pos, float density_over_3,
                       float* inv_mass_out, int n, hipStream_t s);
 // Plain SpMV y = A x (FP32 in, FP32 out) over the tiles, for benchmarks and tests.
-void launch_spmv(const LaunchGeom& g, const DevMatrix& A, const float* x, float* y, hipStream_t s);
+void launch_spmv(const LaunchGeom& g, const DevMatrix& A, const float* x, float* y, int waves, hipStream_t s);
 
 }  // namespace chysx_amd

@@ -51,7 +51,7 @@ template class DeviceArray<int2>;
 template class DeviceArray<float>;
 template class DeviceArray<float4>;
 template class DeviceArray<double>;
-template class DeviceArray<Tile>;
+template class DeviceArray<SliceDesc>;
 template class DeviceArray<InstState>;
 template class DeviceArray<InstParams>;
 
@@ -230,22 +230,23 @@ void Problem::build(const std::vector<const Instance*>& insts) {
         }
     }
 
-    // ---- tiles -------------------------------------------------------------
-    tiles_h_.clear();
+    // ---- slices ------------------------------------------------------------
+    slices_h_.clear();
     std::vector<InstState> st(static_cast<size_t>(ninst_));
     std::memset(st.data(), 0, sizeof(InstState) * st.size());
     for (int k = 0; k < ninst_; ++k) {
         const Instance& I = *insts[size_t(k)];
-        st[size_t(k)].tile0 = int(tiles_h_.size());
-        for (int r = 0; r < I.n; r += 256)
-            tiles_h_.push_back(Tile{k, inst_row0_[size_t(k)] + r, std::min(256, I.n - r), I.part0 + r});
-        st[size_t(k)].ntiles = int(tiles_h_.size()) - st[size_t(k)].tile0;
+        st[size_t(k)].slice0 = inst_slice0_[size_t(k)];
+        st[size_t(k)].nslices = inst_nslices_[size_t(k)];
+        for (int q = 0; q < inst_nslices_[size_t(k)]; ++q)
+            slices_h_.push_back(SliceDesc{k, I.part0 + kSlice * q, std::min(kSlice, I.n - kSlice * q), 0});
         st[size_t(k)].status = kStopped;
     }
+    if (int(slices_h_.size()) != nslices) throw std::logic_error("slice count mismatch");
 
     // ---- upload --------------------------------------------------------------
     hipStream_t s = nullptr;
-    tiles_.assign(tiles_h_.data(), tiles_h_.size(), s);
+    slices_.assign(slices_h_.data(), slices_h_.size(), s);
     state_.assign(st.data(), st.size(), s);
     tri_v_.assign(tri_v.data(), tri_v.size(), s);
     tri_dminv_.assign(reinterpret_cast<const float4*>(tri_dminv.data()), tri_dminv.size() / 4, s);
@@ -280,7 +281,7 @@ void Problem::build(const std::vector<const Instance*>& insts) {
         x_.resize(v3);
         x_.zero(s);
     }
-    partials_.resize(std::max<size_t>(size_t(4) * tiles_h_.size(), 4));
+    partials_.resize(std::max<size_t>(size_t(4) * slices_h_.size(), 4));
     partials_.zero(s);
     params_.resize(size_t(std::max(ninst_, 1)));
     if (host_states_) (void)hipHostFree(host_states_);
@@ -340,17 +341,108 @@ DevMesh Problem::mesh() const {
 double Problem::spmv_bytes() const { return 40.0 * double(nnz_total_) + 64.0 * double(n_rows_) + 4.0; }
 
 void Problem::launch_assemble(float dt, const DevExt& ext, hipStream_t s) {
-    chysx_amd::launch_assemble(geom(), mesh(), matrix(), vectors(), ext, params_.data(), dt, s);
+    chysx_amd::launch_assemble(geom(), mesh(), matrix(), vectors(), ext, params_.data(), dt, cfg_.assemble_waves, s);
 }
 void Problem::launch_pcg_start(hipStream_t s) {
-    launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 0, s);
+    launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 0, cfg_.residual_waves, s);
     launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseStart, s);
 }
 void Problem::launch_pcg_iteration(hipStream_t s) {
-    launch_pcg_spmv(geom(), matrix(), vectors(), state_.data(), s);
+    launch_pcg_spmv(geom(), matrix(), vectors(), state_.data(), cfg_.spmv_waves, cfg_.spmv_unroll, s);
     launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseSigma, s);
-    launch_pcg_update(geom(), matrix(), vectors(), state_.data(), s);
+    launch_pcg_update(geom(), matrix(), vectors(), state_.data(), cfg_.update_waves, s);
     launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseRho, s);
+}
+
+void Problem::set_launch_config(const LaunchConfig& c) {
+    if (!valid_waves(c.assemble_waves) || !valid_waves(c.residual_waves) || !valid_waves(c.spmv_waves) ||
+        !valid_waves(c.update_waves) || !valid_waves(c.finalize_waves) || !valid_unroll(c.spmv_unroll))
+        throw std::invalid_argument("launch config: waves must be 1, 2, 4, 8 or 16 and unroll 1, 2 or 4");
+    cfg_ = c;
+    destroy_graph();
+}
+
+namespace {
+__global__ void prime_states_kernel(InstState* st, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        st[i].status = kRunning;
+        st[i].mode = kModeIter;
+        st[i].skip = 0;
+        st[i].sigma = 1.0;
+        st[i].rho = 1.0;
+        st[i].beta = 0.5;
+    }
+}
+}  // namespace
+
+void Problem::prime_states(InstState* st, hipStream_t s) const {
+    hipLaunchKernelGGL(prime_states_kernel, dim3((ninst_ + 63) / 64), dim3(64), 0, s, st, ninst_);
+    check_hip(hipGetLastError(), "prime_states_kernel");
+}
+
+// Autotuner: times every SpMV-pass shape (waves x unroll) and update-pass
+// shape on this problem's own matrix layout, on scratch vectors and a scratch
+// solver state, and keeps the fastest.  Launch shapes never change results.
+void Problem::autotune(hipStream_t s) {
+    if (!built_ || n_pad_ < kAutotuneMinRows) return;
+    const size_t v3 = size_t(n_pad_) * 3;
+    DeviceArray<float> f[6];
+    for (auto& a : f) {
+        a.resize(v3);
+        a.zero(s);
+    }
+    DeviceArray<double> xs, ps;
+    xs.resize(v3);
+    xs.zero(s);
+    ps.resize(partials_.size());
+    DeviceArray<InstState> sst;
+    sst.resize(size_t(ninst_));
+    check_hip(hipMemcpyAsync(sst.data(), state_.data(), sizeof(InstState) * ninst_, hipMemcpyDeviceToDevice, s),
+              "hipMemcpyAsync(scratch state)");
+    prime_states(sst.data(), s);
+    const DevVectors v{f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data(), f[5].data(), xs.data(),
+                       ps.data()};
+    hipEvent_t e0, e1;
+    check_hip(hipEventCreate(&e0), "hipEventCreate");
+    check_hip(hipEventCreate(&e1), "hipEventCreate");
+    auto time_it = [&](auto&& launch) {
+        for (int i = 0; i < 2; ++i) launch();
+        check_hip(hipEventRecord(e0, s), "hipEventRecord");
+        for (int i = 0; i < kAutotuneReps; ++i) launch();
+        check_hip(hipEventRecord(e1, s), "hipEventRecord");
+        check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.0f;
+        check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        return double(ms) / kAutotuneReps;
+    };
+    const int waves_list[] = {1, 2, 4, 8, 16};
+    const int unroll_list[] = {1, 2, 4};
+    double best = 1e30;
+    tune_log_.clear();
+    for (int w : waves_list)
+        for (int u : unroll_list) {
+            const double t = time_it([&] { launch_pcg_spmv(geom(), matrix(), v, sst.data(), w, u, s); });
+            tune_log_.push_back({0, w, u, t});
+            if (t < best) {
+                best = t;
+                cfg_.spmv_waves = w;
+                cfg_.spmv_unroll = u;
+            }
+        }
+    best = 1e30;
+    for (int w : waves_list) {
+        const double t = time_it([&] { launch_pcg_update(geom(), matrix(), v, sst.data(), w, s); });
+        tune_log_.push_back({1, w, 0, t});
+        if (t < best) {
+            best = t;
+            cfg_.update_waves = w;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(autotune)");
+    destroy_graph();
 }
 void Problem::capture_chunk_graph() {
     if (!cap_stream_) check_hip(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -373,7 +465,7 @@ void Problem::launch_pcg_chunk(hipStream_t s) {
     check_hip(hipGraphLaunch(chunk_exec_, s), "hipGraphLaunch(pcg chunk)");
 }
 void Problem::launch_finalize(float dt, const DevExt& ext, hipStream_t s) {
-    chysx_amd::launch_finalize(geom(), vectors(), ext, params_.data(), dt, s);
+    chysx_amd::launch_finalize(geom(), vectors(), ext, params_.data(), dt, cfg_.finalize_waves, s);
 }
 void Problem::read_states(hipStream_t s) {
     check_hip(hipMemcpyAsync(host_states_, state_.data(), sizeof(InstState) * ninst_, hipMemcpyDeviceToHost, s),
@@ -410,7 +502,7 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
         max_it = std::max(max_it, host_states_[k].iters);
     }
     if (any_stopped) {
-        launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 1, s);
+        launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 1, cfg_.residual_waves, s);
         launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseFinal, s);
         read_states(s);
     }
@@ -622,6 +714,7 @@ void ClothSim::ensure_topology() {
     inst_.n = n_;
     inst_.part0 = 0;
     prob_.build({&inst_});
+    if (autotune_) prob_.autotune(nullptr);
     topo_dirty_ = false;
     params_dirty_ = true;
 }
@@ -766,6 +859,7 @@ void ClothBatch::step(float dt, std::uintptr_t stream) {
     for (auto& I : insts_) v.push_back(I.get());
     if (dirty_ || !prob_.built()) {
         prob_.build(v);
+        if (autotune_) prob_.autotune(s);
         dirty_ = false;
     }
     prob_.upload_params(v, s);

@@ -77,6 +77,24 @@ struct Instance {
     Material material;
 };
 
+// Workgroup shapes of the step's kernels (speed only; results do not depend
+// on them).
+struct LaunchConfig {
+    int assemble_waves = 4;
+    int residual_waves = 4;
+    int spmv_waves = 4;
+    int spmv_unroll = 1;
+    int update_waves = 4;
+    int finalize_waves = 4;
+};
+
+struct TuneSample {
+    int kernel;  // 0: SpMV pass, 1: update pass
+    int waves;
+    int unroll;
+    double ms;
+};
+
 struct SolveStats {
     int iters = 0;
     int restarts = 0;
@@ -107,7 +125,7 @@ public:
     std::vector<SolveStats> stats() const { return stats_; }
     int n_rows_padded() const { return n_pad_; }
     int ninst() const { return ninst_; }
-    int ntiles() const { return int(tiles_h_.size()); }
+    int nslices() const { return int(slices_h_.size()); }
 
     // Reference-layout views of instance `k` (host copies).
     const Pattern& pattern(int k) const { return pats_[size_t(k)]; }
@@ -123,7 +141,7 @@ public:
     void zero_dx(hipStream_t s);
 
     // Kernel-level access for benchmarks.
-    LaunchGeom geom() const { return LaunchGeom{tiles_.data(), int(tiles_h_.size())}; }
+    LaunchGeom geom() const { return LaunchGeom{slices_.data(), int(slices_h_.size())}; }
     DevMatrix matrix() const;
     DevVectors vectors() const;
     DevMesh mesh() const;
@@ -145,6 +163,17 @@ public:
     bool all_done() const;
 
     static constexpr int kChunkIters = 8;
+    static constexpr int kAutotuneMinRows = 1 << 15;
+    static constexpr int kAutotuneReps = 8;
+
+    const LaunchConfig& launch_config() const { return cfg_; }
+    void set_launch_config(const LaunchConfig& c);
+    // Pick the fastest SpMV / update shapes for this problem (no-op for small
+    // problems).  Uses scratch buffers; solver state and vectors are untouched.
+    void autotune(hipStream_t s);
+    const std::vector<TuneSample>& tune_log() const { return tune_log_; }
+    // Mark every instance of `st` as running (timing hooks only).
+    void prime_states(InstState* st, hipStream_t s) const;
 
 private:
     void capture_chunk_graph();
@@ -158,11 +187,11 @@ private:
     std::vector<Pattern> pats_;
     std::vector<std::vector<int>> tri_slots_, spr_slots_, pin_slots_;
     std::vector<int> inst_row0_, inst_n_, inst_slice0_, inst_nslices_;
-    std::vector<Tile> tiles_h_;
+    std::vector<SliceDesc> slices_h_;
     std::vector<int> slice_ptr_h_;
     std::vector<int> row_deg_h_;
 
-    DeviceArray<Tile> tiles_;
+    DeviceArray<SliceDesc> slices_;
     DeviceArray<InstState> state_;
     DeviceArray<InstParams> params_;
     // mesh
@@ -183,6 +212,8 @@ private:
     std::vector<SolveStats> stats_;
     int last_iters_ = 0;
 
+    LaunchConfig cfg_;
+    std::vector<TuneSample> tune_log_;
     hipStream_t cap_stream_ = nullptr;
     hipGraphExec_t chunk_exec_ = nullptr;
 };
@@ -211,6 +242,7 @@ public:
     double pcg_tolerance() const { return inst_.tol; }
     void set_springs_active(bool on);
     void set_external_force_active(bool on) { inst_.force_active = on; params_dirty_ = true; }
+    void set_autotune(bool on) { autotune_ = on; }
     void step(float dt, std::uintptr_t stream);
 
     int num_springs() const { return inst_.has_springs ? int(inst_.edges.size() / 2) : 0; }
@@ -244,6 +276,7 @@ private:
     int n_ = 0;
     bool topo_dirty_ = true;
     bool params_dirty_ = true;
+    bool autotune_ = true;
     int dumped_n_ = 0;
 };
 
@@ -262,6 +295,7 @@ public:
                               std::uintptr_t force) noexcept;
     void set_material(int k, const Material& m);
     void set_options(double tol, int max_iter, bool springs_active, bool force_active);
+    void set_autotune(bool on) { autotune_ = on; }
     void step(float dt, std::uintptr_t stream);
     int count() const { return int(insts_.size()); }
     int total_particles() const { return total_; }
@@ -278,6 +312,7 @@ private:
     int total_ = 0;
     int ext_n_ = 0;
     bool dirty_ = true;
+    bool autotune_ = true;
     bool params_dirty_ = true;
 };
 

@@ -11,6 +11,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/chysx_cloth.h"
 
@@ -296,6 +297,47 @@ PYBIND11_MODULE(_chysx_native, m) {
                 return ms;
             },
             py::arg("which"), py::arg("reps"), py::arg("dt"), py::arg("cuda_stream") = 0)
+        .def(
+            "set_autotune", [](Sim& s, bool on) { check(chysx_cloth_set_autotune(s.p(), on ? 1 : 0)); },
+            py::arg("on"))
+        .def("launch_config",
+             [](const Sim& s) {
+                 int c[6];
+                 check(chysx_cloth_launch_config(s.p(), c));
+                 py::dict d;
+                 d["assemble_waves"] = c[0];
+                 d["residual_waves"] = c[1];
+                 d["spmv_waves"] = c[2];
+                 d["spmv_unroll"] = c[3];
+                 d["update_waves"] = c[4];
+                 d["finalize_waves"] = c[5];
+                 return d;
+             })
+        .def(
+            "set_launch_config",
+            [](Sim& s, py::dict d) {
+                int c[6];
+                check(chysx_cloth_launch_config(s.p(), c));
+                const char* keys[6] = {"assemble_waves", "residual_waves", "spmv_waves",
+                                       "spmv_unroll", "update_waves", "finalize_waves"};
+                for (int i = 0; i < 6; ++i)
+                    if (d.contains(keys[i])) c[i] = d[keys[i]].cast<int>();
+                check(chysx_cloth_set_launch_config(s.p(), c));
+            },
+            py::arg("config"))
+        .def("tune_log",
+             [](const Sim& s) {
+                 int n = 0;
+                 check(chysx_cloth_tune_log(s.p(), &n, nullptr, nullptr));
+                 std::vector<int> kwu(static_cast<size_t>(3 * n));
+                 std::vector<double> ms(static_cast<size_t>(n));
+                 check(chysx_cloth_tune_log(s.p(), &n, kwu.data(), ms.data()));
+                 py::list out;
+                 for (int i = 0; i < n; ++i)
+                     out.append(py::make_tuple(kwu[3 * i] == 0 ? "spmv" : "update", kwu[3 * i + 1], kwu[3 * i + 2],
+                                               ms[i]));
+                 return out;
+             })
         .def("spmv_bytes", [](const Sim& s) {
             double b = 0.0;
             check(chysx_cloth_spmv_bytes(s.p(), &b));
@@ -353,6 +395,9 @@ PYBIND11_MODULE(_chysx_native, m) {
                 check(chysx_batch_set_options(b.h, tol, max_iter, springs ? 1 : 0, force ? 1 : 0));
             },
             py::arg("tol"), py::arg("max_iter"), py::arg("springs_active") = true, py::arg("force_active") = false)
+        .def(
+            "set_autotune", [](Batch& b, bool on) { check(chysx_batch_set_autotune(b.h, on ? 1 : 0)); },
+            py::arg("on"))
         .def(
             "step", [](Batch& b, float dt, std::uintptr_t stream) { check(chysx_batch_step(b.h, dt, stream)); },
             py::arg("dt"), py::arg("cuda_stream") = 0)

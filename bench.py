@@ -42,6 +42,20 @@ def kernel_bytes(n_rows: int, nnz: int):
     }
 
 
+def read_traffic(size):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (recorded in profiles/traffic_cloth<size>.md as `kernel=bytes` lines)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_cloth{size}.md")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            line = line.strip().strip("`")
+            if line.startswith("traffic_bytes_per_launch "):
+                _, name, val = line.split()
+                out[name] = float(val)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +66,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (known bytes) for counter calibration")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -106,6 +122,7 @@ def main():
     t_iter = g.sim.time_kernel(4, args.kernel_reps, s.dt, stream)
     gbs = lambda b, ms: b / (ms * 1e-3) / 1e9  # noqa: E731
 
+    traffic = read_traffic(args.size)
     roofline = {
         "bound": "hbm",
         "kernel": "pcg_spmv_kernel (p = z + beta p_prev; Ap = A p; p.Ap)",
@@ -113,7 +130,7 @@ def main():
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
         "frac": round(gbs(kb["pcg_spmv"], t_pcg_spmv) / PEAK_HBM_GBS, 4),
-        "traffic": None,
+        "traffic": traffic.get("pcg_spmv_kernel"),
         "bytes_per_launch": kb["pcg_spmv"],
         "ms_per_launch": round(t_pcg_spmv, 5),
     }
@@ -172,7 +189,16 @@ def main():
         "roofline": roofline,
         "kernels": extra_kernels,
         "cpu_baseline": cpu,
+        "launch_config": g.sim.launch_config(),
+        "autotune": [list(t) for t in g.sim.tune_log()],
     }
+    if args.calibrate_copy:
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
     print(json.dumps(out))
 
 

@@ -222,3 +222,22 @@ def test_damping_and_uniform_force_closed_form(cuda):
         v = dx / s.dt * decay
     np.testing.assert_allclose(g.positions() - s.pos, np.tile(x, (s.n, 1)), atol=3e-6)
     np.testing.assert_allclose(g.velocities(), np.tile(v, (s.n, 1)), atol=3e-4)
+
+
+def test_launch_shapes_do_not_change_results(cuda):
+    """Workgroup shapes and SpMV unrolling are speed knobs only: every inner
+    product is summed per 64-row slice in slice order."""
+    s = scenes.config2(w=72, steps=5)
+    runs = []
+    for cfg in [dict(spmv_waves=1, spmv_unroll=1, update_waves=16, assemble_waves=2),
+                dict(spmv_waves=16, spmv_unroll=4, update_waves=1, assemble_waves=8),
+                dict(spmv_waves=4, spmv_unroll=2, update_waves=4, residual_waves=1, finalize_waves=16)]:
+        g = make_sim(s, tol=1e-6)
+        g.sim.set_autotune(False)
+        g.sim.set_launch_config(cfg)
+        for _ in range(5):
+            g.sim.step(s.dt)
+        runs.append((g.positions(), g.sim.get_dx64()))
+    for p, x in runs[1:]:
+        np.testing.assert_array_equal(p, runs[0][0])
+        np.testing.assert_array_equal(x, runs[0][1])
