@@ -42,6 +42,88 @@ def kernel_bytes(n_rows: int, nnz: int):
     }
 
 
+def bench_batch(args):
+    """512 cloths, W ~ U{16..128}, per-cloth k, mass and pins, stepped
+    together: one launch per kernel, per-cloth PCG state and convergence."""
+    import numpy as np
+    import torch
+    from chysx import scenes
+    from chysx.harness import make_batch
+
+    sc = scenes.config4(steps=args.warmup + args.steps)
+    t0 = time.perf_counter()
+    b = make_batch(sc, tol=args.tol, max_iter=20000)
+    b.batch.step(sc[0].dt)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    for _ in range(args.warmup - 1):
+        b.batch.step(sc[0].dt)
+    torch.cuda.synchronize()
+    cpu_state = None
+    if not args.no_cpu:
+        cpu_state = (b.pos.cpu().numpy().astype(np.float64), b.vel.cpu().numpy().astype(np.float64))
+    it_mean, it_max = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.batch.step(sc[0].dt)
+        st = [b.batch.stats(k) for k in range(len(sc))]
+        if not all(x["converged"] for x in st):
+            raise RuntimeError("a cloth did not converge")
+        its = [x["iters"] for x in st]
+        it_mean.append(np.mean(its))
+        it_max.append(np.max(its))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n_total = int(sum(s.n for s in sc))
+    cpu = None
+    if cpu_state is not None:
+        from oracle.oracle import oracle_from_scene
+        threads = min(16, os.cpu_count() or 1)
+        pos_all, vel_all = cpu_state
+        secs_total, rows_done, its = 0.0, 0, 0
+        order = np.random.default_rng(0).permutation(len(sc))[:48]
+        for k in order:
+            s = sc[k]
+            a = b.offsets[k]
+            oc, _, _, im, f = oracle_from_scene(s, precision=64, tol=args.tol, max_iter=20000, threads=threads,
+                                                force_active=False)
+            oc.build_topology()
+            p0 = np.ascontiguousarray(pos_all[a:a + s.n])
+            v0 = np.ascontiguousarray(vel_all[a:a + s.n])
+            secs, it = oc.time_steps(p0, v0, im, None, s.dt, 1)
+            secs_total += secs
+            rows_done += s.n
+            its += it
+        est = secs_total * n_total / rows_done
+        cpu = {"value": round(1.0 / est, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle stepping 48 randomly chosen cloths of the batch (one step each, cold start, "
+                         f"{rows_done} particles, {secs_total:.2f} s), scaled by particle count to all 512 cloths"}
+    out = {
+        "metric": METRIC,
+        "value": round(args.steps / elapsed, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batch512: 512 cloths W~U{16..128}, k~10^U(2,4), m~U(0.005,0.02) kg, pins in "
+                               "{corner, two corners, top edge}, dt=1/60, PCG to 1e-6 per cloth",
+                   "particles": n_total, "cloths": len(sc), "parallelism": "1 GPU"},
+        "pcg_iters_per_step_mean": round(float(np.mean(it_mean)), 2),
+        "pcg_iters_per_step_max": round(float(np.mean(it_max)), 2),
+        "setup_s": round(setup_s, 2),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def read_traffic(size):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
     (recorded in profiles/traffic_cloth<size>.md as `kernel=bytes` lines)."""
@@ -61,6 +143,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", choices=["cloth1024", "batch512"], default="cloth1024",
+                    help="cloth1024: one 1024x1024 cloth (default); batch512: 512 uneven cloths in one launch")
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU baseline sample")
@@ -81,6 +165,9 @@ def main():
     from chysx.harness import make_sim
 
     dev = torch.device("cuda:0")
+    if args.workload == "batch512":
+        bench_batch(args)
+        return
     s = scenes.config3(w=args.size, steps=args.warmup + args.steps)
     t0 = time.perf_counter()
     g = make_sim(s, tol=args.tol, max_iter=20000)

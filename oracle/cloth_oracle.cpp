@@ -196,6 +196,7 @@ struct Cloth {
     double tol = 0.0;
     int max_iter = 50;                     // cloth_simulator.h:713
     int threads = 1;
+    bool assemble_only = false;            // test hook: build the step's system, skip solve + finalize
 
     // ---- derived --------------------------------------------------------
     bool topo_dirty = true;
@@ -599,6 +600,7 @@ struct Cloth {
             const T d = mass[i] * inv_dt2;
             diag[i].m[0] += d; diag[i].m[4] += d; diag[i].m[8] += d;
         }
+        if (assemble_only) return;  // system only: no solve, no state change
         // PCG with warm start dx (:1219-1249)
         pcg(rhs, dx);
         // finalize_step_kernel (:222-244)
@@ -778,6 +780,28 @@ int orc_step(void* h, double* pos, double* vel, const double* inv_mass, const do
         }
         from_v3(P, pos);
         from_v3(V, vel);
+    });
+}
+// Assemble the step's linear system at (pos, vel) without solving or
+// changing any state; read it back with orc_get_system.
+int orc_assemble(void* h, const double* pos, const double* vel, const double* inv_mass, const double* force,
+                 double dt) {
+    return dispatch(h, [&](auto& c) {
+        using T = std::remove_reference_t<decltype(c.spr_k[0])>;
+        auto P = to_v3<T>(pos, c.n);
+        auto V = to_v3<T>(vel, c.n);
+        std::vector<T> im;
+        if (inv_mass) im = to_vec<T>(inv_mass, c.n);
+        std::vector<V3<T>> F;
+        if (force) F = to_v3<T>(force, c.n);
+        c.assemble_only = true;
+        try {
+            c.step(P, V, inv_mass ? &im : nullptr, force ? &F : nullptr, T(dt));
+        } catch (...) {
+            c.assemble_only = false;
+            throw;
+        }
+        c.assemble_only = false;
     });
 }
 int orc_nnz(void* h) {

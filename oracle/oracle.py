@@ -44,6 +44,7 @@ def lib():
         L.orc_get_dx.argtypes = [C.c_void_p, _dp]
         L.orc_step.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _ip, _dp, _ip]
         L.orc_nnz.argtypes = [C.c_void_p]
+        L.orc_assemble.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_double]
         L.orc_build_topology.argtypes = [C.c_void_p]
         L.orc_get_pattern.argtypes = [C.c_void_p, _ip, _ip]
         L.orc_get_slots.argtypes = [C.c_void_p, _ip, _ip, _ip]
@@ -170,6 +171,12 @@ class OracleCloth:
         if s < 0:
             _check(2)
         return s, it.value
+
+    def assemble(self, pos, vel, inv_mass=None, force=None, dt=1.0 / 60.0):
+        """Build the step's system at (pos, vel) without solving; read it
+        with system()."""
+        _check(self._L.orc_assemble(self._h, _d(_f64(pos)), _d(_f64(vel)), _d(_f64(inv_mass)),
+                                    _d(_f64(force)), float(dt)))
 
     def build_topology(self):
         _check(self._L.orc_build_topology(self._h))

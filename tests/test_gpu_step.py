@@ -241,3 +241,25 @@ def test_launch_shapes_do_not_change_results(cuda):
     for p, x in runs[1:]:
         np.testing.assert_array_equal(p, runs[0][0])
         np.testing.assert_array_equal(x, runs[0][1])
+
+
+def test_batch_uneven_cloths_each_match_oracle(cuda):
+    """Uneven batch (config-4 recipe, smaller sizes): every cloth converges on
+    its own and each cloth's step matches the oracle from the same state."""
+    sc = scenes.config4(count=24, wmin=8, wmax=40, steps=4, seed=5)
+    b = make_batch(sc, tol=1e-6)
+    for _ in range(3):
+        b.batch.step(1.0 / 60.0)
+    pos0 = b.pos.cpu().numpy().astype(np.float64)
+    vel0 = b.vel.cpu().numpy().astype(np.float64)
+    b.batch.step(1.0 / 60.0)
+    for k in range(0, len(sc), 5):
+        s = sc[k]
+        st = b.batch.stats(k)
+        assert st["converged"] and st["relres"] <= 1e-6
+        a = b.offsets[k]
+        oc, _, _, im, _ = oracle_from_scene(s, precision=64, tol=1e-11, max_iter=100000, force_active=False)
+        p = np.ascontiguousarray(pos0[a:a + s.n])
+        v = np.ascontiguousarray(vel0[a:a + s.n])
+        oc.step(p, v, im, None, 1.0 / 60.0, 1)
+        assert rel(b.positions(k), p) <= 1e-5, k
