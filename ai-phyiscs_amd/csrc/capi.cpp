@@ -21,6 +21,9 @@ struct chysx_cloth {
 struct chysx_batch {
     ClothBatch batch;
 };
+struct chysx_wsnet {
+    WarmStartNet net;
+};
 
 namespace {
 thread_local std::string g_last_error;
@@ -509,6 +512,198 @@ int chysx_batch_set_autotune(chysx_batch* b, int on) {
     return guard([&] {
         need(b, "batch");
         b->batch.set_autotune(on != 0);
+    });
+}
+
+// ---- learned warm start ------------------------------------------------------
+namespace {
+Problem& cloth_prob(chysx_cloth* sim) { return sim->sim.problem(); }
+Problem& batch_prob(chysx_batch* b) { return b->batch.problem(); }
+
+void set_ws_params(Problem& P, const float* theta, int n) {
+    need(theta, "params");
+    if (n != WarmStartNet::num_params()) throw std::invalid_argument("warm start: wrong parameter count");
+    P.net().set_params(theta, nullptr);
+}
+void get_ws_params(Problem& P, float* theta, int n) {
+    need(theta, "params");
+    if (n != WarmStartNet::num_params()) throw std::invalid_argument("warm start: wrong parameter count");
+    if (!P.net().has_params()) throw std::runtime_error("warm start: no parameters set");
+    P.net().get_params(theta);
+}
+}  // namespace
+
+int chysx_ws_num_params(int* out) {
+    return guard([&] {
+        need(out, "out");
+        *out = WarmStartNet::num_params();
+    });
+}
+int chysx_cloth_set_warm_start_mode(chysx_cloth* sim, int mode) {
+    return guard([&] {
+        need(sim, "sim");
+        cloth_prob(sim).set_warm_start_mode(mode);
+    });
+}
+int chysx_cloth_ws_set_params(chysx_cloth* sim, const float* theta, int n) {
+    return guard([&] {
+        need(sim, "sim");
+        set_ws_params(cloth_prob(sim), theta, n);
+    });
+}
+int chysx_cloth_ws_get_params(chysx_cloth* sim, float* theta, int n) {
+    return guard([&] {
+        need(sim, "sim");
+        get_ws_params(cloth_prob(sim), theta, n);
+    });
+}
+int chysx_cloth_ws_init_params(chysx_cloth* sim, uint64_t seed) {
+    return guard([&] {
+        need(sim, "sim");
+        cloth_prob(sim).net().init_params(seed, nullptr);
+    });
+}
+int chysx_cloth_ws_set_training(chysx_cloth* sim, int on, float lr) {
+    return guard([&] {
+        need(sim, "sim");
+        cloth_prob(sim).set_ws_training(on != 0, lr);
+    });
+}
+int chysx_cloth_ws_stats(chysx_cloth* sim, int* used, double* r0_base, double* r0_cand, double* loss) {
+    return guard([&] {
+        need(sim, "sim");
+        const SolveStats st = sim->sim.last_stats();
+        if (used) *used = st.ws_used;
+        if (r0_base) *r0_base = st.ws_r0_base;
+        if (r0_cand) *r0_cand = st.ws_r0_cand;
+        if (loss) *loss = cloth_prob(sim).last_ws_loss();
+    });
+}
+int chysx_batch_set_warm_start_mode(chysx_batch* b, int mode) {
+    return guard([&] {
+        need(b, "batch");
+        batch_prob(b).set_warm_start_mode(mode);
+    });
+}
+int chysx_batch_ws_set_params(chysx_batch* b, const float* theta, int n) {
+    return guard([&] {
+        need(b, "batch");
+        set_ws_params(batch_prob(b), theta, n);
+    });
+}
+int chysx_batch_ws_get_params(chysx_batch* b, float* theta, int n) {
+    return guard([&] {
+        need(b, "batch");
+        get_ws_params(batch_prob(b), theta, n);
+    });
+}
+int chysx_batch_ws_init_params(chysx_batch* b, uint64_t seed) {
+    return guard([&] {
+        need(b, "batch");
+        batch_prob(b).net().init_params(seed, nullptr);
+    });
+}
+int chysx_batch_ws_set_training(chysx_batch* b, int on, float lr) {
+    return guard([&] {
+        need(b, "batch");
+        batch_prob(b).set_ws_training(on != 0, lr);
+    });
+}
+int chysx_batch_ws_stats(chysx_batch* b, int index, int* used, double* r0_base, double* r0_cand, double* loss) {
+    return guard([&] {
+        need(b, "batch");
+        if (index < 0 || index >= b->batch.count()) throw std::out_of_range("batch: cloth index out of range");
+        const SolveStats st = b->batch.stats(index);
+        if (used) *used = st.ws_used;
+        if (r0_base) *r0_base = st.ws_r0_base;
+        if (r0_cand) *r0_cand = st.ws_r0_cand;
+        if (loss) *loss = batch_prob(b).last_ws_loss();
+    });
+}
+
+// ---- stand-alone network (tests) ---------------------------------------------
+namespace {
+struct NetGraph {
+    DeviceArray<SliceDesc> slices;
+    DeviceArray<int> slice_ptr, col, row_deg;
+    WsGraph g{};
+    int rows = 0;
+    void build(int n, const int32_t* ro, const int32_t* cols) {
+        if (n <= 0) throw std::invalid_argument("wsnet: n must be positive");
+        Pattern p;
+        p.n = n;
+        p.row_offsets.assign(ro, ro + n + 1);
+        p.cols.assign(cols, cols + ro[n]);
+        for (int c : p.cols)
+            if (c < 0 || c >= n) throw std::out_of_range("wsnet: column out of range");
+        const DeviceLayout L = build_device_layout(p, nullptr, 0, {}, nullptr, 0, {}, nullptr, 0);
+        std::vector<SliceDesc> sd(static_cast<size_t>(L.nslices));
+        for (int q = 0; q < L.nslices; ++q) sd[size_t(q)] = SliceDesc{0, 64 * q, std::min(64, n - 64 * q), 0};
+        slices.assign(sd.data(), sd.size());
+        slice_ptr.assign(L.slice_ptr.data(), L.slice_ptr.size());
+        col.assign(L.sell_col.data(), L.sell_col.size());
+        row_deg.assign(L.row_deg.data(), L.row_deg.size());
+        g = WsGraph{slices.data(), L.nslices, slice_ptr.data(), col.data(), row_deg.data()};
+        rows = L.n_pad;
+    }
+};
+}  // namespace
+
+int chysx_wsnet_create(chysx_wsnet** out) {
+    return guard([&] {
+        need(out, "out");
+        *out = new chysx_wsnet();
+    });
+}
+int chysx_wsnet_destroy(chysx_wsnet* net) {
+    return guard([&] { delete net; });
+}
+int chysx_wsnet_set_params(chysx_wsnet* net, const float* theta, int n) {
+    return guard([&] {
+        need(net, "net");
+        need(theta, "params");
+        if (n != WarmStartNet::num_params()) throw std::invalid_argument("warm start: wrong parameter count");
+        net->net.set_params(theta, nullptr);
+    });
+}
+int chysx_wsnet_forward(chysx_wsnet* net, int n, const int32_t* row_offsets, const int32_t* cols, const float* F,
+                        const float* E, float* out) {
+    return guard([&] {
+        need(net, "net");
+        need(row_offsets, "row_offsets");
+        need(F, "F");
+        need(E, "E");
+        need(out, "out");
+        NetGraph G;
+        G.build(n, row_offsets, cols);
+        net->net.reserve(G.rows, false);
+        net->net.set_inputs(F, E, n, nullptr);
+        net->net.forward(G.g, nullptr, false);
+        net->net.copy_out(out, n, nullptr);
+    });
+}
+int chysx_wsnet_loss_grad(chysx_wsnet* net, int n, const int32_t* row_offsets, const int32_t* cols, const float* F,
+                          const float* E, const float* target, double* loss, float* grad) {
+    return guard([&] {
+        need(net, "net");
+        need(row_offsets, "row_offsets");
+        need(F, "F");
+        need(E, "E");
+        need(target, "target");
+        need(grad, "grad");
+        NetGraph G;
+        G.build(n, row_offsets, cols);
+        net->net.reserve(G.rows, true);
+        net->net.set_inputs(F, E, n, nullptr);
+        DeviceArray<float> t;
+        std::vector<float> th(size_t(G.rows) * 3, 0.0f);
+        std::copy(target, target + 3 * size_t(n), th.begin());
+        t.assign(th.data(), th.size());
+        net->net.forward(G.g, nullptr, true);
+        net->net.loss(G.g, t.data(), nullptr);
+        net->net.backward(G.g, nullptr);
+        if (loss) *loss = net->net.read_loss(nullptr);
+        net->net.copy_grad(grad, nullptr);
     });
 }
 

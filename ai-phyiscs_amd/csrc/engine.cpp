@@ -54,6 +54,7 @@ template class DeviceArray<double>;
 template class DeviceArray<SliceDesc>;
 template class DeviceArray<InstState>;
 template class DeviceArray<InstParams>;
+template class DeviceArray<WsInst>;
 
 // ---------------------------------------------------------------------------
 // Rest data
@@ -283,6 +284,18 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     }
     partials_.resize(std::max<size_t>(size_t(4) * slices_h_.size(), 4));
     partials_.zero(s);
+    hprev_.resize(v3);
+    hprev_.zero(s);
+    xc_.resize(v3);
+    xc_.zero(s);
+    ws_target_.resize(v3);
+    ws_target_.zero(s);
+    rest_pos_.resize(size_t(std::max(n_rows_, 1)) * 3);
+    ws_inst_.resize(size_t(std::max(ninst_, 1)));
+    ws_inst_.zero(s);
+    ws_host_.assign(size_t(std::max(ninst_, 1)), WsInst{});
+    rest_captured_ = false;
+    ws_age_ = 0;
     params_.resize(size_t(std::max(ninst_, 1)));
     if (host_states_) (void)hipHostFree(host_states_);
     host_states_ = nullptr;
@@ -480,9 +493,55 @@ bool Problem::all_done() const {
     return true;
 }
 
+void Problem::set_warm_start_mode(int mode) {
+    if (mode != kWarmPrevious && mode != kWarmLearned)
+        throw std::invalid_argument("warm start mode: 0 (previous step) or 1 (learned)");
+    ws_mode_ = mode;
+}
+
+WarmStartNet& Problem::net() {
+    if (!net_) net_ = std::make_unique<WarmStartNet>();
+    return *net_;
+}
+
+WsGraph Problem::ws_graph() const {
+    return WsGraph{slices_.data(), int(slices_h_.size()), slice_ptr_.data(), col_.data(), row_deg_.data()};
+}
+
+void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
+    WarmStartNet& N = net();
+    if (!N.has_params()) throw std::runtime_error("learned warm start: no network parameters set");
+    N.reserve(n_pad_, ws_train_);
+    if (!rest_captured_) {
+        check_hip(hipMemcpyAsync(rest_pos_.data(), ext.pos, sizeof(float) * 3 * size_t(n_rows_),
+                                 hipMemcpyDeviceToDevice, s),
+                  "hipMemcpyAsync(rest positions)");
+        rest_captured_ = true;
+    }
+    const DevVectors v = vectors();
+    launch_ws_prep(geom(), matrix(), v, v.x, partials_.data(), s);
+    launch_ws_reduce(ws_inst_.data(), state_.data(), partials_.data(), ninst_, 0, s);
+    launch_ws_features(geom(), matrix(), v, ext, mesh(), rest_pos_.data(), hprev_.data(), ws_inst_.data(),
+                       params_.data(), dt, N.features(), N.edge_features(), s);
+    N.forward(ws_graph(), s, ws_train_);
+    if (ws_mode_ == kWarmLearned) {
+        launch_ws_candidate(geom(), matrix(), v, N.out(), ws_inst_.data(), xc_.data(), partials_.data(), s);
+        launch_ws_reduce(ws_inst_.data(), state_.data(), partials_.data(), ninst_, 1, s);
+        launch_ws_select(geom(), ws_inst_.data(), xc_.data(), x_.data(), s);
+    }
+    ws_ran_ = true;
+}
+
 void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
     if (!built_ || ninst_ == 0) return;
     launch_assemble(dt, ext, s);
+    ws_ran_ = false;
+    // The network's inputs need two previous solutions (dx_prev, dx_prev2).
+    if (ws_mode_ == kWarmLearned || ws_train_) {
+        if (ws_age_ >= 2) warm_start_stage(dt, ext, s);
+        else launch_ws_history(x_.data(), hprev_.data(), x_.size(), s);
+    }
+    ++ws_age_;
     launch_pcg_start(s);
     // Launch about as many iterations as the previous step needed before the
     // first poll; chunks after convergence exit at once.
@@ -506,13 +565,30 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
         launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseFinal, s);
         read_states(s);
     }
+    if (ws_ran_ && ws_train_) {
+        WarmStartNet& N = net();
+        launch_ws_target(geom(), x_.data(), hprev_.data(), ws_inst_.data(), ws_target_.data(), s);
+        N.loss(ws_graph(), ws_target_.data(), s);
+        N.backward(ws_graph(), s);
+        N.adam_step(ws_lr_, s);
+        ws_loss_ = N.read_loss(s);
+    }
     launch_finalize(dt, ext, s);
+    if (ws_ran_) {
+        check_hip(hipMemcpyAsync(ws_host_.data(), ws_inst_.data(), sizeof(WsInst) * ninst_, hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync(warm start stats)");
+        check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(warm start stats)");
+    }
     for (int k = 0; k < ninst_; ++k) {
         SolveStats& q = stats_[size_t(k)];
         q.iters = host_states_[k].iters;
         q.restarts = host_states_[k].restarts;
         q.status = host_states_[k].status;
         q.relres = host_states_[k].relres;
+        const double bb = host_states_[k].bb > 0 ? host_states_[k].bb : 1.0;
+        q.ws_used = ws_ran_ ? ws_host_[size_t(k)].use_cand : 0;
+        q.ws_r0_base = ws_ran_ ? std::sqrt(ws_host_[size_t(k)].rr_base / bb) : 0.0;
+        q.ws_r0_cand = (ws_ran_ && ws_mode_ == kWarmLearned) ? std::sqrt(ws_host_[size_t(k)].rr_cand / bb) : 0.0;
     }
     last_iters_ = max_it;
 }
@@ -573,7 +649,10 @@ void Problem::set_dx64(int k, const double* in, hipStream_t s) {
               "hipMemcpyAsync(dx)");
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(dx)");
 }
-void Problem::zero_dx(hipStream_t s) { x_.zero(s); }
+void Problem::zero_dx(hipStream_t s) {
+    x_.zero(s);
+    ws_age_ = 0;
+}
 
 // ---------------------------------------------------------------------------
 // ClothSim

@@ -19,6 +19,7 @@
 #include "cloth_device.hpp"
 #include "cloth_kernels.hpp"
 #include "topology.hpp"
+#include "warmstart.hpp"
 
 namespace chysx_amd {
 
@@ -100,6 +101,16 @@ struct SolveStats {
     int restarts = 0;
     int status = 0;
     double relres = 0.0;
+    // warm start of this solve
+    int ws_used = 0;         // 1: the network's guess replaced the previous-step guess
+    double ws_r0_base = 0;   // |b - A dx_prev| / |b| (only when the network ran)
+    double ws_r0_cand = 0;   // |b - A dx_net| / |b|
+};
+
+// PCG initial guess.
+enum WarmStartMode : int {
+    kWarmPrevious = 0,  // the previous step's dx (reference behaviour)
+    kWarmLearned = 1,   // the network's prediction, unless its residual is larger
 };
 
 class Problem {
@@ -162,6 +173,16 @@ public:
     void read_states(hipStream_t s);        // blocking
     bool all_done() const;
 
+    // ---- learned warm start ------------------------------------------------
+    void set_warm_start_mode(int mode);
+    int warm_start_mode() const { return ws_mode_; }
+    // Online training: every step runs the network, the solver uses the
+    // previous-step guess (plain solver) unless the mode is learned, and the
+    // converged dx gives one Adam step.
+    void set_ws_training(bool on, float lr) { ws_train_ = on; ws_lr_ = lr; }
+    WarmStartNet& net();
+    double last_ws_loss() const { return ws_loss_; }
+
     static constexpr int kChunkIters = 8;
     static constexpr int kAutotuneMinRows = 1 << 15;
     static constexpr int kAutotuneReps = 8;
@@ -178,6 +199,8 @@ public:
 private:
     void capture_chunk_graph();
     void destroy_graph();
+    void warm_start_stage(float dt, const DevExt& ext, hipStream_t s);
+    WsGraph ws_graph() const;
 
     bool built_ = false;
     int ninst_ = 0;
@@ -214,6 +237,19 @@ private:
 
     LaunchConfig cfg_;
     std::vector<TuneSample> tune_log_;
+    // learned warm start
+    int ws_mode_ = kWarmPrevious;
+    bool ws_train_ = false;
+    float ws_lr_ = 1e-3f;
+    double ws_loss_ = 0.0;
+    bool ws_ran_ = false;
+    int ws_age_ = 0;  // steps since the iterate history was reset
+    bool rest_captured_ = false;
+    std::unique_ptr<WarmStartNet> net_;
+    DeviceArray<float> hprev_, rest_pos_, ws_target_;
+    DeviceArray<double> xc_;
+    DeviceArray<WsInst> ws_inst_;
+    std::vector<WsInst> ws_host_;
     hipStream_t cap_stream_ = nullptr;
     hipGraphExec_t chunk_exec_ = nullptr;
 };

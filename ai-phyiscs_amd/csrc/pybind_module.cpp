@@ -95,6 +95,14 @@ py::dict stats_dict(int it, int rs, int st, double rr) {
     return d;
 }
 
+int ws_mode(const std::string& m) {
+    if (m == "previous") return 0;
+    if (m == "learned") return 1;
+    throw py::value_error("warm start mode must be 'previous' or 'learned'");
+}
+
+py::array_t<float> params_out(int n) { return py::array_t<float>({n}); }
+
 }  // namespace
 
 PYBIND11_MODULE(_chysx_native, m) {
@@ -338,6 +346,42 @@ PYBIND11_MODULE(_chysx_native, m) {
                                                ms[i]));
                  return out;
              })
+        .def(
+            "set_warm_start_mode", [](Sim& s, const std::string& mode) {
+                check(chysx_cloth_set_warm_start_mode(s.p(), ws_mode(mode)));
+            },
+            py::arg("mode"), "'previous' (reference: previous step's dx) or 'learned'.")
+        .def(
+            "ws_set_params",
+            [](Sim& s, FltArr theta) { check(chysx_cloth_ws_set_params(s.p(), theta.data(), int(theta.size()))); },
+            py::arg("theta"))
+        .def("ws_get_params",
+             [](Sim& s) {
+                 int n = 0;
+                 check(chysx_ws_num_params(&n));
+                 auto out = params_out(n);
+                 check(chysx_cloth_ws_get_params(s.p(), out.mutable_data(), n));
+                 return out;
+             })
+        .def(
+            "ws_init_params", [](Sim& s, uint64_t seed) { check(chysx_cloth_ws_init_params(s.p(), seed)); },
+            py::arg("seed") = 0)
+        .def(
+            "ws_set_training",
+            [](Sim& s, bool on, float lr) { check(chysx_cloth_ws_set_training(s.p(), on ? 1 : 0, lr)); },
+            py::arg("on"), py::arg("lr") = 1e-3f)
+        .def("ws_stats",
+             [](Sim& s) {
+                 int used = 0;
+                 double rb = 0, rc = 0, loss = 0;
+                 check(chysx_cloth_ws_stats(s.p(), &used, &rb, &rc, &loss));
+                 py::dict d;
+                 d["used"] = used;
+                 d["r0_previous"] = rb;
+                 d["r0_learned"] = rc;
+                 d["loss"] = loss;
+                 return d;
+             })
         .def("spmv_bytes", [](const Sim& s) {
             double b = 0.0;
             check(chysx_cloth_spmv_bytes(s.p(), &b));
@@ -399,6 +443,44 @@ PYBIND11_MODULE(_chysx_native, m) {
             "set_autotune", [](Batch& b, bool on) { check(chysx_batch_set_autotune(b.h, on ? 1 : 0)); },
             py::arg("on"))
         .def(
+            "set_warm_start_mode", [](Batch& b, const std::string& mode) {
+                check(chysx_batch_set_warm_start_mode(b.h, ws_mode(mode)));
+            },
+            py::arg("mode"))
+        .def(
+            "ws_set_params",
+            [](Batch& b, FltArr theta) { check(chysx_batch_ws_set_params(b.h, theta.data(), int(theta.size()))); },
+            py::arg("theta"))
+        .def("ws_get_params",
+             [](Batch& b) {
+                 int n = 0;
+                 check(chysx_ws_num_params(&n));
+                 auto out = params_out(n);
+                 check(chysx_batch_ws_get_params(b.h, out.mutable_data(), n));
+                 return out;
+             })
+        .def(
+            "ws_init_params", [](Batch& b, uint64_t seed) { check(chysx_batch_ws_init_params(b.h, seed)); },
+            py::arg("seed") = 0)
+        .def(
+            "ws_set_training",
+            [](Batch& b, bool on, float lr) { check(chysx_batch_ws_set_training(b.h, on ? 1 : 0, lr)); },
+            py::arg("on"), py::arg("lr") = 1e-3f)
+        .def(
+            "ws_stats",
+            [](Batch& b, int k) {
+                int used = 0;
+                double rb = 0, rc = 0, loss = 0;
+                check(chysx_batch_ws_stats(b.h, k, &used, &rb, &rc, &loss));
+                py::dict d;
+                d["used"] = used;
+                d["r0_previous"] = rb;
+                d["r0_learned"] = rc;
+                d["loss"] = loss;
+                return d;
+            },
+            py::arg("index"))
+        .def(
             "step", [](Batch& b, float dt, std::uintptr_t stream) { check(chysx_batch_step(b.h, dt, stream)); },
             py::arg("dt"), py::arg("cuda_stream") = 0)
         .def(
@@ -436,5 +518,49 @@ PYBIND11_MODULE(_chysx_native, m) {
         },
         py::arg("n"), py::arg("triangles"), py::arg("include_springs") = false,
         py::arg("pins") = IntArr(0), "Host-only pattern and slot maps of a mesh (no GPU needed).");
+    m.def("ws_num_params", []() {
+        int n = 0;
+        check(chysx_ws_num_params(&n));
+        return n;
+    });
+    m.def(
+        "wsnet_forward",
+        [](FltArr theta, IntArr ro, IntArr cols, FltArr F, FltArr E) {
+            const int n = int(ro.size()) - 1;
+            if (F.ndim() != 2 || F.shape(0) != n || F.shape(1) != 32 || E.ndim() != 2 || E.shape(0) != n ||
+                E.shape(1) != 32)
+                throw py::value_error("wsnet_forward: F and E must be (n, 32)");
+            chysx_wsnet* net = nullptr;
+            check(chysx_wsnet_create(&net));
+            py::array_t<float> out({n, 32});
+            const int rc1 = chysx_wsnet_set_params(net, theta.data(), int(theta.size()));
+            const int rc2 = rc1 ? rc1
+                                : chysx_wsnet_forward(net, n, ro.data(), cols.data(), F.data(), E.data(),
+                                                      out.mutable_data());
+            chysx_wsnet_destroy(net);
+            check(rc2);
+            return out;
+        },
+        py::arg("theta"), py::arg("row_offsets"), py::arg("cols"), py::arg("F"), py::arg("E"));
+    m.def(
+        "wsnet_loss_grad",
+        [](FltArr theta, IntArr ro, IntArr cols, FltArr F, FltArr E, FltArr target) {
+            const int n = int(ro.size()) - 1;
+            if (F.ndim() != 2 || F.shape(0) != n || E.ndim() != 2 || E.shape(0) != n || target.ndim() != 2 ||
+                target.shape(0) != n || target.shape(1) != 3)
+                throw py::value_error("wsnet_loss_grad: F, E (n, 32) and target (n, 3) required");
+            chysx_wsnet* net = nullptr;
+            check(chysx_wsnet_create(&net));
+            py::array_t<float> grad({int(theta.size())});
+            double loss = 0.0;
+            const int rc1 = chysx_wsnet_set_params(net, theta.data(), int(theta.size()));
+            const int rc2 = rc1 ? rc1
+                                : chysx_wsnet_loss_grad(net, n, ro.data(), cols.data(), F.data(), E.data(),
+                                                        target.data(), &loss, grad.mutable_data());
+            chysx_wsnet_destroy(net);
+            check(rc2);
+            return py::make_tuple(loss, grad);
+        },
+        py::arg("theta"), py::arg("row_offsets"), py::arg("cols"), py::arg("F"), py::arg("E"), py::arg("target"));
     m.attr("__version__") = chysx_version();
 }

@@ -1,0 +1,980 @@
+// SPDX-License-Identifier: Apache-2.0
+//
+// Learned PCG warm start on gfx950: BF16 MFMA (v_mfma_f32_16x16x32_bf16)
+// forward / input-gradient / weight-gradient kernels, mean aggregation over
+// the mesh graph and its adjoint, loss, Adam, and the simulator-side feature,
+// candidate and fallback kernels.  See warmstart.hpp for the network.
+//
+// MFMA 16x16x32 BF16 operand layout used throughout (lane l, li = l % 16,
+// lg = l / 16): A[i][k] with i = li, k = 8 lg .. 8 lg + 7; B[k][j] with
+// j = li, k = 8 lg .. 8 lg + 7; D[i][j] with i = 4 lg + r (r = 0..3), j = li.
+
+#include "warmstart.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+
+namespace chysx_amd {
+
+[[noreturn]] void throw_hip(hipError_t err, const char* what);
+void check_hip(hipError_t err, const char* what);
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int FIN = WarmStartNet::kFin;
+constexpr int HID = WarmStartNet::kHidden;
+constexpr int XL = WarmStartNet::kLayerIn;
+constexpr int NOUTD = WarmStartNet::kOut;
+constexpr int NL = WarmStartNet::kLayers;
+
+// Parameter offsets (floats): [We, be, W0, b0, W1, b1, W2, b2, Wd, bd].
+constexpr int OFF_WE = 0;
+constexpr int OFF_BE = OFF_WE + FIN * HID;
+constexpr int OFF_W0 = OFF_BE + HID;
+constexpr int LAYER_STRIDE = XL * HID + HID;
+constexpr int OFF_WD = OFF_W0 + NL * LAYER_STRIDE;
+constexpr int OFF_BD = OFF_WD + HID * NOUTD;
+constexpr int NPARAM = OFF_BD + NOUTD;
+inline int off_w(int l) { return OFF_W0 + l * LAYER_STRIDE; }
+inline int off_b(int l) { return off_w(l) + XL * HID; }
+
+struct MatDesc {
+    int off, K, N;
+};
+constexpr int NMAT = NL + 2;
+const MatDesc kMats[NMAT] = {{OFF_WE, FIN, HID}, {OFF_W0, XL, HID}, {OFF_W0 + LAYER_STRIDE, XL, HID},
+                             {OFF_W0 + 2 * LAYER_STRIDE, XL, HID}, {OFF_WD, HID, NOUTD}};
+
+__device__ __forceinline__ float bf2f(__bf16 v) { return float(v); }
+__device__ __forceinline__ __bf16 f2bf(float v) { return __bf16(v); }
+
+// ---------------------------------------------------------------------------
+// GEMM: Y = act(sum_s X_s W_s + bias) (+ resid), rows x NOUT, X row-major
+// BF16 segments of widths multiple of 32; weights in (NOUT, K) layout.
+// One wave per 64 rows: 4 x NT tiles of 16 x 16, FP32 accumulators.
+// ---------------------------------------------------------------------------
+struct GemmArgs {
+    int rows;
+    const __bf16* seg[3];
+    int segw[3];
+    int nseg;
+    int K;
+    const __bf16* wnk;
+    const float* bias;
+    int relu;
+    const __bf16* resid;
+    float* out_f;
+    __bf16* out_b;
+    __bf16* pre_b;
+};
+
+template <int NT>
+__global__ __launch_bounds__(256) void ws_gemm_kernel(GemmArgs a) {
+    constexpr int NOUT = 16 * NT;
+    const int wave = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int row0 = wave * 64;
+    if (row0 >= a.rows) return;
+    const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    int kg = 0;
+    for (int sg = 0; sg < a.nseg; ++sg) {
+        const __bf16* X = a.seg[sg];
+        const int w = a.segw[sg];
+        for (int k0 = 0; k0 < w; k0 += 32) {
+            bf16x8 af[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+                af[mt] = *reinterpret_cast<const bf16x8*>(X + (size_t)(row0 + mt * 16 + li) * w + k0 + 8 * lg);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const bf16x8 bfr =
+                    *reinterpret_cast<const bf16x8*>(a.wnk + (size_t)(nt * 16 + li) * a.K + kg + k0 + 8 * lg);
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mt], bfr, acc[mt][nt], 0, 0, 0);
+            }
+        }
+        kg += w;
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + mt * 16 + 4 * lg + r;
+                const int n = nt * 16 + li;
+                const size_t o = (size_t)row * NOUT + n;
+                float v = acc[mt][nt][r] + (a.bias ? a.bias[n] : 0.0f);
+                if (a.pre_b) a.pre_b[o] = f2bf(v);
+                if (a.relu) v = fmaxf(v, 0.0f);
+                if (a.resid) v += bf2f(a.resid[o]);
+                if (a.out_f) a.out_f[o] = v;
+                if (a.out_b) a.out_b[o] = f2bf(v);
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: part[wg] = sum over this workgroup's 32-row chunks of
+// X^T dY (K x NOUT) and column sums of dY (bias gradient).  Chunks go to the
+// 4 waves of a workgroup round-robin; waves combine in order through LDS, so
+// the sum is deterministic for a given grid.
+// ---------------------------------------------------------------------------
+struct WgradArgs {
+    int rows;
+    const __bf16* seg[3];
+    int segw[3];
+    int nseg;
+    const __bf16* dy;
+    float* part;
+};
+
+template <int KT, int NT>
+__global__ __launch_bounds__(256) void ws_wgrad_kernel(WgradArgs a) {
+    constexpr int K = 16 * KT, NOUT = 16 * NT, OUTN = K * NOUT + NOUT;
+    __shared__ __attribute__((aligned(16))) __bf16 xs[4][32][K];
+    __shared__ __attribute__((aligned(16))) __bf16 ys[4][32][NOUT];
+    __shared__ float red[OUTN];
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
+    f32x4 acc[KT][NT];
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[kt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
+    const int nchunks = a.rows / 32;
+    for (int cg = blockIdx.x; cg * 4 < nchunks; cg += gridDim.x) {
+        const int c = cg * 4 + wv;
+        const bool live = c < nchunks;
+        const int m0 = c * 32;
+        // stage X (32 x K) and dY (32 x NOUT) of this wave's chunk
+        for (int u = l; u < 32 * (K / 8); u += 64) {
+            const int r = u / (K / 8), kk = (u % (K / 8)) * 8;
+            bf16x8 val = {};
+            if (live) {
+                int base = 0;
+                for (int sg = 0; sg < a.nseg; ++sg) {
+                    if (kk < base + a.segw[sg]) {
+                        val = *reinterpret_cast<const bf16x8*>(a.seg[sg] + (size_t)(m0 + r) * a.segw[sg] + (kk - base));
+                        break;
+                    }
+                    base += a.segw[sg];
+                }
+            }
+            *reinterpret_cast<bf16x8*>(&xs[wv][r][kk]) = val;
+        }
+        for (int u = l; u < 32 * (NOUT / 8); u += 64) {
+            const int r = u / (NOUT / 8), kk = (u % (NOUT / 8)) * 8;
+            bf16x8 val = {};
+            if (live) val = *reinterpret_cast<const bf16x8*>(a.dy + (size_t)(m0 + r) * NOUT + kk);
+            *reinterpret_cast<bf16x8*>(&ys[wv][r][kk]) = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            bf16x8 bfr;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) bfr[t] = ys[wv][8 * lg + t][nt * 16 + li];
+#pragma unroll
+            for (int kt = 0; kt < KT; ++kt) {
+                bf16x8 afr;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) afr[t] = xs[wv][8 * lg + t][kt * 16 + li];
+                acc[kt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr, acc[kt][nt], 0, 0, 0);
+            }
+        }
+        if (l < NOUT)
+            for (int r = 0; r < 32; ++r) bsum += bf2f(ys[wv][r][l]);
+        __syncthreads();
+    }
+    // combine the 4 waves in order
+    for (int w = 0; w < 4; ++w) {
+        if (wv == w) {
+#pragma unroll
+            for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int idx = (kt * 16 + 4 * lg + r) * NOUT + nt * 16 + li;
+                        red[idx] = (w == 0 ? 0.0f : red[idx]) + acc[kt][nt][r];
+                    }
+            if (l < NOUT) red[K * NOUT + l] = (w == 0 ? 0.0f : red[K * NOUT + l]) + bsum;
+        }
+        __syncthreads();
+    }
+    for (int q = threadIdx.x; q < OUTN; q += 256) a.part[(size_t)blockIdx.x * OUTN + q] = red[q];
+}
+
+// grad[woff + q] (q < K*NOUT) and grad[boff + n] from the workgroup partials,
+// summed in workgroup order.
+__global__ void ws_reduce_parts_kernel(const float* __restrict__ part, int nwg, int K, int NOUT, float* __restrict__ grad,
+                                       int woff, int boff) {
+    const int outn = K * NOUT + NOUT;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= outn) return;
+    float s = 0.0f;
+    for (int w = 0; w < nwg; ++w) s += part[(size_t)w * outn + q];
+    if (q < K * NOUT) grad[woff + q] = s;
+    else grad[boff + (q - K * NOUT)] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Graph kernels (one lane per row, one wave per 64-row slice).
+// ---------------------------------------------------------------------------
+// out_i = mean_{j in N(i)} h_j  (64 columns, BF16 in/out, FP32 sums)
+__global__ __launch_bounds__(256) void ws_agg_kernel(WsGraph g, const __bf16* __restrict__ h, __bf16* __restrict__ out) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    float acc[HID];
+#pragma unroll
+    for (int q = 0; q < HID; ++q) acc[q] = 0.0f;
+    const int sb = g.slice_ptr[s], w = g.slice_ptr[s + 1] - sb;
+    const int* colp = g.col + (size_t)sb * 64 + lane;
+    for (int k = 0; k < w; ++k) {
+        const int c = colp[k * 64];
+        if (c < 0) continue;
+        const bf16x8* src = reinterpret_cast<const bf16x8*>(h + (size_t)c * HID);
+#pragma unroll
+        for (int u = 0; u < HID / 8; ++u) {
+            const bf16x8 v = src[u];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) acc[8 * u + t] += bf2f(v[t]);
+        }
+    }
+    const int deg = g.row_deg[row];
+    const float inv = deg > 0 ? 1.0f / float(deg) : 0.0f;
+    bf16x8* dst = reinterpret_cast<bf16x8*>(out + (size_t)row * HID);
+#pragma unroll
+    for (int u = 0; u < HID / 8; ++u) {
+        bf16x8 v;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) v[t] = f2bf(acc[8 * u + t] * inv);
+        dst[u] = v;
+    }
+}
+
+// Backward through a layer's input: dh_out_i = dh_in_i + dx_i[0:64]
+//   + sum_{j in N(i)} dx_j[64:128] / deg_j   (adjoint of the mean aggregation)
+__global__ __launch_bounds__(256) void ws_dh_kernel(WsGraph g, const float* __restrict__ dh_in,
+                                                    const float* __restrict__ dx, float* __restrict__ dh_out) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    float acc[HID];
+    const float4* a4 = reinterpret_cast<const float4*>(dh_in + (size_t)row * HID);
+    const float4* x4 = reinterpret_cast<const float4*>(dx + (size_t)row * XL);
+#pragma unroll
+    for (int u = 0; u < HID / 4; ++u) {
+        const float4 p = a4[u], q = x4[u];
+        acc[4 * u] = p.x + q.x;
+        acc[4 * u + 1] = p.y + q.y;
+        acc[4 * u + 2] = p.z + q.z;
+        acc[4 * u + 3] = p.w + q.w;
+    }
+    const int sb = g.slice_ptr[s], w = g.slice_ptr[s + 1] - sb;
+    const int* colp = g.col + (size_t)sb * 64 + lane;
+    for (int k = 0; k < w; ++k) {
+        const int c = colp[k * 64];
+        if (c < 0) continue;
+        const int dc = g.row_deg[c];
+        const float inv = dc > 0 ? 1.0f / float(dc) : 0.0f;
+        const float4* y4 = reinterpret_cast<const float4*>(dx + (size_t)c * XL + HID);
+#pragma unroll
+        for (int u = 0; u < HID / 4; ++u) {
+            const float4 q = y4[u];
+            acc[4 * u] += q.x * inv;
+            acc[4 * u + 1] += q.y * inv;
+            acc[4 * u + 2] += q.z * inv;
+            acc[4 * u + 3] += q.w * inv;
+        }
+    }
+    float4* o4 = reinterpret_cast<float4*>(dh_out + (size_t)row * HID);
+#pragma unroll
+    for (int u = 0; u < HID / 4; ++u) o4[u] = make_float4(acc[4 * u], acc[4 * u + 1], acc[4 * u + 2], acc[4 * u + 3]);
+}
+
+// dpre = dh * (pre > 0), BF16 out.  n elements.
+__global__ void ws_relu_grad_kernel(const float* __restrict__ dh, const __bf16* __restrict__ pre, __bf16* __restrict__ dpre,
+                                    size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dpre[i] = f2bf(bf2f(pre[i]) > 0.0f ? dh[i] : 0.0f);
+}
+
+// Loss partials (per slice: sum of squared error over valid rows) and
+// dOUT = 2 (out - target) / n_valid in the first 3 columns.
+__global__ __launch_bounds__(256) void ws_loss_kernel(WsGraph g, const float* __restrict__ out,
+                                                      const float* __restrict__ target, float inv_nv,
+                                                      __bf16* __restrict__ dout, double* __restrict__ lpart) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    const bool valid = lane < g.slices[s].nvalid;
+    double e2 = 0.0;
+    for (int c = 0; c < NOUTD; ++c) {
+        float d = 0.0f;
+        if (valid && c < 3) {
+            const float err = out[(size_t)row * NOUTD + c] - target[3 * row + c];
+            e2 += double(err) * err;
+            d = 2.0f * err * inv_nv;
+        }
+        dout[(size_t)row * NOUTD + c] = f2bf(d);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
+    if (lane == 0) lpart[s] = e2;
+}
+
+__global__ void ws_adam_kernel(float* __restrict__ th, const float* __restrict__ gr, float* __restrict__ m,
+                               float* __restrict__ v, int n, float lr, float b1c, float b2c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = gr[i];
+    const float mi = 0.9f * m[i] + 0.1f * g;
+    const float vi = 0.999f * v[i] + 0.001f * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    th[i] -= lr * (mi / b1c) / (sqrtf(vi / b2c) + 1e-8f);
+}
+
+// BF16 copies of every weight matrix: wkn = (K, NOUT) as stored, wnk = transposed.
+__global__ void ws_bf16_kernel(const float* __restrict__ th, __bf16* __restrict__ wnk, __bf16* __restrict__ wkn,
+                               int off, int K, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K * N) return;
+    const int k = i / N, n = i % N;
+    const __bf16 b = f2bf(th[off + i]);
+    wkn[off + i] = b;
+    wnk[off + n * K + k] = b;
+}
+
+int grid_rows(int rows) { return (rows / 64 + 3) / 4; }
+
+__global__ void ws_to_bf16_kernel(const float* __restrict__ in, __bf16* __restrict__ out, size_t n_in, size_t n_out) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    out[i] = f2bf(i < n_in ? in[i] : 0.0f);
+}
+
+template <int NT>
+void run_gemm(const GemmArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(ws_gemm_kernel<NT>, dim3(grid_rows(a.rows)), dim3(256), 0, s, a);
+    check_hip(hipGetLastError(), "ws_gemm_kernel");
+}
+
+constexpr int kWgradBlocks = 240;
+
+template <int KT, int NT>
+void run_wgrad(const WgradArgs& a, float* grad, int woff, int boff, hipStream_t s) {
+    const int nchunk = a.rows / 32;
+    const int nwg = std::max(1, std::min(kWgradBlocks, (nchunk + 3) / 4));
+    hipLaunchKernelGGL((ws_wgrad_kernel<KT, NT>), dim3(nwg), dim3(256), 0, s, a);
+    check_hip(hipGetLastError(), "ws_wgrad_kernel");
+    const int outn = 16 * KT * 16 * NT + 16 * NT;
+    hipLaunchKernelGGL(ws_reduce_parts_kernel, dim3((outn + 255) / 256), dim3(256), 0, s, a.part, nwg, 16 * KT,
+                       16 * NT, grad, woff, boff);
+    check_hip(hipGetLastError(), "ws_reduce_parts_kernel");
+}
+
+template <class T>
+void dalloc(T*& p, size_t n) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    if (n) check_hip(hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T)), "hipMalloc(warm start)");
+}
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// WarmStartNet
+// ---------------------------------------------------------------------------
+int WarmStartNet::num_params() { return NPARAM; }
+
+WarmStartNet::WarmStartNet() {
+    dalloc(theta_, NPARAM);
+    dalloc(grad_, NPARAM);
+    dalloc(m_, NPARAM);
+    dalloc(v_, NPARAM);
+    dalloc(wnk_, NPARAM);
+    dalloc(wkn_, NPARAM);
+    check_hip(hipMemset(theta_, 0, NPARAM * sizeof(float)), "hipMemset");
+    check_hip(hipMemset(grad_, 0, NPARAM * sizeof(float)), "hipMemset");
+    check_hip(hipMemset(m_, 0, NPARAM * sizeof(float)), "hipMemset");
+    check_hip(hipMemset(v_, 0, NPARAM * sizeof(float)), "hipMemset");
+    check_hip(hipMemset(wnk_, 0, NPARAM * sizeof(__bf16)), "hipMemset");
+    check_hip(hipMemset(wkn_, 0, NPARAM * sizeof(__bf16)), "hipMemset");
+}
+
+WarmStartNet::~WarmStartNet() {
+    free_all();
+    dfree(theta_);
+    dfree(grad_);
+    dfree(m_);
+    dfree(v_);
+    dfree(wnk_);
+    dfree(wkn_);
+}
+
+void WarmStartNet::free_all() {
+    dfree(F_);
+    dfree(E_);
+    for (auto& p : H_) dfree(p);
+    for (auto& p : A_) dfree(p);
+    for (auto& p : P_) dfree(p);
+    dfree(out_);
+    dfree(dout_);
+    dfree(dh_);
+    dfree(dh2_);
+    dfree(dpre_);
+    dfree(dx_);
+    dfree(part_);
+    dfree(lpart_);
+    rows_ = 0;
+    training_ = false;
+}
+
+void WarmStartNet::reserve(int rows, bool training) {
+    if (rows % 64) throw std::invalid_argument("warm start: rows must be a multiple of 64");
+    if (rows <= rows_ && (training_ || !training)) return;
+    free_all();
+    const size_t R = size_t(rows);
+    dalloc(F_, R * FIN);
+    dalloc(E_, R * 32);
+    for (auto& p : H_) dalloc(p, R * HID);
+    for (auto& p : A_) dalloc(p, R * HID);
+    for (auto& p : P_) dalloc(p, R * HID);
+    dalloc(out_, R * NOUTD);
+    check_hip(hipMemset(F_, 0, R * FIN * sizeof(__bf16)), "hipMemset");
+    check_hip(hipMemset(E_, 0, R * 32 * sizeof(__bf16)), "hipMemset");
+    if (training) {
+        dalloc(dout_, R * NOUTD);
+        dalloc(dh_, R * HID);
+        dalloc(dh2_, R * HID);
+        dalloc(dpre_, R * HID);
+        dalloc(dx_, R * XL);
+        dalloc(part_, size_t(kWgradBlocks) * (XL * HID + HID));
+        dalloc(lpart_, R / 64);
+        lpart_n_ = int(R / 64);
+    }
+    rows_ = rows;
+    training_ = training;
+}
+
+void WarmStartNet::refresh_bf16(hipStream_t s) {
+    for (const MatDesc& d : kMats) {
+        hipLaunchKernelGGL(ws_bf16_kernel, dim3((d.K * d.N + 255) / 256), dim3(256), 0, s, theta_, wnk_, wkn_, d.off,
+                           d.K, d.N);
+        check_hip(hipGetLastError(), "ws_bf16_kernel");
+    }
+}
+
+void WarmStartNet::set_params(const float* theta, hipStream_t s) {
+    check_hip(hipMemcpyAsync(theta_, theta, NPARAM * sizeof(float), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    check_hip(hipMemsetAsync(m_, 0, NPARAM * sizeof(float), s), "hipMemsetAsync");
+    check_hip(hipMemsetAsync(v_, 0, NPARAM * sizeof(float), s), "hipMemsetAsync");
+    adam_t_ = 0;
+    refresh_bf16(s);
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    has_params_ = true;
+}
+
+void WarmStartNet::get_params(float* theta) const {
+    check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    check_hip(hipMemcpy(theta, theta_, NPARAM * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+}
+
+void WarmStartNet::init_params(uint64_t seed, hipStream_t s) {
+    std::vector<float> th(NPARAM, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (const MatDesc& d : kMats) {
+        std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / float(d.K)));
+        const float sc = (d.off == OFF_WD) ? 0.1f : 1.0f;
+        for (int i = 0; i < d.K * d.N; ++i) th[size_t(d.off + i)] = sc * nd(rng);
+    }
+    set_params(th.data(), s);
+}
+
+void WarmStartNet::set_inputs(const float* F, const float* E, int n, hipStream_t s) {
+    if (n > rows_) throw std::invalid_argument("warm start: reserve() rows first");
+    float* tmp = nullptr;
+    dalloc(tmp, size_t(n) * 32);
+    const size_t nin = size_t(n) * 32, nout = size_t(rows_) * 32;
+    for (int which = 0; which < 2; ++which) {
+        check_hip(hipMemcpyAsync(tmp, which ? E : F, nin * sizeof(float), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        hipLaunchKernelGGL(ws_to_bf16_kernel, dim3(unsigned((nout + 255) / 256)), dim3(256), 0, s, tmp,
+                           which ? E_ : F_, nin, nout);
+        check_hip(hipGetLastError(), "ws_to_bf16_kernel");
+    }
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    dfree(tmp);
+}
+
+void WarmStartNet::copy_out(float* host, int n, hipStream_t s) const {
+    check_hip(hipMemcpyAsync(host, out_, sizeof(float) * size_t(n) * NOUTD, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(out)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+void WarmStartNet::copy_grad(float* host, hipStream_t s) const {
+    check_hip(hipMemcpyAsync(host, grad_, NPARAM * sizeof(float), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep) {
+    const int rows = g.nslices * 64;
+    if (rows > rows_) throw std::invalid_argument("warm start: reserve() rows first");
+    if (keep && !training_) throw std::invalid_argument("warm start: reserve(training) before a kept forward");
+    (void)keep;
+    // encoder
+    GemmArgs a{};
+    a.rows = rows;
+    a.seg[0] = F_;
+    a.segw[0] = FIN;
+    a.nseg = 1;
+    a.K = FIN;
+    a.wnk = wnk_ + OFF_WE;
+    a.bias = theta_ + OFF_BE;
+    a.relu = 1;
+    a.out_b = H_[0];
+    a.pre_b = P_[0];
+    run_gemm<HID / 16>(a, s);
+    for (int l = 0; l < NL; ++l) {
+        hipLaunchKernelGGL(ws_agg_kernel, dim3((g.nslices + 3) / 4), dim3(256), 0, s, g, H_[l], A_[l]);
+        check_hip(hipGetLastError(), "ws_agg_kernel");
+        GemmArgs b{};
+        b.rows = rows;
+        b.seg[0] = H_[l];
+        b.segw[0] = HID;
+        b.seg[1] = A_[l];
+        b.segw[1] = HID;
+        b.seg[2] = E_;
+        b.segw[2] = 32;
+        b.nseg = 3;
+        b.K = XL;
+        b.wnk = wnk_ + off_w(l);
+        b.bias = theta_ + off_b(l);
+        b.relu = 1;
+        b.resid = H_[l];
+        b.out_b = H_[l + 1];
+        b.pre_b = P_[l + 1];
+        run_gemm<HID / 16>(b, s);
+    }
+    GemmArgs d{};
+    d.rows = rows;
+    d.seg[0] = H_[NL];
+    d.segw[0] = HID;
+    d.nseg = 1;
+    d.K = HID;
+    d.wnk = wnk_ + OFF_WD;
+    d.bias = theta_ + OFF_BD;
+    d.out_f = out_;
+    run_gemm<NOUTD / 16>(d, s);
+}
+
+void WarmStartNet::loss(const WsGraph& g, const float* target, hipStream_t s) {
+    if (!training_) throw std::invalid_argument("warm start: reserve(training) first");
+    // number of valid rows
+    std::vector<SliceDesc> sd(static_cast<size_t>(g.nslices));
+    check_hip(hipMemcpyAsync(sd.data(), g.slices, sizeof(SliceDesc) * g.nslices, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(slices)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    long long nv = 0;
+    for (const auto& q : sd) nv += q.nvalid;
+    hipLaunchKernelGGL(ws_loss_kernel, dim3((g.nslices + 3) / 4), dim3(256), 0, s, g, out_, target,
+                       1.0f / float(std::max<long long>(nv, 1)), dout_, lpart_);
+    check_hip(hipGetLastError(), "ws_loss_kernel");
+    lpart_n_ = g.nslices;
+    loss_nv_ = std::max<long long>(nv, 1);
+}
+
+double WarmStartNet::read_loss(hipStream_t s) {
+    std::vector<double> h(static_cast<size_t>(lpart_n_));
+    check_hip(hipMemcpyAsync(h.data(), lpart_, sizeof(double) * lpart_n_, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(loss)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    double t = 0.0;
+    for (double v : h) t += v;
+    return t / double(loss_nv_);
+}
+
+void WarmStartNet::backward(const WsGraph& g, hipStream_t s) {
+    if (!training_) throw std::invalid_argument("warm start: reserve(training) first");
+    const int rows = g.nslices * 64;
+    // decoder: dWd, dbd from H3 and dOUT; dH3 = dOUT Wd^T
+    {
+        WgradArgs w{};
+        w.rows = rows;
+        w.seg[0] = H_[NL];
+        w.segw[0] = HID;
+        w.nseg = 1;
+        w.dy = dout_;
+        w.part = part_;
+        run_wgrad<HID / 16, NOUTD / 16>(w, grad_, OFF_WD, OFF_BD, s);
+        GemmArgs a{};
+        a.rows = rows;
+        a.seg[0] = dout_;
+        a.segw[0] = NOUTD;
+        a.nseg = 1;
+        a.K = NOUTD;
+        a.wnk = wkn_ + OFF_WD;  // (HID, NOUTD) = (NOUT of this product, K)
+        a.out_f = dh_;
+        run_gemm<HID / 16>(a, s);
+    }
+    float* dh = dh_;
+    float* dh_next = dh2_;
+    const size_t nel = size_t(rows) * HID;
+    for (int l = NL - 1; l >= 0; --l) {
+        hipLaunchKernelGGL(ws_relu_grad_kernel, dim3(unsigned((nel + 255) / 256)), dim3(256), 0, s, dh, P_[l + 1],
+                           dpre_, nel);
+        check_hip(hipGetLastError(), "ws_relu_grad_kernel");
+        WgradArgs w{};
+        w.rows = rows;
+        w.seg[0] = H_[l];
+        w.segw[0] = HID;
+        w.seg[1] = A_[l];
+        w.segw[1] = HID;
+        w.seg[2] = E_;
+        w.segw[2] = 32;
+        w.nseg = 3;
+        w.dy = dpre_;
+        w.part = part_;
+        run_wgrad<XL / 16, HID / 16>(w, grad_, off_w(l), off_b(l), s);
+        GemmArgs a{};
+        a.rows = rows;
+        a.seg[0] = dpre_;
+        a.segw[0] = HID;
+        a.nseg = 1;
+        a.K = HID;
+        a.wnk = wkn_ + off_w(l);  // (XL, HID)
+        a.out_f = dx_;
+        run_gemm<XL / 16>(a, s);
+        hipLaunchKernelGGL(ws_dh_kernel, dim3((g.nslices + 3) / 4), dim3(256), 0, s, g, dh, dx_, dh_next);
+        check_hip(hipGetLastError(), "ws_dh_kernel");
+        std::swap(dh, dh_next);
+    }
+    hipLaunchKernelGGL(ws_relu_grad_kernel, dim3(unsigned((nel + 255) / 256)), dim3(256), 0, s, dh, P_[0], dpre_, nel);
+    check_hip(hipGetLastError(), "ws_relu_grad_kernel");
+    WgradArgs w{};
+    w.rows = rows;
+    w.seg[0] = F_;
+    w.segw[0] = FIN;
+    w.nseg = 1;
+    w.dy = dpre_;
+    w.part = part_;
+    run_wgrad<FIN / 16, HID / 16>(w, grad_, OFF_WE, OFF_BE, s);
+}
+
+void WarmStartNet::adam_step(float lr, hipStream_t s) {
+    ++adam_t_;
+    const float b1c = 1.0f - std::pow(0.9f, float(adam_t_));
+    const float b2c = 1.0f - std::pow(0.999f, float(adam_t_));
+    hipLaunchKernelGGL(ws_adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, s, theta_, grad_, m_, v_, NPARAM, lr,
+                       b1c, b2c);
+    check_hip(hipGetLastError(), "ws_adam_kernel");
+    refresh_bf16(s);
+}
+
+// ---------------------------------------------------------------------------
+// Simulator-side kernels
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ void ld3f(const float* p, int i, float& a, float& b, float& c) {
+    a = p[3 * i];
+    b = p[3 * i + 1];
+    c = p[3 * i + 2];
+}
+
+// b - A x with FP64 products and sums; optionally r, z = M^-1 r.
+__global__ __launch_bounds__(256) void ws_prep_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+                                                      const double* __restrict__ x, double* __restrict__ partials,
+                                                      int write_rz) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    double rr = 0.0, xx = 0.0;
+    if (lane < sd.nvalid) {
+        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+        const double x0 = x[3 * row], x1 = x[3 * row + 1], x2 = x[3 * row + 2];
+        double a0 = double(dg[0]) * x0 + double(dg[64]) * x1 + double(dg[128]) * x2;
+        double a1 = double(dg[192]) * x0 + double(dg[256]) * x1 + double(dg[320]) * x2;
+        double a2 = double(dg[384]) * x0 + double(dg[448]) * x1 + double(dg[512]) * x2;
+        const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
+        const int* colp = A.col + (size_t)sb * 64 + lane;
+        const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+        for (int k = 0; k < w; ++k) {
+            const int c = colp[k * 64];
+            if (c >= 0) {
+                const float* bk = vp + k * 9 * 64;
+                const double y0 = x[3 * c], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
+                a0 += double(bk[0]) * y0 + double(bk[64]) * y1 + double(bk[128]) * y2;
+                a1 += double(bk[192]) * y0 + double(bk[256]) * y1 + double(bk[320]) * y2;
+                a2 += double(bk[384]) * y0 + double(bk[448]) * y1 + double(bk[512]) * y2;
+            }
+        }
+        const double r0 = double(v.rhs[3 * row]) - a0, r1 = double(v.rhs[3 * row + 1]) - a1,
+                     r2 = double(v.rhs[3 * row + 2]) - a2;
+        rr = r0 * r0 + r1 * r1 + r2 * r2;
+        xx = x0 * x0 + x1 * x1 + x2 * x2;
+        if (write_rz) {
+            const float f0 = float(r0), f1 = float(r1), f2 = float(r2);
+            const float* mv = A.minv + (size_t)s * 6 * 64 + lane;
+            const float m0 = mv[0], m1 = mv[64], m2 = mv[128], m3 = mv[192], m4 = mv[256], m5 = mv[320];
+            v.r[3 * row] = f0;
+            v.r[3 * row + 1] = f1;
+            v.r[3 * row + 2] = f2;
+            v.z[3 * row] = m0 * f0 + m1 * f1 + m2 * f2;
+            v.z[3 * row + 1] = m1 * f0 + m3 * f1 + m4 * f2;
+            v.z[3 * row + 2] = m2 * f0 + m4 * f1 + m5 * f2;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        rr += __shfl_xor(rr, off, 64);
+        xx += __shfl_xor(xx, off, 64);
+    }
+    if (lane == 0) {
+        partials[4 * s] = rr;
+        partials[4 * s + 1] = xx;
+    }
+}
+
+__global__ __launch_bounds__(1024) void ws_reduce_kernel(WsInst* __restrict__ wi, const InstState* __restrict__ st,
+                                                         const double* __restrict__ partials, int phase) {
+    const InstState& S = st[blockIdx.x];
+    const int s0 = S.slice0, ns = S.nslices;
+    double a = 0.0, b = 0.0;
+    for (int q = threadIdx.x; q < ns; q += 1024) {
+        a += partials[4 * (s0 + q)];
+        b += partials[4 * (s0 + q) + 1];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    __shared__ double sh[16][2];
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6][0] = a;
+        sh[threadIdx.x >> 6][1] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double ta = 0.0, tb = 0.0;
+    for (int w = 0; w < 16; ++w) {
+        ta += sh[w][0];
+        tb += sh[w][1];
+    }
+    WsInst& W = wi[blockIdx.x];
+    if (phase == 0) {
+        W.rr_base = ta;
+        W.scale = fmax(sqrt(tb / (3.0 * 64.0 * double(ns))), 1e-12);
+        W.use_cand = 0;
+    } else {
+        W.rr_cand = ta;
+        W.use_cand = (ta < W.rr_base) ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatrix A, DevVectors v, DevExt e,
+                                                          DevMesh m, const float* __restrict__ rest,
+                                                          float* __restrict__ hprev, const WsInst* __restrict__ wi,
+                                                          const InstParams* __restrict__ params, float dt,
+                                                          __bf16* __restrict__ F, __bf16* __restrict__ E) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    float f[FIN];
+#pragma unroll
+    for (int q = 0; q < FIN; ++q) f[q] = 0.0f;
+    float ef[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (lane < sd.nvalid) {
+        const float is = float(1.0 / wi[sd.inst].scale);
+        const double x0 = v.x[3 * row], x1 = v.x[3 * row + 1], x2 = v.x[3 * row + 2];
+        float h0, h1, h2;
+        ld3f(hprev, row, h0, h1, h2);
+        f[0] = float(x0) * is;
+        f[1] = float(x1) * is;
+        f[2] = float(x2) * is;
+        f[3] = (float(x0) - h0) * is;
+        f[4] = (float(x1) - h1) * is;
+        f[5] = (float(x2) - h2) * is;
+        f[6] = v.z[3 * row] * is;
+        f[7] = v.z[3 * row + 1] * is;
+        f[8] = v.z[3 * row + 2] * is;
+        f[12] = (m.pin_inc_off[row + 1] > m.pin_inc_off[row]) ? 1.0f : 0.0f;
+        const int p = sd.part0 + lane;
+        float mass = 1.0f;
+        if (e.inv_mass != nullptr) {
+            const float w = e.inv_mass[p];
+            mass = (w > 1.0e-12f) ? (1.0f / w) : 1.0e8f;
+        }
+        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+        const float tr = (dg[0] + dg[256] + dg[512]) * (1.0f / 3.0f);
+        f[13] = (tr > 0.0f) ? fminf(mass / (dt * dt) / tr, 1.0f) : 0.0f;
+        f[14] = 1.0f;
+        hprev[3 * row] = float(x0);
+        hprev[3 * row + 1] = float(x1);
+        hprev[3 * row + 2] = float(x2);
+        // mean edge features over the mesh neighbours
+        float px, py, pz, rx, ry, rz;
+        ld3f(e.pos, p, px, py, pz);
+        ld3f(rest, p, rx, ry, rz);
+        const int shift = sd.part0 - 64 * s;  // particle = row + shift within an instance
+        const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
+        const int* colp = A.col + (size_t)sb * 64 + lane;
+        int cnt = 0;
+        for (int k = 0; k < w; ++k) {
+            const int c = colp[k * 64];
+            if (c < 0) continue;
+            const int pj = c + shift;
+            float qx, qy, qz, sx, sy, sz;
+            ld3f(e.pos, pj, qx, qy, qz);
+            ld3f(rest, pj, sx, sy, sz);
+            const float dx = qx - px, dy = qy - py, dz = qz - pz;
+            const float l0 = sqrtf((sx - rx) * (sx - rx) + (sy - ry) * (sy - ry) + (sz - rz) * (sz - rz));
+            const float il = l0 > 1e-12f ? 1.0f / l0 : 0.0f;
+            ef[0] += dx * il;
+            ef[1] += dy * il;
+            ef[2] += dz * il;
+            ef[3] += sqrtf(dx * dx + dy * dy + dz * dz) * il - 1.0f;
+            ++cnt;
+        }
+        if (cnt) {
+            const float ic = 1.0f / float(cnt);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ef[q] *= ic;
+        }
+    }
+    bf16x8* fo = reinterpret_cast<bf16x8*>(F + (size_t)row * FIN);
+#pragma unroll
+    for (int u = 0; u < FIN / 8; ++u) {
+        bf16x8 t;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) t[q] = f2bf(f[8 * u + q]);
+        fo[u] = t;
+    }
+    bf16x8* eo = reinterpret_cast<bf16x8*>(E + (size_t)row * 32);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        bf16x8 t;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) t[q] = f2bf((u == 0 && q < 4) ? ef[q] : 0.0f);
+        eo[u] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void ws_candidate_kernel(LaunchGeom g, const double* __restrict__ x,
+                                                           const float* __restrict__ out,
+                                                           const WsInst* __restrict__ wi, double* __restrict__ xc) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    if (lane >= sd.nvalid) {
+        xc[3 * row] = xc[3 * row + 1] = xc[3 * row + 2] = 0.0;
+        return;
+    }
+    const double sc = wi[sd.inst].scale;
+    for (int c = 0; c < 3; ++c) xc[3 * row + c] = x[3 * row + c] + sc * double(out[(size_t)row * NOUTD + c]);
+}
+
+__global__ __launch_bounds__(256) void ws_select_kernel(LaunchGeom g, const WsInst* __restrict__ wi,
+                                                        const double* __restrict__ xc, double* __restrict__ x) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    if (lane >= sd.nvalid || !wi[sd.inst].use_cand) return;
+    for (int c = 0; c < 3; ++c) x[3 * row + c] = xc[3 * row + c];
+}
+
+__global__ __launch_bounds__(256) void ws_target_kernel(LaunchGeom g, const double* __restrict__ x,
+                                                        const float* __restrict__ hprev, const WsInst* __restrict__ wi,
+                                                        float* __restrict__ target) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    const double is = 1.0 / wi[sd.inst].scale;
+    for (int c = 0; c < 3; ++c)
+        target[3 * row + c] = (lane < sd.nvalid) ? float((x[3 * row + c] - double(hprev[3 * row + c])) * is) : 0.0f;
+}
+
+__global__ void ws_history_kernel(const double* __restrict__ x, float* __restrict__ hprev, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) hprev[i] = float(x[i]);
+}
+
+inline int grid_slices(int nslices) { return (nslices + 3) / 4; }
+
+}  // namespace
+
+void launch_ws_history(const double* x, float* hprev, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(ws_history_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, x, hprev, n);
+    check_hip(hipGetLastError(), "ws_history_kernel");
+}
+
+void launch_ws_prep(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const double* x, double* partials,
+                    hipStream_t s) {
+    if (g.nslices <= 0) return;
+    hipLaunchKernelGGL(ws_prep_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, v, x, partials,
+                       x == v.x ? 1 : 0);
+    check_hip(hipGetLastError(), "ws_prep_kernel");
+}
+void launch_ws_reduce(WsInst* wi, const InstState* st, const double* partials, int ninst, int phase, hipStream_t s) {
+    if (ninst <= 0) return;
+    hipLaunchKernelGGL(ws_reduce_kernel, dim3(ninst), dim3(1024), 0, s, wi, st, partials, phase);
+    check_hip(hipGetLastError(), "ws_reduce_kernel");
+}
+void launch_ws_features(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const DevExt& e,
+                        const DevMesh& m, const float* rest_pos, float* hprev, const WsInst* wi,
+                        const InstParams* params, float dt, __bf16* F, __bf16* E, hipStream_t s) {
+    if (g.nslices <= 0) return;
+    hipLaunchKernelGGL(ws_features_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, v, e, m, rest_pos, hprev,
+                       wi, params, dt, F, E);
+    check_hip(hipGetLastError(), "ws_features_kernel");
+}
+void launch_ws_candidate(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const float* out,
+                         const WsInst* wi, double* xc, double* partials, hipStream_t s) {
+    if (g.nslices <= 0) return;
+    hipLaunchKernelGGL(ws_candidate_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, v.x, out, wi, xc);
+    check_hip(hipGetLastError(), "ws_candidate_kernel");
+    hipLaunchKernelGGL(ws_prep_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, v, xc, partials, 0);
+    check_hip(hipGetLastError(), "ws_prep_kernel(candidate)");
+}
+void launch_ws_select(const LaunchGeom& g, const WsInst* wi, const double* xc, double* x, hipStream_t s) {
+    if (g.nslices <= 0) return;
+    hipLaunchKernelGGL(ws_select_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, wi, xc, x);
+    check_hip(hipGetLastError(), "ws_select_kernel");
+}
+void launch_ws_target(const LaunchGeom& g, const double* x, const float* hprev, const WsInst* wi, float* target,
+                      hipStream_t s) {
+    if (g.nslices <= 0) return;
+    hipLaunchKernelGGL(ws_target_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, x, hprev, wi, target);
+    check_hip(hipGetLastError(), "ws_target_kernel");
+}
+
+}  // namespace chysx_amd

@@ -193,6 +193,40 @@ int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg);
  * waves, unroll} and mean ms. */
 int chysx_cloth_tune_log(const chysx_cloth* sim, int* count, int* kernel_waves_unroll, double* ms);
 int chysx_batch_set_autotune(chysx_batch* b, int on);
+/* ---- learned warm start (extension) --------------------------------------
+ * An edge message-passing network over the mesh graph (BF16 MFMA, FP32
+ * accumulate) predicts the PCG initial guess; the solver keeps the
+ * previous-step guess whenever the prediction's residual is larger.
+ * mode: 0 = previous-step dx (reference behaviour, default), 1 = learned.
+ * Parameters: chysx_ws_num_params() FP32 values.  Training: every step runs
+ * the network; the converged dx drives one Adam step (lr). */
+int chysx_ws_num_params(int* out);
+int chysx_cloth_set_warm_start_mode(chysx_cloth* sim, int mode);
+int chysx_cloth_ws_set_params(chysx_cloth* sim, const float* theta, int n);
+int chysx_cloth_ws_get_params(chysx_cloth* sim, float* theta, int n);
+int chysx_cloth_ws_init_params(chysx_cloth* sim, uint64_t seed);
+int chysx_cloth_ws_set_training(chysx_cloth* sim, int on, float lr);
+/* Last step: prediction used (1/0), relative residuals of the previous-step
+ * and predicted guesses, last training loss. */
+int chysx_cloth_ws_stats(chysx_cloth* sim, int* used, double* r0_base, double* r0_cand, double* loss);
+int chysx_batch_set_warm_start_mode(chysx_batch* b, int mode);
+int chysx_batch_ws_set_params(chysx_batch* b, const float* theta, int n);
+int chysx_batch_ws_get_params(chysx_batch* b, float* theta, int n);
+int chysx_batch_ws_init_params(chysx_batch* b, uint64_t seed);
+int chysx_batch_ws_set_training(chysx_batch* b, int on, float lr);
+int chysx_batch_ws_stats(chysx_batch* b, int index, int* used, double* r0_base, double* r0_cand, double* loss);
+/* Stand-alone network on a graph (n rows, CSR neighbours), for checking the
+ * GPU forward / backward: F and E are n x 32 FP32 inputs, out n x 32, target
+ * n x 3, loss = mean over rows of |out[:, 0:3] - target|^2, grad FP32. */
+typedef struct chysx_wsnet chysx_wsnet;
+int chysx_wsnet_create(chysx_wsnet** out);
+int chysx_wsnet_destroy(chysx_wsnet* net);
+int chysx_wsnet_set_params(chysx_wsnet* net, const float* theta, int n);
+int chysx_wsnet_forward(chysx_wsnet* net, int n, const int32_t* row_offsets, const int32_t* cols, const float* F,
+                        const float* E, float* out);
+int chysx_wsnet_loss_grad(chysx_wsnet* net, int n, const int32_t* row_offsets, const int32_t* cols, const float* F,
+                          const float* E, const float* target, double* loss, float* grad);
+
 /* Algorithmic bytes of one plain SpMV: 40 per off-diagonal block + 64 per row + 4. */
 int chysx_cloth_spmv_bytes(const chysx_cloth* sim, double* bytes);
 
