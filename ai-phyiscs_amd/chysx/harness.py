@@ -94,19 +94,27 @@ class GpuBatch:
         return self.pos[a:a + self.scenes[k].n].detach().cpu().numpy()
 
 
-def make_batch(scenes: List[Scene], tol: float = 1e-6, max_iter: int = 10000, device: str = "cuda:0") -> GpuBatch:
+def make_batch(scenes: List[Scene], tol: float = 1e-6, max_iter: int = 10000, device: str = "cuda:0",
+               force_active: bool = False) -> GpuBatch:
     import torch
     dev = torch.device(device)
     offs = np.cumsum([0] + [s.n for s in scenes]).tolist()
     pos = torch.as_tensor(np.concatenate([s.pos for s in scenes]), dtype=torch.float32).to(dev).contiguous()
     vel = torch.as_tensor(np.concatenate([s.vel for s in scenes]), dtype=torch.float32).to(dev).contiguous()
     im = torch.as_tensor(np.concatenate([s.inv_mass for s in scenes]), dtype=torch.float32).to(dev).contiguous()
+    force = None
+    if force_active:
+        force = torch.as_tensor(np.concatenate([s.force if s.force is not None else np.zeros_like(s.pos)
+                                                for s in scenes]), dtype=torch.float32).to(dev).contiguous()
     b = ClothBatch()
     for s in scenes:
         b.add_cloth(s.tris, s.n, s.spring_k if np.ndim(s.spring_k) else float(s.spring_k),
                     s.stretch_k if np.ndim(s.stretch_k) else float(s.stretch_k), s.pin_idx, s.pin_targets, s.pin_k)
-    b.set_external_buffers(pos.data_ptr(), vel.data_ptr(), int(offs[-1]), im.data_ptr(), 0)
+    b.set_external_buffers(pos.data_ptr(), vel.data_ptr(), int(offs[-1]), im.data_ptr(),
+                           force.data_ptr() if force is not None else 0)
     for k, s in enumerate(scenes):
         b.set_material(k, _material(s))
-    b.set_options(tol, max_iter, True, False)
-    return GpuBatch(b, pos, vel, im, offs, scenes)
+    b.set_options(tol, max_iter, True, force_active)
+    gb = GpuBatch(b, pos, vel, im, offs, scenes)
+    gb.force = force
+    return gb

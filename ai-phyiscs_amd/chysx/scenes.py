@@ -165,6 +165,32 @@ def config4(count: int = 512, wmin: int = 16, wmax: int = 128, steps: int = 500,
     return out
 
 
+def config5_train(count: int = 16, w: int = 256, seed: int = 5) -> List[Scene]:
+    """Warm-start training cloths: W x W (default 256), 1 m, pins ~
+    U{one corner, two top corners, top edge}, seeded wind U(0,5) N/m^2 in a
+    random horizontal direction, k ~ 10^U(2,4) (springs and stretch),
+    m ~ U(0.005, 0.02) kg, dt = 1/60."""
+    rng = np.random.default_rng(seed)
+    kinds = ["one_corner", "two_corners", "top_edge"]
+    out = []
+    pos0, tris = grid(w, 1.0)
+    areas = vertex_areas(pos0, tris)
+    for c in range(count):
+        k = float(10.0 ** rng.uniform(2.0, 4.0))
+        m = float(rng.uniform(0.005, 0.02))
+        kind = kinds[int(rng.integers(0, 3))]
+        p = float(rng.uniform(0.0, 5.0))
+        ang = float(rng.uniform(0.0, 2.0 * np.pi))
+        d = np.array([np.cos(ang), 0.0, np.sin(ang)])
+        force = (areas[:, None] * p * d[None, :]).astype(np.float32)
+        pos = pos0.copy()
+        idx, tgt = _pins(w, kind, pos)
+        out.append(Scene(f"train{c}_{w}", pos, np.zeros_like(pos), np.full(w * w, 1.0 / m, np.float32),
+                         tris, idx, tgt, 1.0e6, k, k, force, 1.0 / 60.0, 0, grid_w=w,
+                         meta={"pins": kind, "k": k, "mass": m, "wind": p}))
+    return out
+
+
 def freefall(w: int = 6) -> Scene:
     """Unpinned cloth with no elastic terms active beyond rest state: every
     particle follows x_n = x_0 + dt^2 g n(n+1)/2 exactly."""
