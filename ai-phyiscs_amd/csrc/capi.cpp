@@ -520,6 +520,15 @@ namespace {
 Problem& cloth_prob(chysx_cloth* sim) { return sim->sim.problem(); }
 Problem& batch_prob(chysx_batch* b) { return b->batch.problem(); }
 
+void fill_ws_stats(const SolveStats& st, double loss, double* out) {
+    out[0] = st.ws_proj_used;
+    out[1] = st.ws_used;
+    out[2] = st.ws_kdim;
+    out[3] = st.ws_r0_prev;
+    out[4] = st.ws_r0_proj;
+    out[5] = st.ws_r0_learned;
+    out[6] = loss;
+}
 void set_ws_params(Problem& P, const float* theta, int n) {
     need(theta, "params");
     if (n != WarmStartNet::num_params()) throw std::invalid_argument("warm start: wrong parameter count");
@@ -569,14 +578,18 @@ int chysx_cloth_ws_set_training(chysx_cloth* sim, int on, float lr) {
         cloth_prob(sim).set_ws_training(on != 0, lr);
     });
 }
-int chysx_cloth_ws_stats(chysx_cloth* sim, int* used, double* r0_base, double* r0_cand, double* loss) {
+int chysx_cloth_ws_set_history(chysx_cloth* sim, int k) {
     return guard([&] {
         need(sim, "sim");
-        const SolveStats st = sim->sim.last_stats();
-        if (used) *used = st.ws_used;
-        if (r0_base) *r0_base = st.ws_r0_base;
-        if (r0_cand) *r0_cand = st.ws_r0_cand;
-        if (loss) *loss = cloth_prob(sim).last_ws_loss();
+        cloth_prob(sim).set_ws_history(k);
+    });
+}
+int chysx_cloth_ws_stats(chysx_cloth* sim, double* out, int n) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        if (n < CHYSX_WS_STATS_LEN) throw std::invalid_argument("ws_stats: out too short");
+        fill_ws_stats(sim->sim.last_stats(), cloth_prob(sim).last_ws_loss(), out);
     });
 }
 int chysx_batch_set_warm_start_mode(chysx_batch* b, int mode) {
@@ -609,15 +622,28 @@ int chysx_batch_ws_set_training(chysx_batch* b, int on, float lr) {
         batch_prob(b).set_ws_training(on != 0, lr);
     });
 }
-int chysx_batch_ws_stats(chysx_batch* b, int index, int* used, double* r0_base, double* r0_cand, double* loss) {
+int chysx_batch_ws_set_history(chysx_batch* b, int k) {
     return guard([&] {
         need(b, "batch");
+        batch_prob(b).set_ws_history(k);
+    });
+}
+int chysx_batch_ws_stats(chysx_batch* b, int index, double* out, int n) {
+    return guard([&] {
+        need(b, "batch");
+        need(out, "out");
+        if (n < CHYSX_WS_STATS_LEN) throw std::invalid_argument("ws_stats: out too short");
         if (index < 0 || index >= b->batch.count()) throw std::out_of_range("batch: cloth index out of range");
-        const SolveStats st = b->batch.stats(index);
-        if (used) *used = st.ws_used;
-        if (r0_base) *r0_base = st.ws_r0_base;
-        if (r0_cand) *r0_cand = st.ws_r0_cand;
-        if (loss) *loss = batch_prob(b).last_ws_loss();
+        fill_ws_stats(b->batch.stats(index), batch_prob(b).last_ws_loss(), out);
+    });
+}
+int chysx_ws_small_solve(int k, const double* G, const double* g, double* c, int* kept) {
+    return guard([&] {
+        need(G, "G");
+        need(g, "g");
+        need(c, "c");
+        const int m = ws_small_solve_host(k, G, g, c);
+        if (kept) *kept = m;
     });
 }
 

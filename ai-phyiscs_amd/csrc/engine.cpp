@@ -284,18 +284,26 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     }
     partials_.resize(std::max<size_t>(size_t(4) * slices_h_.size(), 4));
     partials_.zero(s);
-    hprev_.resize(v3);
-    hprev_.zero(s);
-    xc_.resize(v3);
-    xc_.zero(s);
-    ws_target_.resize(v3);
-    ws_target_.zero(s);
+    // warm-start buffers (allocated on first use)
+    hist_.release();
+    rb_.release();
+    d_.release();
+    ad_.release();
+    xb_.release();
+    ws_part_.release();
+    ws_part2_.release();
+    {
+        int maxs = 1;
+        for (int k = 0; k < ninst_; ++k) maxs = std::max(maxs, inst_nslices_[size_t(k)]);
+        ws_chunks_ = (maxs + kWsChunkSlices - 1) / kWsChunkSlices;
+    }
     rest_pos_.resize(size_t(std::max(n_rows_, 1)) * 3);
     ws_inst_.resize(size_t(std::max(ninst_, 1)));
     ws_inst_.zero(s);
     ws_host_.assign(size_t(std::max(ninst_, 1)), WsInst{});
     rest_captured_ = false;
-    ws_age_ = 0;
+    hist_n_ = 0;
+    hist_head_ = 0;
     params_.resize(size_t(std::max(ninst_, 1)));
     if (host_states_) (void)hipHostFree(host_states_);
     host_states_ = nullptr;
@@ -494,9 +502,47 @@ bool Problem::all_done() const {
 }
 
 void Problem::set_warm_start_mode(int mode) {
-    if (mode != kWarmPrevious && mode != kWarmLearned)
-        throw std::invalid_argument("warm start mode: 0 (previous step) or 1 (learned)");
+    if (mode != kWarmPrevious && mode != kWarmLearned && mode != kWarmProjection)
+        throw std::invalid_argument("warm start mode: 0 (previous step), 1 (learned) or 2 (projection)");
     ws_mode_ = mode;
+}
+
+void Problem::set_ws_history(int k) {
+    if (k < 2 || k > kWsHistMax) throw std::invalid_argument("warm start history: 2 <= k <= 8");
+    if (k != hist_k_) {
+        hist_k_ = k;
+        hist_.release();
+        hist_n_ = 0;
+        hist_head_ = 0;
+    }
+}
+
+WsHist Problem::hist_ptrs() const {
+    WsHist h{};
+    h.k = std::min(hist_n_, hist_k_);
+    const size_t v3 = size_t(n_pad_) * 3;
+    for (int j = 0; j < h.k; ++j) {
+        const int slot = ((hist_head_ - j) % hist_k_ + hist_k_) % hist_k_;
+        h.v[j] = hist_.data() + size_t(slot) * v3;
+    }
+    return h;
+}
+
+WsReduce Problem::ws_reduce() const {
+    return WsReduce{ws_part_.data(), int(slices_h_.size()), ws_part2_.data(), ws_chunks_};
+}
+
+void Problem::push_history(hipStream_t s) {
+    const size_t v3 = size_t(n_pad_) * 3;
+    if (hist_.size() != size_t(hist_k_) * v3) {
+        hist_.resize(size_t(hist_k_) * v3);
+        hist_.zero(s);
+        hist_n_ = 0;
+        hist_head_ = 0;
+    }
+    hist_head_ = (hist_n_ == 0) ? 0 : (hist_head_ + 1) % hist_k_;
+    launch_ws_history(x_.data(), hist_.data() + size_t(hist_head_) * v3, v3, s);
+    hist_n_ = std::min(hist_n_ + 1, hist_k_);
 }
 
 WarmStartNet& Problem::net() {
@@ -509,6 +555,20 @@ WsGraph Problem::ws_graph() const {
 }
 
 void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
+    const size_t v3 = size_t(n_pad_) * 3;
+    const size_t ns = slices_h_.size();
+    if (ws_part_.size() < size_t(kWsMaxValues) * ns) ws_part_.resize(size_t(kWsMaxValues) * ns);
+    if (ws_part2_.size() < size_t(kWsMaxValues) * size_t(ninst_) * size_t(ws_chunks_))
+        ws_part2_.resize(size_t(kWsMaxValues) * size_t(ninst_) * size_t(ws_chunks_));
+    const WsHist h = hist_ptrs();
+    const WsReduce red = ws_reduce();
+    if (ws_mode_ != kWarmPrevious) {
+        launch_ws_gram(geom(), matrix(), rhs_.data(), h, red, s);
+        launch_ws_finish(red, ws_gram_values(h.k), state_.data(), ws_inst_.data(), ninst_, kWsPhaseProj, h.k, s);
+        launch_ws_proj_apply(geom(), h, ws_inst_.data(), x_.data(), s);
+        proj_ran_ = true;
+    }
+    if (ws_mode_ != kWarmLearned && !ws_train_) return;
     WarmStartNet& N = net();
     if (!N.has_params()) throw std::runtime_error("learned warm start: no network parameters set");
     N.reserve(n_pad_, ws_train_);
@@ -518,17 +578,18 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
                   "hipMemcpyAsync(rest positions)");
         rest_captured_ = true;
     }
+    for (DeviceArray<float>* a : {&rb_, &d_, &ad_})
+        if (a->size() != v3) a->resize(v3);
+    if (xb_.size() != v3) xb_.resize(v3);
     const DevVectors v = vectors();
-    launch_ws_prep(geom(), matrix(), v, v.x, partials_.data(), s);
-    launch_ws_reduce(ws_inst_.data(), state_.data(), partials_.data(), ninst_, 0, s);
-    launch_ws_features(geom(), matrix(), v, ext, mesh(), rest_pos_.data(), hprev_.data(), ws_inst_.data(),
-                       params_.data(), dt, N.features(), N.edge_features(), s);
+    launch_ws_base(geom(), matrix(), v, x_.data(), xb_.data(), rb_.data(), red, s);
+    launch_ws_finish(red, 3, state_.data(), ws_inst_.data(), ninst_, kWsPhaseBase, proj_ran_ ? 1 : 0, s);
+    launch_ws_features(geom(), matrix(), v, ext, mesh(), rest_pos_.data(), h, xb_.data(), ws_inst_.data(), dt,
+                       N.features(), N.edge_features(), s);
     N.forward(ws_graph(), s, ws_train_);
-    if (ws_mode_ == kWarmLearned) {
-        launch_ws_candidate(geom(), matrix(), v, N.out(), ws_inst_.data(), xc_.data(), partials_.data(), s);
-        launch_ws_reduce(ws_inst_.data(), state_.data(), partials_.data(), ninst_, 1, s);
-        launch_ws_select(geom(), ws_inst_.data(), xc_.data(), x_.data(), s);
-    }
+    launch_ws_direction(geom(), matrix(), N.out(), ws_inst_.data(), rb_.data(), d_.data(), ad_.data(), red, s);
+    launch_ws_finish(red, 4, state_.data(), ws_inst_.data(), ninst_, kWsPhaseLine, 0, s);
+    if (ws_mode_ == kWarmLearned) launch_ws_line_apply(geom(), ws_inst_.data(), d_.data(), x_.data(), s);
     ws_ran_ = true;
 }
 
@@ -536,12 +597,10 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
     if (!built_ || ninst_ == 0) return;
     launch_assemble(dt, ext, s);
     ws_ran_ = false;
-    // The network's inputs need two previous solutions (dx_prev, dx_prev2).
-    if (ws_mode_ == kWarmLearned || ws_train_) {
-        if (ws_age_ >= 2) warm_start_stage(dt, ext, s);
-        else launch_ws_history(x_.data(), hprev_.data(), x_.size(), s);
-    }
-    ++ws_age_;
+    proj_ran_ = false;
+    const bool ws_on = ws_mode_ != kWarmPrevious || ws_train_;
+    // The projection and the network's inputs need two previous solutions.
+    if (ws_on && hist_n_ >= 2) warm_start_stage(dt, ext, s);
     launch_pcg_start(s);
     // Launch about as many iterations as the previous step needed before the
     // first poll; chunks after convergence exit at once.
@@ -567,14 +626,16 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
     }
     if (ws_ran_ && ws_train_) {
         WarmStartNet& N = net();
-        launch_ws_target(geom(), x_.data(), hprev_.data(), ws_inst_.data(), ws_target_.data(), s);
-        N.loss(ws_graph(), ws_target_.data(), s);
+        const WsReduce red = ws_reduce();
+        launch_ws_energy(geom(), x_.data(), xb_.data(), rb_.data(), red, s);
+        launch_ws_finish(red, 1, state_.data(), ws_inst_.data(), ninst_, kWsPhaseEnergy, 0, s);
+        launch_ws_dout(geom(), ws_inst_.data(), rb_.data(), ad_.data(), N.dout(), s);
         N.backward(ws_graph(), s);
         N.adam_step(ws_lr_, s);
-        ws_loss_ = N.read_loss(s);
     }
+    if (ws_on) push_history(s);
     launch_finalize(dt, ext, s);
-    if (ws_ran_) {
+    if (ws_ran_ || proj_ran_) {
         check_hip(hipMemcpyAsync(ws_host_.data(), ws_inst_.data(), sizeof(WsInst) * ninst_, hipMemcpyDeviceToHost, s),
                   "hipMemcpyAsync(warm start stats)");
         check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(warm start stats)");
@@ -585,10 +646,19 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
         q.restarts = host_states_[k].restarts;
         q.status = host_states_[k].status;
         q.relres = host_states_[k].relres;
-        const double bb = host_states_[k].bb > 0 ? host_states_[k].bb : 1.0;
-        q.ws_used = ws_ran_ ? ws_host_[size_t(k)].use_cand : 0;
-        q.ws_r0_base = ws_ran_ ? std::sqrt(ws_host_[size_t(k)].rr_base / bb) : 0.0;
-        q.ws_r0_cand = (ws_ran_ && ws_mode_ == kWarmLearned) ? std::sqrt(ws_host_[size_t(k)].rr_cand / bb) : 0.0;
+        const WsInst& W = ws_host_[size_t(k)];
+        const double wb = W.bb > 0 ? W.bb : 1.0;
+        q.ws_proj_used = proj_ran_ ? W.use_proj : 0;
+        q.ws_kdim = proj_ran_ ? W.kdim : 0;
+        q.ws_used = (ws_ran_ && ws_mode_ == kWarmLearned) ? W.use_cand : 0;
+        q.ws_r0_prev = (ws_ran_ || proj_ran_) ? std::sqrt(W.rr_prev / wb) : 0.0;
+        q.ws_r0_proj = proj_ran_ ? std::sqrt(W.rr_proj / wb) : 0.0;
+        q.ws_r0_learned = ws_ran_ ? std::sqrt(W.rr_cand / wb) : 0.0;
+    }
+    if (ws_ran_ && ws_train_) {
+        double l = 0.0;
+        for (int k = 0; k < ninst_; ++k) l += ws_host_[size_t(k)].loss;
+        ws_loss_ = l / double(ninst_);
     }
     last_iters_ = max_it;
 }
@@ -648,10 +718,11 @@ void Problem::set_dx64(int k, const double* in, hipStream_t s) {
                              hipMemcpyHostToDevice, s),
               "hipMemcpyAsync(dx)");
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(dx)");
+    reset_history();  // the caller's guess is used as given
 }
 void Problem::zero_dx(hipStream_t s) {
     x_.zero(s);
-    ws_age_ = 0;
+    reset_history();
 }
 
 // ---------------------------------------------------------------------------

@@ -101,16 +101,20 @@ struct SolveStats {
     int restarts = 0;
     int status = 0;
     double relres = 0.0;
-    // warm start of this solve
-    int ws_used = 0;         // 1: the network's guess replaced the previous-step guess
-    double ws_r0_base = 0;   // |b - A dx_prev| / |b| (only when the network ran)
-    double ws_r0_cand = 0;   // |b - A dx_net| / |b|
+    // warm start of this solve (zeros when no warm-start stage ran)
+    int ws_proj_used = 0;     // 1: the history projection replaced dx_prev
+    int ws_used = 0;          // 1: the network's direction was applied
+    int ws_kdim = 0;          // projection directions kept
+    double ws_r0_prev = 0;    // |b - A dx_prev| / |b|
+    double ws_r0_proj = 0;    // |b - A x_proj| / |b| (predicted)
+    double ws_r0_learned = 0; // |b - A (x_b + alpha d)| / |b| (predicted)
 };
 
 // PCG initial guess.
 enum WarmStartMode : int {
-    kWarmPrevious = 0,  // the previous step's dx (reference behaviour)
-    kWarmLearned = 1,   // the network's prediction, unless its residual is larger
+    kWarmPrevious = 0,    // the previous step's dx (reference behaviour)
+    kWarmLearned = 1,     // projection, then the network's line-searched direction
+    kWarmProjection = 2,  // Galerkin projection onto the last K solutions
 };
 
 class Problem {
@@ -173,12 +177,15 @@ public:
     void read_states(hipStream_t s);        // blocking
     bool all_done() const;
 
-    // ---- learned warm start ------------------------------------------------
+    // ---- warm start ----------------------------------------------------------
     void set_warm_start_mode(int mode);
     int warm_start_mode() const { return ws_mode_; }
-    // Online training: every step runs the network, the solver uses the
-    // previous-step guess (plain solver) unless the mode is learned, and the
-    // converged dx gives one Adam step.
+    // Number of previous solutions the projection uses (2..8, default 6).
+    void set_ws_history(int k);
+    int ws_history() const { return hist_k_; }
+    // Online training: every step runs the network; its direction is applied
+    // only in the learned mode; after the solve the line-search loss against
+    // the converged dx gives one Adam step.
     void set_ws_training(bool on, float lr) { ws_train_ = on; ws_lr_ = lr; }
     WarmStartNet& net();
     double last_ws_loss() const { return ws_loss_; }
@@ -200,7 +207,11 @@ private:
     void capture_chunk_graph();
     void destroy_graph();
     void warm_start_stage(float dt, const DevExt& ext, hipStream_t s);
+    void push_history(hipStream_t s);
+    void reset_history() { hist_n_ = 0; }
     WsGraph ws_graph() const;
+    WsHist hist_ptrs() const;
+    WsReduce ws_reduce() const;
 
     bool built_ = false;
     int ninst_ = 0;
@@ -237,17 +248,21 @@ private:
 
     LaunchConfig cfg_;
     std::vector<TuneSample> tune_log_;
-    // learned warm start
+    // warm start
     int ws_mode_ = kWarmPrevious;
     bool ws_train_ = false;
     float ws_lr_ = 1e-3f;
     double ws_loss_ = 0.0;
-    bool ws_ran_ = false;
-    int ws_age_ = 0;  // steps since the iterate history was reset
+    bool ws_ran_ = false;    // network stage ran this step
+    bool proj_ran_ = false;  // projection stage ran this step
+    int hist_k_ = 6;         // history length of the projection
+    int hist_n_ = 0;         // solutions held (<= hist_k_)
+    int hist_head_ = 0;      // slot of the newest
+    int ws_chunks_ = 1;
     bool rest_captured_ = false;
     std::unique_ptr<WarmStartNet> net_;
-    DeviceArray<float> hprev_, rest_pos_, ws_target_;
-    DeviceArray<double> xc_;
+    DeviceArray<float> hist_, rest_pos_, rb_, d_, ad_;
+    DeviceArray<double> xb_, ws_part_, ws_part2_;
     DeviceArray<WsInst> ws_inst_;
     std::vector<WsInst> ws_host_;
     hipStream_t cap_stream_ = nullptr;

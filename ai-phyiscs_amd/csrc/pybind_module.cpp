@@ -98,7 +98,20 @@ py::dict stats_dict(int it, int rs, int st, double rr) {
 int ws_mode(const std::string& m) {
     if (m == "previous") return 0;
     if (m == "learned") return 1;
-    throw py::value_error("warm start mode must be 'previous' or 'learned'");
+    if (m == "projection") return 2;
+    throw py::value_error("warm start mode must be 'previous', 'projection' or 'learned'");
+}
+
+py::dict ws_stats_dict(const double* o) {
+    py::dict d;
+    d["projection_used"] = int(o[0]);
+    d["used"] = int(o[1]);
+    d["projection_dims"] = int(o[2]);
+    d["r0_previous"] = o[3];
+    d["r0_projection"] = o[4];
+    d["r0_learned"] = o[5];
+    d["loss"] = o[6];
+    return d;
 }
 
 py::array_t<float> params_out(int n) { return py::array_t<float>({n}); }
@@ -350,7 +363,12 @@ PYBIND11_MODULE(_chysx_native, m) {
             "set_warm_start_mode", [](Sim& s, const std::string& mode) {
                 check(chysx_cloth_set_warm_start_mode(s.p(), ws_mode(mode)));
             },
-            py::arg("mode"), "'previous' (reference: previous step's dx) or 'learned'.")
+            py::arg("mode"),
+            "'previous' (reference: previous step's dx), 'projection' (Galerkin projection onto the last K "
+            "solutions) or 'learned' (projection plus the network's line-searched direction).")
+        .def(
+            "ws_set_history", [](Sim& s, int k) { check(chysx_cloth_ws_set_history(s.p(), k)); }, py::arg("k"),
+            "Number of previous solutions the projection uses (2..8).")
         .def(
             "ws_set_params",
             [](Sim& s, FltArr theta) { check(chysx_cloth_ws_set_params(s.p(), theta.data(), int(theta.size()))); },
@@ -372,15 +390,9 @@ PYBIND11_MODULE(_chysx_native, m) {
             py::arg("on"), py::arg("lr") = 1e-3f)
         .def("ws_stats",
              [](Sim& s) {
-                 int used = 0;
-                 double rb = 0, rc = 0, loss = 0;
-                 check(chysx_cloth_ws_stats(s.p(), &used, &rb, &rc, &loss));
-                 py::dict d;
-                 d["used"] = used;
-                 d["r0_previous"] = rb;
-                 d["r0_learned"] = rc;
-                 d["loss"] = loss;
-                 return d;
+                 double o[CHYSX_WS_STATS_LEN];
+                 check(chysx_cloth_ws_stats(s.p(), o, CHYSX_WS_STATS_LEN));
+                 return ws_stats_dict(o);
              })
         .def("spmv_bytes", [](const Sim& s) {
             double b = 0.0;
@@ -448,6 +460,8 @@ PYBIND11_MODULE(_chysx_native, m) {
             },
             py::arg("mode"))
         .def(
+            "ws_set_history", [](Batch& b, int k) { check(chysx_batch_ws_set_history(b.h, k)); }, py::arg("k"))
+        .def(
             "ws_set_params",
             [](Batch& b, FltArr theta) { check(chysx_batch_ws_set_params(b.h, theta.data(), int(theta.size()))); },
             py::arg("theta"))
@@ -469,15 +483,9 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def(
             "ws_stats",
             [](Batch& b, int k) {
-                int used = 0;
-                double rb = 0, rc = 0, loss = 0;
-                check(chysx_batch_ws_stats(b.h, k, &used, &rb, &rc, &loss));
-                py::dict d;
-                d["used"] = used;
-                d["r0_previous"] = rb;
-                d["r0_learned"] = rc;
-                d["loss"] = loss;
-                return d;
+                double o[CHYSX_WS_STATS_LEN];
+                check(chysx_batch_ws_stats(b.h, k, o, CHYSX_WS_STATS_LEN));
+                return ws_stats_dict(o);
             },
             py::arg("index"))
         .def(
@@ -518,6 +526,17 @@ PYBIND11_MODULE(_chysx_native, m) {
         },
         py::arg("n"), py::arg("triangles"), py::arg("include_springs") = false,
         py::arg("pins") = IntArr(0), "Host-only pattern and slot maps of a mesh (no GPU needed).");
+    m.def(
+        "ws_small_solve",
+        [](DblArr G, DblArr g) {
+            const int k = int(g.size());
+            if (G.ndim() != 2 || G.shape(0) != k || G.shape(1) != k) throw py::value_error("G must be (k, k)");
+            py::array_t<double> c({k});
+            int kept = 0;
+            check(chysx_ws_small_solve(k, G.data(), g.data(), c.mutable_data(), &kept));
+            return py::make_tuple(c, kept);
+        },
+        py::arg("G"), py::arg("g"));
     m.def("ws_num_params", []() {
         int n = 0;
         check(chysx_ws_num_params(&n));

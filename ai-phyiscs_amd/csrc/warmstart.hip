@@ -696,6 +696,73 @@ void WarmStartNet::adam_step(float lr, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Simulator-side kernels
 // ---------------------------------------------------------------------------
+
+// Cyclic Jacobi eigen-decomposition of the symmetric k x k matrix G (k <= 8)
+// and the eigen-truncated solve c = sum_{lam > 1e-12 lam_max} (u^T g / lam) u.
+// The directions dropped are those where the history vectors are numerically
+// collinear.  Returns the number of directions kept.
+__host__ __device__ inline int ws_small_solve(int k, const double* G, const double* g, double* c) {
+    double a[kWsHistMax][kWsHistMax], v[kWsHistMax][kWsHistMax];
+    for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) {
+            a[i][j] = 0.5 * (G[i * k + j] + G[j * k + i]);
+            v[i][j] = (i == j) ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0.0, dia = 0.0;
+        for (int p = 0; p < k; ++p) {
+            dia += a[p][p] * a[p][p];
+            for (int q = p + 1; q < k; ++q) off += a[p][q] * a[p][q];
+        }
+        if (!(off > 1e-30 * dia)) break;
+        for (int p = 0; p < k; ++p)
+            for (int q = p + 1; q < k; ++q) {
+                if (a[p][q] == 0.0) continue;
+                const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                const double t = fabs(theta) > 1e150 ? 0.5 / theta
+                                                      : (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+                for (int r = 0; r < k; ++r) {
+                    const double arp = a[r][p], arq = a[r][q];
+                    a[r][p] = cs * arp - sn * arq;
+                    a[r][q] = sn * arp + cs * arq;
+                }
+                for (int r = 0; r < k; ++r) {
+                    const double apr = a[p][r], aqr = a[q][r];
+                    a[p][r] = cs * apr - sn * aqr;
+                    a[q][r] = sn * apr + cs * aqr;
+                }
+                for (int r = 0; r < k; ++r) {
+                    const double vrp = v[r][p], vrq = v[r][q];
+                    v[r][p] = cs * vrp - sn * vrq;
+                    v[r][q] = sn * vrp + cs * vrq;
+                }
+            }
+    }
+    double lmax = 0.0;
+    for (int i = 0; i < k; ++i) {
+        c[i] = 0.0;
+        lmax = fmax(lmax, a[i][i]);
+    }
+    int kept = 0;
+    if (!(lmax > 0.0)) return 0;
+    for (int m = 0; m < k; ++m) {
+        const double lam = a[m][m];
+        if (!(lam > 1e-12 * lmax)) continue;
+        double pr = 0.0;
+        for (int i = 0; i < k; ++i) pr += v[i][m] * g[i];
+        const double cf = pr / lam;
+        for (int i = 0; i < k; ++i) c[i] += cf * v[i][m];
+        ++kept;
+    }
+    return kept;
+}
+
+int ws_small_solve_host(int k, const double* G, const double* g, double* c) {
+    if (k < 1 || k > kWsHistMax) throw std::invalid_argument("ws_small_solve: 1 <= k <= 8");
+    return ws_small_solve(k, G, g, c);
+}
+
 namespace {
 
 __device__ __forceinline__ void ld3f(const float* p, int i, float& a, float& b, float& c) {
@@ -704,18 +771,223 @@ __device__ __forceinline__ void ld3f(const float* p, int i, float& a, float& b, 
     c = p[3 * i + 2];
 }
 
-// b - A x with FP64 products and sums; optionally r, z = M^-1 r.
-__global__ __launch_bounds__(256) void ws_prep_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                      const double* __restrict__ x, double* __restrict__ partials,
-                                                      int write_rz) {
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Wave-sum `val` and store it as value `o` of slice `s`.
+__device__ __forceinline__ void put_partial(const WsReduce& red, int o, int s, int lane, double val) {
+    const double t = wave_sum(val);
+    if (lane == 0) red.part[(size_t)o * red.ldp + s] = t;
+}
+
+// Gram sums of K history vectors: per row y_j = (A v_j)_row in FP64, then
+// [V^T A V (packed i <= j), (AV)^T (AV) (packed), V^T b, (AV)^T b, b^T b].
+template <int K>
+__global__ __launch_bounds__(256) void ws_gram_kernel(LaunchGeom g, DevMatrix A, const float* __restrict__ rhs,
+                                                      WsHist h, WsReduce red) {
     const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
-    double rr = 0.0, xx = 0.0;
+    double y[K][3];
+    float vr[K][3];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            y[j][c] = 0.0;
+            vr[j][c] = 0.0f;
+        }
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
     if (lane < sd.nvalid) {
         const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
-        const double x0 = x[3 * row], x1 = x[3 * row + 1], x2 = x[3 * row + 2];
+        double m[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m[q] = double(dg[64 * q]);
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            ld3f(h.v[j], row, vr[j][0], vr[j][1], vr[j][2]);
+            const double x0 = vr[j][0], x1 = vr[j][1], x2 = vr[j][2];
+            y[j][0] = m[0] * x0 + m[1] * x1 + m[2] * x2;
+            y[j][1] = m[3] * x0 + m[4] * x1 + m[5] * x2;
+            y[j][2] = m[6] * x0 + m[7] * x1 + m[8] * x2;
+        }
+        const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
+        const int* colp = A.col + (size_t)sb * 64 + lane;
+        const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+        for (int k = 0; k < w; ++k) {
+            const int c = colp[k * 64];
+            if (c < 0) continue;
+            const float* bk = vp + k * 9 * 64;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) m[q] = double(bk[64 * q]);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                float u0, u1, u2;
+                ld3f(h.v[j], c, u0, u1, u2);
+                y[j][0] += m[0] * double(u0) + m[1] * double(u1) + m[2] * double(u2);
+                y[j][1] += m[3] * double(u0) + m[4] * double(u1) + m[5] * double(u2);
+                y[j][2] += m[6] * double(u0) + m[7] * double(u1) + m[8] * double(u2);
+            }
+        }
+        b0 = rhs[3 * row];
+        b1 = rhs[3 * row + 1];
+        b2 = rhs[3 * row + 2];
+    }
+    int o = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+#pragma unroll
+        for (int j = i; j < K; ++j)
+            put_partial(red, o++, s, lane,
+                        double(vr[i][0]) * y[j][0] + double(vr[i][1]) * y[j][1] + double(vr[i][2]) * y[j][2]);
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+#pragma unroll
+        for (int j = i; j < K; ++j)
+            put_partial(red, o++, s, lane, y[i][0] * y[j][0] + y[i][1] * y[j][1] + y[i][2] * y[j][2]);
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+        put_partial(red, o++, s, lane, double(vr[i][0]) * b0 + double(vr[i][1]) * b1 + double(vr[i][2]) * b2);
+#pragma unroll
+    for (int i = 0; i < K; ++i) put_partial(red, o++, s, lane, y[i][0] * b0 + y[i][1] * b1 + y[i][2] * b2);
+    put_partial(red, o++, s, lane, b0 * b0 + b1 * b1 + b2 * b2);
+}
+
+// Sums of `nv` per-slice partials over chunk blockIdx.y of instance
+// blockIdx.x; fixed order, so the result does not depend on timing.
+__global__ __launch_bounds__(256) void ws_chunk_kernel(WsReduce red, int nv, const InstState* __restrict__ st) {
+    const int inst = blockIdx.x, c = blockIdx.y;
+    const int s0 = st[inst].slice0, ns = st[inst].nslices;
+    const int a = c * kWsChunkSlices, e = min(ns, a + kWsChunkSlices);
+    const int wv = int(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    __shared__ double sh[4][kWsMaxValues];
+    for (int v = 0; v < nv; ++v) {
+        double acc = 0.0;
+        for (int q = a + int(threadIdx.x); q < e; q += 256) acc += red.part[(size_t)v * red.ldp + s0 + q];
+        acc = wave_sum(acc);
+        if (lane == 0) sh[wv][v] = acc;
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < nv) red.part2[((size_t)inst * red.chunks + c) * nv + t] = ((sh[0][t] + sh[1][t]) + sh[2][t]) + sh[3][t];
+}
+
+__global__ __launch_bounds__(128) void ws_finish_kernel(WsReduce red, int nv, const InstState* __restrict__ st,
+                                                        WsInst* __restrict__ wi, int phase, int hk, double inv_ninst) {
+    const int inst = blockIdx.x;
+    __shared__ double tot[kWsMaxValues];
+    const int t = threadIdx.x;
+    if (t < nv) {
+        double a = 0.0;
+        for (int c = 0; c < red.chunks; ++c) a += red.part2[((size_t)inst * red.chunks + c) * nv + t];
+        tot[t] = a;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    WsInst& W = wi[inst];
+    if (phase == kWsPhaseProj) {
+        const int K = hk;
+        double G[kWsHistMax * kWsHistMax], Q[kWsHistMax * kWsHistMax], gv[kWsHistMax], bav[kWsHistMax],
+            c[kWsHistMax];
+        int o = 0;
+        for (int i = 0; i < K; ++i)
+            for (int j = i; j < K; ++j) G[i * K + j] = G[j * K + i] = tot[o++];
+        for (int i = 0; i < K; ++i)
+            for (int j = i; j < K; ++j) Q[i * K + j] = Q[j * K + i] = tot[o++];
+        for (int i = 0; i < K; ++i) gv[i] = tot[o++];
+        for (int i = 0; i < K; ++i) bav[i] = tot[o++];
+        const double bb = tot[o++];
+        const int kept = ws_small_solve(K, G, gv, c);
+        double phi = 0.0, rr = bb;
+        for (int i = 0; i < K; ++i) {
+            double gc = 0.0, qc = 0.0;
+            for (int j = 0; j < K; ++j) {
+                gc += G[i * K + j] * c[j];
+                qc += Q[i * K + j] * c[j];
+            }
+            phi += c[i] * (0.5 * gc - gv[i]);
+            rr += c[i] * (qc - 2.0 * bav[i]);
+        }
+        const double phi_prev = 0.5 * G[0] - gv[0];
+        W.bb = bb;
+        W.rr_prev = fmax(bb - 2.0 * bav[0] + Q[0], 0.0);
+        W.rr_proj = fmax(rr, 0.0);
+        W.kdim = kept;
+        W.use_proj = (kept > 0 && phi < phi_prev) ? 1 : 0;
+        for (int i = 0; i < kWsHistMax; ++i) W.coef[i] = (i < K) ? c[i] : 0.0;
+    } else if (phase == kWsPhaseBase) {
+        W.rr_base = tot[0];
+        W.scale = fmax(sqrt(tot[1] / (3.0 * 64.0 * double(max(1, st[inst].nslices)))), 1e-12);
+        W.bb = tot[2];
+        if (!hk) {  // no projection this step: the base guess is dx_prev
+            W.rr_prev = tot[0];
+            W.use_proj = 0;
+        }
+        W.use_cand = 0;
+        W.alpha = 0.0;
+    } else if (phase == kWsPhaseLine) {
+        const double u = tot[0], w = tot[1], q = tot[2], a2 = tot[3];
+        W.u = u;
+        W.w = w;
+        const double al = (w > 0.0 && isfinite(u / w)) ? u / w : 0.0;
+        const double rr = W.rr_base - 2.0 * al * q + al * al * a2;
+        W.rr_cand = fmax(rr, 0.0);
+        W.use_cand = (al != 0.0 && rr < W.rr_base) ? 1 : 0;
+        W.alpha = W.use_cand ? al : 0.0;
+    } else {  // kWsPhaseEnergy
+        const double E = tot[0], u = W.u, w = W.w;
+        W.energy = E;
+        if (w > 0.0 && E > 0.0) {
+            W.loss = 1.0 - u * u / (w * E);
+            W.gcoef = -2.0 * u / (w * E) * W.scale * inv_ninst;
+        } else {
+            W.loss = 0.0;
+            W.gcoef = 0.0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void ws_proj_apply_kernel(LaunchGeom g, WsHist h, const WsInst* __restrict__ wi,
+                                                            double* __restrict__ x) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    const WsInst& W = wi[sd.inst];
+    if (lane >= sd.nvalid || !W.use_proj) return;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int j = 0; j < h.k; ++j) {
+        const double cj = W.coef[j];
+        a0 += cj * double(h.v[j][3 * row]);
+        a1 += cj * double(h.v[j][3 * row + 1]);
+        a2 += cj * double(h.v[j][3 * row + 2]);
+    }
+    x[3 * row] = a0;
+    x[3 * row + 1] = a1;
+    x[3 * row + 2] = a2;
+}
+
+// r_b = b - A x with FP64 products and sums (stored FP32), z = M^-1 r_b,
+// xb <- x; partials [|r_b|^2, |x|^2, |b|^2].
+__global__ __launch_bounds__(256) void ws_base_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+                                                      const double* __restrict__ x, double* __restrict__ xb,
+                                                      float* __restrict__ rb, WsReduce red) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    double rr = 0.0, xx = 0.0, bb = 0.0;
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    if (lane < sd.nvalid) {
+        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+        x0 = x[3 * row];
+        x1 = x[3 * row + 1];
+        x2 = x[3 * row + 2];
         double a0 = double(dg[0]) * x0 + double(dg[64]) * x1 + double(dg[128]) * x2;
         double a1 = double(dg[192]) * x0 + double(dg[256]) * x1 + double(dg[320]) * x2;
         double a2 = double(dg[384]) * x0 + double(dg[448]) * x1 + double(dg[512]) * x2;
@@ -732,74 +1004,40 @@ __global__ __launch_bounds__(256) void ws_prep_kernel(LaunchGeom g, DevMatrix A,
                 a2 += double(bk[384]) * y0 + double(bk[448]) * y1 + double(bk[512]) * y2;
             }
         }
-        const double r0 = double(v.rhs[3 * row]) - a0, r1 = double(v.rhs[3 * row + 1]) - a1,
-                     r2 = double(v.rhs[3 * row + 2]) - a2;
+        const double q0 = double(v.rhs[3 * row]), q1 = double(v.rhs[3 * row + 1]), q2 = double(v.rhs[3 * row + 2]);
+        const double r0 = q0 - a0, r1 = q1 - a1, r2 = q2 - a2;
         rr = r0 * r0 + r1 * r1 + r2 * r2;
         xx = x0 * x0 + x1 * x1 + x2 * x2;
-        if (write_rz) {
-            const float f0 = float(r0), f1 = float(r1), f2 = float(r2);
-            const float* mv = A.minv + (size_t)s * 6 * 64 + lane;
-            const float m0 = mv[0], m1 = mv[64], m2 = mv[128], m3 = mv[192], m4 = mv[256], m5 = mv[320];
-            v.r[3 * row] = f0;
-            v.r[3 * row + 1] = f1;
-            v.r[3 * row + 2] = f2;
-            v.z[3 * row] = m0 * f0 + m1 * f1 + m2 * f2;
-            v.z[3 * row + 1] = m1 * f0 + m3 * f1 + m4 * f2;
-            v.z[3 * row + 2] = m2 * f0 + m4 * f1 + m5 * f2;
-        }
+        bb = q0 * q0 + q1 * q1 + q2 * q2;
+        f0 = float(r0);
+        f1 = float(r1);
+        f2 = float(r2);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        rr += __shfl_xor(rr, off, 64);
-        xx += __shfl_xor(xx, off, 64);
-    }
-    if (lane == 0) {
-        partials[4 * s] = rr;
-        partials[4 * s + 1] = xx;
-    }
+    const float* mv = A.minv + (size_t)s * 6 * 64 + lane;
+    const float m0 = mv[0], m1 = mv[64], m2 = mv[128], m3 = mv[192], m4 = mv[256], m5 = mv[320];
+    const bool live = lane < sd.nvalid;
+    rb[3 * row] = f0;
+    rb[3 * row + 1] = f1;
+    rb[3 * row + 2] = f2;
+    v.z[3 * row] = live ? m0 * f0 + m1 * f1 + m2 * f2 : 0.0f;
+    v.z[3 * row + 1] = live ? m1 * f0 + m3 * f1 + m4 * f2 : 0.0f;
+    v.z[3 * row + 2] = live ? m2 * f0 + m4 * f1 + m5 * f2 : 0.0f;
+    xb[3 * row] = x0;
+    xb[3 * row + 1] = x1;
+    xb[3 * row + 2] = x2;
+    put_partial(red, 0, s, lane, rr);
+    put_partial(red, 1, s, lane, xx);
+    put_partial(red, 2, s, lane, bb);
 }
 
-__global__ __launch_bounds__(1024) void ws_reduce_kernel(WsInst* __restrict__ wi, const InstState* __restrict__ st,
-                                                         const double* __restrict__ partials, int phase) {
-    const InstState& S = st[blockIdx.x];
-    const int s0 = S.slice0, ns = S.nslices;
-    double a = 0.0, b = 0.0;
-    for (int q = threadIdx.x; q < ns; q += 1024) {
-        a += partials[4 * (s0 + q)];
-        b += partials[4 * (s0 + q) + 1];
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_xor(a, off, 64);
-        b += __shfl_xor(b, off, 64);
-    }
-    __shared__ double sh[16][2];
-    if ((threadIdx.x & 63) == 0) {
-        sh[threadIdx.x >> 6][0] = a;
-        sh[threadIdx.x >> 6][1] = b;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    double ta = 0.0, tb = 0.0;
-    for (int w = 0; w < 16; ++w) {
-        ta += sh[w][0];
-        tb += sh[w][1];
-    }
-    WsInst& W = wi[blockIdx.x];
-    if (phase == 0) {
-        W.rr_base = ta;
-        W.scale = fmax(sqrt(tb / (3.0 * 64.0 * double(ns))), 1e-12);
-        W.use_cand = 0;
-    } else {
-        W.rr_cand = ta;
-        W.use_cand = (ta < W.rr_base) ? 1 : 0;
-    }
-}
-
+// Features (scaled by 1 / RMS(x_b)):
+//   0-2 dx_prev, 3-5 dx_prev - dx_prev2, 6-8 z = M^-1 r_b, 9-11 x_b - dx_prev,
+//   12 pinned, 13 inertia share of the diagonal, 14 constant 1.
+// Edge features (mean over mesh neighbours): (x_j - x_i)/L0 (3), |x_j - x_i|/L0 - 1.
 __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatrix A, DevVectors v, DevExt e,
-                                                          DevMesh m, const float* __restrict__ rest,
-                                                          float* __restrict__ hprev, const WsInst* __restrict__ wi,
-                                                          const InstParams* __restrict__ params, float dt,
+                                                          DevMesh m, const float* __restrict__ rest, WsHist h,
+                                                          const double* __restrict__ xb,
+                                                          const WsInst* __restrict__ wi, float dt,
                                                           __bf16* __restrict__ F, __bf16* __restrict__ E) {
     const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
     if (s >= g.nslices) return;
@@ -810,19 +1048,22 @@ __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatri
     for (int q = 0; q < FIN; ++q) f[q] = 0.0f;
     float ef[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (lane < sd.nvalid) {
-        const float is = float(1.0 / wi[sd.inst].scale);
-        const double x0 = v.x[3 * row], x1 = v.x[3 * row + 1], x2 = v.x[3 * row + 2];
-        float h0, h1, h2;
-        ld3f(hprev, row, h0, h1, h2);
-        f[0] = float(x0) * is;
-        f[1] = float(x1) * is;
-        f[2] = float(x2) * is;
-        f[3] = (float(x0) - h0) * is;
-        f[4] = (float(x1) - h1) * is;
-        f[5] = (float(x2) - h2) * is;
+        const float is = float(1.0 / fmax(wi[sd.inst].scale, 1e-30));
+        float h0, h1, h2, p0, p1, p2;
+        ld3f(h.v[0], row, h0, h1, h2);
+        ld3f(h.v[1], row, p0, p1, p2);
+        f[0] = h0 * is;
+        f[1] = h1 * is;
+        f[2] = h2 * is;
+        f[3] = (h0 - p0) * is;
+        f[4] = (h1 - p1) * is;
+        f[5] = (h2 - p2) * is;
         f[6] = v.z[3 * row] * is;
         f[7] = v.z[3 * row + 1] * is;
         f[8] = v.z[3 * row + 2] * is;
+        f[9] = float(xb[3 * row] - double(h0)) * is;
+        f[10] = float(xb[3 * row + 1] - double(h1)) * is;
+        f[11] = float(xb[3 * row + 2] - double(h2)) * is;
         f[12] = (m.pin_inc_off[row + 1] > m.pin_inc_off[row]) ? 1.0f : 0.0f;
         const int p = sd.part0 + lane;
         float mass = 1.0f;
@@ -834,10 +1075,6 @@ __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatri
         const float tr = (dg[0] + dg[256] + dg[512]) * (1.0f / 3.0f);
         f[13] = (tr > 0.0f) ? fminf(mass / (dt * dt) / tr, 1.0f) : 0.0f;
         f[14] = 1.0f;
-        hprev[3 * row] = float(x0);
-        hprev[3 * row + 1] = float(x1);
-        hprev[3 * row + 2] = float(x2);
-        // mean edge features over the mesh neighbours
         float px, py, pz, rx, ry, rz;
         ld3f(e.pos, p, px, py, pz);
         ld3f(rest, p, rx, ry, rz);
@@ -885,96 +1122,209 @@ __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatri
     }
 }
 
-__global__ __launch_bounds__(256) void ws_candidate_kernel(LaunchGeom g, const double* __restrict__ x,
-                                                           const float* __restrict__ out,
-                                                           const WsInst* __restrict__ wi, double* __restrict__ xc) {
+// d = scale * out[:, 0:3] (FP32; padding rows 0).
+__global__ __launch_bounds__(256) void ws_dir_write_kernel(LaunchGeom g, const float* __restrict__ out,
+                                                           const WsInst* __restrict__ wi, float* __restrict__ d) {
     const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
-    if (lane >= sd.nvalid) {
-        xc[3 * row] = xc[3 * row + 1] = xc[3 * row + 2] = 0.0;
-        return;
+    const bool live = lane < sd.nvalid;
+    const float sc = float(wi[sd.inst].scale);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) d[3 * row + c] = live ? sc * out[(size_t)row * NOUTD + c] : 0.0f;
+}
+
+// A d (FP64 sums, stored FP32) and partials [d.r_b, d.Ad, Ad.r_b, Ad.Ad].
+__global__ __launch_bounds__(256) void ws_dir_spmv_kernel(LaunchGeom g, DevMatrix A, const float* __restrict__ d,
+                                                          const float* __restrict__ rb, float* __restrict__ ad,
+                                                          WsReduce red) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    if (lane < sd.nvalid) {
+        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+        d0 = d[3 * row];
+        d1 = d[3 * row + 1];
+        d2 = d[3 * row + 2];
+        a0 = double(dg[0]) * d0 + double(dg[64]) * d1 + double(dg[128]) * d2;
+        a1 = double(dg[192]) * d0 + double(dg[256]) * d1 + double(dg[320]) * d2;
+        a2 = double(dg[384]) * d0 + double(dg[448]) * d1 + double(dg[512]) * d2;
+        const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
+        const int* colp = A.col + (size_t)sb * 64 + lane;
+        const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+        for (int k = 0; k < w; ++k) {
+            const int c = colp[k * 64];
+            if (c >= 0) {
+                const float* bk = vp + k * 9 * 64;
+                const double y0 = d[3 * c], y1 = d[3 * c + 1], y2 = d[3 * c + 2];
+                a0 += double(bk[0]) * y0 + double(bk[64]) * y1 + double(bk[128]) * y2;
+                a1 += double(bk[192]) * y0 + double(bk[256]) * y1 + double(bk[320]) * y2;
+                a2 += double(bk[384]) * y0 + double(bk[448]) * y1 + double(bk[512]) * y2;
+            }
+        }
+        r0 = rb[3 * row];
+        r1 = rb[3 * row + 1];
+        r2 = rb[3 * row + 2];
     }
-    const double sc = wi[sd.inst].scale;
-    for (int c = 0; c < 3; ++c) xc[3 * row + c] = x[3 * row + c] + sc * double(out[(size_t)row * NOUTD + c]);
+    ad[3 * row] = float(a0);
+    ad[3 * row + 1] = float(a1);
+    ad[3 * row + 2] = float(a2);
+    const double f0 = float(a0), f1 = float(a1), f2 = float(a2);  // the stored A d
+    put_partial(red, 0, s, lane, d0 * r0 + d1 * r1 + d2 * r2);
+    put_partial(red, 1, s, lane, d0 * a0 + d1 * a1 + d2 * a2);
+    put_partial(red, 2, s, lane, f0 * r0 + f1 * r1 + f2 * r2);
+    put_partial(red, 3, s, lane, f0 * f0 + f1 * f1 + f2 * f2);
 }
 
-__global__ __launch_bounds__(256) void ws_select_kernel(LaunchGeom g, const WsInst* __restrict__ wi,
-                                                        const double* __restrict__ xc, double* __restrict__ x) {
+__global__ __launch_bounds__(256) void ws_line_apply_kernel(LaunchGeom g, const WsInst* __restrict__ wi,
+                                                            const float* __restrict__ d, double* __restrict__ x) {
     const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
-    if (lane >= sd.nvalid || !wi[sd.inst].use_cand) return;
-    for (int c = 0; c < 3; ++c) x[3 * row + c] = xc[3 * row + c];
+    const WsInst& W = wi[sd.inst];
+    if (lane >= sd.nvalid || !W.use_cand) return;
+    const double al = W.alpha;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[3 * row + c] += al * double(d[3 * row + c]);
 }
 
-__global__ __launch_bounds__(256) void ws_target_kernel(LaunchGeom g, const double* __restrict__ x,
-                                                        const float* __restrict__ hprev, const WsInst* __restrict__ wi,
-                                                        float* __restrict__ target) {
+__global__ __launch_bounds__(256) void ws_energy_kernel(LaunchGeom g, const double* __restrict__ x,
+                                                        const double* __restrict__ xb, const float* __restrict__ rb,
+                                                        WsReduce red) {
     const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
-    const double is = 1.0 / wi[sd.inst].scale;
-    for (int c = 0; c < 3; ++c)
-        target[3 * row + c] = (lane < sd.nvalid) ? float((x[3 * row + c] - double(hprev[3 * row + c])) * is) : 0.0f;
+    double e = 0.0;
+    if (lane < sd.nvalid)
+        for (int c = 0; c < 3; ++c) e += (x[3 * row + c] - xb[3 * row + c]) * double(rb[3 * row + c]);
+    put_partial(red, 0, s, lane, e);
 }
 
-__global__ void ws_history_kernel(const double* __restrict__ x, float* __restrict__ hprev, size_t n) {
+__global__ __launch_bounds__(256) void ws_dout_kernel(LaunchGeom g, const WsInst* __restrict__ wi,
+                                                      const float* __restrict__ rb, const float* __restrict__ ad,
+                                                      __bf16* __restrict__ dout) {
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    float gr[3] = {0.0f, 0.0f, 0.0f};
+    if (lane < sd.nvalid) {
+        const WsInst& W = wi[sd.inst];
+        const double gc = W.gcoef, uw = (W.w > 0.0) ? W.u / W.w : 0.0;
+        for (int c = 0; c < 3; ++c) gr[c] = float(gc * (double(rb[3 * row + c]) - uw * double(ad[3 * row + c])));
+    }
+    bf16x8* o = reinterpret_cast<bf16x8*>(dout + (size_t)row * NOUTD);
+#pragma unroll
+    for (int u = 0; u < NOUTD / 8; ++u) {
+        bf16x8 t;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) t[q] = f2bf((u == 0 && q < 3) ? gr[q] : 0.0f);
+        o[u] = t;
+    }
+}
+
+__global__ void ws_history_kernel(const double* __restrict__ x, float* __restrict__ hist, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) hprev[i] = float(x[i]);
+    if (i < n) hist[i] = float(x[i]);
 }
 
-inline int grid_slices(int nslices) { return (nslices + 3) / 4; }
+inline unsigned grid_slices(int nslices) { return unsigned((nslices + 3) / 4); }
+
+template <int K>
+void gram_k(const LaunchGeom& g, const DevMatrix& A, const float* rhs, const WsHist& h, const WsReduce& red,
+            hipStream_t s) {
+    hipLaunchKernelGGL(ws_gram_kernel<K>, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, rhs, h, red);
+}
 
 }  // namespace
 
-void launch_ws_history(const double* x, float* hprev, size_t n, hipStream_t s) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(ws_history_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, x, hprev, n);
-    check_hip(hipGetLastError(), "ws_history_kernel");
-}
-
-void launch_ws_prep(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const double* x, double* partials,
+void launch_ws_gram(const LaunchGeom& g, const DevMatrix& A, const float* rhs, const WsHist& h, const WsReduce& red,
                     hipStream_t s) {
     if (g.nslices <= 0) return;
-    hipLaunchKernelGGL(ws_prep_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, v, x, partials,
-                       x == v.x ? 1 : 0);
-    check_hip(hipGetLastError(), "ws_prep_kernel");
+    switch (h.k) {
+        case 2: gram_k<2>(g, A, rhs, h, red, s); break;
+        case 3: gram_k<3>(g, A, rhs, h, red, s); break;
+        case 4: gram_k<4>(g, A, rhs, h, red, s); break;
+        case 5: gram_k<5>(g, A, rhs, h, red, s); break;
+        case 6: gram_k<6>(g, A, rhs, h, red, s); break;
+        case 7: gram_k<7>(g, A, rhs, h, red, s); break;
+        case 8: gram_k<8>(g, A, rhs, h, red, s); break;
+        default: throw std::invalid_argument("warm start: history length must be 2..8");
+    }
+    check_hip(hipGetLastError(), "ws_gram_kernel");
 }
-void launch_ws_reduce(WsInst* wi, const InstState* st, const double* partials, int ninst, int phase, hipStream_t s) {
+
+void launch_ws_finish(const WsReduce& red, int nvalues, const InstState* st, WsInst* wi, int ninst, int phase,
+                      int hist_k, hipStream_t s) {
     if (ninst <= 0) return;
-    hipLaunchKernelGGL(ws_reduce_kernel, dim3(ninst), dim3(1024), 0, s, wi, st, partials, phase);
-    check_hip(hipGetLastError(), "ws_reduce_kernel");
+    if (nvalues < 1 || nvalues > kWsMaxValues) throw std::invalid_argument("ws_finish: bad value count");
+    hipLaunchKernelGGL(ws_chunk_kernel, dim3(unsigned(ninst), unsigned(red.chunks)), dim3(256), 0, s, red, nvalues, st);
+    check_hip(hipGetLastError(), "ws_chunk_kernel");
+    hipLaunchKernelGGL(ws_finish_kernel, dim3(unsigned(ninst)), dim3(128), 0, s, red, nvalues, st, wi, phase, hist_k,
+                       1.0 / double(ninst));
+    check_hip(hipGetLastError(), "ws_finish_kernel");
 }
-void launch_ws_features(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const DevExt& e,
-                        const DevMesh& m, const float* rest_pos, float* hprev, const WsInst* wi,
-                        const InstParams* params, float dt, __bf16* F, __bf16* E, hipStream_t s) {
+
+void launch_ws_proj_apply(const LaunchGeom& g, const WsHist& h, const WsInst* wi, double* x, hipStream_t s) {
     if (g.nslices <= 0) return;
-    hipLaunchKernelGGL(ws_features_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, v, e, m, rest_pos, hprev,
-                       wi, params, dt, F, E);
+    hipLaunchKernelGGL(ws_proj_apply_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, h, wi, x);
+    check_hip(hipGetLastError(), "ws_proj_apply_kernel");
+}
+
+void launch_ws_base(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const double* x, double* xb,
+                    float* rb, const WsReduce& red, hipStream_t s) {
+    if (g.nslices <= 0) return;
+    hipLaunchKernelGGL(ws_base_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, v, x, xb, rb, red);
+    check_hip(hipGetLastError(), "ws_base_kernel");
+}
+
+void launch_ws_features(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const DevExt& e,
+                        const DevMesh& m, const float* rest_pos, const WsHist& h, const double* xb,
+                        const WsInst* wi, float dt, __bf16* F, __bf16* E, hipStream_t s) {
+    if (g.nslices <= 0) return;
+    hipLaunchKernelGGL(ws_features_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, v, e, m, rest_pos, h,
+                       xb, wi, dt, F, E);
     check_hip(hipGetLastError(), "ws_features_kernel");
 }
-void launch_ws_candidate(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const float* out,
-                         const WsInst* wi, double* xc, double* partials, hipStream_t s) {
+
+void launch_ws_direction(const LaunchGeom& g, const DevMatrix& A, const float* out, const WsInst* wi,
+                         const float* rb, float* d, float* ad, const WsReduce& red, hipStream_t s) {
     if (g.nslices <= 0) return;
-    hipLaunchKernelGGL(ws_candidate_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, v.x, out, wi, xc);
-    check_hip(hipGetLastError(), "ws_candidate_kernel");
-    hipLaunchKernelGGL(ws_prep_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, v, xc, partials, 0);
-    check_hip(hipGetLastError(), "ws_prep_kernel(candidate)");
+    hipLaunchKernelGGL(ws_dir_write_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, out, wi, d);
+    check_hip(hipGetLastError(), "ws_dir_write_kernel");
+    hipLaunchKernelGGL(ws_dir_spmv_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, d, rb, ad, red);
+    check_hip(hipGetLastError(), "ws_dir_spmv_kernel");
 }
-void launch_ws_select(const LaunchGeom& g, const WsInst* wi, const double* xc, double* x, hipStream_t s) {
+
+void launch_ws_line_apply(const LaunchGeom& g, const WsInst* wi, const float* d, double* x, hipStream_t s) {
     if (g.nslices <= 0) return;
-    hipLaunchKernelGGL(ws_select_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, wi, xc, x);
-    check_hip(hipGetLastError(), "ws_select_kernel");
+    hipLaunchKernelGGL(ws_line_apply_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, wi, d, x);
+    check_hip(hipGetLastError(), "ws_line_apply_kernel");
 }
-void launch_ws_target(const LaunchGeom& g, const double* x, const float* hprev, const WsInst* wi, float* target,
+
+void launch_ws_energy(const LaunchGeom& g, const double* x, const double* xb, const float* rb, const WsReduce& red,
                       hipStream_t s) {
     if (g.nslices <= 0) return;
-    hipLaunchKernelGGL(ws_target_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, x, hprev, wi, target);
-    check_hip(hipGetLastError(), "ws_target_kernel");
+    hipLaunchKernelGGL(ws_energy_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, x, xb, rb, red);
+    check_hip(hipGetLastError(), "ws_energy_kernel");
+}
+
+void launch_ws_dout(const LaunchGeom& g, const WsInst* wi, const float* rb, const float* ad, __bf16* dout,
+                    hipStream_t s) {
+    if (g.nslices <= 0) return;
+    hipLaunchKernelGGL(ws_dout_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, wi, rb, ad, dout);
+    check_hip(hipGetLastError(), "ws_dout_kernel");
+}
+
+void launch_ws_history(const double* x, float* hist, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(ws_history_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, x, hist, n);
+    check_hip(hipGetLastError(), "ws_history_kernel");
 }
 
 }  // namespace chysx_amd

@@ -114,37 +114,96 @@ private:
 };
 
 // ---- simulator-side kernels ------------------------------------------------
+//
+// Warm-start pipeline of one step (all instances in one launch each):
+//   projection: Galerkin projection of the system onto the span of the last
+//     K converged dx (newest first): G = V^T A V, g = V^T b, x = V c with
+//     c = G^+ g (eigen-truncated), accepted when its energy
+//     phi(x) = x^T A x / 2 - b^T x is lower than phi(dx_prev);
+//   learned: from the base guess x_b (projection or dx_prev) the network
+//     predicts a direction d; x = x_b + alpha d with the A-norm line search
+//     alpha = d^T r_b / d^T A d, accepted when the predicted residual
+//     |r_b - alpha A d| is smaller than |r_b|;
+//   training: loss 1 - (d^T r_b)^2 / ((d^T A d)(e^T r_b)), e = x_conv - x_b,
+//     the fraction of the base guess's A-norm error left after the line search.
+
+constexpr int kWsHistMax = 8;
+
+struct WsHist {
+    const float* v[kWsHistMax];  // newest first, 3 * rows floats each
+    int k;
+};
 
 // Per-instance warm-start bookkeeping (device).
 struct WsInst {
-    double rr_base;   // |b - A x_base|^2
-    double rr_cand;   // |b - A x_cand|^2
-    double scale;     // RMS of dx_prev over the instance's rows
-    int use_cand;     // 1: the network's guess replaces the base guess
+    double coef[kWsHistMax];  // projection coefficients (newest first)
+    double bb;        // |b|^2
+    double rr_prev;   // |b - A dx_prev|^2
+    double rr_proj;   // |b - A x_proj|^2 (predicted from the Gram sums)
+    double rr_base;   // |b - A x_b|^2 (learned stage, exact)
+    double rr_cand;   // |b - A (x_b + alpha d)|^2 (predicted)
+    double scale;     // RMS of x_b over the instance's rows
+    double u, w;      // d.r_b, d.A d
+    double alpha;     // line-search step (0: rejected)
+    double energy;    // e.r_b = e^T A e (training)
+    double loss;      // training loss of this instance
+    double gcoef;     // dLoss/dd = gcoef * (r_b - (u/w) A d)   (scaled by 1/ninst)
+    int use_proj;     // 1: the projection replaced dx_prev
+    int use_cand;     // 1: the learned direction was applied
+    int kdim;         // projection directions kept
     int pad;
 };
 
-// Residual of the base guess x (FP64): r (FP32) and z = M^-1 r into v.r / v.z;
-// per-slice partials [|r|^2, |x|^2].
-void launch_ws_prep(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const double* x,
-                    double* partials, hipStream_t s);
-// Per-instance reduction of launch_ws_prep / launch_ws_cand partials.
-void launch_ws_reduce(WsInst* wi, const InstState* st, const double* partials, int ninst, int phase, hipStream_t s);
-// Network inputs for every row: features F and mean edge features E (BF16);
-// also rotates the history (hprev <- dx_prev as FP32).
+// Number of Gram sums for k history vectors: two packed triangles (V^T A V,
+// (AV)^T (AV)), V^T b, (AV)^T b and b^T b.
+constexpr int ws_gram_values(int k) { return k * (k + 1) + 2 * k + 1; }
+constexpr int kWsMaxValues = ws_gram_values(kWsHistMax);
+constexpr int kWsChunkSlices = 1024;  // slices per reduction workgroup
+
+// Phases of launch_ws_finish.
+enum WsPhase : int { kWsPhaseProj = 0, kWsPhaseBase = 1, kWsPhaseLine = 2, kWsPhaseEnergy = 3 };
+
+struct WsReduce {
+    double* part;   // SoA [values][ldp]
+    int ldp;        // >= nslices
+    double* part2;  // [ninst][chunks][values]
+    int chunks;     // ceil(max slices of an instance / kWsChunkSlices)
+};
+
+// Gram sums of the history against the current system (per-slice partials).
+void launch_ws_gram(const LaunchGeom& g, const DevMatrix& A, const float* rhs, const WsHist& h, const WsReduce& red,
+                    hipStream_t s);
+// Per-instance sums of `nvalues` per-slice partials (deterministic, fixed
+// chunking) followed by the phase's per-instance decision.
+void launch_ws_finish(const WsReduce& red, int nvalues, const InstState* st, WsInst* wi, int ninst, int phase,
+                      int hist_k, hipStream_t s);
+// x <- V c for instances whose projection was accepted.
+void launch_ws_proj_apply(const LaunchGeom& g, const WsHist& h, const WsInst* wi, double* x, hipStream_t s);
+// Base guess of the learned stage: r_b = b - A x (FP64 sums, stored FP32),
+// z = M^-1 r_b into v.z, xb <- x; partials [|r_b|^2, |x|^2].
+void launch_ws_base(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const double* x, double* xb,
+                    float* rb, const WsReduce& red, hipStream_t s);
+// Network inputs for every row: features F and mean edge features E (BF16).
 void launch_ws_features(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const DevExt& e,
-                        const DevMesh& m, const float* rest_pos, float* hprev, const WsInst* wi,
-                        const InstParams* params, float dt, __bf16* F, __bf16* E, hipStream_t s);
-// Candidate guess xc = x + scale * out[:, 0:3] (FP64) and its residual partials.
-void launch_ws_candidate(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const float* out,
-                         const WsInst* wi, double* xc, double* partials, hipStream_t s);
-// x <- xc for instances whose candidate residual is smaller.
-void launch_ws_select(const LaunchGeom& g, const WsInst* wi, const double* xc, double* x, hipStream_t s);
-// hprev <- x (FP32), n values: keeps the previous solution while the
-// network's inputs are still being filled.
-void launch_ws_history(const double* x, float* hprev, size_t n, hipStream_t s);
-// Training target (x_converged - x_base) / scale, x_base = hprev (FP32).
-void launch_ws_target(const LaunchGeom& g, const double* x, const float* hprev, const WsInst* wi, float* target,
+                        const DevMesh& m, const float* rest_pos, const WsHist& h, const double* xb,
+                        const WsInst* wi, float dt, __bf16* F, __bf16* E, hipStream_t s);
+// d = scale * out[:, 0:3] and A d; partials [d.r_b, d.Ad, Ad.r_b, Ad.Ad].
+void launch_ws_direction(const LaunchGeom& g, const DevMatrix& A, const float* out, const WsInst* wi,
+                         const float* rb, float* d, float* ad, const WsReduce& red, hipStream_t s);
+// x <- x + alpha d for instances whose direction was accepted.
+void launch_ws_line_apply(const LaunchGeom& g, const WsInst* wi, const float* d, double* x, hipStream_t s);
+// Training: partials [(x - xb).r_b].
+void launch_ws_energy(const LaunchGeom& g, const double* x, const double* xb, const float* rb, const WsReduce& red,
                       hipStream_t s);
+// Training: upstream gradient of the network output (BF16, rows x 32).
+void launch_ws_dout(const LaunchGeom& g, const WsInst* wi, const float* rb, const float* ad, __bf16* dout,
+                    hipStream_t s);
+// hist <- x as FP32 (n values).
+void launch_ws_history(const double* x, float* hist, size_t n, hipStream_t s);
+
+// Host twin of the per-instance projection solve (tests): coefficients c of
+// the eigen-truncated solve G c = g (G symmetric k x k, row-major); returns
+// the number of eigen-directions kept.
+int ws_small_solve_host(int k, const double* G, const double* g, double* c);
 
 }  // namespace chysx_amd

@@ -76,6 +76,12 @@ def bench_batch(args):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     n_total = int(sum(s.n for s in sc))
+    ws_cmp = None
+    if args.ws_steps > 0:
+        del b
+        ws_cmp = batch_warm_start_comparison(args, sc, make_batch)
+        b = None
+
     cpu = None
     if cpu_state is not None:
         from oracle.oracle import oracle_from_scene
@@ -85,7 +91,7 @@ def bench_batch(args):
         order = np.random.default_rng(0).permutation(len(sc))[:48]
         for k in order:
             s = sc[k]
-            a = b.offsets[k]
+            a = int(np.cumsum([0] + [q.n for q in sc])[k])
             oc, _, _, im, f = oracle_from_scene(s, precision=64, tol=args.tol, max_iter=20000, threads=threads,
                                                 force_active=False)
             oc.build_topology()
@@ -117,11 +123,111 @@ def bench_batch(args):
                    "particles": n_total, "cloths": len(sc), "parallelism": "1 GPU"},
         "pcg_iters_per_step_mean": round(float(np.mean(it_mean)), 2),
         "pcg_iters_per_step_max": round(float(np.mean(it_max)), 2),
+        "warm_start_comparison": ws_cmp,
         "setup_s": round(setup_s, 2),
         "roofline": None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def warm_start_comparison(args, scene, make_sim):
+    """Same workload and step window with each warm start: previous-step dx
+    (the reference's), Galerkin projection onto the last 6 solutions, and the
+    projection plus the learned direction (committed trained parameters)."""
+    import numpy as np
+    import torch
+    from chysx import warmstart as ws
+    path = args.ws_weights or ws.DEFAULT_WEIGHTS
+    theta, meta = (None, None)
+    if os.path.exists(path):
+        theta, meta = ws.load_params(path)
+    out = {"warmup": args.warmup, "steps": args.ws_steps, "weights": os.path.relpath(path, ROOT) if theta is not None
+           else None}
+    if meta is not None:
+        out["trained_pairs"] = meta.get("pairs")
+        out["train_seconds"] = meta.get("train_seconds")
+    pos = {}
+    for mode in ("previous", "projection", "learned"):
+        if mode == "learned" and theta is None:
+            out[mode] = None
+            continue
+        g = make_sim(scene, tol=args.tol, max_iter=20000)
+        if theta is not None:
+            g.sim.ws_set_params(theta)
+        g.sim.set_warm_start_mode(mode)
+        for _ in range(args.warmup):
+            g.sim.step(scene.dt)
+        torch.cuda.synchronize()
+        its, used, proj = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(args.ws_steps):
+            g.sim.step(scene.dt)
+            st = g.sim.last_solve_stats()
+            if not st["converged"]:
+                raise RuntimeError(f"PCG did not converge ({mode}): {st}")
+            its.append(st["iters"])
+            w = g.sim.ws_stats()
+            used.append(w["used"])
+            proj.append(w["projection_used"])
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        pos[mode] = g.positions()
+        out[mode] = {"steps_per_s": round(args.ws_steps / el, 3), "pcg_iters_per_step": round(float(np.mean(its)), 2),
+                     "projection_used_frac": round(float(np.mean(proj)), 3),
+                     "learned_used_frac": round(float(np.mean(used)), 3)}
+        del g
+    for mode in ("projection", "learned"):
+        if out.get(mode):
+            out[mode]["iters_ratio_vs_previous"] = round(out["previous"]["pcg_iters_per_step"] /
+                                                         out[mode]["pcg_iters_per_step"], 3)
+            out[mode]["speedup_vs_previous"] = round(out[mode]["steps_per_s"] / out["previous"]["steps_per_s"], 3)
+            out[mode]["max_pos_diff_vs_previous_m"] = float(np.abs(pos[mode] - pos["previous"]).max())
+    return out
+
+
+def batch_warm_start_comparison(args, sc, make_batch):
+    """The batch stepped with each warm start (fresh batch per mode)."""
+    import numpy as np
+    import torch
+    from chysx import warmstart as ws
+    path = args.ws_weights or ws.DEFAULT_WEIGHTS
+    theta = ws.load_params(path)[0] if os.path.exists(path) else None
+    out = {"warmup": args.warmup, "steps": args.ws_steps,
+           "weights": os.path.relpath(path, ROOT) if theta is not None else None}
+    for mode in ("previous", "projection", "learned"):
+        if mode == "learned" and theta is None:
+            out[mode] = None
+            continue
+        b = make_batch(sc, tol=args.tol, max_iter=20000)
+        if theta is not None:
+            b.batch.ws_set_params(theta)
+        b.batch.set_warm_start_mode(mode)
+        for _ in range(args.warmup):
+            b.batch.step(sc[0].dt)
+        torch.cuda.synchronize()
+        it_mean, it_max, used = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(args.ws_steps):
+            b.batch.step(sc[0].dt)
+            st = [b.batch.stats(k) for k in range(len(sc))]
+            if not all(x["converged"] for x in st):
+                raise RuntimeError(f"a cloth did not converge ({mode})")
+            its = [x["iters"] for x in st]
+            it_mean.append(np.mean(its))
+            it_max.append(np.max(its))
+            used.append(np.mean([b.batch.ws_stats(k)["used"] for k in range(len(sc))]))
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[mode] = {"steps_per_s": round(args.ws_steps / el, 3),
+                     "pcg_iters_per_step_mean": round(float(np.mean(it_mean)), 2),
+                     "pcg_iters_per_step_max": round(float(np.mean(it_max)), 2),
+                     "learned_used_frac": round(float(np.mean(used)), 3)}
+        del b
+    for mode in ("projection", "learned"):
+        if out.get(mode):
+            out[mode]["speedup_vs_previous"] = round(out[mode]["steps_per_s"] / out["previous"]["steps_per_s"], 3)
+    return out
 
 
 def read_traffic(size):
@@ -150,13 +256,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--ws-steps", type=int, default=40,
+                    help="timed steps per warm-start mode in the projection / learned comparison (0: skip)")
+    ap.add_argument("--ws-weights", default=None, help="learned warm-start parameters (.npz)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (known bytes) for counter calibration")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: every box here has one GPU and this build has no "
-              "multi-GPU path (see the sharding notes in DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the cloth step "
+              "would shard by row bands with a one-row halo per PCG iteration).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -231,6 +340,10 @@ def main():
                           "GB/s_ref_formulation": round(gbs(kb["pcg_iteration_ref"], t_iter), 1)},
     }
 
+    ws_cmp = None
+    if args.ws_steps > 0:
+        ws_cmp = warm_start_comparison(args, s, make_sim)
+
     cpu = None
     if cpu_state is not None:
         from oracle.oracle import oracle_from_scene
@@ -271,7 +384,7 @@ def main():
         "pcg_iters_per_step": round(float(np.mean(iters)), 2),
         "pcg_restarts_per_step": round(float(np.mean(restarts)), 3),
         "max_relres": float(np.max(relres)),
-        "learned_warm_start": None,
+        "warm_start_comparison": ws_cmp,
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
         "kernels": extra_kernels,

@@ -193,28 +193,45 @@ int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg);
  * waves, unroll} and mean ms. */
 int chysx_cloth_tune_log(const chysx_cloth* sim, int* count, int* kernel_waves_unroll, double* ms);
 int chysx_batch_set_autotune(chysx_batch* b, int on);
-/* ---- learned warm start (extension) --------------------------------------
- * An edge message-passing network over the mesh graph (BF16 MFMA, FP32
- * accumulate) predicts the PCG initial guess; the solver keeps the
- * previous-step guess whenever the prediction's residual is larger.
- * mode: 0 = previous-step dx (reference behaviour, default), 1 = learned.
- * Parameters: chysx_ws_num_params() FP32 values.  Training: every step runs
- * the network; the converged dx drives one Adam step (lr). */
+/* ---- PCG warm start (extension) ------------------------------------------
+ * mode 0: the previous step's dx (reference behaviour, default).
+ * mode 2 (projection): Galerkin projection of the new system onto the span
+ *   of the last K converged dx (K = chysx_*_ws_set_history, 2..8, default 6),
+ *   accepted when it lowers the energy x^T A x / 2 - b^T x below that of
+ *   dx_prev (the A-norm error PCG minimises).
+ * mode 1 (learned): the projection, then an edge message-passing network
+ *   over the mesh graph (BF16 MFMA, FP32 accumulate) predicts a direction d;
+ *   x += alpha d with alpha = d.r / d.Ad, kept only when the predicted
+ *   residual is smaller than without it.
+ * Both start once two solutions are held; set/zero_warm_start resets the
+ * history.  Parameters: chysx_ws_num_params() FP32 values.  Training: every
+ * step runs the network; after the solve the loss
+ * 1 - (d.r)^2 / ((d.Ad)(e.r)), e = dx_converged - x_base, drives one Adam
+ * step (lr). */
+#define CHYSX_WS_STATS_LEN 7
 int chysx_ws_num_params(int* out);
 int chysx_cloth_set_warm_start_mode(chysx_cloth* sim, int mode);
+int chysx_cloth_ws_set_history(chysx_cloth* sim, int k);
 int chysx_cloth_ws_set_params(chysx_cloth* sim, const float* theta, int n);
 int chysx_cloth_ws_get_params(chysx_cloth* sim, float* theta, int n);
 int chysx_cloth_ws_init_params(chysx_cloth* sim, uint64_t seed);
 int chysx_cloth_ws_set_training(chysx_cloth* sim, int on, float lr);
-/* Last step: prediction used (1/0), relative residuals of the previous-step
- * and predicted guesses, last training loss. */
-int chysx_cloth_ws_stats(chysx_cloth* sim, int* used, double* r0_base, double* r0_cand, double* loss);
+/* Last step, out[0..6]: projection used (1/0), learned direction used (1/0),
+ * projection directions kept, relative residuals |b - A x0| / |b| of dx_prev,
+ * of the projection and of the learned guess (0 when that stage did not run),
+ * mean training loss of the step.  n >= CHYSX_WS_STATS_LEN. */
+int chysx_cloth_ws_stats(chysx_cloth* sim, double* out, int n);
 int chysx_batch_set_warm_start_mode(chysx_batch* b, int mode);
+int chysx_batch_ws_set_history(chysx_batch* b, int k);
 int chysx_batch_ws_set_params(chysx_batch* b, const float* theta, int n);
 int chysx_batch_ws_get_params(chysx_batch* b, float* theta, int n);
 int chysx_batch_ws_init_params(chysx_batch* b, uint64_t seed);
 int chysx_batch_ws_set_training(chysx_batch* b, int on, float lr);
-int chysx_batch_ws_stats(chysx_batch* b, int index, int* used, double* r0_base, double* r0_cand, double* loss);
+int chysx_batch_ws_stats(chysx_batch* b, int index, double* out, int n);
+/* Projection solve on the host (tests): c = G^+ g over the eigen-directions
+ * of the symmetric k x k G (row-major) above 1e-12 of the largest; *kept =
+ * directions used. */
+int chysx_ws_small_solve(int k, const double* G, const double* g, double* c, int* kept);
 /* Stand-alone network on a graph (n rows, CSR neighbours), for checking the
  * GPU forward / backward: F and E are n x 32 FP32 inputs, out n x 32, target
  * n x 3, loss = mean over rows of |out[:, 0:3] - target|^2, grad FP32. */

@@ -2,7 +2,7 @@
 
 Only the tests may use this module: it checks the GPU forward and backward
 kernels.  The network (no reference counterpart; the reference has no learned
-component, SURVEY 0.3):
+component):
 
     h0      = relu(F We + be)                                F: (N, 32)
     layer l = 0..2:
@@ -16,8 +16,17 @@ eagg (N, 32) holds the mean edge features (first 4 used).  Weights are stored
 as (K, NOUT) matrices.  The parameter vector order is
 [We, be, W0, b0, W1, b1, W2, b2, Wd, bd].
 
+The warm-start pipeline around it (no reference counterpart either):
+  * galerkin_projection: x = V c, c the eigen-truncated solution of
+    (V^T A V) c = V^T b over the eigen-directions above 1e-12 of the largest;
+  * linesearch_loss_grad: L = 1 - (d.r)^2 / ((d.Ad)(e.r)) and dL/dd, the
+    training loss of the network's direction d (r = b - A x_b, e = x* - x_b).
+
 Pinned by tests/test_oracle_warmstart.py: backward vs central finite
-differences of forward; the mean aggregation vs a dense adjacency product.
+differences of forward; the mean aggregation vs a dense adjacency product;
+the projection vs the exact solution when it lies in span(V) and vs energy
+minimality under perturbation; the line-search gradient vs central finite
+differences, L = 0 for d parallel to e and L = 1 for d A-orthogonal to e.
 """
 
 from __future__ import annotations
@@ -125,3 +134,31 @@ def loss_and_grad(theta, F, eagg, target, valid, ro, co):
     dout = np.zeros_like(out)
     dout[:, :3] = 2.0 * diff / nv
     return L, backward(theta, cache, dout, eagg, ro, co), out
+
+
+def galerkin_projection(A, b, V, cutoff=1e-12):
+    """x = V c minimising x^T A x / 2 - b^T x over span(V) (A SPD, V (n, k)).
+
+    Eigen-truncated: directions of V^T A V below `cutoff` times its largest
+    eigenvalue are dropped.  Returns (x, c, kept)."""
+    V = np.asarray(V, np.float64)
+    AV = A @ V
+    G = V.T @ AV
+    G = 0.5 * (G + G.T)
+    g = V.T @ b
+    lam, U = np.linalg.eigh(G)
+    keep = lam > cutoff * lam.max()
+    c = U[:, keep] @ ((U[:, keep].T @ g) / lam[keep])
+    return V @ c, c, int(keep.sum())
+
+
+def linesearch_loss_grad(d, r, Ad, e_dot_r):
+    """L = 1 - (d.r)^2 / ((d.Ad)(e.r)): the share of the A-norm error e of
+    the base guess left after x_b + alpha d, alpha = d.r / d.Ad (r = A e).
+    Returns (L, dL/dd)."""
+    u = float(d @ r)
+    w = float(d @ Ad)
+    E = float(e_dot_r)
+    L = 1.0 - u * u / (w * E)
+    grad = -(2.0 * u / (w * E)) * (r - (u / w) * Ad)
+    return L, grad

@@ -161,3 +161,28 @@ def test_mesh_edges_and_counts_without_gpu():
     assert sim.num_springs() == 0 and sim.num_fem_stretch_triangles() == 0
     with pytest.raises(RuntimeError):
         sim.build_springs_from_current_positions(1e3)  # positions not set
+
+
+@pytest.mark.parametrize("k", [2, 3, 5, 8])
+def test_projection_small_solve_matches_eigen_truncated_pinv(k):
+    """The per-instance projection solve (cyclic Jacobi, host twin of the
+    device code) against numpy's eigh-based truncated solve."""
+    from oracle import warmstart_oracle as W
+    rng = np.random.default_rng(k)
+    n = 50
+    M = rng.normal(size=(n, n))
+    A = M @ M.T + n * np.eye(n)
+    V = rng.normal(size=(n, k))
+    V[:, -1] = V[:, 0] * 0.5 + V[:, 1] * 1e-14  # numerically collinear column
+    b = rng.normal(size=n)
+    G = V.T @ A @ V
+    g = V.T @ b
+    c, kept = chysx._chysx_native.ws_small_solve(np.ascontiguousarray(G), g)
+    x_ref, _, kept_ref = W.galerkin_projection(A, b, V)
+    assert kept == kept_ref == k - 1
+    np.testing.assert_allclose(V @ c, x_ref, rtol=1e-8, atol=1e-10)
+
+
+def test_projection_small_solve_rejects_bad_size():
+    with pytest.raises(ValueError):
+        chysx._chysx_native.ws_small_solve(np.eye(9), np.ones(9))
