@@ -38,6 +38,8 @@ class Scene:
     damping: float = 0.0
     grid_w: int = 0
     meta: dict = field(default_factory=dict)
+    shear_k: Optional[float] = None     # FEM shear stiffness (None: no shear elements)
+    bend_k: Optional[float] = None      # dihedral bending stiffness (None: no bending)
 
     @property
     def n(self) -> int:

@@ -532,12 +532,31 @@ __global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* _
     const int s0 = S.slice0, ns = S.nslices;
     const double2* p2 = reinterpret_cast<const double2*>(partials);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int q = threadIdx.x; q < ns; q += kReduceThreads) {
-        const double2 lo = p2[2 * (s0 + q)], hi = p2[2 * (s0 + q) + 1];
-        acc[0] += lo.x;
-        acc[1] += lo.y;
-        acc[2] += hi.x;
-        acc[3] += hi.y;
+    // Loads of 8 slices are issued before they are summed (in slice order, so
+    // the sum is the same as a plain loop's).
+    constexpr int kU = 8;
+    for (int q0 = threadIdx.x; q0 < ns; q0 += kU * kReduceThreads) {
+        double2 lo[kU], hi[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int q = q0 + u * kReduceThreads;
+            if (q < ns) {
+                lo[u] = p2[2 * (s0 + q)];
+                hi[u] = p2[2 * (s0 + q) + 1];
+            } else {
+                lo[u] = make_double2(0.0, 0.0);
+                hi[u] = make_double2(0.0, 0.0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (q0 + u * kReduceThreads < ns) {
+                acc[0] += lo[u].x;
+                acc[1] += lo[u].y;
+                acc[2] += hi[u].x;
+                acc[3] += hi[u].y;
+            }
+        }
     }
     __shared__ double sh[kReduceThreads / 64][4];
 #pragma unroll

@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -61,6 +62,9 @@ template <class T> inline V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x,
 template <class T> inline V3<T> operator*(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
 template <class T> inline T dot3(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <class T> inline T len3(V3<T> a) { return std::sqrt(dot3(a, a)); }
+template <class T> inline V3<T> cross3(V3<T> a, V3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
 
 template <class T> struct M3 {
     T m[9];
@@ -182,10 +186,20 @@ struct Cloth {
     // springs
     std::vector<int> spr_a, spr_b;
     std::vector<T> spr_rest, spr_k;
-    // stretch triangles (theta = 0)
-    std::vector<int> tri_idx;              // 3 per installed triangle
-    std::vector<T> tri_dminv;              // 4 per triangle, row-major 2x2
-    std::vector<T> tri_area, tri_k;
+    // Baraff-Witkin triangle sets (TriangleStretchConstraint): stretch
+    // (theta = 0) and shear (theta = pi/4, cloth_simulator.cu:423-449).
+    struct TriSet {
+        std::vector<int> idx;    // 3 per installed triangle
+        std::vector<T> dminv;    // 4 per triangle, row-major 2x2 (rotation applied)
+        std::vector<T> area, k;
+        std::vector<int> slots;  // 9 per triangle
+    };
+    TriSet st, sh;
+    // bending dihedrals (BendingConstraint, bending_constraint.h:1-129)
+    std::vector<int> dih;                  // 4 per dihedral (v0, v1, v2, v3)
+    std::vector<T> dih_rest;
+    T bend_k = T(1.0e-3);                  // bending_constraint.h:120
+    std::vector<int> dih_slots;            // 16 per dihedral
     // pins
     std::vector<int> pin_idx;
     std::vector<T> pin_tgt;                // 3 per pin
@@ -201,7 +215,7 @@ struct Cloth {
     // ---- derived --------------------------------------------------------
     bool topo_dirty = true;
     Pattern pat;
-    std::vector<int> tri_slots, spr_slots, pin_slots;
+    std::vector<int> spr_slots, pin_slots;
     std::vector<M3<T>> diag, vals;
     std::vector<V3<T>> rhs, xn, xt;
     std::vector<V3<double>> dx;            // PCG iterate: always FP64 (see pcg())
@@ -230,11 +244,15 @@ struct Cloth {
         topo_dirty = true;
     }
 
-    // triangle_rest_shape_kernel with theta = 0 (triangle_stretch_constraint.cu:50-110).
-    void build_stretch(const std::vector<V3<T>>& pos, const T* kper, T kscalar) {
+    // triangle_rest_shape_kernel (triangle_stretch_constraint.cu:50-110):
+    // rest frame [[lab, u_c], [0, v_c]], its inverse (identity when
+    // det <= 1e-6), area lab v_c / 2, times R(theta)^-1 = [[c, s], [-s, c]]
+    // with c, s the FP32 cos / sin of theta (:340-341).
+    void build_tri_set(TriSet& S, const std::vector<V3<T>>& pos, const T* kper, T kscalar, float theta) {
         const size_t Tn = tris.size() / 3;
-        tri_idx = tris;
-        tri_dminv.resize(4 * Tn); tri_area.resize(Tn); tri_k.resize(Tn);
+        const T cs = T(std::cos(theta)), sn = T(std::sin(theta));
+        S.idx = tris;
+        S.dminv.resize(4 * Tn); S.area.resize(Tn); S.k.resize(Tn);
         for (size_t t = 0; t < Tn; ++t) {
             const V3<T> a = pos[tris[3 * t]], b = pos[tris[3 * t + 1]], c = pos[tris[3 * t + 2]];
             const V3<T> ab = b - a, ac = c - a;
@@ -243,7 +261,7 @@ struct Cloth {
             const T uc = dot3(abh, ac);
             const T vc2 = lac * lac - uc * uc;
             const T vc = (vc2 > T(0)) ? std::sqrt(vc2) : T(0);
-            tri_area[t] = T(0.5) * lab * vc;
+            S.area[t] = T(0.5) * lab * vc;
             const T det = lab * vc;
             T d00 = 1, d01 = 0, d10 = 0, d11 = 1;
             if (det > T(kMinNorm)) {
@@ -254,9 +272,98 @@ struct Cloth {
                 d10 = -T(0) * inv_det;
                 d11 = lab * inv_det;
             }
-            tri_dminv[4 * t + 0] = d00; tri_dminv[4 * t + 1] = d01;
-            tri_dminv[4 * t + 2] = d10; tri_dminv[4 * t + 3] = d11;
-            tri_k[t] = kper ? kper[t] : kscalar;
+            S.dminv[4 * t + 0] = d00 * cs + d01 * (-sn);
+            S.dminv[4 * t + 1] = d00 * sn + d01 * cs;
+            S.dminv[4 * t + 2] = d10 * cs + d11 * (-sn);
+            S.dminv[4 * t + 3] = d10 * sn + d11 * cs;
+            S.k[t] = kper ? kper[t] : kscalar;
+        }
+        topo_dirty = true;
+    }
+    void build_stretch(const std::vector<V3<T>>& pos, const T* kper, T kscalar) {
+        build_tri_set(st, pos, kper, kscalar, 0.0f);
+    }
+    // build_fem_shear_from_current_positions (cloth_simulator.cu:423-449).
+    void build_shear(const std::vector<V3<T>>& pos, const T* kper, T kscalar) {
+        build_tri_set(sh, pos, kper, kscalar, 0.7853981633974483f);
+    }
+
+    // Dihedral detection of build_bending_from_current_positions
+    // (cloth_simulator.cu:451-514): each directed triangle edge records its
+    // third vertex (a later triangle overwrites an earlier one); an edge
+    // v0 < v1 with both directions present gives (v0, v1, third(v0->v1),
+    // third(v1->v0)).  The reference emits them in hash-map order; here
+    // they are sorted by (v0, v1) so the order is deterministic.
+    static std::vector<int> find_dihedrals(const std::vector<int>& tri) {
+        std::map<std::pair<int, int>, int> third;
+        const size_t Tn = tri.size() / 3;
+        for (size_t t = 0; t < Tn; ++t) {
+            const int v[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
+            third[{v[0], v[1]}] = v[2];
+            third[{v[1], v[2]}] = v[0];
+            third[{v[2], v[0]}] = v[1];
+        }
+        std::vector<int> out;
+        for (const auto& kv : third) {
+            const int v0 = kv.first.first, v1 = kv.first.second;
+            if (v0 >= v1) continue;
+            auto rev = third.find({v1, v0});
+            if (rev == third.end()) continue;
+            out.insert(out.end(), {v0, v1, kv.second, rev->second});
+        }
+        return out;
+    }
+
+    // compute_dihedral_frame (bending_constraint.cu:50-127): dihedral angle
+    // theta and a_i = d theta / d x_i; invalid for a collapsed hinge or a
+    // wing with no height (floor 1e-12).
+    struct DihFrame {
+        T theta;
+        V3<T> a[4];
+        bool valid;
+    };
+    static DihFrame dihedral_frame(V3<T> v0, V3<T> v1, V3<T> v2, V3<T> v3) {
+        DihFrame F;
+        F.valid = false;
+        F.theta = T(0);
+        const T kFloor = T(1.0e-12);
+        const V3<T> e10 = v1 - v0, e20 = v2 - v0, e30 = v3 - v0;
+        const T l = len3(e10);
+        if (l < kFloor) return F;
+        const T inv_l = T(1) / l;
+        const V3<T> e10h = e10 * inv_l;
+        const V3<T> n1r = cross3(e20, e10), n2r = cross3(e10, e30);
+        const T n1l = len3(n1r), n2l = len3(n2r);
+        if (n1l < kFloor || n2l < kFloor) return F;
+        const V3<T> n1 = n1r * (T(1) / n1l), n2 = n2r * (T(1) / n2l);
+        T sin_t = dot3(cross3(n1, n2), e10h);
+        if (sin_t > T(1)) sin_t = T(1);
+        if (sin_t < T(-1)) sin_t = T(-1);
+        const T cos_t = dot3(n1, n2);
+        F.theta = std::atan2(sin_t, cos_t);
+        const T inv_l2 = inv_l * inv_l;
+        const T om1 = dot3(e10, e20) * inv_l2, om2 = dot3(e10, e30) * inv_l2;
+        const T h1 = len3(e20 - e10 * om1), h2 = len3(e30 - e10 * om2);
+        if (h1 < kFloor || h2 < kFloor) return F;
+        const T ih1 = T(1) / h1, ih2 = T(1) / h2;
+        const T t1[4] = {(om1 - T(1)) * ih1, -om1 * ih1, T(1) * ih1, T(0)};
+        const T t2[4] = {(om2 - T(1)) * ih2, -om2 * ih2, T(0), T(1) * ih2};
+        for (int i = 0; i < 4; ++i) F.a[i] = n1 * t1[i] + n2 * t2[i];
+        F.valid = true;
+        return F;
+    }
+    DihFrame dih_frame(size_t d, const std::vector<V3<T>>& x) const {
+        return dihedral_frame(x[dih[4 * d]], x[dih[4 * d + 1]], x[dih[4 * d + 2]], x[dih[4 * d + 3]]);
+    }
+    // set_dihedrals_from_positions + bending_rest_angle_kernel
+    // (bending_constraint.cu:136-152, 260-290).
+    void build_bending(const std::vector<V3<T>>& pos, T k) {
+        bend_k = k;
+        dih = find_dihedrals(tris);
+        dih_rest.resize(dih.size() / 4);
+        for (size_t d = 0; d < dih_rest.size(); ++d) {
+            const DihFrame F = dih_frame(d, pos);
+            dih_rest[d] = F.valid ? F.theta : T(0);
         }
         topo_dirty = true;
     }
@@ -268,12 +375,22 @@ struct Cloth {
     void ensure_topology() {
         if (!topo_dirty && pat.n == n) return;
         std::vector<int> rows, cols;
-        const size_t Tn = tri_idx.size() / 3;
-        rows.reserve(6 * Tn); cols.reserve(6 * Tn);
-        for (size_t t = 0; t < Tn; ++t) {
-            const int v[3] = {tri_idx[3 * t], tri_idx[3 * t + 1], tri_idx[3 * t + 2]};
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b)
+        auto tri_pairs = [&](const TriSet& S) {
+            const size_t Tn = S.idx.size() / 3;
+            for (size_t t = 0; t < Tn; ++t) {
+                const int v[3] = {S.idx[3 * t], S.idx[3 * t + 1], S.idx[3 * t + 2]};
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        if (a != b && v[a] != v[b]) { rows.push_back(v[a]); cols.push_back(v[b]); }
+            }
+        };
+        tri_pairs(st);
+        tri_pairs(sh);
+        // bending: all 12 directed pairs of the 4 dihedral vertices (:741-761)
+        for (size_t d = 0; d < dih.size() / 4; ++d) {
+            const int* v = &dih[4 * d];
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
                     if (a != b && v[a] != v[b]) { rows.push_back(v[a]); cols.push_back(v[b]); }
         }
         if (springs_active) {
@@ -287,11 +404,19 @@ struct Cloth {
         // slot[c*N^2 + a*N + b] for (v_a, v_b).
         pin_slots.resize(pin_idx.size());
         for (size_t c = 0; c < pin_idx.size(); ++c) pin_slots[c] = resolve_slot(pat, pin_idx[c], pin_idx[c]);
-        tri_slots.resize(9 * Tn);
-        for (size_t t = 0; t < Tn; ++t)
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b)
-                    tri_slots[9 * t + 3 * a + b] = resolve_slot(pat, tri_idx[3 * t + a], tri_idx[3 * t + b]);
+        for (TriSet* S : {&st, &sh}) {
+            const size_t Tn = S->idx.size() / 3;
+            S->slots.resize(9 * Tn);
+            for (size_t t = 0; t < Tn; ++t)
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        S->slots[9 * t + 3 * a + b] = resolve_slot(pat, S->idx[3 * t + a], S->idx[3 * t + b]);
+        }
+        dih_slots.resize(dih.size() * 4);
+        for (size_t d = 0; d < dih.size() / 4; ++d)
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                    dih_slots[16 * d + 4 * a + b] = resolve_slot(pat, dih[4 * d + a], dih[4 * d + b]);
         spr_slots.clear();
         if (springs_active) {
             spr_slots.resize(4 * spr_a.size());
@@ -345,11 +470,11 @@ struct Cloth {
     };
     // Shared prologue of triangle_gradient_kernel / triangle_hessian_scatter_kernel
     // (triangle_stretch_constraint.cu:153-196, 231-280).
-    TriLocal tri_local(size_t t, const std::vector<V3<T>>& x) const {
+    TriLocal tri_local(const TriSet& S, size_t t, const std::vector<V3<T>>& x) const {
         TriLocal L;
-        const V3<T> x0 = x[tri_idx[3 * t]], x1 = x[tri_idx[3 * t + 1]], x2 = x[tri_idx[3 * t + 2]];
-        const T D00 = tri_dminv[4 * t + 0], D01 = tri_dminv[4 * t + 1];
-        const T D10 = tri_dminv[4 * t + 2], D11 = tri_dminv[4 * t + 3];
+        const V3<T> x0 = x[S.idx[3 * t]], x1 = x[S.idx[3 * t + 1]], x2 = x[S.idx[3 * t + 2]];
+        const T D00 = S.dminv[4 * t + 0], D01 = S.dminv[4 * t + 1];
+        const T D10 = S.dminv[4 * t + 2], D11 = S.dminv[4 * t + 3];
         const V3<T> a = x1 - x0, b = x2 - x0;
         const V3<T> wu = a * D00 + b * D10;
         const V3<T> wv = a * D01 + b * D11;
@@ -364,7 +489,7 @@ struct Cloth {
         L.wvh = wv * L.inv_lv;
         L.dwu[0] = -D00 - D10; L.dwu[1] = D00; L.dwu[2] = D10;
         L.dwv[0] = -D01 - D11; L.dwv[1] = D01; L.dwv[2] = D11;
-        L.keff = tri_area[t] * tri_k[t];
+        L.keff = S.area[t] * S.k[t];
         return L;
     }
     V3<T> tri_grad(const TriLocal& L, int a) const {
@@ -561,13 +686,21 @@ struct Cloth {
                 rhs[spr_b[e]] = rhs[spr_b[e]] - ge;
             }
         }
-        const size_t Tn = tri_idx.size() / 3;
-        for (size_t t = 0; t < Tn; ++t) {
-            const TriLocal L = tri_local(t, xn);
-            for (int a = 0; a < 3; ++a) {
-                const int v = tri_idx[3 * t + a];
-                rhs[v] = rhs[v] + tri_grad(L, a);
+        for (const TriSet* S : {&st, &sh}) {  // stretch, then shear (:922-923)
+            const size_t Tn = S->idx.size() / 3;
+            for (size_t t = 0; t < Tn; ++t) {
+                const TriLocal L = tri_local(*S, t, xn);
+                for (int a = 0; a < 3; ++a) {
+                    const int v = S->idx[3 * t + a];
+                    rhs[v] = rhs[v] + tri_grad(L, a);
+                }
             }
+        }
+        for (size_t d = 0; d < dih.size() / 4; ++d) {  // bending_gradient_kernel (bending_constraint.cu:175-203)
+            const DihFrame F = dih_frame(d, xn);
+            if (!F.valid) continue;
+            const T coeff = bend_k * (F.theta - dih_rest[d]);
+            for (int i = 0; i < 4; ++i) rhs[dih[4 * d + i]] = rhs[dih[4 * d + i]] + F.a[i] * coeff;
         }
         // assemble_rhs_kernel (:127-142)
         const T inv_dt2 = T(1) / (dt * dt);
@@ -591,10 +724,20 @@ struct Cloth {
                 scatter(spr_slots[4 * e + 3], H);
             }
         }
-        for (size_t t = 0; t < Tn; ++t) {
-            const TriLocal L = tri_local(t, xn);
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b) scatter(tri_slots[9 * t + 3 * a + b], tri_hess(L, a, b));
+        for (const TriSet* S : {&st, &sh}) {
+            const size_t Tn = S->idx.size() / 3;
+            for (size_t t = 0; t < Tn; ++t) {
+                const TriLocal L = tri_local(*S, t, xn);
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) scatter(S->slots[9 * t + 3 * a + b], tri_hess(L, a, b));
+            }
+        }
+        for (size_t d = 0; d < dih.size() / 4; ++d) {  // bending_hessian_scatter_kernel (:214-246)
+            const DihFrame F = dih_frame(d, xn);
+            if (!F.valid) continue;
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j)
+                    scatter(dih_slots[16 * d + 4 * i + j], m3_scale(m3_outer(F.a[i], F.a[j]), bend_k));
         }
         for (int i = 0; i < N; ++i) {  // add_inertia_diag_kernel (:199-210)
             const T d = mass[i] * inv_dt2;
@@ -819,7 +962,7 @@ int orc_get_pattern(void* h, int* row_offsets, int* cols) {
 }
 int orc_get_slots(void* h, int* tri_slots, int* spr_slots, int* pin_slots) {
     return dispatch(h, [&](auto& c) {
-        if (tri_slots) std::copy(c.tri_slots.begin(), c.tri_slots.end(), tri_slots);
+        if (tri_slots) std::copy(c.st.slots.begin(), c.st.slots.end(), tri_slots);
         if (spr_slots) std::copy(c.spr_slots.begin(), c.spr_slots.end(), spr_slots);
         if (pin_slots) std::copy(c.pin_slots.begin(), c.pin_slots.end(), pin_slots);
     });
@@ -838,8 +981,38 @@ int orc_get_system(void* h, double* diag, double* values, double* rhs, double* d
 int orc_get_rest(void* h, double* spr_rest, double* tri_dminv, double* tri_area) {
     return dispatch(h, [&](auto& c) {
         if (spr_rest) for (size_t e = 0; e < c.spr_rest.size(); ++e) spr_rest[e] = double(c.spr_rest[e]);
-        if (tri_dminv) for (size_t i = 0; i < c.tri_dminv.size(); ++i) tri_dminv[i] = double(c.tri_dminv[i]);
-        if (tri_area) for (size_t i = 0; i < c.tri_area.size(); ++i) tri_area[i] = double(c.tri_area[i]);
+        if (tri_dminv) for (size_t i = 0; i < c.st.dminv.size(); ++i) tri_dminv[i] = double(c.st.dminv[i]);
+        if (tri_area) for (size_t i = 0; i < c.st.area.size(); ++i) tri_area[i] = double(c.st.area[i]);
+    });
+}
+int orc_build_shear(void* h, const double* pos, const double* k_per_tri, double k) {
+    return dispatch(h, [&](auto& c) {
+        using T = std::remove_reference_t<decltype(c.spr_k[0])>;
+        auto P = to_v3<T>(pos, c.n);
+        std::vector<T> kv;
+        if (k_per_tri) kv = to_vec<T>(k_per_tri, int(c.tris.size() / 3));
+        c.build_shear(P, k_per_tri ? kv.data() : nullptr, T(k));
+    });
+}
+int orc_build_bending(void* h, const double* pos, double k) {
+    return dispatch(h, [&](auto& c) {
+        using T = std::remove_reference_t<decltype(c.spr_k[0])>;
+        auto P = to_v3<T>(pos, c.n);
+        c.build_bending(P, T(k));
+    });
+}
+int orc_num_dihedrals(void* h) {
+    Handle* H = static_cast<Handle*>(h);
+    return int(H->precision == 32 ? H->f.dih.size() / 4 : H->d.dih.size() / 4);
+}
+// Dihedrals (4 per), rest angles, shear triangle slots (9 per), dihedral
+// slots (16 per); null pointers are skipped.
+int orc_get_bending(void* h, int* dih, double* rest, int* shear_slots, int* dih_slots) {
+    return dispatch(h, [&](auto& c) {
+        if (dih) std::copy(c.dih.begin(), c.dih.end(), dih);
+        if (rest) for (size_t d = 0; d < c.dih_rest.size(); ++d) rest[d] = double(c.dih_rest[d]);
+        if (shear_slots) std::copy(c.sh.slots.begin(), c.sh.slots.end(), shear_slots);
+        if (dih_slots) std::copy(c.dih_slots.begin(), c.dih_slots.end(), dih_slots);
     });
 }
 int orc_last_stats(void* h, int* iters, double* relres, int* converged, int* restarts) {
@@ -866,14 +1039,14 @@ void orc_spring_element(const double* x, double L, double k, double* grad_a, dou
 // Triangle element: rest positions xr (9) define Dm_inv/area; evaluate at x (9).
 // Outputs grad (9 = 3 vertices) and hess (81 = 3x3 blocks of 3x3, block (a,b)
 // at offset 27a + 9b), plus dminv (4) and area (1).
-void orc_triangle_element(const double* xr, const double* x, double k, double* grad, double* hess,
+void orc_triangle_element(const double* xr, const double* x, double k, float theta, double* grad, double* hess,
                           double* dminv, double* area) {
     Cloth<double> c;
     c.tris = {0, 1, 2};
     std::vector<V3<double>> R = {{xr[0], xr[1], xr[2]}, {xr[3], xr[4], xr[5]}, {xr[6], xr[7], xr[8]}};
-    c.build_stretch(R, nullptr, k);
+    c.build_tri_set(c.st, R, nullptr, k, theta);
     std::vector<V3<double>> X = {{x[0], x[1], x[2]}, {x[3], x[4], x[5]}, {x[6], x[7], x[8]}};
-    const auto L = c.tri_local(0, X);
+    const auto L = c.tri_local(c.st, 0, X);
     for (int a = 0; a < 3; ++a) {
         const V3<double> g = c.tri_grad(L, a);
         grad[3 * a] = g.x; grad[3 * a + 1] = g.y; grad[3 * a + 2] = g.z;
@@ -882,8 +1055,30 @@ void orc_triangle_element(const double* xr, const double* x, double k, double* g
             std::copy(h.m, h.m + 9, hess + 27 * a + 9 * b);
         }
     }
-    std::copy(c.tri_dminv.begin(), c.tri_dminv.end(), dminv);
-    *area = c.tri_area[0];
+    std::copy(c.st.dminv.begin(), c.st.dminv.end(), dminv);
+    *area = c.st.area[0];
+}
+// One dihedral (x: 4 x 3): angle theta, a_i = d theta / d x_i (4 x 3),
+// validity (compute_dihedral_frame, bending_constraint.cu:50-127).
+int orc_dihedral_element(const double* x, double* theta, double* a) {
+    std::vector<V3<double>> X(4);
+    for (int i = 0; i < 4; ++i) X[size_t(i)] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+    const auto F = Cloth<double>::dihedral_frame(X[0], X[1], X[2], X[3]);
+    *theta = F.theta;
+    for (int i = 0; i < 4; ++i) {
+        a[3 * i] = F.valid ? F.a[i].x : 0.0;
+        a[3 * i + 1] = F.valid ? F.a[i].y : 0.0;
+        a[3 * i + 2] = F.valid ? F.a[i].z : 0.0;
+    }
+    return F.valid ? 1 : 0;
+}
+// Dihedrals of a triangle list (4 per), sorted by hinge; returns the count
+// and fills `out` when it is not null.
+int orc_find_dihedrals(const int* tris, int ntri, int* out) {
+    const std::vector<int> t(tris, tris + 3 * size_t(ntri));
+    const std::vector<int> d = Cloth<double>::find_dihedrals(t);
+    if (out) std::copy(d.begin(), d.end(), out);
+    return int(d.size() / 4);
 }
 void orc_mat3_inverse(const double* m, double* out) {
     M3<double> a;

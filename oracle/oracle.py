@@ -52,7 +52,13 @@ def lib():
         L.orc_get_rest.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_last_stats.argtypes = [C.c_void_p, _ip, _dp, _ip, _ip]
         L.orc_spring_element.argtypes = [_dp, C.c_double, C.c_double, _dp, _dp]
-        L.orc_triangle_element.argtypes = [_dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_triangle_element.argtypes = [_dp, _dp, C.c_double, C.c_float, _dp, _dp, _dp, _dp]
+        L.orc_build_shear.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
+        L.orc_build_bending.argtypes = [C.c_void_p, _dp, C.c_double]
+        L.orc_num_dihedrals.argtypes = [C.c_void_p]
+        L.orc_get_bending.argtypes = [C.c_void_p, _ip, _dp, _ip, _ip]
+        L.orc_dihedral_element.argtypes = [_dp, _dp, _dp]
+        L.orc_find_dihedrals.argtypes = [_ip, C.c_int, _ip]
         L.orc_mat3_inverse.argtypes = [_dp, _dp]
         L.orc_pcg_solve.argtypes = [C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, _ip, _dp]
         L.orc_time_steps.restype = C.c_double
@@ -124,6 +130,32 @@ class OracleCloth:
         else:
             kk = _f64(k)
             _check(self._L.orc_build_stretch(self._h, _d(p), _d(kk), 0.0))
+
+    def build_shear(self, pos, k):
+        """FEM shear: the stretch element with the material axes rotated by pi/4."""
+        p = _f64(pos)
+        if np.ndim(k) == 0:
+            _check(self._L.orc_build_shear(self._h, _d(p), None, float(k)))
+        else:
+            kk = _f64(k)
+            _check(self._L.orc_build_shear(self._h, _d(p), _d(kk), 0.0))
+
+    def build_bending(self, pos, k):
+        _check(self._L.orc_build_bending(self._h, _d(_f64(pos)), float(k)))
+
+    def num_dihedrals(self):
+        return int(self._L.orc_num_dihedrals(self._h))
+
+    def bending(self, ntri):
+        """(dihedrals (D,4), rest angles (D,), shear slots (T,9), dihedral slots (D,16));
+        slots need build_topology() first."""
+        nd = self.num_dihedrals()
+        d = np.zeros((nd, 4), np.int32)
+        r = np.zeros(nd)
+        ss = np.zeros((ntri, 9), np.int32)
+        ds = np.zeros((nd, 16), np.int32)
+        _check(self._L.orc_get_bending(self._h, _i(d), _d(r), _i(ss), _i(ds)))
+        return d, r, ss, ds
 
     def set_pins(self, idx, targets, k=1.0e6):
         i = np.ascontiguousarray(idx, np.int32)
@@ -230,6 +262,10 @@ def oracle_from_scene(scene, precision=64, springs_active=True, force_active=Tru
     vel = np.ascontiguousarray(scene.vel, np.float64)
     oc.build_springs(pos, scene.spring_k)
     oc.build_stretch(pos, scene.stretch_k)
+    if getattr(scene, "shear_k", None) is not None:
+        oc.build_shear(pos, scene.shear_k)
+    if getattr(scene, "bend_k", None) is not None:
+        oc.build_bending(pos, scene.bend_k)
     oc.set_pins(scene.pin_idx, scene.pin_targets, scene.pin_k)
     oc.set_params(scene.gravity, scene.damping, springs_active, force_active and scene.force is not None,
                   tol, max_iter, threads)
@@ -246,15 +282,37 @@ def spring_element(xa, xb, rest, k):
     return g, H.reshape(3, 3)
 
 
-def triangle_element(x_rest, x, k):
+def triangle_element(x_rest, x, k, theta=0.0):
+    """Stretch (theta = 0) or shear (theta = pi/4) triangle: gradient (3,3),
+    Hessian blocks (3,3,3,3) [a][b], D_m^-1 R^-1 (2,2) and rest area."""
     xr = np.ascontiguousarray(x_rest, np.float64).reshape(9)
     xx = np.ascontiguousarray(x, np.float64).reshape(9)
     g = np.zeros(9)
     H = np.zeros(81)
     dm = np.zeros(4)
     ar = np.zeros(1)
-    lib().orc_triangle_element(_d(xr), _d(xx), float(k), _d(g), _d(H), _d(dm), _d(ar))
+    lib().orc_triangle_element(_d(xr), _d(xx), float(k), float(theta), _d(g), _d(H), _d(dm), _d(ar))
     return g.reshape(3, 3), H.reshape(3, 3, 3, 3), dm.reshape(2, 2), float(ar[0])
+
+
+SHEAR_THETA = 0.7853981633974483  # pi/4, cloth_simulator.cu:428
+
+
+def dihedral_element(x):
+    """(theta, a (4,3), valid) of the dihedral x (4,3): v0, v1 hinge, v2, v3 wings."""
+    xx = np.ascontiguousarray(x, np.float64).reshape(12)
+    th = np.zeros(1)
+    a = np.zeros(12)
+    ok = lib().orc_dihedral_element(_d(xx), _d(th), _d(a))
+    return float(th[0]), a.reshape(4, 3), bool(ok)
+
+
+def find_dihedrals(tris):
+    t = np.ascontiguousarray(tris, np.int32)
+    n = lib().orc_find_dihedrals(_i(t), t.shape[0], None)
+    out = np.zeros((n, 4), np.int32)
+    lib().orc_find_dihedrals(_i(t), t.shape[0], _i(out))
+    return out
 
 
 def mat3_inverse(m):
