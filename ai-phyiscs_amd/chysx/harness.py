@@ -71,6 +71,10 @@ def make_sim(scene: Scene, tol: float = 1e-6, max_iter: int = 10000, springs_act
     sim.build_fem_stretch_from_current_positions(float(k_t) if np.ndim(k_t) == 0 else 1.0)
     if np.ndim(k_t) != 0:
         sim.set_fem_stretch_stiffnesses(np.asarray(k_t, np.float32))
+    if scene.shear_k is not None:
+        sim.build_fem_shear_from_current_positions(float(scene.shear_k))
+    if scene.bend_k is not None:
+        sim.build_bending_from_current_positions(float(scene.bend_k))
     if scene.pin_idx.size:
         sim.set_pins(scene.pin_idx, scene.pin_targets, scene.pin_k)
     sim.set_springs_active(springs_active)

@@ -193,6 +193,27 @@ def config5_train(count: int = 16, w: int = 256, seed: int = 5) -> List[Scene]:
     return out
 
 
+def drape(w: int = 21, size: float = 0.8, seed: Optional[int] = None) -> Scene:
+    """The reference demo's cloth (scenes/cloth_freefall.cu:36-110): W x W grid
+    of side 0.8 m, FEM stretch k = 5000, shear k = 500, bending k = 0.01,
+    area-lumped mass of density 0.1 kg/m^2, damping 2, the two top corners
+    pinned; springs are built but, as in the reference step, not active.
+    With a seed the rest shape is crumpled out of plane (+-30% of the grid
+    spacing), so rest dihedral angles are non-zero.  dt = 1/60."""
+    pos, tris = grid(w, size)
+    if seed is not None:
+        h = size / (w - 1)
+        rng = np.random.default_rng(seed)
+        pos = pos.copy()
+        pos[:, 1] += rng.uniform(-0.3 * h, 0.3 * h, pos.shape[0]).astype(np.float32)
+    mass = 0.1 * vertex_areas(pos, tris)
+    inv_mass = (1.0 / mass).astype(np.float32)
+    idx, tgt = _pins(w, "two_corners", pos)
+    return Scene(f"drape_{w}x{w}" + ("" if seed is None else f"_crumpled{seed}"), pos, np.zeros_like(pos), inv_mass,
+                 tris, idx, tgt, 1.0e6, 5000.0, 5000.0, None, 1.0 / 60.0, 100, damping=2.0, grid_w=w,
+                 shear_k=500.0, bend_k=0.01)
+
+
 def freefall(w: int = 6) -> Scene:
     """Unpinned cloth with no elastic terms active beyond rest state: every
     particle follows x_n = x_0 + dt^2 g n(n+1)/2 exactly."""

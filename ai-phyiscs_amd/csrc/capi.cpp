@@ -129,6 +129,42 @@ int chysx_cloth_build_fem_stretch(chysx_cloth* sim, float k, uintptr_t stream) {
         sim->sim.build_fem_stretch(k, stream);
     });
 }
+int chysx_cloth_build_fem_shear(chysx_cloth* sim, float k, uintptr_t stream) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.build_fem_shear(k, stream);
+    });
+}
+int chysx_cloth_num_fem_shear(const chysx_cloth* sim, int* out) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        *out = sim->sim.num_fem_shear();
+    });
+}
+int chysx_cloth_build_bending(chysx_cloth* sim, float k, uintptr_t stream) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.build_bending(k, stream);
+    });
+}
+int chysx_cloth_num_bending(const chysx_cloth* sim, int* out) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        *out = sim->sim.num_bending();
+    });
+}
+int chysx_cloth_get_dihedrals(const chysx_cloth* sim, int32_t* dihedrals, float* rest_angles) {
+    return guard([&] {
+        need(sim, "sim");
+        const Instance& I = sim->sim.instance();
+        if (!I.has_bending) return;
+        if (dihedrals && !I.dih.empty()) std::memcpy(dihedrals, I.dih.data(), I.dih.size() * sizeof(int));
+        if (rest_angles && !I.dih_rest.empty())
+            std::memcpy(rest_angles, I.dih_rest.data(), I.dih_rest.size() * sizeof(float));
+    });
+}
 int chysx_cloth_num_fem_stretch(const chysx_cloth* sim, int* out) {
     return guard([&] {
         need(sim, "sim");
@@ -335,6 +371,26 @@ int chysx_topology_query(int n, const int32_t* tris, int ntri, int include_sprin
     });
 }
 
+int chysx_topology_query_bending(int n, const int32_t* tris, int ntri, int* nnz, int* ndih, int32_t* row_offsets,
+                                 int32_t* col_indices, int32_t* dihedrals, int32_t* dihedral_slots) {
+    return guard([&] {
+        if (n < 0 || ntri < 0) throw std::invalid_argument("topology_query: negative size");
+        if (ntri > 0) need(tris, "triangles");
+        for (int q = 0; q < 3 * ntri; ++q)
+            if (tris[q] < 0 || tris[q] >= n) throw std::out_of_range("topology_query: vertex index out of range");
+        const std::vector<int> d = find_dihedrals(tris, ntri);
+        const int D = int(d.size() / 4);
+        const Pattern p = build_pattern(n, tris, ntri, nullptr, 0, D ? d.data() : nullptr, D);
+        const std::vector<int> ds = build_dihedral_slots(p, D ? d.data() : nullptr, D);
+        if (nnz) *nnz = int(p.cols.size());
+        if (ndih) *ndih = D;
+        if (row_offsets) std::memcpy(row_offsets, p.row_offsets.data(), p.row_offsets.size() * sizeof(int));
+        if (col_indices && !p.cols.empty()) std::memcpy(col_indices, p.cols.data(), p.cols.size() * sizeof(int));
+        if (dihedrals && !d.empty()) std::memcpy(dihedrals, d.data(), d.size() * sizeof(int));
+        if (dihedral_slots && !ds.empty()) std::memcpy(dihedral_slots, ds.data(), ds.size() * sizeof(int));
+    });
+}
+
 int chysx_batch_create(chysx_batch** out) {
     return guard([&] {
         need(out, "out");
@@ -437,7 +493,7 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
                     case 0: launch_spmv(g, A, v.p0, v.ap, c.spmv_waves, s); break;
                     case 1: launch_pcg_spmv(g, A, v, P.states(), c.spmv_waves, c.spmv_unroll, s); break;
                     case 2: launch_pcg_update(g, A, v, P.states(), c.update_waves, s); break;
-                    case 3: launch_assemble(g, P.mesh(), A, v, ext, P.params(), dt, c.assemble_waves, s); break;
+                    case 3: launch_assemble(g, P.mesh(), A, v, ext, P.params(), dt, c.assemble_waves, P.max_width(), s); break;
                     case 4: P.launch_pcg_iteration(s); break;
                 }
             }

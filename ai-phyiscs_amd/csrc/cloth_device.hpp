@@ -68,6 +68,9 @@ struct alignas(16) InstParams {
     float pin_k;
     int force_active;   // 1: external force enters the predictor
     int springs_active; // 1: spring gradient/Hessian enter the step
+    int tri_stretch;    // 1: stretch triangles installed
+    int tri_shear;      // 1: shear triangles installed
+    float bend_k;       // bending stiffness (dihedrals only exist when installed)
 };
 
 }  // namespace chysx_amd

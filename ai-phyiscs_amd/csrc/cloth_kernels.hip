@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -29,6 +30,7 @@ namespace {
 constexpr float kMinLength = 1.0e-6f;  // spring_constraint.cu:35
 constexpr float kMinNorm = 1.0e-6f;    // triangle_stretch_constraint.cu:39
 constexpr double kGuard = 1.0e-37;     // pcg_solver.cu:421-423,451
+constexpr size_t kAssembleLdsBytes = 64 * 1024;
 
 // Slice owned by this wavefront (>= nslices: nothing to do).
 template <int WAVES>
@@ -144,29 +146,152 @@ __global__ void area_mass_kernel(const int* __restrict__ vt_off, const int* __re
 // ---------------------------------------------------------------------------
 // Assembly: one lane per block row.  The lane computes the row's right-hand
 // side and every Hessian block of its row from the elements incident to it
-// (pins, springs, stretch triangles, inertia), writes the diagonal and its
-// symmetric inverse, and accumulates off-diagonal blocks into its own ELL
-// entries.  No atomics; the incidence order is fixed, so the result is
-// deterministic.
+// (pins, springs, stretch and shear triangles, bending dihedrals, inertia),
+// writes the diagonal and its symmetric inverse, and accumulates off-diagonal
+// blocks for its own ELL entries.  No atomics; the incidence order is fixed,
+// so the result is deterministic.
 //
 // Element formulas restate:
 //   pins      pin_constraint.cu:61-92
 //   springs   spring_constraint.cu:76-151
-//   stretch   triangle_stretch_constraint.cu:153-311
+//   stretch   triangle_stretch_constraint.cu:153-311 (shear: the same with
+//             the pi/4-rotated frame, cloth_simulator.cu:423-449)
+//   bending   bending_constraint.cu:50-246
 //   inertia   cloth_simulator.cu:199-210
 //   rhs       cloth_simulator.cu:83-142, written as m*(v/dt + g + f/m) - grad,
 //             the same quantity as (m/dt^2)(x_tilde - x_n) - grad without the
 //             FP32 cancellation of x_tilde - x_n.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void add_block(float* __restrict__ vrow, int k, const float (&H)[9], float sgn) {
+__device__ __forceinline__ void add_block(float* __restrict__ acc, int k, const float (&H)[9], float sgn) {
 #pragma unroll
-    for (int c = 0; c < 9; ++c) vrow[(k * 9 + c) * 64] += sgn * H[c];
+    for (int c = 0; c < 9; ++c) acc[(k * 9 + c) * 64] += sgn * H[c];
 }
 
-template <int WAVES>
+// One Baraff-Witkin triangle (stretch or shear, by its D_m^-1) seen from its
+// vertex a: gradient into `grad`, the three blocks H_ab into D (b == a or a
+// repeated vertex) or the row accumulator (triangle_gradient_kernel and
+// triangle_hessian_scatter_kernel, triangle_stretch_constraint.cu:153-311).
+__device__ __forceinline__ void tri_element(const float4 Dm, float keff, int a, F3 x0, F3 x1, F3 x2, int packed,
+                                            F3& grad, float (&D)[9], float* __restrict__ acc) {
+    const F3 ea = x1 - x0, eb = x2 - x0;
+    const F3 wu = ea * Dm.x + eb * Dm.z;
+    const F3 wv = ea * Dm.y + eb * Dm.w;
+    float lu = sqrtf(dot(wu, wu));
+    float lv = sqrtf(dot(wv, wv));
+    if (lu < kMinNorm) lu = kMinNorm;
+    if (lv < kMinNorm) lv = kMinNorm;
+    const float inv_lu = 1.0f / lu, inv_lv = 1.0f / lv;
+    const float cu = lu - 1.0f, cv = lv - 1.0f;
+    const F3 uh = wu * inv_lu, vh = wv * inv_lv;
+    const float dwu[3] = {-Dm.x - Dm.z, Dm.x, Dm.z};
+    const float dwv[3] = {-Dm.y - Dm.w, Dm.y, Dm.w};
+    grad = grad + (uh * (cu * dwu[a]) + vh * (cv * dwv[a])) * keff;
+    // H_ab = dwu_a dwu_b Mu + dwv_a dwv_b Mv with
+    // Mu = keff uh uh^T + [cu > 0] keff cu / lu (I - uh uh^T), Mv likewise.
+    const float su = (cu > 0.0f) ? keff * cu * inv_lu : 0.0f;
+    const float sv = (cv > 0.0f) ? keff * cv * inv_lv : 0.0f;
+    const float qu = keff - su, qv = keff - sv;
+    float Mu[6], Mv[6];
+    Mu[0] = qu * uh.x * uh.x + su;
+    Mu[1] = qu * uh.x * uh.y;
+    Mu[2] = qu * uh.x * uh.z;
+    Mu[3] = qu * uh.y * uh.y + su;
+    Mu[4] = qu * uh.y * uh.z;
+    Mu[5] = qu * uh.z * uh.z + su;
+    Mv[0] = qv * vh.x * vh.x + sv;
+    Mv[1] = qv * vh.x * vh.y;
+    Mv[2] = qv * vh.x * vh.z;
+    Mv[3] = qv * vh.y * vh.y + sv;
+    Mv[4] = qv * vh.y * vh.z;
+    Mv[5] = qv * vh.z * vh.z + sv;
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) {
+        const float cu_ab = dwu[a] * dwu[bb], cv_ab = dwv[a] * dwv[bb];
+        float H[9];
+        H[0] = cu_ab * Mu[0] + cv_ab * Mv[0];
+        H[1] = cu_ab * Mu[1] + cv_ab * Mv[1];
+        H[2] = cu_ab * Mu[2] + cv_ab * Mv[2];
+        H[4] = cu_ab * Mu[3] + cv_ab * Mv[3];
+        H[5] = cu_ab * Mu[4] + cv_ab * Mv[4];
+        H[8] = cu_ab * Mu[5] + cv_ab * Mv[5];
+        H[3] = H[1];
+        H[6] = H[2];
+        H[7] = H[5];
+        const int kb = (packed >> (10 * bb)) & 1023;
+        if (bb == a || kb == 1023) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) D[c] += H[c];
+        } else {
+            add_block(acc, kb, H, 1.0f);
+        }
+    }
+}
+
+// compute_dihedral_frame (bending_constraint.cu:50-127): dihedral angle theta
+// and a_i = d theta / d x_i; invalid for a collapsed hinge or a wing with no
+// height (floor 1e-12).
+struct DihFrame {
+    float theta;
+    F3 a[4];
+    bool valid;
+};
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ DihFrame dihedral_frame(F3 v0, F3 v1, F3 v2, F3 v3) {
+    constexpr float kFloor = 1.0e-12f;
+    DihFrame F;
+    F.valid = false;
+    F.theta = 0.0f;
+    const F3 e10 = v1 - v0, e20 = v2 - v0, e30 = v3 - v0;
+    const float l = sqrtf(dot(e10, e10));
+    if (l < kFloor) return F;
+    const float inv_l = 1.0f / l;
+    const F3 e10h = e10 * inv_l;
+    const F3 n1r = cross(e20, e10), n2r = cross(e10, e30);
+    const float n1l = sqrtf(dot(n1r, n1r)), n2l = sqrtf(dot(n2r, n2r));
+    if (n1l < kFloor || n2l < kFloor) return F;
+    const F3 n1 = n1r * (1.0f / n1l), n2 = n2r * (1.0f / n2l);
+    float sin_t = dot(cross(n1, n2), e10h);
+    sin_t = fminf(fmaxf(sin_t, -1.0f), 1.0f);
+    const float cos_t = dot(n1, n2);
+    F.theta = atan2f(sin_t, cos_t);
+    const float inv_l2 = inv_l * inv_l;
+    const float om1 = dot(e10, e20) * inv_l2, om2 = dot(e10, e30) * inv_l2;
+    const F3 p1 = e20 - e10 * om1, p2 = e30 - e10 * om2;
+    const float h1 = sqrtf(dot(p1, p1)), h2 = sqrtf(dot(p2, p2));
+    if (h1 < kFloor || h2 < kFloor) return F;
+    const float ih1 = 1.0f / h1, ih2 = 1.0f / h2;
+    const float t1[4] = {(om1 - 1.0f) * ih1, -om1 * ih1, ih1, 0.0f};
+    const float t2[4] = {(om2 - 1.0f) * ih2, -om2 * ih2, 0.0f, ih2};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) F.a[i] = n1 * t1[i] + n2 * t2[i];
+    F.valid = true;
+    return F;
+}
+
+// bending_rest_angle_kernel (bending_constraint.cu:136-152).
+__global__ void rest_angle_kernel(const int4* __restrict__ dih, const float* __restrict__ pos,
+                                  float* __restrict__ rest, int n) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n) return;
+    const int4 q = dih[d];
+    const DihFrame F = dihedral_frame(ld3(pos, q.x), ld3(pos, q.y), ld3(pos, q.z), ld3(pos, q.w));
+    rest[d] = F.valid ? F.theta : 0.0f;
+}
+
+// Row-owned assembly: the lane of row i evaluates every element touching i
+// (pins, springs, stretch and shear triangles, bending dihedrals) in a fixed
+// incidence order and adds the row's blocks into an accumulator with the
+// CSR-slot layout of its slice, stride 64 floats.  With LDS = true the
+// accumulator is the wave's slab of LDS (maxw blocks per lane) and each
+// block is written to HBM once at the end; otherwise it is the matrix row
+// itself (zeroed first).  No atomics: every slot has one owner lane.
+template <int WAVES, bool LDS>
 __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevMesh m, DevMatrix A, DevVectors v,
                                                              DevExt e, const InstParams* __restrict__ params,
-                                                             float dt) {
+                                                             float dt, int maxw) {
+    extern __shared__ float lds_acc[];
     const int s = my_slice<WAVES>(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
@@ -178,9 +303,10 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
     const int sb = A.slice_ptr[s];
     const int deg = A.row_deg[row];
     float* __restrict__ vrow = A.val + (size_t)sb * 9 * 64 + lane;
+    float* __restrict__ acc = LDS ? lds_acc + (size_t)(threadIdx.x >> 6) * maxw * 9 * 64 + lane : vrow;
     for (int k = 0; k < deg; ++k)
 #pragma unroll
-        for (int c = 0; c < 9; ++c) vrow[(k * 9 + c) * 64] = 0.0f;
+        for (int c = 0; c < 9; ++c) acc[(k * 9 + c) * 64] = 0.0f;
 
     const F3 xi = ld3(e.pos, p);
     const F3 vi = ld3(e.vel, p);
@@ -190,13 +316,13 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
         const float w = e.inv_mass[p];
         mass = (w > 1.0e-12f) ? (1.0f / w) : 1.0e8f;
     }
-    F3 acc = f3(pr.gx, pr.gy, pr.gz);
+    F3 ac = f3(pr.gx, pr.gy, pr.gz);
     if (pr.force_active && e.force != nullptr) {
         const float im = (mass > 0.0f) ? (1.0f / mass) : 0.0f;
-        acc = acc + ld3(e.force, p) * im;
+        ac = ac + ld3(e.force, p) * im;
     }
     const float inv_dt = 1.0f / dt;
-    const F3 b = (vi * inv_dt + acc) * mass;
+    const F3 b = (vi * inv_dt + ac) * mass;
 
     float D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     F3 grad = f3(0.0f, 0.0f, 0.0f);
@@ -243,72 +369,59 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
 #pragma unroll
                 for (int c = 0; c < 9; ++c) D[c] -= H[c];
             } else {
-                add_block(vrow, ko, H, -1.0f);
+                add_block(acc, ko, H, -1.0f);
             }
         }
     }
-    // stretch triangles
+    // stretch, then shear triangles (cloth_simulator.cu:922-923, 1136-1137)
     for (int q = m.tri_inc_off[row]; q < m.tri_inc_off[row + 1]; ++q) {
         const int2 ent = m.tri_inc[q];
         const int ti = ent.x >> 2, a = ent.x & 3;
         const F3 x0 = ld3(e.pos, m.tri_v[3 * ti]);
         const F3 x1 = ld3(e.pos, m.tri_v[3 * ti + 1]);
         const F3 x2 = ld3(e.pos, m.tri_v[3 * ti + 2]);
-        const float4 Dm = m.tri_dminv[ti];
-        const F3 ea = x1 - x0, eb = x2 - x0;
-        const F3 wu = ea * Dm.x + eb * Dm.z;
-        const F3 wv = ea * Dm.y + eb * Dm.w;
-        float lu = sqrtf(dot(wu, wu));
-        float lv = sqrtf(dot(wv, wv));
-        if (lu < kMinNorm) lu = kMinNorm;
-        if (lv < kMinNorm) lv = kMinNorm;
-        const float inv_lu = 1.0f / lu, inv_lv = 1.0f / lv;
-        const float cu = lu - 1.0f, cv = lv - 1.0f;
-        const F3 uh = wu * inv_lu, vh = wv * inv_lv;
-        const float dwu[3] = {-Dm.x - Dm.z, Dm.x, Dm.z};
-        const float dwv[3] = {-Dm.y - Dm.w, Dm.y, Dm.w};
-        const float keff = m.tri_area[ti] * m.tri_k[ti];
-        grad = grad + (uh * (cu * dwu[a]) + vh * (cv * dwv[a])) * keff;
-        // H_ab = dwu_a dwu_b Mu + dwv_a dwv_b Mv with
-        // Mu = keff uh uh^T + [cu > 0] keff cu / lu (I - uh uh^T), Mv likewise.
-        const float su = (cu > 0.0f) ? keff * cu * inv_lu : 0.0f;
-        const float sv = (cv > 0.0f) ? keff * cv * inv_lv : 0.0f;
-        const float qu = keff - su, qv = keff - sv;
-        float Mu[6], Mv[6];
-        Mu[0] = qu * uh.x * uh.x + su;
-        Mu[1] = qu * uh.x * uh.y;
-        Mu[2] = qu * uh.x * uh.z;
-        Mu[3] = qu * uh.y * uh.y + su;
-        Mu[4] = qu * uh.y * uh.z;
-        Mu[5] = qu * uh.z * uh.z + su;
-        Mv[0] = qv * vh.x * vh.x + sv;
-        Mv[1] = qv * vh.x * vh.y;
-        Mv[2] = qv * vh.x * vh.z;
-        Mv[3] = qv * vh.y * vh.y + sv;
-        Mv[4] = qv * vh.y * vh.z;
-        Mv[5] = qv * vh.z * vh.z + sv;
-        const int packed = ent.y;
+        if (pr.tri_stretch)
+            tri_element(m.tri_dminv[ti], m.tri_area[ti] * m.tri_k[ti], a, x0, x1, x2, ent.y, grad, D, acc);
+        if (pr.tri_shear)
+            tri_element(m.sh_dminv[ti], m.sh_area[ti] * m.sh_k[ti], a, x0, x1, x2, ent.y, grad, D, acc);
+    }
+    // bending dihedrals: gradient k (theta - theta_rest) a_i, Gauss-Newton
+    // blocks k a_i a_j^T (bending_constraint.cu:175-246)
+    for (int q = m.bend_inc_off[row]; q < m.bend_inc_off[row + 1]; ++q) {
+        const int4 ent = m.bend_inc[q];
+        const int di = ent.x >> 2, a = ent.x & 3;
+        const int4 dv = m.dih_v[di];
+        const DihFrame F = dihedral_frame(ld3(e.pos, dv.x), ld3(e.pos, dv.y), ld3(e.pos, dv.z), ld3(e.pos, dv.w));
+        if (!F.valid) continue;
+        const float kb = pr.bend_k;
+        grad = grad + F.a[a] * (kb * (F.theta - m.dih_rest[di]));
+        const F3 ka = F.a[a] * kb;
+        const int kk[4] = {ent.y & 0xFFFF, (ent.y >> 16) & 0xFFFF, ent.z & 0xFFFF, (ent.z >> 16) & 0xFFFF};
 #pragma unroll
-        for (int bb = 0; bb < 3; ++bb) {
-            const float cu_ab = dwu[a] * dwu[bb], cv_ab = dwv[a] * dwv[bb];
+        for (int bb = 0; bb < 4; ++bb) {
+            const F3 ab = F.a[bb];
             float H[9];
-            H[0] = cu_ab * Mu[0] + cv_ab * Mv[0];
-            H[1] = cu_ab * Mu[1] + cv_ab * Mv[1];
-            H[2] = cu_ab * Mu[2] + cv_ab * Mv[2];
-            H[4] = cu_ab * Mu[3] + cv_ab * Mv[3];
-            H[5] = cu_ab * Mu[4] + cv_ab * Mv[4];
-            H[8] = cu_ab * Mu[5] + cv_ab * Mv[5];
-            H[3] = H[1];
-            H[6] = H[2];
-            H[7] = H[5];
-            const int kb = (packed >> (10 * bb)) & 1023;
-            if (bb == a || kb == 1023) {
+            H[0] = ka.x * ab.x;
+            H[1] = ka.x * ab.y;
+            H[2] = ka.x * ab.z;
+            H[3] = ka.y * ab.x;
+            H[4] = ka.y * ab.y;
+            H[5] = ka.y * ab.z;
+            H[6] = ka.z * ab.x;
+            H[7] = ka.z * ab.y;
+            H[8] = ka.z * ab.z;
+            if (bb == a || kk[bb] == 0xFFFF) {
 #pragma unroll
                 for (int c = 0; c < 9; ++c) D[c] += H[c];
             } else {
-                add_block(vrow, kb, H, 1.0f);
+                add_block(acc, kk[bb], H, 1.0f);
             }
         }
+    }
+    if (LDS) {
+        for (int k = 0; k < deg; ++k)
+#pragma unroll
+            for (int c = 0; c < 9; ++c) vrow[(k * 9 + c) * 64] = acc[(k * 9 + c) * 64];
     }
     // inertia (add_inertia_diag_kernel, cloth_simulator.cu:199-210)
     const float mi = mass * (1.0f / (dt * dt));
@@ -722,11 +835,28 @@ void launch_area_mass(const int* vt_off, const int* vt_ids, const int* tri_v, co
     CHYSX_LAUNCH_CHECK();
 }
 void launch_assemble(const LaunchGeom& g, const DevMesh& m, const DevMatrix& A, const DevVectors& v,
-                     const DevExt& e, const InstParams* params, float dt, int waves, hipStream_t s) {
+                     const DevExt& e, const InstParams* params, float dt, int waves, int maxw, hipStream_t s) {
     if (g.nslices <= 0) return;
-#define L(W) assemble_kernel<W><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, m, A, v, e, params, dt)
-    CHYSX_WAVES_SWITCH(waves, L)
+    // LDS accumulator: waves * maxw blocks of 9 x 64 floats per workgroup,
+    // kept within 64 KiB by lowering the waves per workgroup; rows wider
+    // than 28 blocks accumulate in the matrix itself.
+    const size_t slab = size_t(std::max(maxw, 1)) * 9 * 64 * sizeof(float);
+    if (slab * 1 <= kAssembleLdsBytes) {
+        while (waves > 1 && slab * size_t(waves) > kAssembleLdsBytes) waves >>= 1;
+        const size_t bytes = slab * size_t(waves);
+#define L(W) assemble_kernel<W, true><<<grid_for(g.nslices, W), 64 * W, bytes, s>>>(g, m, A, v, e, params, dt, maxw)
+        CHYSX_WAVES_SWITCH(waves, L)
 #undef L
+    } else {
+#define L(W) assemble_kernel<W, false><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, m, A, v, e, params, dt, maxw)
+        CHYSX_WAVES_SWITCH(waves, L)
+#undef L
+    }
+    CHYSX_LAUNCH_CHECK();
+}
+void launch_rest_angles(const int4* dih, const float* pos, float* rest, int n, hipStream_t s) {
+    if (n <= 0) return;
+    rest_angle_kernel<<<(n + 255) / 256, 256, 0, s>>>(dih, pos, rest, n);
     CHYSX_LAUNCH_CHECK();
 }
 void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,

@@ -27,6 +27,15 @@ struct DevMesh {
     const int2* spr_inc;
     const int* pin_inc_off;
     const int* pin_inc;
+    // shear triangles (same vertices and incidence as tri_*)
+    const float4* sh_dminv;
+    const float* sh_area;
+    const float* sh_k;
+    // bending dihedrals and their row incidence
+    const int4* dih_v;
+    const float* dih_rest;
+    const int* bend_inc_off;
+    const int4* bend_inc;
 };
 
 // Wave-sliced ELL matrix: entry (s, k, lane) of component c lives at
@@ -75,8 +84,11 @@ bool valid_unroll(int unroll);
 void launch_rest_lengths(const int* spr_v, const float* pos, float* rest, int nspring, hipStream_t s);
 void launch_rest_shapes(const int* tri_v, const float* pos, float4* dminv, float* area, int ntri, float cos_t,
                         float sin_t, hipStream_t s);
+// maxw: widest slice (blocks per row) of the problem, sizes the LDS accumulator.
 void launch_assemble(const LaunchGeom& g, const DevMesh& m, const DevMatrix& A, const DevVectors& v,
-                     const DevExt& e, const InstParams* params, float dt, int waves, hipStream_t s);
+                     const DevExt& e, const InstParams* params, float dt, int waves, int maxw, hipStream_t s);
+// Rest dihedral angles (bending_rest_angle_kernel).
+void launch_rest_angles(const int4* dih, const float* pos, float* rest, int n, hipStream_t s);
 // Residual pass: report == 0 starts a solve (true residual, r, z, p = z);
 // report == 1 only measures the true residual of stopped instances.
 void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,

@@ -50,6 +50,7 @@ template class DeviceArray<int>;
 template class DeviceArray<int2>;
 template class DeviceArray<float>;
 template class DeviceArray<float4>;
+template class DeviceArray<int4>;
 template class DeviceArray<double>;
 template class DeviceArray<SliceDesc>;
 template class DeviceArray<InstState>;
@@ -102,6 +103,56 @@ void compute_rest_stretch(Instance& inst, const float* pos, const float* k_per_t
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(rest shapes)");
 }
 
+// FEM shear (build_fem_shear_from_current_positions, cloth_simulator.cu:423-449):
+// the stretch rest frame with the material axes rotated by pi/4; cos and sin
+// are the FP32 values of the FP32 angle (triangle_stretch_constraint.cu:340-341).
+void compute_rest_shear(Instance& inst, const float* pos, const float* k_per_tri, float k, hipStream_t s) {
+    const int T = int(inst.tris.size() / 3);
+    inst.sh_dminv.assign(size_t(4) * T, 0.0f);
+    inst.sh_area.assign(size_t(T), 0.0f);
+    inst.sh_k.assign(size_t(T), k);
+    if (k_per_tri) std::copy(k_per_tri, k_per_tri + T, inst.sh_k.begin());
+    inst.has_shear = true;
+    if (T == 0) return;
+    constexpr float kQuarterPi = 0.7853981633974483f;
+    std::vector<int> v(inst.tris);
+    for (int& i : v) i += inst.part0;
+    DeviceArray<int> dv;
+    DeviceArray<float4> dm;
+    DeviceArray<float> ar;
+    dv.assign(v.data(), v.size(), s);
+    dm.resize(size_t(T));
+    ar.resize(size_t(T));
+    launch_rest_shapes(dv.data(), pos, dm.data(), ar.data(), T, std::cos(kQuarterPi), std::sin(kQuarterPi), s);
+    check_hip(hipMemcpyAsync(inst.sh_dminv.data(), dm.data(), sizeof(float4) * T, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(shear rest shapes)");
+    check_hip(hipMemcpyAsync(inst.sh_area.data(), ar.data(), sizeof(float) * T, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(shear rest areas)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(shear rest shapes)");
+}
+
+// Bending (build_bending_from_current_positions, cloth_simulator.cu:451-514):
+// dihedrals from the mesh, rest angles from the current positions.
+void compute_rest_bending(Instance& inst, const float* pos, float k, hipStream_t s) {
+    const int T = int(inst.tris.size() / 3);
+    inst.dih = find_dihedrals(T ? inst.tris.data() : nullptr, T);
+    const int D = int(inst.dih.size() / 4);
+    inst.dih_rest.assign(size_t(D), 0.0f);
+    inst.bend_k = k;
+    inst.has_bending = true;
+    if (D == 0) return;
+    std::vector<int> v(inst.dih);
+    for (int& i : v) i += inst.part0;
+    DeviceArray<int4> dv;
+    DeviceArray<float> rest;
+    dv.assign(reinterpret_cast<const int4*>(v.data()), size_t(D), s);
+    rest.resize(size_t(D));
+    launch_rest_angles(dv.data(), pos, rest.data(), D, s);
+    check_hip(hipMemcpyAsync(inst.dih_rest.data(), rest.data(), sizeof(float) * D, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync(rest angles)");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(rest angles)");
+}
+
 // ---------------------------------------------------------------------------
 // Problem
 // ---------------------------------------------------------------------------
@@ -124,6 +175,7 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     ninst_ = int(insts.size());
     pats_.assign(size_t(ninst_), Pattern());
     tri_slots_.assign(size_t(ninst_), {});
+    dih_slots_.assign(size_t(ninst_), {});
     spr_slots_.assign(size_t(ninst_), {});
     pin_slots_.assign(size_t(ninst_), {});
     inst_row0_.assign(size_t(ninst_), 0);
@@ -139,16 +191,20 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     size_t ntri_tot = 0, nspr_tot = 0;
     for (int k = 0; k < ninst_; ++k) {
         const Instance& I = *insts[size_t(k)];
-        const int T = I.has_stretch ? int(I.tris.size() / 3) : 0;
+        const int T = (I.has_stretch || I.has_shear) ? int(I.tris.size() / 3) : 0;
         const int E = (I.has_springs && I.springs_active) ? int(I.edges.size() / 2) : 0;
         const int P = int(I.pin_idx.size());
+        const int D = I.has_bending ? int(I.dih.size() / 4) : 0;
         const int* tris = T ? I.tris.data() : nullptr;
         const int* spr = E ? I.edges.data() : nullptr;
-        pats_[size_t(k)] = build_pattern(I.n, tris, T, spr, E);
+        const int* dih = D ? I.dih.data() : nullptr;
+        pats_[size_t(k)] = build_pattern(I.n, tris, T, spr, E, dih, D);
         build_slot_maps(pats_[size_t(k)], tris, T, spr, E, P ? I.pin_idx.data() : nullptr, P,
                         tri_slots_[size_t(k)], spr_slots_[size_t(k)], pin_slots_[size_t(k)]);
+        dih_slots_[size_t(k)] = build_dihedral_slots(pats_[size_t(k)], dih, D);
         lay[size_t(k)] = build_device_layout(pats_[size_t(k)], tris, T, tri_slots_[size_t(k)], spr, E,
-                                             spr_slots_[size_t(k)], P ? I.pin_idx.data() : nullptr, P);
+                                             spr_slots_[size_t(k)], P ? I.pin_idx.data() : nullptr, P, dih, D,
+                                             &dih_slots_[size_t(k)]);
         inst_row0_[size_t(k)] = row0;
         inst_n_[size_t(k)] = I.n;
         inst_slice0_[size_t(k)] = slice0;
@@ -169,13 +225,15 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     slice_ptr_h_.assign(size_t(nslices) + 1, 0);
     row_deg_h_.assign(size_t(n_pad_), 0);
     std::vector<int> tri_v, spr_v, pin_v, tri_inc_off(size_t(n_pad_) + 1, 0), spr_inc_off(size_t(n_pad_) + 1, 0),
-        pin_inc_off(size_t(n_pad_) + 1, 0), pin_inc;
+        pin_inc_off(size_t(n_pad_) + 1, 0), pin_inc, bend_inc_off(size_t(n_pad_) + 1, 0);
     std::vector<int2> tri_inc, spr_inc;
-    std::vector<float> tri_dminv, tri_area, tri_k, spr_rest, spr_k, pin_tgt;
+    std::vector<int4> dih_v, bend_inc;
+    std::vector<float> tri_dminv, tri_area, tri_k, sh_dminv, sh_area, sh_k, spr_rest, spr_k, pin_tgt, dih_rest;
     tri_v.reserve(3 * ntri_tot);
     spr_v.reserve(2 * nspr_tot);
     int entries = 0;
-    int tri_base = 0, spr_base = 0;
+    int tri_base = 0, spr_base = 0, dih_base = 0;
+    max_width_ = 0;
     for (int k = 0; k < ninst_; ++k) {
         const Instance& I = *insts[size_t(k)];
         const DeviceLayout& L = lay[size_t(k)];
@@ -183,13 +241,31 @@ void Problem::build(const std::vector<const Instance*>& insts) {
         for (int s = 0; s < L.nslices; ++s) slice_ptr_h_[size_t(s0 + s) + 1] = entries + L.slice_ptr[size_t(s) + 1];
         entries += L.slice_ptr[size_t(L.nslices)];
         for (int r = 0; r < L.n_pad; ++r) row_deg_h_[size_t(r0 + r)] = L.row_deg[size_t(r)];
-        const int T = I.has_stretch ? int(I.tris.size() / 3) : 0;
+        max_width_ = std::max(max_width_, L.max_width);
+        const int T = (I.has_stretch || I.has_shear) ? int(I.tris.size() / 3) : 0;
         const int E = (I.has_springs && I.springs_active) ? int(I.edges.size() / 2) : 0;
         const int P = int(I.pin_idx.size());
+        const int D = I.has_bending ? int(I.dih.size() / 4) : 0;
         for (int q = 0; q < 3 * T; ++q) tri_v.push_back(I.tris[size_t(q)] + I.part0);
-        tri_dminv.insert(tri_dminv.end(), I.tri_dminv.begin(), I.tri_dminv.begin() + 4 * T);
-        tri_area.insert(tri_area.end(), I.tri_area.begin(), I.tri_area.begin() + T);
-        tri_k.insert(tri_k.end(), I.tri_k.begin(), I.tri_k.begin() + T);
+        // absent sets contribute zero-stiffness elements (the kernel skips them per instance)
+        auto append_set = [&](bool has, const std::vector<float>& dm, const std::vector<float>& ar,
+                              const std::vector<float>& kk, std::vector<float>& odm, std::vector<float>& oar,
+                              std::vector<float>& ok) {
+            if (has && (dm.size() < size_t(4) * T || ar.size() < size_t(T) || kk.size() < size_t(T)))
+                throw std::logic_error("triangle rest data does not match the mesh: rebuild after set_mesh");
+            for (int t = 0; t < T; ++t) {
+                for (int c = 0; c < 4; ++c) odm.push_back(has ? dm[size_t(4) * t + c] : (c == 0 || c == 3 ? 1.0f : 0.0f));
+                oar.push_back(has ? ar[size_t(t)] : 0.0f);
+                ok.push_back(has ? kk[size_t(t)] : 0.0f);
+            }
+        };
+        append_set(I.has_stretch, I.tri_dminv, I.tri_area, I.tri_k, tri_dminv, tri_area, tri_k);
+        append_set(I.has_shear, I.sh_dminv, I.sh_area, I.sh_k, sh_dminv, sh_area, sh_k);
+        for (int d = 0; d < D; ++d) {
+            dih_v.push_back(make_int4(I.dih[size_t(4) * d] + I.part0, I.dih[size_t(4) * d + 1] + I.part0,
+                                      I.dih[size_t(4) * d + 2] + I.part0, I.dih[size_t(4) * d + 3] + I.part0));
+            dih_rest.push_back(I.dih_rest[size_t(d)]);
+        }
         for (int q = 0; q < 2 * E; ++q) spr_v.push_back(I.edges[size_t(q)] + I.part0);
         if (E) {
             spr_rest.insert(spr_rest.end(), I.spr_rest.begin(), I.spr_rest.begin() + E);
@@ -200,11 +276,18 @@ void Problem::build(const std::vector<const Instance*>& insts) {
             for (int d = 0; d < 3; ++d) pin_tgt.push_back(I.pin_tgt[size_t(3 * c + d)]);
         }
         // incidence, rows offset by r0, element ids offset by bases
-        const int tb = int(tri_inc.size()), sbs = int(spr_inc.size()), pb = int(pin_inc.size());
+        const int tb = int(tri_inc.size()), sbs = int(spr_inc.size()), pb = int(pin_inc.size()),
+                  bb = int(bend_inc.size());
         for (int r = 0; r < L.n_pad; ++r) {
             tri_inc_off[size_t(r0 + r) + 1] = tb + L.tri_inc_off[size_t(r) + 1];
             spr_inc_off[size_t(r0 + r) + 1] = sbs + L.spr_inc_off[size_t(r) + 1];
             pin_inc_off[size_t(r0 + r) + 1] = pb + L.pin_inc_off[size_t(r) + 1];
+            bend_inc_off[size_t(r0 + r) + 1] = bb + L.bend_inc_off[size_t(r) + 1];
+        }
+        for (size_t q = 0; q < L.bend_inc.size() / 4; ++q) {
+            const int id = L.bend_inc[4 * q];
+            bend_inc.push_back(make_int4(((id >> 2) + dih_base) * 4 + (id & 3), L.bend_inc[4 * q + 1],
+                                         L.bend_inc[4 * q + 2], 0));
         }
         for (size_t q = 0; q < L.tri_inc.size() / 2; ++q) {
             const int id = L.tri_inc[2 * q];
@@ -217,7 +300,8 @@ void Problem::build(const std::vector<const Instance*>& insts) {
         for (int c : L.pin_inc) pin_inc.push_back(c + pin_offset_[size_t(k)]);
         tri_base += T;
         spr_base += E;
-        if (tri_base > (1 << 29)) throw std::invalid_argument("too many triangles");
+        dih_base += D;
+        if (tri_base > (1 << 29) || dih_base > (1 << 29)) throw std::invalid_argument("too many elements");
     }
     // fill leading offsets of rows with no later entry (monotone already by construction)
     std::vector<int> sell_col(size_t(entries) * kSlice, -1);
@@ -253,6 +337,13 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     tri_dminv_.assign(reinterpret_cast<const float4*>(tri_dminv.data()), tri_dminv.size() / 4, s);
     tri_area_.assign(tri_area.data(), tri_area.size(), s);
     tri_k_.assign(tri_k.data(), tri_k.size(), s);
+    sh_dminv_.assign(reinterpret_cast<const float4*>(sh_dminv.data()), sh_dminv.size() / 4, s);
+    sh_area_.assign(sh_area.data(), sh_area.size(), s);
+    sh_k_.assign(sh_k.data(), sh_k.size(), s);
+    dih_v_.assign(dih_v.data(), dih_v.size(), s);
+    dih_rest_.assign(dih_rest.data(), dih_rest.size(), s);
+    bend_inc_off_.assign(bend_inc_off.data(), bend_inc_off.size(), s);
+    bend_inc_.assign(bend_inc.data(), bend_inc.size(), s);
     spr_v_.assign(spr_v.data(), spr_v.size(), s);
     spr_rest_.assign(spr_rest.data(), spr_rest.size(), s);
     spr_k_.assign(spr_k.data(), spr_k.size(), s);
@@ -330,6 +421,9 @@ void Problem::upload_params(const std::vector<const Instance*>& insts, hipStream
         q.pin_k = I.pin_k;
         q.force_active = I.force_active ? 1 : 0;
         q.springs_active = (I.springs_active && I.has_springs) ? 1 : 0;
+        q.tri_stretch = I.has_stretch ? 1 : 0;
+        q.tri_shear = I.has_shear ? 1 : 0;
+        q.bend_k = I.has_bending ? I.bend_k : 0.0f;
     }
     if (p.empty()) return;
     check_hip(hipMemcpyAsync(params_.data(), p.data(), sizeof(InstParams) * p.size(), hipMemcpyHostToDevice, s),
@@ -354,15 +448,37 @@ DevVectors Problem::vectors() const {
                       partials_.data()};
 }
 DevMesh Problem::mesh() const {
-    return DevMesh{tri_v_.data(),       tri_dminv_.data(),   tri_area_.data(), tri_k_.data(),       spr_v_.data(),
-                   spr_rest_.data(),    spr_k_.data(),       pin_v_.data(),    pin_tgt_.data(),     tri_inc_off_.data(),
-                   tri_inc_.data(),     spr_inc_off_.data(), spr_inc_.data(),  pin_inc_off_.data(), pin_inc_.data()};
+    DevMesh m{};
+    m.tri_v = tri_v_.data();
+    m.tri_dminv = tri_dminv_.data();
+    m.tri_area = tri_area_.data();
+    m.tri_k = tri_k_.data();
+    m.spr_v = spr_v_.data();
+    m.spr_rest = spr_rest_.data();
+    m.spr_k = spr_k_.data();
+    m.pin_v = pin_v_.data();
+    m.pin_tgt = pin_tgt_.data();
+    m.tri_inc_off = tri_inc_off_.data();
+    m.tri_inc = tri_inc_.data();
+    m.spr_inc_off = spr_inc_off_.data();
+    m.spr_inc = spr_inc_.data();
+    m.pin_inc_off = pin_inc_off_.data();
+    m.pin_inc = pin_inc_.data();
+    m.sh_dminv = sh_dminv_.data();
+    m.sh_area = sh_area_.data();
+    m.sh_k = sh_k_.data();
+    m.dih_v = dih_v_.data();
+    m.dih_rest = dih_rest_.data();
+    m.bend_inc_off = bend_inc_off_.data();
+    m.bend_inc = bend_inc_.data();
+    return m;
 }
 
 double Problem::spmv_bytes() const { return 40.0 * double(nnz_total_) + 64.0 * double(n_rows_) + 4.0; }
 
 void Problem::launch_assemble(float dt, const DevExt& ext, hipStream_t s) {
-    chysx_amd::launch_assemble(geom(), mesh(), matrix(), vectors(), ext, params_.data(), dt, cfg_.assemble_waves, s);
+    chysx_amd::launch_assemble(geom(), mesh(), matrix(), vectors(), ext, params_.data(), dt, cfg_.assemble_waves,
+                               max_width_, s);
 }
 void Problem::launch_pcg_start(hipStream_t s) {
     launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 0, cfg_.residual_waves, s);
@@ -743,6 +859,8 @@ void ClothSim::set_mesh(const int* tris, int ntri) {
     if (ntri < 0) throw std::invalid_argument("ClothSimulator::set_mesh: negative triangle count");
     inst_.tris.assign(tris, tris + 3 * size_t(ntri));
     inst_.edges = build_edges(tris, ntri);
+    // Element sets built on a previous mesh are dropped (rebuild them).
+    inst_.has_springs = inst_.has_stretch = inst_.has_shear = inst_.has_bending = false;
     topo_dirty_ = true;
 }
 
@@ -778,6 +896,30 @@ void ClothSim::build_fem_stretch(float k, std::uintptr_t stream) {
     inst_.part0 = 0;
     compute_rest_stretch(inst_, reinterpret_cast<const float*>(pos_), nullptr, k,
                          reinterpret_cast<hipStream_t>(stream));
+    topo_dirty_ = true;
+}
+
+void ClothSim::build_fem_shear(float k, std::uintptr_t stream) {
+    const int T = int(inst_.tris.size() / 3);
+    if (T > 0 && pos_ == 0)
+        throw std::runtime_error(
+            "ClothSimulator::build_fem_shear_from_current_positions: external positions must be set first via "
+            "set_external_buffers().");
+    check_indices(inst_.tris, n_, "build_fem_shear_from_current_positions");
+    inst_.part0 = 0;
+    compute_rest_shear(inst_, reinterpret_cast<const float*>(pos_), nullptr, k, reinterpret_cast<hipStream_t>(stream));
+    topo_dirty_ = true;
+}
+
+void ClothSim::build_bending(float k, std::uintptr_t stream) {
+    const int T = int(inst_.tris.size() / 3);
+    if (T > 0 && pos_ == 0)
+        throw std::runtime_error(
+            "ClothSimulator::build_bending_from_current_positions: external positions must be set first via "
+            "set_external_buffers().");
+    check_indices(inst_.tris, n_, "build_bending_from_current_positions");
+    inst_.part0 = 0;
+    compute_rest_bending(inst_, reinterpret_cast<const float*>(pos_), k, reinterpret_cast<hipStream_t>(stream));
     topo_dirty_ = true;
 }
 

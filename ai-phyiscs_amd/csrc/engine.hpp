@@ -66,6 +66,15 @@ struct Instance {
     bool has_stretch = false;
     std::vector<float> tri_dminv;  // 4 per triangle
     std::vector<float> tri_area, tri_k;
+    // shear triangles (theta = pi/4): the same triangles, rotated frame
+    bool has_shear = false;
+    std::vector<float> sh_dminv;   // 4 per triangle
+    std::vector<float> sh_area, sh_k;
+    // bending dihedrals (v0, v1, v2, v3), rest angles, one stiffness
+    bool has_bending = false;
+    std::vector<int> dih;
+    std::vector<float> dih_rest;
+    float bend_k = 1.0e-3f;
     // pins
     std::vector<int> pin_idx;
     std::vector<float> pin_tgt;
@@ -166,6 +175,7 @@ public:
     // 40 per off-diagonal block + 64 per row + 4.
     double spmv_bytes() const;
     long long nnz_total() const { return nnz_total_; }
+    int max_width() const { return max_width_; }
     int rows_total() const { return n_rows_; }
 
     // Launch pieces (used by step and by the benchmark).
@@ -230,8 +240,12 @@ private:
     DeviceArray<InstParams> params_;
     // mesh
     DeviceArray<int> tri_v_, spr_v_, pin_v_;
-    DeviceArray<float4> tri_dminv_;
-    DeviceArray<float> tri_area_, tri_k_, spr_rest_, spr_k_, pin_tgt_;
+    DeviceArray<float4> tri_dminv_, sh_dminv_;
+    DeviceArray<float> tri_area_, tri_k_, sh_area_, sh_k_, spr_rest_, spr_k_, pin_tgt_, dih_rest_;
+    DeviceArray<int4> dih_v_, bend_inc_;
+    DeviceArray<int> bend_inc_off_;
+    std::vector<std::vector<int>> dih_slots_;
+    int max_width_ = 0;
     DeviceArray<int> tri_inc_off_, spr_inc_off_, pin_inc_off_, pin_inc_;
     DeviceArray<int2> tri_inc_, spr_inc_;
     std::vector<int> pin_offset_;  // first global pin of each instance
@@ -298,6 +312,10 @@ public:
 
     int num_springs() const { return inst_.has_springs ? int(inst_.edges.size() / 2) : 0; }
     int num_fem_stretch() const { return inst_.has_stretch ? int(inst_.tris.size() / 3) : 0; }
+    int num_fem_shear() const { return inst_.has_shear ? int(inst_.tris.size() / 3) : 0; }
+    int num_bending() const { return inst_.has_bending ? int(inst_.dih.size() / 4) : 0; }
+    void build_fem_shear(float k, std::uintptr_t stream);
+    void build_bending(float k, std::uintptr_t stream);
     int num_pins() const { return int(inst_.pin_idx.size()); }
     int num_edges() const { return int(inst_.edges.size() / 2); }
     const std::vector<int>& edges() const { return inst_.edges; }
@@ -373,5 +391,7 @@ void check_hip(hipError_t err, const char* what);
 // (particle space; the instance's particles start at inst.part0).
 void compute_rest_springs(Instance& inst, const float* pos, const float* k_per_edge, float k, hipStream_t s);
 void compute_rest_stretch(Instance& inst, const float* pos, const float* k_per_tri, float k, hipStream_t s);
+void compute_rest_shear(Instance& inst, const float* pos, const float* k_per_tri, float k, hipStream_t s);
+void compute_rest_bending(Instance& inst, const float* pos, float k, hipStream_t s);
 
 }  // namespace chysx_amd

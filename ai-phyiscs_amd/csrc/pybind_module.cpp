@@ -214,6 +214,39 @@ PYBIND11_MODULE(_chysx_native, m) {
                  return n;
              })
         .def(
+            "build_fem_shear_from_current_positions",
+            [](Sim& s, float k, std::uintptr_t stream) { check(chysx_cloth_build_fem_shear(s.p(), k, stream)); },
+            py::arg("stiffness"), py::arg("cuda_stream") = 0,
+            "One Baraff-Witkin shear element per face: the stretch element with its material axes rotated by "
+            "pi/4 (per-element weight area * stiffness).")
+        .def("num_fem_shear_triangles",
+             [](const Sim& s) {
+                 int n = 0;
+                 check(chysx_cloth_num_fem_shear(s.p(), &n));
+                 return n;
+             })
+        .def(
+            "build_bending_from_current_positions",
+            [](Sim& s, float k, std::uintptr_t stream) { check(chysx_cloth_build_bending(s.p(), k, stream)); },
+            py::arg("stiffness"), py::arg("cuda_stream") = 0,
+            "One dihedral bending element per interior edge, rest angles from the current positions.")
+        .def("num_bending_dihedrals",
+             [](const Sim& s) {
+                 int n = 0;
+                 check(chysx_cloth_num_bending(s.p(), &n));
+                 return n;
+             })
+        .def("bending_dihedrals",
+             [](const Sim& s) {
+                 int n = 0;
+                 check(chysx_cloth_num_bending(s.p(), &n));
+                 py::array_t<int32_t> d({n, 4});
+                 py::array_t<float> r({n});
+                 check(chysx_cloth_get_dihedrals(s.p(), d.mutable_data(), r.mutable_data()));
+                 return py::make_tuple(d, r);
+             },
+             "(dihedrals (D, 4), rest angles (D,)) of the installed bending elements.")
+        .def(
             "set_pcg_iterations", [](Sim& s, int it) { check(chysx_cloth_set_pcg_iterations(s.p(), it)); },
             py::arg("max_iter"), "Set the maximum number of PCG iterations per step.")
         .def("pcg_iterations",
@@ -526,6 +559,25 @@ PYBIND11_MODULE(_chysx_native, m) {
         },
         py::arg("n"), py::arg("triangles"), py::arg("include_springs") = false,
         py::arg("pins") = IntArr(0), "Host-only pattern and slot maps of a mesh (no GPU needed).");
+    m.def(
+        "topology_query_bending",
+        [](int n, IntArr tris) {
+            if (tris.ndim() != 2 || tris.shape(1) != 3) throw py::value_error("topology_query: triangles must be (M, 3)");
+            const int T = int(tris.shape(0));
+            int nnz = 0, nd = 0;
+            check(chysx_topology_query_bending(n, tris.data(), T, &nnz, &nd, nullptr, nullptr, nullptr, nullptr));
+            py::array_t<int32_t> ro({n + 1}), cols({nnz}), dih({nd, 4}), ds({nd, 16});
+            check(chysx_topology_query_bending(n, tris.data(), T, &nnz, &nd, ro.mutable_data(), cols.mutable_data(),
+                                               dih.mutable_data(), ds.mutable_data()));
+            py::dict d;
+            d["row_offsets"] = ro;
+            d["col_indices"] = cols;
+            d["dihedrals"] = dih;
+            d["dihedral_slots"] = ds;
+            return d;
+        },
+        py::arg("n"), py::arg("triangles"),
+        "Host-only pattern with FEM triangles and bending dihedrals, dihedral list and slots (no GPU needed).");
     m.def(
         "ws_small_solve",
         [](DblArr G, DblArr g) {

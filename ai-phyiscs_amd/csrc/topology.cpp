@@ -65,8 +65,64 @@ std::vector<int> build_edges(const int* tris, int ntri) {
     return edges;
 }
 
-Pattern build_pattern(int n, const int* tris, int ntri, const int* springs, int nspring) {
-    if (n < 0 || ntri < 0 || nspring < 0) throw std::invalid_argument("build_pattern: negative size");
+std::vector<int> find_dihedrals(const int* tris, int ntri) {
+    // Directed edges (from, to, third, triangle order), bucketed by `from`.
+    int nv = 0;
+    for (int q = 0; q < 3 * ntri; ++q) {
+        if (tris[q] < 0) throw std::out_of_range("find_dihedrals: negative vertex index");
+        nv = std::max(nv, tris[q] + 1);
+    }
+    std::vector<int> off(size_t(nv) + 1, 0);
+    for (int q = 0; q < 3 * ntri; ++q) off[size_t(tris[q]) + 1]++;
+    for (int i = 0; i < nv; ++i) off[size_t(i) + 1] += off[size_t(i)];
+    struct DE {
+        int to, third, order;
+    };
+    std::vector<DE> de(size_t(3) * ntri);
+    {
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int t = 0; t < ntri; ++t) {
+            const int v[3] = {tris[3 * t], tris[3 * t + 1], tris[3 * t + 2]};
+            for (int e = 0; e < 3; ++e) {
+                const int from = v[e], to = v[(e + 1) % 3], third = v[(e + 2) % 3];
+                de[size_t(fill[size_t(from)]++)] = DE{to, third, 3 * t + e};
+            }
+        }
+    }
+    // Per bucket: sort by (to, order) and keep the last record of each `to`
+    // (the latest write wins).
+    std::vector<int> len(size_t(nv), 0);
+    for (int i = 0; i < nv; ++i) {
+        auto b = de.begin() + off[size_t(i)], e = de.begin() + off[size_t(i) + 1];
+        std::sort(b, e, [](const DE& x, const DE& y) { return x.to != y.to ? x.to < y.to : x.order < y.order; });
+        int w = 0;
+        for (auto it = b; it != e; ++it) {
+            if (it + 1 != e && (it + 1)->to == it->to) continue;
+            *(b + w++) = *it;
+        }
+        len[size_t(i)] = w;
+    }
+    auto lookup = [&](int from, int to) -> const DE* {
+        auto b = de.begin() + off[size_t(from)], e = b + len[size_t(from)];
+        auto it = std::lower_bound(b, e, to, [](const DE& x, int v) { return x.to < v; });
+        return (it != e && it->to == to) ? &*it : nullptr;
+    };
+    std::vector<int> out;
+    for (int v0 = 0; v0 < nv; ++v0) {
+        auto b = de.begin() + off[size_t(v0)];
+        for (int q = 0; q < len[size_t(v0)]; ++q) {
+            const DE& d = *(b + q);
+            if (v0 >= d.to) continue;
+            const DE* r = lookup(d.to, v0);
+            if (!r) continue;
+            out.insert(out.end(), {v0, d.to, d.third, r->third});
+        }
+    }
+    return out;
+}
+
+Pattern build_pattern(int n, const int* tris, int ntri, const int* springs, int nspring, const int* dih, int ndih) {
+    if (n < 0 || ntri < 0 || nspring < 0 || ndih < 0) throw std::invalid_argument("build_pattern: negative size");
     std::vector<int> rows, cols;
     rows.reserve(size_t(6) * ntri + size_t(2) * nspring);
     cols.reserve(rows.capacity());
@@ -82,6 +138,12 @@ Pattern build_pattern(int n, const int* tris, int ntri, const int* springs, int 
             for (int b = 0; b < 3; ++b)
                 if (a != b && v[a] != v[b]) push(v[a], v[b]);
                 else if (a != b) push(v[a], v[b]);  // v[a]==v[b]: range-checked, then dropped
+    }
+    for (int d = 0; dih && d < ndih; ++d) {
+        const int* v = dih + 4 * d;
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b)
+                if (a != b) push(v[a], v[b]);  // v[a] == v[b]: range-checked, then dropped
     }
     for (int e = 0; springs && e < nspring; ++e) {
         push(springs[2 * e], springs[2 * e + 1]);
@@ -123,9 +185,18 @@ void build_slot_maps(const Pattern& p, const int* tris, int ntri, const int* spr
     for (int c = 0; c < npin; ++c) pin_slots[size_t(c)] = resolve_slot(p, pins[c], pins[c]);
 }
 
+std::vector<int> build_dihedral_slots(const Pattern& p, const int* dih, int ndih) {
+    std::vector<int> out(size_t(16) * ndih);
+    for (int d = 0; d < ndih; ++d)
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) out[size_t(16) * d + 4 * a + b] = resolve_slot(p, dih[4 * d + a], dih[4 * d + b]);
+    return out;
+}
+
 DeviceLayout build_device_layout(const Pattern& p, const int* tris, int ntri, const std::vector<int>& tri_slots,
                                  const int* springs, int nspring, const std::vector<int>& spr_slots,
-                                 const int* pins, int npin) {
+                                 const int* pins, int npin, const int* dih, int ndih,
+                                 const std::vector<int>* dih_slots) {
     DeviceLayout L;
     const int n = p.n;
     L.n = n;
@@ -207,6 +278,34 @@ DeviceLayout build_device_layout(const Pattern& p, const int* tris, int ntri, co
     {
         std::vector<int> fill(cnt.begin(), cnt.end() - 1);
         for (int c = 0; c < npin; ++c) L.pin_inc[size_t(fill[size_t(pins[c])]++)] = c;
+    }
+    // Dihedral incidence, per row in dihedral order.
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int q = 0; dih && q < 4 * ndih; ++q) cnt[size_t(dih[q]) + 1]++;
+    for (int i = 0; i < L.n_pad; ++i) cnt[size_t(i) + 1] += cnt[size_t(i)];
+    L.bend_inc_off = cnt;
+    L.bend_inc.assign(size_t(4) * cnt[size_t(L.n_pad)], 0);
+    if (dih && ndih > 0) {
+        if (!dih_slots || dih_slots->size() != size_t(16) * ndih)
+            throw std::invalid_argument("device layout: dihedral slots missing");
+        auto local16 = [&](int slot, int row) -> int {
+            if (slot < 0) return 0xFFFF;
+            const int k = slot - p.row_offsets[size_t(row)];
+            if (k >= 0xFFFF) throw std::invalid_argument("device layout: row degree >= 65535");
+            return k;
+        };
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (int d = 0; d < ndih; ++d)
+            for (int a = 0; a < 4; ++a) {
+                const int row = dih[4 * d + a];
+                int k[4];
+                for (int b = 0; b < 4; ++b) k[b] = (b == a) ? 0xFFFF : local16((*dih_slots)[size_t(16) * d + 4 * a + b], row);
+                const size_t e = size_t(fill[size_t(row)]++);
+                L.bend_inc[4 * e] = d * 4 + a;
+                L.bend_inc[4 * e + 1] = int(unsigned(k[0]) | (unsigned(k[1]) << 16));
+                L.bend_inc[4 * e + 2] = int(unsigned(k[2]) | (unsigned(k[3]) << 16));
+                L.bend_inc[4 * e + 3] = 0;
+            }
     }
     return L;
 }

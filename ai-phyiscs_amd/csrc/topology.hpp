@@ -30,10 +30,20 @@ struct Pattern {
     std::vector<int> cols;         // nnz, ascending per row, no diagonal
 };
 
+// Bending dihedrals (v0, v1, v2, v3) of a triangle list, 4 ints each
+// (build_bending_from_current_positions, cloth_simulator.cu:451-514): every
+// directed triangle edge records its triangle's third vertex (a later
+// triangle overwrites an earlier one); an edge v0 < v1 present in both
+// directions gives (v0, v1, third(v0 -> v1), third(v1 -> v0)).  The
+// reference emits them in hash-map order; here they are sorted by (v0, v1).
+std::vector<int> find_dihedrals(const int* tris, int ntri);
+
 // Pattern from the ordered off-diagonal pairs of every triangle
-// (a != b, v_a != v_b) plus both directions of every spring edge when
-// `springs` is non-null.  Throws std::out_of_range on an index outside [0, n).
-Pattern build_pattern(int n, const int* tris, int ntri, const int* springs, int nspring);
+// (a != b, v_a != v_b), of every dihedral (4 vertices) when `dih` is
+// non-null, plus both directions of every spring edge when `springs` is
+// non-null.  Throws std::out_of_range on an index outside [0, n).
+Pattern build_pattern(int n, const int* tris, int ntri, const int* springs, int nspring, const int* dih = nullptr,
+                      int ndih = 0);
 
 // Reference slot encoding: -r-1 for a diagonal block, else the CSR index of
 // (r, c).  Throws std::out_of_range if (r, c) is not in the pattern.
@@ -44,6 +54,8 @@ int resolve_slot(const Pattern& p, int r, int c);
 void build_slot_maps(const Pattern& p, const int* tris, int ntri, const int* springs, int nspring,
                      const int* pins, int npin, std::vector<int>& tri_slots, std::vector<int>& spr_slots,
                      std::vector<int>& pin_slots);
+// Dihedral slot map 16d + 4a + b (ConstraintN<4>::bind_hessian_layout).
+std::vector<int> build_dihedral_slots(const Pattern& p, const int* dih, int ndih);
 
 // Device layout of one instance, row space padded to a multiple of kSlice.
 //
@@ -75,11 +87,16 @@ struct DeviceLayout {
     // pin incidence: pin indices per row
     std::vector<int> pin_inc_off;
     std::vector<int> pin_inc;
+    // dihedral incidence: 4 ints per entry {d*4 + a, k_b0 | k_b1 << 16,
+    // k_b2 | k_b3 << 16, 0} with 0xFFFF for a diagonal block
+    std::vector<int> bend_inc_off;
+    std::vector<int> bend_inc;
 };
 
 DeviceLayout build_device_layout(const Pattern& p, const int* tris, int ntri, const std::vector<int>& tri_slots,
                                  const int* springs, int nspring, const std::vector<int>& spr_slots,
-                                 const int* pins, int npin);
+                                 const int* pins, int npin, const int* dih = nullptr, int ndih = 0,
+                                 const std::vector<int>* dih_slots = nullptr);
 
 // Per-vertex triangle incidence (for the mass redistribution helper):
 // for each vertex, the triangles touching it, in triangle order.

@@ -87,6 +87,22 @@ int chysx_cloth_num_springs(const chysx_cloth* sim, int* out);
 int chysx_cloth_build_fem_stretch(chysx_cloth* sim, float stiffness, uintptr_t stream);
 /* num_fem_stretch_triangles (bindings.cpp:260-265). */
 int chysx_cloth_num_fem_stretch(const chysx_cloth* sim, int* out);
+/* build_fem_shear_from_current_positions (cloth_simulator.cu:423-449;
+ * bindings.cpp:266-285): the stretch element with the material axes rotated
+ * by pi/4, one per face. */
+int chysx_cloth_build_fem_shear(chysx_cloth* sim, float stiffness, uintptr_t stream);
+/* num_fem_shear_triangles (bindings.cpp:286-290). */
+int chysx_cloth_num_fem_shear(const chysx_cloth* sim, int* out);
+/* build_bending_from_current_positions (cloth_simulator.cu:451-514;
+ * bindings.cpp:291-313): one dihedral per interior edge, rest angles from the
+ * current positions.  Dihedrals are ordered by hinge (v0, v1); the
+ * reference's order is its hash map's. */
+int chysx_cloth_build_bending(chysx_cloth* sim, float stiffness, uintptr_t stream);
+/* num_bending_dihedrals (bindings.cpp:314-318). */
+int chysx_cloth_num_bending(const chysx_cloth* sim, int* out);
+/* Dihedrals (4 ints each: hinge v0 < v1, wings v2, v3) and rest angles of the
+ * installed bending elements; either output may be null. */
+int chysx_cloth_get_dihedrals(const chysx_cloth* sim, int32_t* dihedrals, float* rest_angles);
 
 /* set_pins / clear_pins / num_pins / update_pin_targets
  * (cloth_simulator.cu:277-313; bindings.cpp:85-166).  targets: n x 3 floats. */
@@ -154,6 +170,14 @@ int chysx_cloth_get_edges(const chysx_cloth* sim, int32_t* out);
 int chysx_topology_query(int n, const int32_t* triangles, int ntri, int include_springs, const int32_t* pins,
                          int npin, int* nnz, int* nedges, int32_t* row_offsets, int32_t* col_indices,
                          int32_t* edges, int32_t* tri_slots, int32_t* spring_slots, int32_t* pin_slots);
+
+/* Host-only topology query with FEM triangles and bending dihedrals: the
+ * pattern of ensure_hessian_topology (cloth_simulator.cu:698-793) over
+ * triangle pairs and the 12 directed pairs of every dihedral, the dihedral
+ * list and its slot map (16 per dihedral).  Null outputs give sizes. */
+int chysx_topology_query_bending(int n, const int32_t* triangles, int ntri, int* nnz, int* ndih,
+                                 int32_t* row_offsets, int32_t* col_indices, int32_t* dihedrals,
+                                 int32_t* dihedral_slots);
 
 /* ---- batched cloths (extension) ---------------------------------------- */
 typedef struct chysx_batch chysx_batch;

@@ -186,3 +186,31 @@ def test_projection_small_solve_matches_eigen_truncated_pinv(k):
 def test_projection_small_solve_rejects_bad_size():
     with pytest.raises(ValueError):
         chysx._chysx_native.ws_small_solve(np.eye(9), np.ones(9))
+
+
+@pytest.mark.parametrize("case", ["grid7", "grid20", "random"])
+def test_topology_with_bending_bit_identical_to_oracle(case):
+    from oracle.oracle import OracleCloth as OC
+    if case.startswith("grid"):
+        w = int(case[4:])
+        pos, tris = scenes.grid(w)
+        n = w * w
+    else:
+        rng = np.random.default_rng(11)
+        n = 80
+        tris = rng.integers(0, n, size=(200, 3)).astype(np.int32)
+        tris[::17, 1] = tris[::17, 0]  # degenerate triangles
+        pos = rng.normal(size=(n, 3)).astype(np.float32)
+    d = chysx._chysx_native.topology_query_bending(n, tris)
+    oc = OC(64)
+    oc.set_mesh(tris, n)
+    oc.build_stretch(pos, 1e3)
+    oc.build_bending(pos, 1e-2)
+    oc.set_params((0, -9.8, 0), 0.0, False, False, 0.0, 1)
+    oc.build_topology()
+    ro, co = oc.pattern()
+    dih, _, _, ds = oc.bending(tris.shape[0])
+    np.testing.assert_array_equal(d["row_offsets"], ro)
+    np.testing.assert_array_equal(d["col_indices"], co)
+    np.testing.assert_array_equal(d["dihedrals"], dih)
+    np.testing.assert_array_equal(d["dihedral_slots"], ds)
