@@ -155,6 +155,65 @@ int chysx_cloth_num_bending(const chysx_cloth* sim, int* out) {
         *out = sim->sim.num_bending();
     });
 }
+int chysx_cloth_add_static_plane(chysx_cloth* sim, const float* n, float d) {
+    return guard([&] {
+        need(sim, "sim");
+        need(n, "n");
+        sim->sim.add_static_plane(n, d);
+    });
+}
+int chysx_cloth_add_static_box(chysx_cloth* sim, const float* c, const float* h, const float* ex, const float* ey,
+                               const float* ez) {
+    return guard([&] {
+        need(sim, "sim");
+        need(c, "center");
+        need(h, "half_ext");
+        need(ex, "ex");
+        need(ey, "ey");
+        need(ez, "ez");
+        sim->sim.add_static_box(c, h, ex, ey, ez);
+    });
+}
+int chysx_cloth_clear_static_shapes(chysx_cloth* sim) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.clear_static_shapes();
+    });
+}
+int chysx_cloth_static_shape_counts(const chysx_cloth* sim, int* planes, int* boxes) {
+    return guard([&] {
+        need(sim, "sim");
+        if (planes) *planes = int(sim->sim.instance().planes.size() / 4);
+        if (boxes) *boxes = int(sim->sim.instance().boxes.size() / 16);
+    });
+}
+int chysx_cloth_set_static_contact(chysx_cloth* sim, int which, float value) {
+    return guard([&] {
+        need(sim, "sim");
+        Instance& I = sim->sim.contact_instance();
+        switch (which) {
+            case 0: I.contact_h = value; break;
+            case 1: I.contact_k = value; break;
+            case 2: I.contact_mu = value; break;
+            case 3: I.contact_eps = value; break;
+            default: throw std::invalid_argument("set_static_contact: which must be 0..3");
+        }
+    });
+}
+int chysx_cloth_get_static_contact(const chysx_cloth* sim, int which, float* value) {
+    return guard([&] {
+        need(sim, "sim");
+        need(value, "value");
+        const Instance& I = sim->sim.instance();
+        switch (which) {
+            case 0: *value = I.contact_h; break;
+            case 1: *value = I.contact_k; break;
+            case 2: *value = I.contact_mu; break;
+            case 3: *value = I.contact_eps; break;
+            default: throw std::invalid_argument("get_static_contact: which must be 0..3");
+        }
+    });
+}
 int chysx_cloth_get_dihedrals(const chysx_cloth* sim, int32_t* dihedrals, float* rest_angles) {
     return guard([&] {
         need(sim, "sim");

@@ -71,6 +71,10 @@ struct alignas(16) InstParams {
     int tri_stretch;    // 1: stretch triangles installed
     int tri_shear;      // 1: shear triangles installed
     float bend_k;       // bending stiffness (dihedrals only exist when installed)
+    // static-shape contact: thickness h, stiffness k, friction mu, eps_u; the
+    // instance's planes (1 float4 each) and boxes (4 float4 each) in the shape table
+    float contact_h, contact_k, contact_mu, contact_eps;
+    int plane0, nplane, box0, nbox;
 };
 
 }  // namespace chysx_amd

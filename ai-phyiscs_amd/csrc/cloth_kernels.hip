@@ -270,6 +270,41 @@ __device__ __forceinline__ DihFrame dihedral_frame(F3 v0, F3 v1, F3 v2, F3 v3) {
     return F;
 }
 
+// box_signed_dist (static_contact.cu:55-99): signed distance of x to an
+// oriented box and the unit outward normal (towards the closest face when
+// inside).
+__device__ __forceinline__ float box_sdf(const float4* __restrict__ b, F3 x, F3& n) {
+    const float4 b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3];
+    const F3 c = f3(b0.x, b0.y, b0.z), he = f3(b0.w, b1.x, b1.y);
+    const F3 ex = f3(b1.z, b1.w, b2.x), ey = f3(b2.y, b2.z, b2.w), ez = f3(b3.x, b3.y, b3.z);
+    const F3 rel = x - c;
+    const float qx = dot(rel, ex), qy = dot(rel, ey), qz = dot(rel, ez);
+    const float dx = fabsf(qx) - he.x, dy = fabsf(qy) - he.y, dz = fabsf(qz) - he.z;
+    if (dx <= 0.0f && dy <= 0.0f && dz <= 0.0f) {
+        if (dx >= dy && dx >= dz) {
+            n = (qx >= 0.0f) ? ex : ex * -1.0f;
+            return dx;
+        }
+        if (dy >= dz) {
+            n = (qy >= 0.0f) ? ey : ey * -1.0f;
+            return dy;
+        }
+        n = (qz >= 0.0f) ? ez : ez * -1.0f;
+        return dz;
+    }
+    const float cx = dx > 0.0f ? dx : 0.0f, cy = dy > 0.0f ? dy : 0.0f, cz = dz > 0.0f ? dz : 0.0f;
+    const float dist = sqrtf(cx * cx + cy * cy + cz * cz);
+    const float inv = 1.0f / fmaxf(dist, 1.0e-30f);
+    const float sx = qx > 0.0f ? cx : -cx, sy = qy > 0.0f ? cy : -cy, sz = qz > 0.0f ? cz : -cz;
+    n = ex * (sx * inv) + ey * (sy * inv) + ez * (sz * inv);
+    return dist;
+}
+
+// ipc_f1_sf_over_x (friction.cuh:28-36).
+__device__ __forceinline__ float f1_sf_over_x(float u, float eps) {
+    return (u > eps) ? 1.0f / u : (-u / eps + 2.0f) / eps;
+}
+
 // bending_rest_angle_kernel (bending_constraint.cu:136-152).
 __global__ void rest_angle_kernel(const int4* __restrict__ dih, const float* __restrict__ pos,
                                   float* __restrict__ rest, int n) {
@@ -423,13 +458,78 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
 #pragma unroll
             for (int c = 0; c < 9; ++c) vrow[(k * 9 + c) * 64] = acc[(k * 9 + c) * 64];
     }
+    // static-shape contact at x_n (detect_kernel + scatter_gradient_kernel,
+    // static_contact.cu:107-213): the deepest plane / box penetration below
+    // the thickness h; penalty force k depth n and, with friction, the
+    // lagged tangential term alpha u_t on the gradient.
+    float cdepth = 0.0f, calpha = 0.0f;
+    F3 cn = f3(0.0f, 0.0f, 0.0f);
+    if (pr.contact_k > 0.0f && pr.nplane + pr.nbox > 0) {
+        for (int q = 0; q < pr.nplane; ++q) {
+            const float4 pl = m.shapes[pr.plane0 + q];
+            const F3 nq = f3(pl.x, pl.y, pl.z);
+            const float depth = pr.contact_h - (dot(nq, xi) + pl.w);
+            if (depth > cdepth) {
+                cdepth = depth;
+                cn = nq;
+            }
+        }
+        for (int q = 0; q < pr.nbox; ++q) {
+            F3 nq;
+            const float depth = pr.contact_h - box_sdf(m.shapes + pr.box0 + 4 * q, xi, nq);
+            if (depth > cdepth) {
+                cdepth = depth;
+                cn = nq;
+            }
+        }
+        if (cdepth > 0.0f) {
+            grad = grad + cn * (-pr.contact_k * cdepth);
+            if (pr.contact_mu > 0.0f) {
+                const F3 dxv = vi * dt;
+                const F3 ut = dxv - cn * dot(cn, dxv);
+                calpha = pr.contact_mu * (pr.contact_k * cdepth) * f1_sf_over_x(sqrtf(dot(ut, ut)), pr.contact_eps);
+                grad = grad + ut * (-calpha);
+            }
+        }
+    }
     // inertia (add_inertia_diag_kernel, cloth_simulator.cu:199-210)
     const float mi = mass * (1.0f / (dt * dt));
     D[0] += mi;
     D[4] += mi;
     D[8] += mi;
 
-    st3(v.rhs, row, b - grad);
+    F3 rr = b - grad;
+    if (cdepth > 0.0f) {
+        // bake_diag_kernel (static_contact.cu:232-300): k n n^T + alpha (I - n n^T)
+        const float kc = pr.contact_k;
+        D[0] += kc * cn.x * cn.x + calpha * (1.0f - cn.x * cn.x);
+        D[1] += kc * cn.x * cn.y - calpha * cn.x * cn.y;
+        D[2] += kc * cn.x * cn.z - calpha * cn.x * cn.z;
+        D[3] += kc * cn.x * cn.y - calpha * cn.x * cn.y;
+        D[4] += kc * cn.y * cn.y + calpha * (1.0f - cn.y * cn.y);
+        D[5] += kc * cn.y * cn.z - calpha * cn.y * cn.z;
+        D[6] += kc * cn.x * cn.z - calpha * cn.x * cn.z;
+        D[7] += kc * cn.y * cn.z - calpha * cn.y * cn.z;
+        D[8] += kc * cn.z * cn.z + calpha * (1.0f - cn.z * cn.z);
+        // apply_coulomb_friction_kernel (static_contact.cu:325-380): the
+        // tangential part of F0 = rhs - k depth n onto the cone of radius
+        // mu max(0, -F0 . n).
+        if (pr.contact_mu > 0.0f) {
+            const F3 F0 = rr - cn * (kc * cdepth);
+            const float f0n = dot(F0, cn);
+            const float fn = fmaxf(0.0f, -f0n);
+            if (fn > 0.0f) {
+                const F3 F0t = F0 - cn * f0n;
+                const float ft2 = dot(F0t, F0t);
+                if (ft2 > 0.0f) {
+                    const float ft = sqrtf(ft2);
+                    const float cone = pr.contact_mu * fn;
+                    rr = rr - F0t * ((ft <= cone) ? 1.0f : cone / ft);
+                }
+            }
+        }
+    }
+    st3(v.rhs, row, rr);
     float* dg = A.diag + (size_t)s * 9 * 64 + lane;
 #pragma unroll
     for (int c = 0; c < 9; ++c) dg[c * 64] = D[c];

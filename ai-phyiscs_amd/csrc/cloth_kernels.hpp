@@ -36,6 +36,9 @@ struct DevMesh {
     const float* dih_rest;
     const int* bend_inc_off;
     const int4* bend_inc;
+    // static shapes: planes (n.xyz, d); boxes as 4 float4
+    // (c.xyz h.x) (h.yz ex.xy) (ex.z ey.xyz) (ez.xyz 0)
+    const float4* shapes;
 };
 
 // Wave-sliced ELL matrix: entry (s, k, lane) of component c lives at

@@ -409,9 +409,22 @@ void Problem::build(const std::vector<const Instance*>& insts) {
 
 void Problem::upload_params(const std::vector<const Instance*>& insts, hipStream_t s) {
     std::vector<InstParams> p(insts.size());
+    std::vector<float4> shapes;
     for (size_t k = 0; k < insts.size(); ++k) {
         const Instance& I = *insts[k];
         InstParams& q = p[k];
+        q.contact_h = I.contact_h;
+        q.contact_k = I.contact_k;
+        q.contact_mu = I.contact_mu;
+        q.contact_eps = I.contact_eps;
+        q.plane0 = int(shapes.size());
+        q.nplane = int(I.planes.size() / 4);
+        for (size_t j = 0; j < I.planes.size(); j += 4)
+            shapes.push_back(make_float4(I.planes[j], I.planes[j + 1], I.planes[j + 2], I.planes[j + 3]));
+        q.box0 = int(shapes.size());
+        q.nbox = int(I.boxes.size() / 16);
+        for (size_t j = 0; j < I.boxes.size(); j += 4)
+            shapes.push_back(make_float4(I.boxes[j], I.boxes[j + 1], I.boxes[j + 2], I.boxes[j + 3]));
         q.tol2 = I.tol > 0.0 ? I.tol * I.tol : 0.0;
         q.max_iter = I.max_iter;
         q.gx = I.material.gx;
@@ -426,6 +439,7 @@ void Problem::upload_params(const std::vector<const Instance*>& insts, hipStream
         q.bend_k = I.has_bending ? I.bend_k : 0.0f;
     }
     if (p.empty()) return;
+    if (!shapes.empty()) shapes_.assign(shapes.data(), shapes.size(), s);
     check_hip(hipMemcpyAsync(params_.data(), p.data(), sizeof(InstParams) * p.size(), hipMemcpyHostToDevice, s),
               "hipMemcpyAsync(params)");
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(params)");
@@ -471,6 +485,7 @@ DevMesh Problem::mesh() const {
     m.dih_rest = dih_rest_.data();
     m.bend_inc_off = bend_inc_off_.data();
     m.bend_inc = bend_inc_.data();
+    m.shapes = shapes_.data();
     return m;
 }
 
@@ -921,6 +936,23 @@ void ClothSim::build_bending(float k, std::uintptr_t stream) {
     inst_.part0 = 0;
     compute_rest_bending(inst_, reinterpret_cast<const float*>(pos_), k, reinterpret_cast<hipStream_t>(stream));
     topo_dirty_ = true;
+}
+
+void ClothSim::add_static_plane(const float* n, float d) {
+    inst_.planes.insert(inst_.planes.end(), {n[0], n[1], n[2], d});
+    params_dirty_ = true;
+}
+
+void ClothSim::add_static_box(const float* c, const float* h, const float* ex, const float* ey, const float* ez) {
+    inst_.boxes.insert(inst_.boxes.end(), {c[0], c[1], c[2], h[0], h[1], h[2], ex[0], ex[1], ex[2], ey[0], ey[1], ey[2],
+                                           ez[0], ez[1], ez[2], 0.0f});
+    params_dirty_ = true;
+}
+
+void ClothSim::clear_static_shapes() {
+    inst_.planes.clear();
+    inst_.boxes.clear();
+    params_dirty_ = true;
 }
 
 void ClothSim::set_spring_stiffnesses(const float* k, int n) {

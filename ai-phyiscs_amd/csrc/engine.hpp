@@ -75,6 +75,10 @@ struct Instance {
     std::vector<int> dih;
     std::vector<float> dih_rest;
     float bend_k = 1.0e-3f;
+    // static-shape contact (StaticContactSet): planes 4 floats (n, d), boxes
+    // 16 floats (center, half extents, ex, ey, ez, 0)
+    std::vector<float> planes, boxes;
+    float contact_h = 0.0f, contact_k = 0.0f, contact_mu = 0.0f, contact_eps = 1.0e-4f;
     // pins
     std::vector<int> pin_idx;
     std::vector<float> pin_tgt;
@@ -243,6 +247,7 @@ private:
     DeviceArray<float4> tri_dminv_, sh_dminv_;
     DeviceArray<float> tri_area_, tri_k_, sh_area_, sh_k_, spr_rest_, spr_k_, pin_tgt_, dih_rest_;
     DeviceArray<int4> dih_v_, bend_inc_;
+    DeviceArray<float4> shapes_;
     DeviceArray<int> bend_inc_off_;
     std::vector<std::vector<int>> dih_slots_;
     int max_width_ = 0;
@@ -316,6 +321,11 @@ public:
     int num_bending() const { return inst_.has_bending ? int(inst_.dih.size() / 4) : 0; }
     void build_fem_shear(float k, std::uintptr_t stream);
     void build_bending(float k, std::uintptr_t stream);
+    // static-shape contact (cloth_simulator.h:398-450)
+    void add_static_plane(const float* n, float d);
+    void add_static_box(const float* center, const float* half_ext, const float* ex, const float* ey, const float* ez);
+    void clear_static_shapes();
+    Instance& contact_instance() { params_dirty_ = true; return inst_; }
     int num_pins() const { return int(inst_.pin_idx.size()); }
     int num_edges() const { return int(inst_.edges.size() / 2); }
     const std::vector<int>& edges() const { return inst_.edges; }

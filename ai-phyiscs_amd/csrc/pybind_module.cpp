@@ -236,6 +236,74 @@ PYBIND11_MODULE(_chysx_native, m) {
                  check(chysx_cloth_num_bending(s.p(), &n));
                  return n;
              })
+        .def(
+            "add_static_plane",
+            [](Sim& s, FltArr n, float d) {
+                if (n.ndim() != 1 || n.shape(0) != 3)
+                    throw py::value_error("ClothSimulator.add_static_plane: n must be a (3,) float32 vector");
+                check(chysx_cloth_add_static_plane(s.p(), n.data(), d));
+            },
+            py::arg("n"), py::arg("d"),
+            "Static plane dot(n, x) + d == 0; n points into the half-space the cloth stays in.")
+        .def(
+            "add_static_box",
+            [](Sim& s, FltArr c, FltArr h, FltArr ex, FltArr ey, FltArr ez) {
+                for (auto* a : {&c, &h, &ex, &ey, &ez})
+                    if (a->ndim() != 1 || a->shape(0) != 3)
+                        throw py::value_error("ClothSimulator.add_static_box: every argument must be a (3,) vector");
+                check(chysx_cloth_add_static_box(s.p(), c.data(), h.data(), ex.data(), ey.data(), ez.data()));
+            },
+            py::arg("center"), py::arg("half_ext"), py::arg("ex"), py::arg("ey"), py::arg("ez"),
+            "Static oriented box: unit columns (ex, ey, ez), half extents along them.")
+        .def("clear_static_shapes", [](Sim& s) { check(chysx_cloth_clear_static_shapes(s.p())); })
+        .def("static_plane_count",
+             [](const Sim& s) {
+                 int p = 0;
+                 check(chysx_cloth_static_shape_counts(s.p(), &p, nullptr));
+                 return p;
+             })
+        .def("static_box_count",
+             [](const Sim& s) {
+                 int b = 0;
+                 check(chysx_cloth_static_shape_counts(s.p(), nullptr, &b));
+                 return b;
+             })
+        .def(
+            "set_static_contact_thickness", [](Sim& s, float v) { check(chysx_cloth_set_static_contact(s.p(), 0, v)); },
+            py::arg("thickness"))
+        .def("static_contact_thickness",
+             [](const Sim& s) {
+                 float v = 0.0f;
+                 check(chysx_cloth_get_static_contact(s.p(), 0, &v));
+                 return v;
+             })
+        .def(
+            "set_static_contact_stiffness", [](Sim& s, float v) { check(chysx_cloth_set_static_contact(s.p(), 1, v)); },
+            py::arg("stiffness"))
+        .def("static_contact_stiffness",
+             [](const Sim& s) {
+                 float v = 0.0f;
+                 check(chysx_cloth_get_static_contact(s.p(), 1, &v));
+                 return v;
+             })
+        .def(
+            "set_static_contact_friction", [](Sim& s, float v) { check(chysx_cloth_set_static_contact(s.p(), 2, v)); },
+            py::arg("mu"))
+        .def("static_contact_friction",
+             [](const Sim& s) {
+                 float v = 0.0f;
+                 check(chysx_cloth_get_static_contact(s.p(), 2, &v));
+                 return v;
+             })
+        .def(
+            "set_static_contact_friction_epsilon", [](Sim& s, float v) { check(chysx_cloth_set_static_contact(s.p(), 3, v)); },
+            py::arg("eps_u"))
+        .def("static_contact_friction_epsilon",
+             [](const Sim& s) {
+                 float v = 0.0f;
+                 check(chysx_cloth_get_static_contact(s.p(), 3, &v));
+                 return v;
+             })
         .def("bending_dihedrals",
              [](const Sim& s) {
                  int n = 0;

@@ -100,6 +100,21 @@ int chysx_cloth_num_fem_shear(const chysx_cloth* sim, int* out);
 int chysx_cloth_build_bending(chysx_cloth* sim, float stiffness, uintptr_t stream);
 /* num_bending_dihedrals (bindings.cpp:314-318). */
 int chysx_cloth_num_bending(const chysx_cloth* sim, int* out);
+/* ---- static-shape contact (cloth_simulator.h:398-450; bindings.cpp:595-690) ----
+ * Penalty contact with planes dot(n, x) + d = 0 (n points to the free side)
+ * and oriented boxes (center, half extents along the unit columns ex, ey, ez):
+ * force k (h - dist) n on the deepest contact below thickness h, block
+ * k n n^T on the diagonal, and with friction mu > 0 the lagged IPC friction
+ * block alpha (I - n n^T) plus the Coulomb-cone projection of the force
+ * (static_contact.cu:107-380).  Active when stiffness > 0 and a shape exists. */
+int chysx_cloth_add_static_plane(chysx_cloth* sim, const float* n, float d);
+int chysx_cloth_add_static_box(chysx_cloth* sim, const float* center, const float* half_ext, const float* ex,
+                               const float* ey, const float* ez);
+int chysx_cloth_clear_static_shapes(chysx_cloth* sim);
+int chysx_cloth_static_shape_counts(const chysx_cloth* sim, int* planes, int* boxes);
+/* which: 0 thickness, 1 stiffness, 2 friction mu, 3 friction epsilon (default 1e-4). */
+int chysx_cloth_set_static_contact(chysx_cloth* sim, int which, float value);
+int chysx_cloth_get_static_contact(const chysx_cloth* sim, int which, float* value);
 /* Dihedrals (4 ints each: hinge v0 < v1, wings v2, v3) and rest angles of the
  * installed bending elements; either output may be null. */
 int chysx_cloth_get_dihedrals(const chysx_cloth* sim, int32_t* dihedrals, float* rest_angles);

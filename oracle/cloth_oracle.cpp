@@ -211,6 +211,16 @@ struct Cloth {
     int max_iter = 50;                     // cloth_simulator.h:713
     int threads = 1;
     bool assemble_only = false;            // test hook: build the step's system, skip solve + finalize
+    // static-shape contact (StaticContactSet, collision/static_contact.h:79-312)
+    struct Plane { T n[3]; T d; };
+    struct Box { T c[3], h[3], ex[3], ey[3], ez[3]; };
+    std::vector<Plane> planes;
+    std::vector<Box> boxes;
+    T contact_h = T(0), contact_k = T(0), contact_mu = T(0), contact_eps = T(1.0e-4);
+    bool contact_active() const { return contact_k > T(0) && (!planes.empty() || !boxes.empty()); }
+    // per-particle contact cache of the current step
+    std::vector<V3<T>> c_n, c_slip;
+    std::vector<T> c_depth, c_slipn;
 
     // ---- derived --------------------------------------------------------
     bool topo_dirty = true;
@@ -645,6 +655,59 @@ struct Cloth {
         last_converged = (tol > 0.0) && (last_relres <= tol);
     }
 
+    // ---- static-shape contact ---------------------------------------------
+    // box_signed_dist (static_contact.cu:55-99): local coordinates q,
+    // d_i = |q_i| - h_i; inside: the largest d_i and that face's normal;
+    // outside: |max(d, 0)| and the normalised clamped local offset.
+    static T box_sdf(const Box& b, V3<T> x, V3<T>& n) {
+        const V3<T> ex{b.ex[0], b.ex[1], b.ex[2]}, ey{b.ey[0], b.ey[1], b.ey[2]}, ez{b.ez[0], b.ez[1], b.ez[2]};
+        const V3<T> rel = x - V3<T>{b.c[0], b.c[1], b.c[2]};
+        const T qx = dot3(rel, ex), qy = dot3(rel, ey), qz = dot3(rel, ez);
+        const T dx = std::fabs(qx) - b.h[0], dy = std::fabs(qy) - b.h[1], dz = std::fabs(qz) - b.h[2];
+        if (dx <= T(0) && dy <= T(0) && dz <= T(0)) {
+            if (dx >= dy && dx >= dz) { n = (qx >= T(0)) ? ex : ex * T(-1); return dx; }
+            if (dy >= dz) { n = (qy >= T(0)) ? ey : ey * T(-1); return dy; }
+            n = (qz >= T(0)) ? ez : ez * T(-1);
+            return dz;
+        }
+        const T cx = dx > T(0) ? dx : T(0), cy = dy > T(0) ? dy : T(0), cz = dz > T(0) ? dz : T(0);
+        const T dist = std::sqrt(cx * cx + cy * cy + cz * cz);
+        const T inv = T(1) / std::max(dist, T(1.0e-30));
+        const T sx = qx > T(0) ? cx : -cx, sy = qy > T(0) ? cy : -cy, sz = qz > T(0) ? cz : -cz;
+        n = ex * (sx * inv) + ey * (sy * inv) + ez * (sz * inv);
+        return dist;
+    }
+    // detect_kernel (static_contact.cu:107-165): the deepest active contact
+    // (largest thickness - signed distance > 0) and the lagged tangential
+    // slip u_t = (v dt) - n (n . v dt).
+    void contact_detect(int i, const V3<T>& x, const V3<T>& v, T dt) {
+        T best = T(0);
+        V3<T> bn{0, 0, 0};
+        for (const Plane& pl : planes) {
+            const V3<T> n{pl.n[0], pl.n[1], pl.n[2]};
+            const T depth = contact_h - (dot3(n, x) + pl.d);
+            if (depth > best) { best = depth; bn = n; }
+        }
+        for (const Box& b : boxes) {
+            V3<T> n;
+            const T depth = contact_h - box_sdf(b, x, n);
+            if (depth > best) { best = depth; bn = n; }
+        }
+        c_depth[i] = best;
+        c_n[i] = bn;
+        V3<T> ut{0, 0, 0};
+        T un = T(0);
+        if (best > T(0) && dt > T(0)) {
+            const V3<T> dxv = v * dt;
+            ut = dxv - bn * dot3(bn, dxv);
+            un = len3(ut);
+        }
+        c_slip[i] = ut;
+        c_slipn[i] = un;
+    }
+    // ipc_f1_sf_over_x (friction.cuh:28-36).
+    static T f1_sf_over_x(T u, T eps) { return (u > eps) ? T(1) / u : (-u / eps + T(2)) / eps; }
+
     // ---- the step -------------------------------------------------------
     // ClothSimulator::step (src/cloth/cloth_simulator.cu:814-1261), PCG branch.
     void step(std::vector<V3<T>>& pos, std::vector<V3<T>>& vel, const std::vector<T>* inv_mass,
@@ -702,11 +765,50 @@ struct Cloth {
             const T coeff = bend_k * (F.theta - dih_rest[d]);
             for (int i = 0; i < 4; ++i) rhs[dih[4 * d + i]] = rhs[dih[4 * d + i]] + F.a[i] * coeff;
         }
+        // static contact (:989-1008): detect at x_n with the current velocity;
+        // scatter_gradient_kernel (static_contact.cu:182-213)
+        const bool contact = contact_active();
+        if (contact) {
+            c_n.assign(size_t(N), V3<T>{0, 0, 0});
+            c_slip.assign(size_t(N), V3<T>{0, 0, 0});
+            c_depth.assign(size_t(N), T(0));
+            c_slipn.assign(size_t(N), T(0));
+            for (int i = 0; i < N; ++i) {
+                contact_detect(i, xn[i], vel[i], dt);
+                const T depth = c_depth[i];
+                if (depth <= T(0)) continue;
+                rhs[i] = rhs[i] + c_n[i] * (-contact_k * depth);
+                if (contact_mu > T(0)) {
+                    const T alpha = contact_mu * (contact_k * depth) * f1_sf_over_x(c_slipn[i], contact_eps);
+                    rhs[i] = rhs[i] + c_slip[i] * (-alpha);
+                }
+            }
+        }
         // assemble_rhs_kernel (:127-142)
         const T inv_dt2 = T(1) / (dt * dt);
         for (int i = 0; i < N; ++i) {
             const T mi = mass[i] * inv_dt2;
             rhs[i] = (xt[i] - xn[i]) * mi - rhs[i];
+        }
+        // apply_coulomb_friction_kernel (static_contact.cu:325-380): project
+        // the inertial + elastic force's tangential part onto the cone of
+        // radius mu * max(0, -F0 . n).
+        if (contact && contact_mu > T(0)) {
+            for (int i = 0; i < N; ++i) {
+                const T depth = c_depth[i];
+                if (depth <= T(0)) continue;
+                const V3<T> n = c_n[i];
+                const V3<T> F0 = rhs[i] - n * (contact_k * depth);
+                const T F0n = dot3(F0, n);
+                const T fn = std::max(T(0), -F0n);
+                if (fn <= T(0)) continue;
+                const V3<T> F0t = F0 - n * F0n;
+                const T ft2 = dot3(F0t, F0t);
+                if (ft2 <= T(0)) continue;
+                const T ft = std::sqrt(ft2);
+                const T cone = contact_mu * fn;
+                rhs[i] = rhs[i] - F0t * ((ft <= cone) ? T(1) : cone / ft);
+            }
         }
         // topology (:1118-1121) and Hessian (:1129-1217)
         ensure_topology();
@@ -742,6 +844,19 @@ struct Cloth {
         for (int i = 0; i < N; ++i) {  // add_inertia_diag_kernel (:199-210)
             const T d = mass[i] * inv_dt2;
             diag[i].m[0] += d; diag[i].m[4] += d; diag[i].m[8] += d;
+        }
+        if (contact) {  // bake_diag_kernel (static_contact.cu:232-300), after inertia (:1189-1193)
+            for (int i = 0; i < N; ++i) {
+                const T depth = c_depth[i];
+                if (depth <= T(0)) continue;
+                const V3<T> n = c_n[i];
+                M3<T> H = m3_scale(m3_outer(n, n), contact_k);
+                if (contact_mu > T(0)) {
+                    const T alpha = contact_mu * (contact_k * depth) * f1_sf_over_x(c_slipn[i], contact_eps);
+                    H = m3_add(H, m3_scale(m3_sub(m3_ident<T>(), m3_outer(n, n)), alpha));
+                }
+                diag[i] = m3_add(diag[i], H);
+            }
         }
         if (assemble_only) return;  // system only: no solve, no state change
         // PCG with warm start dx (:1219-1249)
@@ -1057,6 +1172,46 @@ void orc_triangle_element(const double* xr, const double* x, double k, float the
     }
     std::copy(c.st.dminv.begin(), c.st.dminv.end(), dminv);
     *area = c.st.area[0];
+}
+// Static shapes and contact parameters (add_static_plane / add_static_box,
+// cloth_simulator.h:398-450).  box: center, half extents, ex, ey, ez (15).
+int orc_add_plane(void* h, const double* n, double d) {
+    return dispatch(h, [&](auto& c) {
+        using Cl = std::remove_reference_t<decltype(c)>;
+        typename Cl::Plane p;
+        for (int k = 0; k < 3; ++k) p.n[k] = n[k];
+        p.d = d;
+        c.planes.push_back(p);
+    });
+}
+int orc_add_box(void* h, const double* b15) {
+    return dispatch(h, [&](auto& c) {
+        using Cl = std::remove_reference_t<decltype(c)>;
+        typename Cl::Box b;
+        for (int k = 0; k < 3; ++k) {
+            b.c[k] = b15[k]; b.h[k] = b15[3 + k]; b.ex[k] = b15[6 + k]; b.ey[k] = b15[9 + k]; b.ez[k] = b15[12 + k];
+        }
+        c.boxes.push_back(b);
+    });
+}
+int orc_clear_shapes(void* h) {
+    return dispatch(h, [&](auto& c) { c.planes.clear(); c.boxes.clear(); });
+}
+int orc_set_contact(void* h, double thickness, double stiffness, double friction, double eps) {
+    return dispatch(h, [&](auto& c) {
+        c.contact_h = thickness; c.contact_k = stiffness; c.contact_mu = friction; c.contact_eps = eps;
+    });
+}
+// Signed distance and outward normal of a box (15 values as above).
+double orc_box_sdf(const double* b15, const double* x, double* n) {
+    Cloth<double>::Box b;
+    for (int k = 0; k < 3; ++k) {
+        b.c[k] = b15[k]; b.h[k] = b15[3 + k]; b.ex[k] = b15[6 + k]; b.ey[k] = b15[9 + k]; b.ez[k] = b15[12 + k];
+    }
+    V3<double> nn;
+    const double d = Cloth<double>::box_sdf(b, V3<double>{x[0], x[1], x[2]}, nn);
+    n[0] = nn.x; n[1] = nn.y; n[2] = nn.z;
+    return d;
 }
 // One dihedral (x: 4 x 3): angle theta, a_i = d theta / d x_i (4 x 3),
 // validity (compute_dihedral_frame, bending_constraint.cu:50-127).

@@ -59,6 +59,12 @@ def lib():
         L.orc_get_bending.argtypes = [C.c_void_p, _ip, _dp, _ip, _ip]
         L.orc_dihedral_element.argtypes = [_dp, _dp, _dp]
         L.orc_find_dihedrals.argtypes = [_ip, C.c_int, _ip]
+        L.orc_add_plane.argtypes = [C.c_void_p, _dp, C.c_double]
+        L.orc_add_box.argtypes = [C.c_void_p, _dp]
+        L.orc_clear_shapes.argtypes = [C.c_void_p]
+        L.orc_set_contact.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_box_sdf.argtypes = [_dp, _dp, _dp]
+        L.orc_box_sdf.restype = C.c_double
         L.orc_mat3_inverse.argtypes = [_dp, _dp]
         L.orc_pcg_solve.argtypes = [C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, _ip, _dp]
         L.orc_time_steps.restype = C.c_double
@@ -156,6 +162,20 @@ class OracleCloth:
         ds = np.zeros((nd, 16), np.int32)
         _check(self._L.orc_get_bending(self._h, _i(d), _d(r), _i(ss), _i(ds)))
         return d, r, ss, ds
+
+    def add_plane(self, n, d):
+        _check(self._L.orc_add_plane(self._h, _d(_f64(n).reshape(3)), float(d)))
+
+    def add_box(self, center, half_ext, ex=(1, 0, 0), ey=(0, 1, 0), ez=(0, 0, 1)):
+        b = np.concatenate([_f64(v).reshape(3) for v in (center, half_ext, ex, ey, ez)])
+        _check(self._L.orc_add_box(self._h, _d(b)))
+
+    def clear_shapes(self):
+        _check(self._L.orc_clear_shapes(self._h))
+
+    def set_contact(self, thickness, stiffness, friction=0.0, friction_epsilon=1e-4):
+        _check(self._L.orc_set_contact(self._h, float(thickness), float(stiffness), float(friction),
+                                       float(friction_epsilon)))
 
     def set_pins(self, idx, targets, k=1.0e6):
         i = np.ascontiguousarray(idx, np.int32)
@@ -305,6 +325,13 @@ def dihedral_element(x):
     a = np.zeros(12)
     ok = lib().orc_dihedral_element(_d(xx), _d(th), _d(a))
     return float(th[0]), a.reshape(4, 3), bool(ok)
+
+
+def box_sdf(center, half_ext, ex, ey, ez, x):
+    b = np.concatenate([np.asarray(v, np.float64).reshape(3) for v in (center, half_ext, ex, ey, ez)])
+    n = np.zeros(3)
+    d = lib().orc_box_sdf(_d(b), _d(np.ascontiguousarray(x, np.float64).reshape(3)), _d(n))
+    return float(d), n
 
 
 def find_dihedrals(tris):
