@@ -550,7 +550,9 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
             for (int i = 0; i < n; ++i) {
                 switch (which) {
                     case 0: launch_spmv(g, A, v.p0, v.ap, c.spmv_waves, s); break;
-                    case 1: launch_pcg_spmv(g, A, v, P.states(), c.spmv_waves, c.spmv_unroll, s); break;
+                    case 1:
+                        launch_pcg_spmv(g, A, v, P.states(), c.spmv_waves, c.spmv_unroll, s, c.spmv_branch_free);
+                        break;
                     case 2: launch_pcg_update(g, A, v, P.states(), c.update_waves, s); break;
                     case 3: launch_assemble(g, P.mesh(), A, v, ext, P.params(), dt, c.assemble_waves, P.max_width(), s); break;
                     case 4: P.launch_pcg_iteration(s); break;
@@ -578,29 +580,30 @@ int chysx_cloth_set_autotune(chysx_cloth* sim, int on) {
     });
 }
 
-int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg6) {
+int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg7) {
     return guard([&] {
         need(sim, "sim");
-        need(cfg6, "cfg");
+        need(cfg7, "cfg");
         const LaunchConfig& c = const_cast<chysx_cloth*>(sim)->sim.problem().launch_config();
-        const int v[6] = {c.assemble_waves, c.residual_waves, c.spmv_waves, c.spmv_unroll, c.update_waves,
-                          c.finalize_waves};
-        std::memcpy(cfg6, v, sizeof(v));
+        const int v[7] = {c.assemble_waves, c.residual_waves, c.spmv_waves,    c.spmv_unroll,
+                          c.update_waves,   c.finalize_waves, c.spmv_branch_free};
+        std::memcpy(cfg7, v, sizeof(v));
     });
 }
 
-int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg6) {
+int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg7) {
     return guard([&] {
         need(sim, "sim");
-        need(cfg6, "cfg");
+        need(cfg7, "cfg");
         sim->sim.ensure_topology();
         LaunchConfig c;
-        c.assemble_waves = cfg6[0];
-        c.residual_waves = cfg6[1];
-        c.spmv_waves = cfg6[2];
-        c.spmv_unroll = cfg6[3];
-        c.update_waves = cfg6[4];
-        c.finalize_waves = cfg6[5];
+        c.assemble_waves = cfg7[0];
+        c.residual_waves = cfg7[1];
+        c.spmv_waves = cfg7[2];
+        c.spmv_unroll = cfg7[3];
+        c.update_waves = cfg7[4];
+        c.finalize_waves = cfg7[5];
+        c.spmv_branch_free = cfg7[6];
         sim->sim.problem().set_launch_config(c);
     });
 }

@@ -629,7 +629,7 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_residual_kernel(LaunchGeom g, 
 // gathers.  Mode kModeCheck: the recursive residual passed the test, so this
 // pass computes the true residual instead.
 // ---------------------------------------------------------------------------
-template <int WAVES, int UNROLL>
+template <int WAVES, int UNROLL, bool BF>
 __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
                                                              const InstState* __restrict__ st) {
     const int s = my_slice<WAVES>(g.nslices);
@@ -667,7 +667,18 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
             for (int u = 0; u < UNROLL; ++u) c[u] = (k0 + u < w) ? colp[(k0 + u) * 64] : -1;
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
-                if (c[u] >= 0) {
+                if (BF) {
+                    // Padding entries hold zero blocks: gather the row's own
+                    // vector instead of branching (k0 + u < w is wave-uniform).
+                    if (k0 + u < w) {
+                        const int cc = (c[u] >= 0) ? c[u] : row;
+                        const float* bk = vp + (k0 + u) * 9 * 64;
+                        const F3 pj = ld3(z, cc) + ld3(pold, cc) * beta;
+                        a0 += bk[0] * pj.x + bk[64] * pj.y + bk[128] * pj.z;
+                        a1 += bk[192] * pj.x + bk[256] * pj.y + bk[320] * pj.z;
+                        a2 += bk[384] * pj.x + bk[448] * pj.y + bk[512] * pj.z;
+                    }
+                } else if (c[u] >= 0) {
                     const float* bk = vp + (k0 + u) * 9 * 64;
                     const F3 pj = ld3(z, c[u]) + ld3(pold, c[u]) * beta;
                     a0 += bk[0] * pj.x + bk[64] * pj.y + bk[128] * pj.z;
@@ -968,23 +979,38 @@ void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVecto
     CHYSX_LAUNCH_CHECK();
 }
 void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
-                     int unroll, hipStream_t s) {
+                     int unroll, hipStream_t s, int branch_free) {
     if (g.nslices <= 0) return;
-#define L1(W) pcg_spmv_kernel<W, 1><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-#define L2(W) pcg_spmv_kernel<W, 2><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-#define L4(W) pcg_spmv_kernel<W, 4><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-    if (unroll == 1) {
-        CHYSX_WAVES_SWITCH(waves, L1)
-    } else if (unroll == 2) {
-        CHYSX_WAVES_SWITCH(waves, L2)
-    } else if (unroll == 4) {
-        CHYSX_WAVES_SWITCH(waves, L4)
+#define L1(W) pcg_spmv_kernel<W, 1, false><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define L2(W) pcg_spmv_kernel<W, 2, false><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define L4(W) pcg_spmv_kernel<W, 4, false><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define B1(W) pcg_spmv_kernel<W, 1, true><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define B2(W) pcg_spmv_kernel<W, 2, true><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define B4(W) pcg_spmv_kernel<W, 4, true><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+    if (!valid_unroll(unroll)) throw std::invalid_argument("unsupported unroll");
+    if (!branch_free) {
+        if (unroll == 1) {
+            CHYSX_WAVES_SWITCH(waves, L1)
+        } else if (unroll == 2) {
+            CHYSX_WAVES_SWITCH(waves, L2)
+        } else {
+            CHYSX_WAVES_SWITCH(waves, L4)
+        }
     } else {
-        throw std::invalid_argument("unsupported unroll");
+        if (unroll == 1) {
+            CHYSX_WAVES_SWITCH(waves, B1)
+        } else if (unroll == 2) {
+            CHYSX_WAVES_SWITCH(waves, B2)
+        } else {
+            CHYSX_WAVES_SWITCH(waves, B4)
+        }
     }
 #undef L1
 #undef L2
 #undef L4
+#undef B1
+#undef B2
+#undef B4
     CHYSX_LAUNCH_CHECK();
 }
 void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,

@@ -98,7 +98,7 @@ void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVecto
                          int report, int waves, hipStream_t s);
 // SpMV pass (direction update, A p, p . A p; or the true-residual check).
 void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
-                     int unroll, hipStream_t s);
+                     int unroll, hipStream_t s, int branch_free = 0);
 // Update pass (x, r, z; r . z, r . r).
 void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
                        hipStream_t s);

@@ -500,7 +500,8 @@ void Problem::launch_pcg_start(hipStream_t s) {
     launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseStart, s);
 }
 void Problem::launch_pcg_iteration(hipStream_t s) {
-    launch_pcg_spmv(geom(), matrix(), vectors(), state_.data(), cfg_.spmv_waves, cfg_.spmv_unroll, s);
+    launch_pcg_spmv(geom(), matrix(), vectors(), state_.data(), cfg_.spmv_waves, cfg_.spmv_unroll, s,
+                    cfg_.spmv_branch_free);
     launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseSigma, s);
     launch_pcg_update(geom(), matrix(), vectors(), state_.data(), cfg_.update_waves, s);
     launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseRho, s);
@@ -508,7 +509,8 @@ void Problem::launch_pcg_iteration(hipStream_t s) {
 
 void Problem::set_launch_config(const LaunchConfig& c) {
     if (!valid_waves(c.assemble_waves) || !valid_waves(c.residual_waves) || !valid_waves(c.spmv_waves) ||
-        !valid_waves(c.update_waves) || !valid_waves(c.finalize_waves) || !valid_unroll(c.spmv_unroll))
+        !valid_waves(c.update_waves) || !valid_waves(c.finalize_waves) || !valid_unroll(c.spmv_unroll) ||
+        (c.spmv_branch_free != 0 && c.spmv_branch_free != 1))
         throw std::invalid_argument("launch config: waves must be 1, 2, 4, 8 or 16 and unroll 1, 2 or 4");
     cfg_ = c;
     destroy_graph();
@@ -572,16 +574,18 @@ void Problem::autotune(hipStream_t s) {
     const int unroll_list[] = {1, 2, 4};
     double best = 1e30;
     tune_log_.clear();
-    for (int w : waves_list)
-        for (int u : unroll_list) {
-            const double t = time_it([&] { launch_pcg_spmv(geom(), matrix(), v, sst.data(), w, u, s); });
-            tune_log_.push_back({0, w, u, t});
-            if (t < best) {
-                best = t;
-                cfg_.spmv_waves = w;
-                cfg_.spmv_unroll = u;
+    for (int bf = 0; bf < 2; ++bf)
+        for (int w : waves_list)
+            for (int u : unroll_list) {
+                const double t = time_it([&] { launch_pcg_spmv(geom(), matrix(), v, sst.data(), w, u, s, bf); });
+                tune_log_.push_back({bf ? 2 : 0, w, u, t});
+                if (t < best) {
+                    best = t;
+                    cfg_.spmv_waves = w;
+                    cfg_.spmv_unroll = u;
+                    cfg_.spmv_branch_free = bf;
+                }
             }
-        }
     best = 1e30;
     for (int w : waves_list) {
         const double t = time_it([&] { launch_pcg_update(geom(), matrix(), v, sst.data(), w, s); });

@@ -98,12 +98,13 @@ struct LaunchConfig {
     int residual_waves = 4;
     int spmv_waves = 4;
     int spmv_unroll = 1;
+    int spmv_branch_free = 0;  // 1: padding entries gather the row's own vector (zero blocks) instead of a branch
     int update_waves = 4;
     int finalize_waves = 4;
 };
 
 struct TuneSample {
-    int kernel;  // 0: SpMV pass, 1: update pass
+    int kernel;  // 0: SpMV pass, 1: update pass, 2: branch-free SpMV pass
     int waves;
     int unroll;
     double ms;

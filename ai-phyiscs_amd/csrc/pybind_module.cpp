@@ -424,7 +424,7 @@ PYBIND11_MODULE(_chysx_native, m) {
             py::arg("on"))
         .def("launch_config",
              [](const Sim& s) {
-                 int c[6];
+                 int c[7];
                  check(chysx_cloth_launch_config(s.p(), c));
                  py::dict d;
                  d["assemble_waves"] = c[0];
@@ -433,16 +433,17 @@ PYBIND11_MODULE(_chysx_native, m) {
                  d["spmv_unroll"] = c[3];
                  d["update_waves"] = c[4];
                  d["finalize_waves"] = c[5];
+                 d["spmv_branch_free"] = c[6];
                  return d;
              })
         .def(
             "set_launch_config",
             [](Sim& s, py::dict d) {
-                int c[6];
+                int c[7];
                 check(chysx_cloth_launch_config(s.p(), c));
-                const char* keys[6] = {"assemble_waves", "residual_waves", "spmv_waves",
-                                       "spmv_unroll", "update_waves", "finalize_waves"};
-                for (int i = 0; i < 6; ++i)
+                const char* keys[7] = {"assemble_waves", "residual_waves", "spmv_waves",      "spmv_unroll",
+                                       "update_waves",   "finalize_waves", "spmv_branch_free"};
+                for (int i = 0; i < 7; ++i)
                     if (d.contains(keys[i])) c[i] = d[keys[i]].cast<int>();
                 check(chysx_cloth_set_launch_config(s.p(), c));
             },
@@ -456,8 +457,10 @@ PYBIND11_MODULE(_chysx_native, m) {
                  check(chysx_cloth_tune_log(s.p(), &n, kwu.data(), ms.data()));
                  py::list out;
                  for (int i = 0; i < n; ++i)
-                     out.append(py::make_tuple(kwu[3 * i] == 0 ? "spmv" : "update", kwu[3 * i + 1], kwu[3 * i + 2],
-                                               ms[i]));
+                     out.append(py::make_tuple(kwu[3 * i] == 0   ? "spmv"
+                                               : kwu[3 * i] == 2 ? "spmv_branch_free"
+                                                                 : "update",
+                                               kwu[3 * i + 1], kwu[3 * i + 2], ms[i]));
                  return out;
              })
         .def(

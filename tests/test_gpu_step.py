@@ -231,7 +231,9 @@ def test_launch_shapes_do_not_change_results(cuda):
     runs = []
     for cfg in [dict(spmv_waves=1, spmv_unroll=1, update_waves=16, assemble_waves=2),
                 dict(spmv_waves=16, spmv_unroll=4, update_waves=1, assemble_waves=8),
-                dict(spmv_waves=4, spmv_unroll=2, update_waves=4, residual_waves=1, finalize_waves=16)]:
+                dict(spmv_waves=4, spmv_unroll=2, update_waves=4, residual_waves=1, finalize_waves=16),
+                dict(spmv_waves=8, spmv_unroll=4, spmv_branch_free=1),
+                dict(spmv_waves=2, spmv_unroll=1, spmv_branch_free=1)]:
         g = make_sim(s, tol=1e-6)
         g.sim.set_autotune(False)
         g.sim.set_launch_config(cfg)
