@@ -543,7 +543,7 @@ void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep) {
     const int rows = g.nslices * 64;
     if (rows > rows_) throw std::invalid_argument("warm start: reserve() rows first");
     if (keep && !training_) throw std::invalid_argument("warm start: reserve(training) before a kept forward");
-    (void)keep;
+    // Pre-activations are only needed by backward().
     // encoder
     GemmArgs a{};
     a.rows = rows;
@@ -555,7 +555,7 @@ void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep) {
     a.bias = theta_ + OFF_BE;
     a.relu = 1;
     a.out_b = H_[0];
-    a.pre_b = P_[0];
+    a.pre_b = keep ? P_[0] : nullptr;
     run_gemm<HID / 16>(a, s);
     for (int l = 0; l < NL; ++l) {
         hipLaunchKernelGGL(ws_agg_kernel, dim3((g.nslices + 3) / 4), dim3(256), 0, s, g, H_[l], A_[l]);
@@ -575,7 +575,7 @@ void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep) {
         b.relu = 1;
         b.resid = H_[l];
         b.out_b = H_[l + 1];
-        b.pre_b = P_[l + 1];
+        b.pre_b = keep ? P_[l + 1] : nullptr;
         run_gemm<HID / 16>(b, s);
     }
     GemmArgs d{};
