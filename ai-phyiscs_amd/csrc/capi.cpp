@@ -551,7 +551,8 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
                 switch (which) {
                     case 0: launch_spmv(g, A, v.p0, v.ap, c.spmv_waves, s); break;
                     case 1:
-                        launch_pcg_spmv(g, A, v, P.states(), c.spmv_waves, c.spmv_unroll, s, c.spmv_branch_free);
+                        launch_pcg_spmv(g, A, v, P.states(), c.spmv_waves, c.spmv_unroll, s, c.spmv_branch_free,
+                                        c.spmv_symmetric);
                         break;
                     case 2: launch_pcg_update(g, A, v, P.states(), c.update_waves, s); break;
                     case 3: launch_assemble(g, P.mesh(), A, v, ext, P.params(), dt, c.assemble_waves, P.max_width(), s); break;
@@ -580,30 +581,31 @@ int chysx_cloth_set_autotune(chysx_cloth* sim, int on) {
     });
 }
 
-int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg7) {
+int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg8) {
     return guard([&] {
         need(sim, "sim");
-        need(cfg7, "cfg");
+        need(cfg8, "cfg");
         const LaunchConfig& c = const_cast<chysx_cloth*>(sim)->sim.problem().launch_config();
-        const int v[7] = {c.assemble_waves, c.residual_waves, c.spmv_waves,    c.spmv_unroll,
-                          c.update_waves,   c.finalize_waves, c.spmv_branch_free};
-        std::memcpy(cfg7, v, sizeof(v));
+        const int v[8] = {c.assemble_waves, c.residual_waves, c.spmv_waves,       c.spmv_unroll,
+                          c.update_waves,   c.finalize_waves, c.spmv_branch_free, c.spmv_symmetric};
+        std::memcpy(cfg8, v, sizeof(v));
     });
 }
 
-int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg7) {
+int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg8) {
     return guard([&] {
         need(sim, "sim");
-        need(cfg7, "cfg");
+        need(cfg8, "cfg");
         sim->sim.ensure_topology();
         LaunchConfig c;
-        c.assemble_waves = cfg7[0];
-        c.residual_waves = cfg7[1];
-        c.spmv_waves = cfg7[2];
-        c.spmv_unroll = cfg7[3];
-        c.update_waves = cfg7[4];
-        c.finalize_waves = cfg7[5];
-        c.spmv_branch_free = cfg7[6];
+        c.assemble_waves = cfg8[0];
+        c.residual_waves = cfg8[1];
+        c.spmv_waves = cfg8[2];
+        c.spmv_unroll = cfg8[3];
+        c.update_waves = cfg8[4];
+        c.finalize_waves = cfg8[5];
+        c.spmv_branch_free = cfg8[6];
+        c.spmv_symmetric = cfg8[7];
         sim->sim.problem().set_launch_config(c);
     });
 }

@@ -629,7 +629,7 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_residual_kernel(LaunchGeom g, 
 // gathers.  Mode kModeCheck: the recursive residual passed the test, so this
 // pass computes the true residual instead.
 // ---------------------------------------------------------------------------
-template <int WAVES, int UNROLL, bool BF>
+template <int WAVES, int UNROLL, bool BF, bool SYM>
 __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
                                                              const InstState* __restrict__ st) {
     const int s = my_slice<WAVES>(g.nslices);
@@ -660,6 +660,7 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
         float a2 = dg[384] * pi.x + dg[448] * pi.y + dg[512] * pi.z;
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
+        const int* mirp = A.mir + (size_t)sb * 64 + lane;
         const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
         for (int k0 = 0; k0 < w; k0 += UNROLL) {
             int c[UNROLL];
@@ -667,23 +668,25 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
             for (int u = 0; u < UNROLL; ++u) c[u] = (k0 + u < w) ? colp[(k0 + u) * 64] : -1;
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
-                if (BF) {
-                    // Padding entries hold zero blocks: gather the row's own
-                    // vector instead of branching (k0 + u < w is wave-uniform).
-                    if (k0 + u < w) {
-                        const int cc = (c[u] >= 0) ? c[u] : row;
+                // BF: padding entries hold zero blocks and gather the row's
+                // own vector instead of branching (k0 + u < w is wave-uniform).
+                const bool live = BF ? (k0 + u < w) : (c[u] >= 0);
+                if (live) {
+                    const int cc = (c[u] >= 0) ? c[u] : row;
+                    const F3 pj = ld3(z, cc) + ld3(pold, cc) * beta;
+                    if (SYM && cc < row) {
+                        // lower block (row, cc) = transpose of (cc, row), which
+                        // row cc's lane streamed moments ago (L2-resident)
+                        const float* bt = A.val + mirp[(k0 + u) * 64];
+                        a0 += bt[0] * pj.x + bt[192] * pj.y + bt[384] * pj.z;
+                        a1 += bt[64] * pj.x + bt[256] * pj.y + bt[448] * pj.z;
+                        a2 += bt[128] * pj.x + bt[320] * pj.y + bt[512] * pj.z;
+                    } else {
                         const float* bk = vp + (k0 + u) * 9 * 64;
-                        const F3 pj = ld3(z, cc) + ld3(pold, cc) * beta;
                         a0 += bk[0] * pj.x + bk[64] * pj.y + bk[128] * pj.z;
                         a1 += bk[192] * pj.x + bk[256] * pj.y + bk[320] * pj.z;
                         a2 += bk[384] * pj.x + bk[448] * pj.y + bk[512] * pj.z;
                     }
-                } else if (c[u] >= 0) {
-                    const float* bk = vp + (k0 + u) * 9 * 64;
-                    const F3 pj = ld3(z, c[u]) + ld3(pold, c[u]) * beta;
-                    a0 += bk[0] * pj.x + bk[64] * pj.y + bk[128] * pj.z;
-                    a1 += bk[192] * pj.x + bk[256] * pj.y + bk[320] * pj.z;
-                    a2 += bk[384] * pj.x + bk[448] * pj.y + bk[512] * pj.z;
                 }
             }
         }
@@ -978,39 +981,38 @@ void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVecto
 #undef L
     CHYSX_LAUNCH_CHECK();
 }
-void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
-                     int unroll, hipStream_t s, int branch_free) {
-    if (g.nslices <= 0) return;
-#define L1(W) pcg_spmv_kernel<W, 1, false><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-#define L2(W) pcg_spmv_kernel<W, 2, false><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-#define L4(W) pcg_spmv_kernel<W, 4, false><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-#define B1(W) pcg_spmv_kernel<W, 1, true><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-#define B2(W) pcg_spmv_kernel<W, 2, true><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-#define B4(W) pcg_spmv_kernel<W, 4, true><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-    if (!valid_unroll(unroll)) throw std::invalid_argument("unsupported unroll");
-    if (!branch_free) {
-        if (unroll == 1) {
-            CHYSX_WAVES_SWITCH(waves, L1)
-        } else if (unroll == 2) {
-            CHYSX_WAVES_SWITCH(waves, L2)
-        } else {
-            CHYSX_WAVES_SWITCH(waves, L4)
-        }
+namespace {
+template <bool BF, bool SYM>
+void pcg_spmv_dispatch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
+                       int unroll, hipStream_t s) {
+#define L1(W) pcg_spmv_kernel<W, 1, BF, SYM><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define L2(W) pcg_spmv_kernel<W, 2, BF, SYM><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define L4(W) pcg_spmv_kernel<W, 4, BF, SYM><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+    if (unroll == 1) {
+        CHYSX_WAVES_SWITCH(waves, L1)
+    } else if (unroll == 2) {
+        CHYSX_WAVES_SWITCH(waves, L2)
     } else {
-        if (unroll == 1) {
-            CHYSX_WAVES_SWITCH(waves, B1)
-        } else if (unroll == 2) {
-            CHYSX_WAVES_SWITCH(waves, B2)
-        } else {
-            CHYSX_WAVES_SWITCH(waves, B4)
-        }
+        CHYSX_WAVES_SWITCH(waves, L4)
     }
 #undef L1
 #undef L2
 #undef L4
-#undef B1
-#undef B2
-#undef B4
+}
+}  // namespace
+
+void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
+                     int unroll, hipStream_t s, int branch_free, int symmetric) {
+    if (g.nslices <= 0) return;
+    if (!valid_unroll(unroll)) throw std::invalid_argument("unsupported unroll");
+    if (symmetric && A.mir == nullptr) throw std::invalid_argument("symmetric SpMV needs the mirror offsets");
+    if (branch_free) {
+        if (symmetric) pcg_spmv_dispatch<true, true>(g, A, v, st, waves, unroll, s);
+        else pcg_spmv_dispatch<true, false>(g, A, v, st, waves, unroll, s);
+    } else {
+        if (symmetric) pcg_spmv_dispatch<false, true>(g, A, v, st, waves, unroll, s);
+        else pcg_spmv_dispatch<false, false>(g, A, v, st, waves, unroll, s);
+    }
     CHYSX_LAUNCH_CHECK();
 }
 void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,

@@ -52,6 +52,9 @@ struct DevMatrix {
     float* val;
     float* diag;
     float* minv;
+    // For an entry (i, j) with j < i: float offset of component 0 of the
+    // block (j, i) in val (its transpose, stored by row j); -1 otherwise.
+    const int* mir;
 };
 
 // Solver vectors in padded row space, 3 components per row (AoS).
@@ -98,7 +101,7 @@ void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVecto
                          int report, int waves, hipStream_t s);
 // SpMV pass (direction update, A p, p . A p; or the true-residual check).
 void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
-                     int unroll, hipStream_t s, int branch_free = 0);
+                     int unroll, hipStream_t s, int branch_free = 0, int symmetric = 0);
 // Update pass (x, r, z; r . z, r . r).
 void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
                        hipStream_t s);

@@ -1,6 +1,8 @@
 // SPDX-License-Identifier: Apache-2.0
 #include "engine.hpp"
 
+#include <climits>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -314,6 +316,31 @@ void Problem::build(const std::vector<const Instance*>& insts) {
             sell_col[base + q] = (c >= 0) ? c + r0 : -1;
         }
     }
+    // Mirror offsets of the lower entries: block (i, j), j < i, is the
+    // transpose of block (j, i) stored by row j.
+    std::vector<int> mir(sell_col.size(), -1);
+    if (size_t(entries) * 9 * kSlice > size_t(INT32_MAX)) throw std::invalid_argument("matrix too large");
+    for (int sl = 0; sl < nslices; ++sl) {
+        const int w = slice_ptr_h_[size_t(sl) + 1] - slice_ptr_h_[size_t(sl)];
+        for (int lane = 0; lane < kSlice; ++lane) {
+            const int i = sl * kSlice + lane;
+            for (int k = 0; k < w; ++k) {
+                const size_t e = (size_t(slice_ptr_h_[size_t(sl)]) + k) * kSlice + lane;
+                const int j = sell_col[e];
+                if (j < 0 || j >= i) continue;
+                const int sj = j / kSlice, lj = j % kSlice;
+                const int wj = slice_ptr_h_[size_t(sj) + 1] - slice_ptr_h_[size_t(sj)];
+                int kk = -1;
+                for (int q = 0; q < wj; ++q)
+                    if (sell_col[(size_t(slice_ptr_h_[size_t(sj)]) + q) * kSlice + lj] == i) {
+                        kk = q;
+                        break;
+                    }
+                if (kk < 0) throw std::logic_error("pattern is not symmetric");
+                mir[e] = int((size_t(slice_ptr_h_[size_t(sj)]) + kk) * 9 * kSlice + lj);
+            }
+        }
+    }
 
     // ---- slices ------------------------------------------------------------
     slices_h_.clear();
@@ -357,6 +384,7 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     pin_inc_.assign(pin_inc.data(), pin_inc.size(), s);
     slice_ptr_.assign(slice_ptr_h_.data(), slice_ptr_h_.size(), s);
     col_.assign(sell_col.data(), sell_col.size(), s);
+    mir_.assign(mir.data(), mir.size(), s);
     row_deg_.assign(row_deg_h_.data(), row_deg_h_.size(), s);
     val_.resize(size_t(entries) * kSlice * 9);
     val_.zero(s);
@@ -455,7 +483,8 @@ void Problem::update_pin_targets(const Instance& inst, int k, hipStream_t s) {
 }
 
 DevMatrix Problem::matrix() const {
-    return DevMatrix{slice_ptr_.data(), col_.data(), row_deg_.data(), val_.data(), diag_.data(), minv_.data()};
+    return DevMatrix{slice_ptr_.data(), col_.data(), row_deg_.data(), val_.data(), diag_.data(), minv_.data(),
+                     mir_.data()};
 }
 DevVectors Problem::vectors() const {
     return DevVectors{rhs_.data(), r_.data(), z_.data(), ap_.data(), p0_.data(), p1_.data(), x_.data(),
@@ -501,7 +530,7 @@ void Problem::launch_pcg_start(hipStream_t s) {
 }
 void Problem::launch_pcg_iteration(hipStream_t s) {
     launch_pcg_spmv(geom(), matrix(), vectors(), state_.data(), cfg_.spmv_waves, cfg_.spmv_unroll, s,
-                    cfg_.spmv_branch_free);
+                    cfg_.spmv_branch_free, cfg_.spmv_symmetric);
     launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseSigma, s);
     launch_pcg_update(geom(), matrix(), vectors(), state_.data(), cfg_.update_waves, s);
     launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseRho, s);
@@ -510,7 +539,7 @@ void Problem::launch_pcg_iteration(hipStream_t s) {
 void Problem::set_launch_config(const LaunchConfig& c) {
     if (!valid_waves(c.assemble_waves) || !valid_waves(c.residual_waves) || !valid_waves(c.spmv_waves) ||
         !valid_waves(c.update_waves) || !valid_waves(c.finalize_waves) || !valid_unroll(c.spmv_unroll) ||
-        (c.spmv_branch_free != 0 && c.spmv_branch_free != 1))
+        (c.spmv_branch_free != 0 && c.spmv_branch_free != 1) || (c.spmv_symmetric != 0 && c.spmv_symmetric != 1))
         throw std::invalid_argument("launch config: waves must be 1, 2, 4, 8 or 16 and unroll 1, 2 or 4");
     cfg_ = c;
     destroy_graph();
@@ -577,7 +606,8 @@ void Problem::autotune(hipStream_t s) {
     for (int bf = 0; bf < 2; ++bf)
         for (int w : waves_list)
             for (int u : unroll_list) {
-                const double t = time_it([&] { launch_pcg_spmv(geom(), matrix(), v, sst.data(), w, u, s, bf); });
+                const double t = time_it(
+                    [&] { launch_pcg_spmv(geom(), matrix(), v, sst.data(), w, u, s, bf, cfg_.spmv_symmetric); });
                 tune_log_.push_back({bf ? 2 : 0, w, u, t});
                 if (t < best) {
                     best = t;

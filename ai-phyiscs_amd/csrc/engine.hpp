@@ -99,6 +99,10 @@ struct LaunchConfig {
     int spmv_waves = 4;
     int spmv_unroll = 1;
     int spmv_branch_free = 0;  // 1: padding entries gather the row's own vector (zero blocks) instead of a branch
+    // 1: the SpMV pass reads each lower block as the transpose of its upper
+    // mirror (streams half the off-diagonal values; changes results at FP32
+    // rounding level, so it is a fixed choice, never picked by the autotuner)
+    int spmv_symmetric = 0;
     int update_waves = 4;
     int finalize_waves = 4;
 };
@@ -256,7 +260,7 @@ private:
     DeviceArray<int2> tri_inc_, spr_inc_;
     std::vector<int> pin_offset_;  // first global pin of each instance
     // matrix
-    DeviceArray<int> slice_ptr_, col_, row_deg_;
+    DeviceArray<int> slice_ptr_, col_, row_deg_, mir_;
     DeviceArray<float> val_, diag_, minv_;
     // vectors
     DeviceArray<float> rhs_, r_, z_, ap_, p0_, p1_;

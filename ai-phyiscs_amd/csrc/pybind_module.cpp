@@ -424,7 +424,7 @@ PYBIND11_MODULE(_chysx_native, m) {
             py::arg("on"))
         .def("launch_config",
              [](const Sim& s) {
-                 int c[7];
+                 int c[8];
                  check(chysx_cloth_launch_config(s.p(), c));
                  py::dict d;
                  d["assemble_waves"] = c[0];
@@ -434,16 +434,18 @@ PYBIND11_MODULE(_chysx_native, m) {
                  d["update_waves"] = c[4];
                  d["finalize_waves"] = c[5];
                  d["spmv_branch_free"] = c[6];
+                 d["spmv_symmetric"] = c[7];
                  return d;
              })
         .def(
             "set_launch_config",
             [](Sim& s, py::dict d) {
-                int c[7];
+                int c[8];
                 check(chysx_cloth_launch_config(s.p(), c));
-                const char* keys[7] = {"assemble_waves", "residual_waves", "spmv_waves",      "spmv_unroll",
-                                       "update_waves",   "finalize_waves", "spmv_branch_free"};
-                for (int i = 0; i < 7; ++i)
+                const char* keys[8] = {"assemble_waves", "residual_waves",   "spmv_waves",    "spmv_unroll",
+                                       "update_waves",   "finalize_waves",   "spmv_branch_free",
+                                       "spmv_symmetric"};
+                for (int i = 0; i < 8; ++i)
                     if (d.contains(keys[i])) c[i] = d[keys[i]].cast<int>();
                 check(chysx_cloth_set_launch_config(s.p(), c));
             },

@@ -220,9 +220,12 @@ int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts
  * The solver state is left unusable for warm starts; call before a step. */
 int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uintptr_t stream, double* ms);
 /* Kernel launch shapes (speed only: results never depend on them).
- * cfg[7] = {assemble waves, residual waves, SpMV-pass waves, SpMV unroll,
- * update-pass waves, finalize waves, SpMV branch-free padding (0/1)}; waves
- * in {1,2,4,8,16} wavefronts of 64 lanes per workgroup, unroll in {1,2,4}.  By default an autotuner times the
+ * cfg[8] = {assemble waves, residual waves, SpMV-pass waves, SpMV unroll,
+ * update-pass waves, finalize waves, SpMV branch-free padding (0/1), SpMV
+ * symmetric (0/1: lower blocks read as transposes of their upper mirrors;
+ * changes results at FP32 rounding level, never chosen by the autotuner)};
+ * waves in {1,2,4,8,16} wavefronts of 64 lanes per workgroup, unroll in
+ * {1,2,4}.  By default an autotuner times the
  * SpMV and update shapes on the problem's own matrix when its topology is
  * built (problems of 32768 rows or more). */
 int chysx_cloth_set_autotune(chysx_cloth* sim, int on);
