@@ -317,6 +317,12 @@ def main():
     t_spmv = g.sim.time_kernel(0, args.kernel_reps, s.dt, stream)
     t_iter = g.sim.time_kernel(4, args.kernel_reps, s.dt, stream)
     gbs = lambda b, ms: b / (ms * 1e-3) / 1e9  # noqa: E731
+    # The other SpMV storage mode, same matrix and shapes (speed only).
+    cfg0 = g.sim.launch_config()
+    g.sim.set_launch_config(dict(spmv_symmetric=1 - cfg0["spmv_symmetric"]))
+    t_alt = g.sim.time_kernel(1, args.kernel_reps, s.dt, stream)
+    t_alt_iter = g.sim.time_kernel(4, args.kernel_reps, s.dt, stream)
+    g.sim.set_launch_config(dict(spmv_symmetric=cfg0["spmv_symmetric"]))
 
     traffic = read_traffic(args.size)
     roofline = {
@@ -338,6 +344,9 @@ def main():
                        "GB/s": round(gbs(kb["pcg_update"], t_pcg_update), 1)},
         "pcg_iteration": {"ref_bytes": kb["pcg_iteration_ref"], "ms": round(t_iter, 5),
                           "GB/s_ref_formulation": round(gbs(kb["pcg_iteration_ref"], t_iter), 1)},
+        ("pcg_spmv_symmetric" if not cfg0["spmv_symmetric"] else "pcg_spmv_general"): {
+            "ms": round(t_alt, 5), "GB/s_algorithmic": round(gbs(kb["pcg_spmv"], t_alt), 1),
+            "frac": round(gbs(kb["pcg_spmv"], t_alt) / PEAK_HBM_GBS, 4), "iteration_ms": round(t_alt_iter, 5)},
     }
 
     ws_cmp = None
