@@ -133,21 +133,35 @@ def test_projection_matches_numpy_galerkin(cuda):
 
 
 def test_projection_cuts_iterations_same_solution(cuda):
+    """Fewer iterations, and every step still the oracle's step: at steps 15,
+    25 and 35 the oracle steps from the GPU's state (tol 1e-11) and the GPU's
+    next positions must match to 1e-5.  (Whole trajectories of this wind cloth
+    are not compared across warm starts: it amplifies 1e-6-level solve
+    differences chaotically.)"""
+    from oracle.oracle import oracle_from_scene
     s = scenes.config3(w=64, steps=40)
-    its, pos = {}, {}
+    its = {}
     for mode in ["previous", "projection"]:
         g = make_sim(s, tol=1e-6)
         g.sim.set_warm_start_mode(mode)
+        oc, _, _, im, f = oracle_from_scene(s, precision=64, tol=1e-11, max_iter=100000)
         it = []
         for k in range(40):
+            check = mode == "projection" and k in (15, 25, 35)
+            if check:
+                p0 = g.positions().astype(np.float64)
+                v0 = g.velocities().astype(np.float64)
+                dx0 = g.sim.get_dx64()
             g.sim.step(s.dt)
             st = g.sim.last_solve_stats()
             assert st["converged"], (mode, k, st)
             it.append(st["iters"])
+            if check:
+                oc.set_dx(dx0)
+                oc.step(p0, v0, im, f, s.dt, 1)
+                assert rel(g.positions(), p0) <= 1e-5, k
         its[mode] = float(np.mean(it[10:]))
-        pos[mode] = g.positions()
     assert its["projection"] < 0.9 * its["previous"], its
-    assert np.abs(pos["projection"] - pos["previous"]).max() < 1e-5
 
 
 def test_zero_network_leaves_projection_unchanged(cuda):
