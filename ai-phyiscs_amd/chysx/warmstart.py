@@ -2,11 +2,14 @@
 
 The network itself lives in the library (BF16 MFMA forward and backward
 kernels, Adam on the GPU).  Training pairs (state, converged dx) come from
-stepping random 256x256 cloths (random pins, wind, stiffness and mass) in the
-learned mode with training on: each step of a batch of cloths gives one pair
-per cloth and one Adam step over the whole batch on the line-search loss
-(the share of the base guess's A-norm error the network's direction leaves).
-The solution each step converges to does not depend on the warm start.
+stepping random 256x256 cloths (random pins, wind, stiffness and mass) with
+training on: each step of a batch of cloths gives one pair per cloth and one
+Adam step over the whole batch on the line-search loss (the share of the
+base guess's A-norm error the network's direction leaves).  The base guess
+is the previous step's dx for the 'learned' mode and the Galerkin
+projection for 'projection_learned'; a parameter file is trained for one of
+them.  The solution each step converges to does not depend on the warm
+start.
 """
 
 from __future__ import annotations
@@ -46,7 +49,8 @@ def load_params(path: str = DEFAULT_WEIGHTS):
 
 def train(pairs: int = 50000, cloths: int = 16, width: int = 256, episode: int = 120, lr: float = 1e-3,
           lr_final: float = 1e-4, seed: int = 5, tol: float = 1e-6, log_every: int = 50,
-          theta0: Optional[np.ndarray] = None, time_budget_s: Optional[float] = None, verbose: bool = True):
+          theta0: Optional[np.ndarray] = None, time_budget_s: Optional[float] = None, verbose: bool = True,
+          mode: str = "learned"):
     """Online training on fresh batches of random cloths.  Returns (theta,
     meta) where meta records the pairs seen, the loss curve and timing."""
     import torch
@@ -65,7 +69,7 @@ def train(pairs: int = 50000, cloths: int = 16, width: int = 256, episode: int =
             b.batch.ws_init_params(seed)
         else:
             b.batch.ws_set_params(theta)
-        b.batch.set_warm_start_mode("learned")
+        b.batch.set_warm_start_mode(mode)
         for k in range(episode):
             frac = min(step_global / max(steps_total - 1, 1), 1.0)
             cur_lr = lr * (lr_final / lr) ** frac
@@ -89,7 +93,7 @@ def train(pairs: int = 50000, cloths: int = 16, width: int = 256, episode: int =
         episode_idx += 1
         if time_budget_s is not None and time.time() - t0 > time_budget_s:
             break
-    meta = {"pairs": int(seen), "cloths_per_batch": cloths, "width": width, "episode_steps": episode,
+    meta = {"mode": mode, "pairs": int(seen), "cloths_per_batch": cloths, "width": width, "episode_steps": episode,
             "lr": lr, "lr_final": lr_final, "seed": seed, "tol": tol, "train_seconds": round(time.time() - t0, 1),
             "loss_first": float(np.mean(losses[:20])) if losses else None,
             "loss_last": float(np.mean(losses[-20:])) if losses else None,
@@ -97,7 +101,7 @@ def train(pairs: int = 50000, cloths: int = 16, width: int = 256, episode: int =
     return theta, meta
 
 
-MODES = ("previous", "projection", "learned")
+MODES = ("previous", "learned", "projection", "projection_learned")
 
 
 def compare(sim_factory, steps: int, warmup: int, theta: np.ndarray, modes=MODES):
@@ -149,6 +153,7 @@ def _main(argv=None):
     t.add_argument("--seed", type=int, default=5)
     t.add_argument("--time-budget", type=float, default=None)
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
+    t.add_argument("--mode", choices=["learned", "projection_learned"], default="learned")
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
     e.add_argument("--width", type=int, default=1024)
@@ -158,7 +163,7 @@ def _main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "train":
         theta, meta = train(a.pairs, a.cloths, a.width, a.episode, a.lr, a.lr_final, a.seed,
-                            time_budget_s=a.time_budget)
+                            time_budget_s=a.time_budget, mode=a.mode)
         save_params(a.out, theta, meta)
         print(json.dumps({k: v for k, v in meta.items() if k != "loss_curve"}), flush=True)
     else:
@@ -170,7 +175,7 @@ def _main(argv=None):
         r = compare(lambda: make_sim(scene, tol=1e-6), a.steps, a.warmup, theta)
         out = {m: {k: round(v, 4) for k, v in r[m].items() if k != "pos"} for m in r}
         out["scene"] = scene.name
-        for m in ("projection", "learned"):
+        for m in MODES[1:]:
             out[m]["iter_ratio_vs_previous"] = round(r["previous"]["iters"] / max(r[m]["iters"], 1e-9), 4)
             out[m]["time_ratio_vs_previous"] = round(r[m]["steps_per_s"] / r["previous"]["steps_per_s"], 4)
             out[m]["max_pos_diff_m"] = float(np.abs(r["previous"]["pos"] - r[m]["pos"]).max())

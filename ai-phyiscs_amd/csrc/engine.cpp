@@ -667,8 +667,9 @@ bool Problem::all_done() const {
 }
 
 void Problem::set_warm_start_mode(int mode) {
-    if (mode != kWarmPrevious && mode != kWarmLearned && mode != kWarmProjection)
-        throw std::invalid_argument("warm start mode: 0 (previous step), 1 (learned) or 2 (projection)");
+    if (mode != kWarmPrevious && mode != kWarmLearned && mode != kWarmProjection && mode != kWarmProjLearned)
+        throw std::invalid_argument(
+            "warm start mode: 0 (previous step), 1 (learned), 2 (projection) or 3 (projection + learned)");
     ws_mode_ = mode;
 }
 
@@ -727,13 +728,14 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
         ws_part2_.resize(size_t(kWsMaxValues) * size_t(ninst_) * size_t(ws_chunks_));
     const WsHist h = hist_ptrs();
     const WsReduce red = ws_reduce();
-    if (ws_mode_ != kWarmPrevious) {
+    const bool learned = ws_mode_ == kWarmLearned || ws_mode_ == kWarmProjLearned;
+    if (ws_mode_ == kWarmProjection || ws_mode_ == kWarmProjLearned) {
         launch_ws_gram(geom(), matrix(), rhs_.data(), h, red, s);
         launch_ws_finish(red, ws_gram_values(h.k), state_.data(), ws_inst_.data(), ninst_, kWsPhaseProj, h.k, s);
         launch_ws_proj_apply(geom(), h, ws_inst_.data(), x_.data(), s);
         proj_ran_ = true;
     }
-    if (ws_mode_ != kWarmLearned && !ws_train_) return;
+    if (!learned && !ws_train_) return;
     WarmStartNet& N = net();
     if (!N.has_params()) throw std::runtime_error("learned warm start: no network parameters set");
     N.reserve(n_pad_, ws_train_);
@@ -754,7 +756,7 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
     N.forward(ws_graph(), s, ws_train_);
     launch_ws_direction(geom(), matrix(), N.out(), ws_inst_.data(), rb_.data(), d_.data(), ad_.data(), red, s);
     launch_ws_finish(red, 4, state_.data(), ws_inst_.data(), ninst_, kWsPhaseLine, 0, s);
-    if (ws_mode_ == kWarmLearned) launch_ws_line_apply(geom(), ws_inst_.data(), d_.data(), x_.data(), s);
+    if (learned) launch_ws_line_apply(geom(), ws_inst_.data(), d_.data(), x_.data(), s);
     ws_ran_ = true;
 }
 
@@ -815,7 +817,7 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
         const double wb = W.bb > 0 ? W.bb : 1.0;
         q.ws_proj_used = proj_ran_ ? W.use_proj : 0;
         q.ws_kdim = proj_ran_ ? W.kdim : 0;
-        q.ws_used = (ws_ran_ && ws_mode_ == kWarmLearned) ? W.use_cand : 0;
+        q.ws_used = (ws_ran_ && (ws_mode_ == kWarmLearned || ws_mode_ == kWarmProjLearned)) ? W.use_cand : 0;
         q.ws_r0_prev = (ws_ran_ || proj_ran_) ? std::sqrt(W.rr_prev / wb) : 0.0;
         q.ws_r0_proj = proj_ran_ ? std::sqrt(W.rr_proj / wb) : 0.0;
         q.ws_r0_learned = ws_ran_ ? std::sqrt(W.rr_cand / wb) : 0.0;

@@ -130,9 +130,10 @@ struct SolveStats {
 
 // PCG initial guess.
 enum WarmStartMode : int {
-    kWarmPrevious = 0,    // the previous step's dx (reference behaviour)
-    kWarmLearned = 1,     // projection, then the network's line-searched direction
-    kWarmProjection = 2,  // Galerkin projection onto the last K solutions
+    kWarmPrevious = 0,        // the previous step's dx (reference behaviour)
+    kWarmLearned = 1,         // dx_prev plus the network's line-searched direction
+    kWarmProjection = 2,      // Galerkin projection onto the last K solutions
+    kWarmProjLearned = 3,     // the projection plus the network's direction
 };
 
 class Problem {

@@ -99,7 +99,8 @@ int ws_mode(const std::string& m) {
     if (m == "previous") return 0;
     if (m == "learned") return 1;
     if (m == "projection") return 2;
-    throw py::value_error("warm start mode must be 'previous', 'projection' or 'learned'");
+    if (m == "projection_learned") return 3;
+    throw py::value_error("warm start mode must be 'previous', 'learned', 'projection' or 'projection_learned'");
 }
 
 py::dict ws_stats_dict(const double* o) {
@@ -470,8 +471,8 @@ PYBIND11_MODULE(_chysx_native, m) {
                 check(chysx_cloth_set_warm_start_mode(s.p(), ws_mode(mode)));
             },
             py::arg("mode"),
-            "'previous' (reference: previous step's dx), 'projection' (Galerkin projection onto the last K "
-            "solutions) or 'learned' (projection plus the network's line-searched direction).")
+            "'previous' (reference: previous step's dx), 'learned' (dx_prev plus the network's line-searched "
+            "direction), 'projection' (Galerkin projection onto the last K solutions) or 'projection_learned'.")
         .def(
             "ws_set_history", [](Sim& s, int k) { check(chysx_cloth_ws_set_history(s.p(), k)); }, py::arg("k"),
             "Number of previous solutions the projection uses (2..8).")

@@ -120,10 +120,12 @@ private:
 //     K converged dx (newest first): G = V^T A V, g = V^T b, x = V c with
 //     c = G^+ g (eigen-truncated), accepted when its energy
 //     phi(x) = x^T A x / 2 - b^T x is lower than phi(dx_prev);
-//   learned: from the base guess x_b (projection or dx_prev) the network
-//     predicts a direction d; x = x_b + alpha d with the A-norm line search
-//     alpha = d^T r_b / d^T A d, accepted when the predicted residual
-//     |r_b - alpha A d| is smaller than |r_b|;
+//   learned: from the base guess x_b (dx_prev, or the projection) the
+//     network predicts a direction d; x = x_b + alpha d with the A-norm line
+//     search alpha = d^T r_b / d^T A d, accepted when it lowers the energy
+//     (the residual in the A^-1 norm), which the exact line search does
+//     unless d is A-orthogonal to the error; the 2-norm residual
+//     |r_b - alpha A d| is predicted and reported;
 //   training: loss 1 - (d^T r_b)^2 / ((d^T A d)(e^T r_b)), e = x_conv - x_b,
 //     the fraction of the base guess's A-norm error left after the line search.
 

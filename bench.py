@@ -133,8 +133,9 @@ def bench_batch(args):
 
 def warm_start_comparison(args, scene, make_sim):
     """Same workload and step window with each warm start: previous-step dx
-    (the reference's), Galerkin projection onto the last 6 solutions, and the
-    projection plus the learned direction (committed trained parameters)."""
+    (the reference's), dx_prev plus the learned direction (committed trained
+    parameters), Galerkin projection onto the last 6 solutions, and the
+    projection plus the learned direction."""
     import numpy as np
     import torch
     from chysx import warmstart as ws
@@ -148,8 +149,9 @@ def warm_start_comparison(args, scene, make_sim):
         out["trained_pairs"] = meta.get("pairs")
         out["train_seconds"] = meta.get("train_seconds")
     pos = {}
-    for mode in ("previous", "projection", "learned"):
-        if mode == "learned" and theta is None:
+    modes = ("previous", "learned", "projection", "projection_learned")
+    for mode in modes:
+        if mode.endswith("learned") and theta is None:
             out[mode] = None
             continue
         g = make_sim(scene, tol=args.tol, max_iter=20000)
@@ -177,7 +179,7 @@ def warm_start_comparison(args, scene, make_sim):
                      "projection_used_frac": round(float(np.mean(proj)), 3),
                      "learned_used_frac": round(float(np.mean(used)), 3)}
         del g
-    for mode in ("projection", "learned"):
+    for mode in modes[1:]:
         if out.get(mode):
             out[mode]["iters_ratio_vs_previous"] = round(out["previous"]["pcg_iters_per_step"] /
                                                          out[mode]["pcg_iters_per_step"], 3)
@@ -195,8 +197,9 @@ def batch_warm_start_comparison(args, sc, make_batch):
     theta = ws.load_params(path)[0] if os.path.exists(path) else None
     out = {"warmup": args.warmup, "steps": args.ws_steps,
            "weights": os.path.relpath(path, ROOT) if theta is not None else None}
-    for mode in ("previous", "projection", "learned"):
-        if mode == "learned" and theta is None:
+    modes = ("previous", "learned", "projection", "projection_learned")
+    for mode in modes:
+        if mode.endswith("learned") and theta is None:
             out[mode] = None
             continue
         b = make_batch(sc, tol=args.tol, max_iter=20000)
@@ -224,7 +227,7 @@ def batch_warm_start_comparison(args, sc, make_batch):
                      "pcg_iters_per_step_max": round(float(np.mean(it_max)), 2),
                      "learned_used_frac": round(float(np.mean(used)), 3)}
         del b
-    for mode in ("projection", "learned"):
+    for mode in modes[1:]:
         if out.get(mode):
             out[mode]["speedup_vs_previous"] = round(out[mode]["steps_per_s"] / out["previous"]["steps_per_s"], 3)
     return out

@@ -242,10 +242,12 @@ int chysx_batch_set_autotune(chysx_batch* b, int on);
  *   of the last K converged dx (K = chysx_*_ws_set_history, 2..8, default 6),
  *   accepted when it lowers the energy x^T A x / 2 - b^T x below that of
  *   dx_prev (the A-norm error PCG minimises).
- * mode 1 (learned): the projection, then an edge message-passing network
- *   over the mesh graph (BF16 MFMA, FP32 accumulate) predicts a direction d;
- *   x += alpha d with alpha = d.r / d.Ad, kept only when the predicted
- *   residual is smaller than without it.
+ * mode 1 (learned): an edge message-passing network over the mesh graph
+ *   (BF16 MFMA, FP32 accumulate) predicts a direction d from dx_prev;
+ *   x = dx_prev + alpha d with alpha = d.r / d.Ad, kept when it lowers the
+ *   energy (the residual in the A^-1 norm); else dx_prev is used.
+ * mode 3 (projection + learned): the projection, then the network's
+ *   direction in the same way.
  * Both start once two solutions are held; set/zero_warm_start resets the
  * history.  Parameters: chysx_ws_num_params() FP32 values.  Training: every
  * step runs the network; after the solve the loss

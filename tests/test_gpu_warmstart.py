@@ -164,54 +164,70 @@ def test_projection_cuts_iterations_same_solution(cuda):
     assert its["projection"] < 0.9 * its["previous"], its
 
 
-def test_zero_network_leaves_projection_unchanged(cuda):
-    """A network with zero output proposes no direction: the learned mode must
-    run bitwise like the projection mode."""
+@pytest.mark.parametrize("base,mode", [("previous", "learned"), ("projection", "projection_learned")])
+def test_zero_network_leaves_base_guess_unchanged(cuda, base, mode):
+    """A network with zero output proposes no direction: each learned mode
+    must run bitwise like its base mode."""
     s = scenes.config2(w=40, steps=8)
     runs = []
-    for mode in ["projection", "learned"]:
+    for m in [base, mode]:
         g = make_sim(s, tol=1e-6)
         g.sim.ws_set_params(np.zeros(chysx._chysx_native.ws_num_params(), np.float32))
-        g.sim.set_warm_start_mode(mode)
+        g.sim.set_warm_start_mode(m)
         for k in range(8):
             g.sim.step(s.dt)
-            if mode == "learned":
+            if m == mode:
                 assert g.sim.ws_stats()["used"] == 0
         runs.append(g.positions())
     np.testing.assert_array_equal(runs[0], runs[1])
 
 
-def test_learned_direction_only_kept_when_residual_drops(cuda):
+def test_learned_direction_applied_and_steps_stay_exact(cuda):
+    """A random network's direction lowers the energy whenever it is not
+    A-orthogonal to the error, so it is applied; every solve still converges
+    and each checked step matches the oracle's step from the same state."""
+    from oracle.oracle import oracle_from_scene
     s = scenes.config2(w=40, steps=12)
     g = make_sim(s, tol=1e-6)
     g.sim.ws_init_params(3)
     g.sim.set_warm_start_mode("learned")
-    seen = 0
+    oc, _, _, im, f = oracle_from_scene(s, precision=64, tol=1e-11, max_iter=100000)
+    used = []
     for k in range(12):
+        check = k in (5, 10)
+        if check:
+            p0 = g.positions().astype(np.float64)
+            v0 = g.velocities().astype(np.float64)
+            dx0 = g.sim.get_dx64()
         g.sim.step(s.dt)
         assert g.sim.last_solve_stats()["converged"]
         st = g.sim.ws_stats()
-        if k >= 2:  # the warm-start stages run once two solutions exist
-            seen += 1
-            base = st["r0_projection"] if st["projection_used"] else st["r0_previous"]
-            assert st["r0_previous"] > 0 and st["r0_learned"] > 0
-            if st["used"]:
-                assert st["r0_learned"] < base * (1 + 1e-9), st
-    assert seen == 10
+        if k >= 2:  # the network runs once two solutions exist
+            used.append(st["used"])
+            assert st["r0_previous"] > 0 and st["r0_learned"] > 0 and st["projection_used"] == 0
+        if check:
+            oc.set_dx(dx0)
+            oc.step(p0, v0, im, f, s.dt, 1)
+            assert rel(g.positions(), p0) <= 1e-5, k
+    assert np.mean(used) > 0.5
 
 
 def test_online_training_reduces_loss(cuda):
-    s = scenes.config2(w=48, steps=80, seed=9)
+    """Online training from dx_prev: the line-search loss (share of the A-norm
+    error of dx_prev the direction leaves) starts near 1 for a fresh network
+    and falls well below it."""
+    s = scenes.config2(w=48, steps=120, seed=9)
     g = make_sim(s, tol=1e-6)
     g.sim.ws_init_params(1)
     g.sim.set_warm_start_mode("learned")
     g.sim.ws_set_training(True, 2e-3)
     losses = []
-    for k in range(80):
+    for k in range(120):
         g.sim.step(s.dt)
         assert g.sim.last_solve_stats()["converged"]
         if k >= 2:
             losses.append(g.sim.ws_stats()["loss"])
     losses = np.array(losses)
     assert np.all((losses >= -1e-6) & (losses <= 1 + 1e-6))
-    assert np.mean(losses[-10:]) < np.mean(losses[:10]) - 0.02, (losses[:10], losses[-10:])
+    assert losses[0] > 0.8
+    assert np.mean(losses[-20:]) < 0.6 * np.mean(losses[:5]), (losses[:5], losses[-20:])
