@@ -231,3 +231,28 @@ def test_online_training_reduces_loss(cuda):
     assert np.all((losses >= -1e-6) & (losses <= 1 + 1e-6))
     assert losses[0] > 0.8
     assert np.mean(losses[-20:]) < 0.6 * np.mean(losses[:5]), (losses[:5], losses[-20:])
+
+
+def test_trained_network_gains_over_previous_guess(cuda):
+    """The committed parameters (trained on the GPU from dx_prev, 50k pairs on
+    256x256 cloths) on a held-out 128x128 cloth of the training distribution:
+    the learned warm start needs fewer PCG iterations than the previous-step
+    guess (measured 55.4 vs 61.0), and the projection fewer still."""
+    from chysx import warmstart
+    theta, meta = warmstart.load_params()
+    assert meta["mode"] == "learned" and meta["pairs"] >= 50000
+    scene = scenes.config5_train(count=1, w=128, seed=12345)[0]
+    its = {}
+    for mode in ["previous", "learned", "projection"]:
+        g = make_sim(scene, tol=1e-6)
+        g.sim.ws_set_params(theta)
+        g.sim.set_warm_start_mode(mode)
+        it = []
+        for k in range(40):
+            g.sim.step(scene.dt)
+            st = g.sim.last_solve_stats()
+            assert st["converged"], (mode, k)
+            it.append(st["iters"])
+        its[mode] = float(np.mean(it[10:]))
+    assert its["learned"] <= 0.95 * its["previous"], its
+    assert its["projection"] <= 0.85 * its["previous"], its

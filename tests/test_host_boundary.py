@@ -214,3 +214,11 @@ def test_topology_with_bending_bit_identical_to_oracle(case):
     np.testing.assert_array_equal(d["col_indices"], co)
     np.testing.assert_array_equal(d["dihedrals"], dih)
     np.testing.assert_array_equal(d["dihedral_slots"], ds)
+
+
+def test_committed_warm_start_parameters_load():
+    from chysx import warmstart
+    theta, meta = warmstart.load_params()
+    assert theta.dtype == np.float32 and theta.size == chysx._chysx_native.ws_num_params()
+    assert np.all(np.isfinite(theta))
+    assert meta["mode"] == "learned" and meta["pairs"] >= 50000 and meta["loss_last"] < meta["loss_first"]
