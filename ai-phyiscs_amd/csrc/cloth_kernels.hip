@@ -22,6 +22,7 @@
 #include <stdexcept>
 
 #include "cloth_kernels.hpp"
+#include "matrix_access.hpp"
 
 namespace chysx_amd {
 
@@ -454,9 +455,12 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
         }
     }
     if (LDS) {
-        for (int k = 0; k < deg; ++k)
+        const int* colr = A.col + (size_t)sb * 64 + lane;
+        for (int k = 0; k < deg; ++k) {
+            if (A.sym && colr[k * 64] < row) continue;  // read through the upper mirror
 #pragma unroll
             for (int c = 0; c < 9; ++c) vrow[(k * 9 + c) * 64] = acc[(k * 9 + c) * 64];
+        }
     }
     // static-shape contact at x_n (detect_kernel + scatter_gradient_kernel,
     // static_contact.cu:107-213): the deepest plane / box penetration below
@@ -577,15 +581,17 @@ __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVector
     double a2 = double(dg[384]) * x0 + double(dg[448]) * x1 + double(dg[512]) * x2;
     const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
     const int* colp = A.col + (size_t)sb * 64 + lane;
+    const int* mirp = A.mir + (size_t)sb * 64 + lane;
     const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
     for (int k = 0; k < w; ++k) {
         const int c = colp[k * 64];
         if (c >= 0) {
-            const float* bk = vp + k * 9 * 64;
+            float m[9];
+            load_block(A, vp + k * 9 * 64, mirp + k * 64, c, row, m);
             const double y0 = x[3 * c], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
-            a0 += double(bk[0]) * y0 + double(bk[64]) * y1 + double(bk[128]) * y2;
-            a1 += double(bk[192]) * y0 + double(bk[256]) * y1 + double(bk[320]) * y2;
-            a2 += double(bk[384]) * y0 + double(bk[448]) * y1 + double(bk[512]) * y2;
+            a0 += double(m[0]) * y0 + double(m[1]) * y1 + double(m[2]) * y2;
+            a1 += double(m[3]) * y0 + double(m[4]) * y1 + double(m[5]) * y2;
+            a2 += double(m[6]) * y0 + double(m[7]) * y1 + double(m[8]) * y2;
         }
     }
     const F3 bf = ld3(v.rhs, row);
@@ -1005,6 +1011,7 @@ void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& 
                      int unroll, hipStream_t s, int branch_free, int symmetric) {
     if (g.nslices <= 0) return;
     if (!valid_unroll(unroll)) throw std::invalid_argument("unsupported unroll");
+    if (symmetric != A.sym) throw std::invalid_argument("symmetric SpMV: launch flag and matrix mode differ");
     if (symmetric && A.mir == nullptr) throw std::invalid_argument("symmetric SpMV needs the mirror offsets");
     if (branch_free) {
         if (symmetric) pcg_spmv_dispatch<true, true>(g, A, v, st, waves, unroll, s);

@@ -55,6 +55,8 @@ struct DevMatrix {
     // For an entry (i, j) with j < i: float offset of component 0 of the
     // block (j, i) in val (its transpose, stored by row j); -1 otherwise.
     const int* mir;
+    // 1: lower blocks are read through mir (assembly writes upper blocks only)
+    int sym;
 };
 
 // Solver vectors in padded row space, 3 components per row (AoS).

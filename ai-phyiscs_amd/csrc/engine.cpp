@@ -385,6 +385,7 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     slice_ptr_.assign(slice_ptr_h_.data(), slice_ptr_h_.size(), s);
     col_.assign(sell_col.data(), sell_col.size(), s);
     mir_.assign(mir.data(), mir.size(), s);
+    mir_h_ = std::move(mir);
     row_deg_.assign(row_deg_h_.data(), row_deg_h_.size(), s);
     val_.resize(size_t(entries) * kSlice * 9);
     val_.zero(s);
@@ -483,8 +484,8 @@ void Problem::update_pin_targets(const Instance& inst, int k, hipStream_t s) {
 }
 
 DevMatrix Problem::matrix() const {
-    return DevMatrix{slice_ptr_.data(), col_.data(), row_deg_.data(), val_.data(), diag_.data(), minv_.data(),
-                     mir_.data()};
+    return DevMatrix{slice_ptr_.data(), col_.data(),  row_deg_.data(), val_.data(),
+                     diag_.data(),      minv_.data(), mir_.data(),     cfg_.spmv_symmetric};
 }
 DevVectors Problem::vectors() const {
     return DevVectors{rhs_.data(), r_.data(), z_.data(), ap_.data(), p0_.data(), p1_.data(), x_.data(),
@@ -855,12 +856,21 @@ void Problem::copy_values(int k, float* out) const {
     const std::vector<float> v = download(val_.data(), val_.size());
     const Pattern& P = pats_[size_t(k)];
     const int r0 = inst_row0_[size_t(k)];
+    const bool sym = cfg_.spmv_symmetric != 0;
     for (int i = 0; i < P.n; ++i) {
         const int row = r0 + i, s = row / kSlice, lane = row % kSlice;
         const int sb = slice_ptr_h_[size_t(s)];
         for (int q = P.row_offsets[size_t(i)]; q < P.row_offsets[size_t(i) + 1]; ++q) {
             const int kk = q - P.row_offsets[size_t(i)];
-            for (int c = 0; c < 9; ++c) out[9 * size_t(q) + c] = v[((size_t(sb) + kk) * 9 + c) * kSlice + lane];
+            const size_t e = (size_t(sb) + kk) * kSlice + lane;
+            if (sym && P.cols[size_t(q)] < i) {
+                // the operator's lower block: transpose of the upper mirror
+                const size_t m = size_t(mir_h_[e]);
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) out[9 * size_t(q) + 3 * a + b] = v[m + size_t(3 * b + a) * kSlice];
+            } else {
+                for (int c = 0; c < 9; ++c) out[9 * size_t(q) + c] = v[((size_t(sb) + kk) * 9 + c) * kSlice + lane];
+            }
         }
     }
 }

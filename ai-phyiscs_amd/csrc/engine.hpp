@@ -102,7 +102,7 @@ struct LaunchConfig {
     // 1: the SpMV pass reads each lower block as the transpose of its upper
     // mirror (streams half the off-diagonal values; changes results at FP32
     // rounding level, so it is a fixed choice, never picked by the autotuner)
-    int spmv_symmetric = 0;
+    int spmv_symmetric = 1;
     int update_waves = 4;
     int finalize_waves = 4;
 };
@@ -262,6 +262,7 @@ private:
     std::vector<int> pin_offset_;  // first global pin of each instance
     // matrix
     DeviceArray<int> slice_ptr_, col_, row_deg_, mir_;
+    std::vector<int> mir_h_;
     DeviceArray<float> val_, diag_, minv_;
     // vectors
     DeviceArray<float> rhs_, r_, z_, ap_, p0_, p1_;

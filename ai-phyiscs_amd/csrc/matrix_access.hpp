@@ -1,0 +1,37 @@
+// SPDX-License-Identifier: Apache-2.0
+//
+// Block access on the wave-sliced ELL matrix shared by every kernel that
+// multiplies by A.  With A.sym set the lower blocks (row, c), c < row, are
+// read as transposes of their upper mirrors (c, row); assembly then writes
+// only the upper blocks, and every product (iterations, true residual, warm
+// start) uses the same symmetric operator.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cloth_kernels.hpp"
+
+namespace chysx_amd {
+
+// The 3x3 block of entry k of `row` (vp_k = that entry's component 0 in its
+// slice, stride 64 floats), row-major into m.
+__device__ __forceinline__ void load_block(const DevMatrix& A, const float* __restrict__ vp_k,
+                                           const int* __restrict__ mir_k, int c, int row, float (&m)[9]) {
+    if (A.sym && c < row) {
+        const float* bt = A.val + *mir_k;
+        m[0] = bt[0];
+        m[1] = bt[192];
+        m[2] = bt[384];
+        m[3] = bt[64];
+        m[4] = bt[256];
+        m[5] = bt[448];
+        m[6] = bt[128];
+        m[7] = bt[320];
+        m[8] = bt[512];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m[q] = vp_k[q * 64];
+    }
+}
+
+}  // namespace chysx_amd

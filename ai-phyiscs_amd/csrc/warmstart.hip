@@ -10,6 +10,7 @@
 // j = li, k = 8 lg .. 8 lg + 7; D[i][j] with i = 4 lg + r (r = 0..3), j = li.
 
 #include "warmstart.hpp"
+#include "matrix_access.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -817,13 +818,15 @@ __global__ __launch_bounds__(256) void ws_gram_kernel(LaunchGeom g, DevMatrix A,
         }
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
+        const int* mirp = A.mir + (size_t)sb * 64 + lane;
         const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
         for (int k = 0; k < w; ++k) {
             const int c = colp[k * 64];
             if (c < 0) continue;
-            const float* bk = vp + k * 9 * 64;
+            float mf[9];
+            load_block(A, vp + k * 9 * 64, mirp + k * 64, c, row, mf);
 #pragma unroll
-            for (int q = 0; q < 9; ++q) m[q] = double(bk[64 * q]);
+            for (int q = 0; q < 9; ++q) m[q] = double(mf[q]);
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 float u0, u1, u2;
@@ -997,15 +1000,17 @@ __global__ __launch_bounds__(256) void ws_base_kernel(LaunchGeom g, DevMatrix A,
         double a2 = double(dg[384]) * x0 + double(dg[448]) * x1 + double(dg[512]) * x2;
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
+        const int* mirp = A.mir + (size_t)sb * 64 + lane;
         const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
         for (int k = 0; k < w; ++k) {
             const int c = colp[k * 64];
             if (c >= 0) {
-                const float* bk = vp + k * 9 * 64;
+                float m[9];
+                load_block(A, vp + k * 9 * 64, mirp + k * 64, c, row, m);
                 const double y0 = x[3 * c], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
-                a0 += double(bk[0]) * y0 + double(bk[64]) * y1 + double(bk[128]) * y2;
-                a1 += double(bk[192]) * y0 + double(bk[256]) * y1 + double(bk[320]) * y2;
-                a2 += double(bk[384]) * y0 + double(bk[448]) * y1 + double(bk[512]) * y2;
+                a0 += double(m[0]) * y0 + double(m[1]) * y1 + double(m[2]) * y2;
+                a1 += double(m[3]) * y0 + double(m[4]) * y1 + double(m[5]) * y2;
+                a2 += double(m[6]) * y0 + double(m[7]) * y1 + double(m[8]) * y2;
             }
         }
         const double q0 = double(v.rhs[3 * row]), q1 = double(v.rhs[3 * row + 1]), q2 = double(v.rhs[3 * row + 2]);
@@ -1158,15 +1163,17 @@ __global__ __launch_bounds__(256) void ws_dir_spmv_kernel(LaunchGeom g, DevMatri
         a2 = double(dg[384]) * d0 + double(dg[448]) * d1 + double(dg[512]) * d2;
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
+        const int* mirp = A.mir + (size_t)sb * 64 + lane;
         const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
         for (int k = 0; k < w; ++k) {
             const int c = colp[k * 64];
             if (c >= 0) {
-                const float* bk = vp + k * 9 * 64;
+                float m[9];
+                load_block(A, vp + k * 9 * 64, mirp + k * 64, c, row, m);
                 const double y0 = d[3 * c], y1 = d[3 * c + 1], y2 = d[3 * c + 2];
-                a0 += double(bk[0]) * y0 + double(bk[64]) * y1 + double(bk[128]) * y2;
-                a1 += double(bk[192]) * y0 + double(bk[256]) * y1 + double(bk[320]) * y2;
-                a2 += double(bk[384]) * y0 + double(bk[448]) * y1 + double(bk[512]) * y2;
+                a0 += double(m[0]) * y0 + double(m[1]) * y1 + double(m[2]) * y2;
+                a1 += double(m[3]) * y0 + double(m[4]) * y1 + double(m[5]) * y2;
+                a2 += double(m[6]) * y0 + double(m[7]) * y1 + double(m[8]) * y2;
             }
         }
         r0 = rb[3 * row];
