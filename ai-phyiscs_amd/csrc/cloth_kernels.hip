@@ -68,9 +68,12 @@ struct S6 {
 };
 
 // Sum over the 64 lanes of a wave (fixed butterfly order); lane 0 writes the
-// slice's partial slots.  Every lane of the wave must call it.
+// slice's partial slots, slot j of every slice contiguous
+// (partials[j * nslices + slice]) so a reduction reads only the slots it
+// needs, coalesced.  Every lane of the wave must call it.
 template <int N>
-__device__ __forceinline__ void wave_partials(double (&v)[N], double* __restrict__ partials, int slice) {
+__device__ __forceinline__ void wave_partials(double (&v)[N], double* __restrict__ partials, int slice,
+                                              int nslices) {
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         double a = v[j];
@@ -80,7 +83,7 @@ __device__ __forceinline__ void wave_partials(double (&v)[N], double* __restrict
     }
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) partials[4 * slice + j] = v[j];
+        for (int j = 0; j < N; ++j) partials[(size_t)j * nslices + slice] = v[j];
     }
 }
 
@@ -622,7 +625,7 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_residual_kernel(LaunchGeom g, 
     const int lane = threadIdx.x & 63;
     double part[4] = {0.0, 0.0, 0.0, 0.0};
     if (lane < sd.nvalid) residual_row(A, v, pcur, s * 64 + lane, report != 0, part);
-    wave_partials<4>(part, v.partials, s);
+    wave_partials<4>(part, v.partials, s, g.nslices);
 }
 
 // ---------------------------------------------------------------------------
@@ -650,7 +653,7 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
     if (S.mode == kModeCheck) {
         double part[4] = {0.0, 0.0, 0.0, 0.0};
         if (valid) residual_row(A, v, parity ? v.p1 : v.p0, row, false, part);
-        wave_partials<4>(part, v.partials, s);
+        wave_partials<4>(part, v.partials, s, g.nslices);
         return;
     }
     const float beta = float(S.beta);
@@ -700,7 +703,7 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
         st3(pnew, row, pi);
         part[0] = double(pi.x) * a0 + double(pi.y) * a1 + double(pi.z) * a2;
     }
-    wave_partials<1>(part, v.partials, s);
+    wave_partials<1>(part, v.partials, s, g.nslices);
 }
 
 // ---------------------------------------------------------------------------
@@ -736,7 +739,7 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_update_kernel(LaunchGeom g, De
         part[0] = double(ri.x) * zi.x + double(ri.y) * zi.y + double(ri.z) * zi.z;
         part[1] = double(ri.x) * ri.x + double(ri.y) * ri.y + double(ri.z) * ri.z;
     }
-    wave_partials<2>(part, v.partials, s);
+    wave_partials<2>(part, v.partials, s, g.nslices);
 }
 
 // ---------------------------------------------------------------------------
@@ -753,7 +756,7 @@ constexpr int kReduceThreads = 1024;
 __global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* __restrict__ st,
                                                                     const InstParams* __restrict__ params,
                                                                     const double* __restrict__ partials,
-                                                                    int phase) {
+                                                                    int nslices, int phase) {
     InstState& S = st[blockIdx.x];
     const InstParams& pr = params[blockIdx.x];
     if ((phase == kPhaseSigma || phase == kPhaseRho) && S.status != kRunning) return;
@@ -763,32 +766,29 @@ __global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* _
         return;
     }
     const int s0 = S.slice0, ns = S.nslices;
-    const double2* p2 = reinterpret_cast<const double2*>(partials);
+    // Slots this phase reads: start r.r, b.b, r.z; sigma p.Ap (check mode:
+    // r.r and r.z of the true residual); rho r.z, r.r; final r.r.
+    const int need = (phase == kPhaseStart) ? 7
+                     : (phase == kPhaseSigma) ? (S.mode == kModeCheck ? 5 : 1)
+                     : (phase == kPhaseRho) ? 3 : 1;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     // Loads of 8 slices are issued before they are summed (in slice order, so
     // the sum is the same as a plain loop's).
     constexpr int kU = 8;
-    for (int q0 = threadIdx.x; q0 < ns; q0 += kU * kReduceThreads) {
-        double2 lo[kU], hi[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int q = q0 + u * kReduceThreads;
-            if (q < ns) {
-                lo[u] = p2[2 * (s0 + q)];
-                hi[u] = p2[2 * (s0 + q) + 1];
-            } else {
-                lo[u] = make_double2(0.0, 0.0);
-                hi[u] = make_double2(0.0, 0.0);
-            }
-        }
+    for (int j = 0; j < 4; ++j) {
+        if (!((need >> j) & 1)) continue;
+        const double* pj = partials + (size_t)j * nslices + s0;
+        for (int q0 = threadIdx.x; q0 < ns; q0 += kU * kReduceThreads) {
+            double t[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            if (q0 + u * kReduceThreads < ns) {
-                acc[0] += lo[u].x;
-                acc[1] += lo[u].y;
-                acc[2] += hi[u].x;
-                acc[3] += hi[u].y;
+            for (int u = 0; u < kU; ++u) {
+                const int q = q0 + u * kReduceThreads;
+                t[u] = (q < ns) ? pj[q] : 0.0;
             }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (q0 + u * kReduceThreads < ns) acc[j] += t[u];
         }
     }
     __shared__ double sh[kReduceThreads / 64][4];
@@ -1030,10 +1030,10 @@ void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors
 #undef L
     CHYSX_LAUNCH_CHECK();
 }
-void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int phase,
-                       hipStream_t s) {
+void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int nslices,
+                       int phase, hipStream_t s) {
     if (ninst <= 0) return;
-    pcg_reduce_kernel<<<ninst, kReduceThreads, 0, s>>>(st, params, partials, phase);
+    pcg_reduce_kernel<<<ninst, kReduceThreads, 0, s>>>(st, params, partials, nslices, phase);
     CHYSX_LAUNCH_CHECK();
 }
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,

@@ -108,8 +108,8 @@ void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& 
 void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
                        hipStream_t s);
 // One workgroup per instance: finish the reductions of `phase` (PcgPhase).
-void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int phase,
-                       hipStream_t s);
+void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int nslices,
+                       int phase, hipStream_t s);
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
                      int waves, hipStream_t s);
 void launch_area_mass(const int* vt_off, const int* vt_ids, const int* tri_v, const float* pos, float density_over_3,

@@ -527,14 +527,14 @@ void Problem::launch_assemble(float dt, const DevExt& ext, hipStream_t s) {
 }
 void Problem::launch_pcg_start(hipStream_t s) {
     launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 0, cfg_.residual_waves, s);
-    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseStart, s);
+    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, int(slices_h_.size()), kPhaseStart, s);
 }
 void Problem::launch_pcg_iteration(hipStream_t s) {
     launch_pcg_spmv(geom(), matrix(), vectors(), state_.data(), cfg_.spmv_waves, cfg_.spmv_unroll, s,
                     cfg_.spmv_branch_free, cfg_.spmv_symmetric);
-    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseSigma, s);
+    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, int(slices_h_.size()), kPhaseSigma, s);
     launch_pcg_update(geom(), matrix(), vectors(), state_.data(), cfg_.update_waves, s);
-    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseRho, s);
+    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, int(slices_h_.size()), kPhaseRho, s);
 }
 
 void Problem::set_launch_config(const LaunchConfig& c) {
@@ -789,7 +789,7 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
     }
     if (any_stopped) {
         launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 1, cfg_.residual_waves, s);
-        launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, kPhaseFinal, s);
+        launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, int(slices_h_.size()), kPhaseFinal, s);
         read_states(s);
     }
     if (ws_ran_ && ws_train_) {
