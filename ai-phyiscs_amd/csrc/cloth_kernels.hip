@@ -753,60 +753,89 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_update_kernel(LaunchGeom g, De
 // ---------------------------------------------------------------------------
 constexpr int kReduceThreads = 1024;
 
+// Sums slots slot[0..NS) of an instance's slice partials.  Thread t adds
+// slices t, t + T, t + 2T, ... in order; the waves' sums are then combined
+// in wave order, so the result is one fixed sum whatever KU is.  All NS x KU
+// loads of a round are issued before any of them is added.  The total of
+// slot slot[j] lands in tot[slot[j]] on thread 0 only.
+template <int NS, int KU>
+__device__ __forceinline__ void block_sum(const double* __restrict__ partials, int nslices, int s0, int ns,
+                                          const int (&slot)[NS], double (&tot)[4]) {
+    double acc[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) acc[j] = 0.0;
+    for (int q0 = threadIdx.x; q0 < ns; q0 += KU * kReduceThreads) {
+        double t[NS][KU];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const double* pj = partials + (size_t)slot[j] * nslices + s0;
+#pragma unroll
+            for (int u = 0; u < KU; ++u) {
+                const int q = q0 + u * kReduceThreads;
+                t[j][u] = (q < ns) ? pj[q] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+#pragma unroll
+            for (int u = 0; u < KU; ++u)
+                if (q0 + u * kReduceThreads < ns) acc[j] += t[j][u];
+    }
+    __shared__ double sh[kReduceThreads / 64][NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) sh[threadIdx.x >> 6][j] = acc[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kReduceThreads / 64; ++w)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) tot[slot[j]] += sh[w][j];
+    }
+    __syncthreads();  // sh may be refilled by a second call
+}
+
+// single_ns > 0: one instance owning slices [0, single_ns) (the partial
+// loads then need no state read first); 0: bounds from the instance state.
+// The phase's partials are loaded before the state is inspected, so the two
+// loads overlap; an instance that is not live discards them.
 __global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* __restrict__ st,
                                                                     const InstParams* __restrict__ params,
                                                                     const double* __restrict__ partials,
-                                                                    int nslices, int phase) {
+                                                                    int nslices, int phase, int single_ns) {
     InstState& S = st[blockIdx.x];
     const InstParams& pr = params[blockIdx.x];
+    const int s0 = single_ns > 0 ? 0 : S.slice0;
+    const int ns = single_ns > 0 ? single_ns : S.nslices;
+    // Slots: start r.r, b.b, r.z; sigma p.Ap (check mode: r.r and r.z of the
+    // true residual); rho r.z, r.r; final r.r.
+    double tot[4] = {0.0, 0.0, 0.0, 0.0};
+    if (phase == kPhaseStart) {
+        const int sl[3] = {0, 1, 2};
+        block_sum<3, 8>(partials, nslices, s0, ns, sl, tot);
+    } else if (phase == kPhaseRho) {
+        const int sl[2] = {0, 1};
+        block_sum<2, 16>(partials, nslices, s0, ns, sl, tot);
+    } else {
+        const int sl[1] = {0};
+        block_sum<1, 16>(partials, nslices, s0, ns, sl, tot);
+    }
     if ((phase == kPhaseSigma || phase == kPhaseRho) && S.status != kRunning) return;
     if (phase == kPhaseFinal && S.status != kStopped) return;
     if (phase == kPhaseRho && S.skip) {
         if (threadIdx.x == 0) S.skip = 0;
         return;
     }
-    const int s0 = S.slice0, ns = S.nslices;
-    // Slots this phase reads: start r.r, b.b, r.z; sigma p.Ap (check mode:
-    // r.r and r.z of the true residual); rho r.z, r.r; final r.r.
-    const int need = (phase == kPhaseStart) ? 7
-                     : (phase == kPhaseSigma) ? (S.mode == kModeCheck ? 5 : 1)
-                     : (phase == kPhaseRho) ? 3 : 1;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    // Loads of 8 slices are issued before they are summed (in slice order, so
-    // the sum is the same as a plain loop's).
-    constexpr int kU = 8;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (!((need >> j) & 1)) continue;
-        const double* pj = partials + (size_t)j * nslices + s0;
-        for (int q0 = threadIdx.x; q0 < ns; q0 += kU * kReduceThreads) {
-            double t[kU];
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const int q = q0 + u * kReduceThreads;
-                t[u] = (q < ns) ? pj[q] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < kU; ++u)
-                if (q0 + u * kReduceThreads < ns) acc[j] += t[u];
-        }
+    if (phase == kPhaseSigma && S.mode == kModeCheck) {
+        const int sl[1] = {2};
+        block_sum<1, 16>(partials, nslices, s0, ns, sl, tot);
     }
-    __shared__ double sh[kReduceThreads / 64][4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) sh[threadIdx.x >> 6][j] = acc[j];
-    }
-    __syncthreads();
     if (threadIdx.x != 0) return;
-    double tot[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int w = 0; w < kReduceThreads / 64; ++w)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) tot[j] += sh[w][j];
     const double tol2 = pr.tol2;
     switch (phase) {
         case kPhaseStart: {
@@ -1033,7 +1062,8 @@ void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors
 void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int nslices,
                        int phase, hipStream_t s) {
     if (ninst <= 0) return;
-    pcg_reduce_kernel<<<ninst, kReduceThreads, 0, s>>>(st, params, partials, nslices, phase);
+    pcg_reduce_kernel<<<ninst, kReduceThreads, 0, s>>>(st, params, partials, nslices, phase,
+                                                       ninst == 1 ? nslices : 0);
     CHYSX_LAUNCH_CHECK();
 }
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
