@@ -638,6 +638,17 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_residual_kernel(LaunchGeom g, 
 // gathers.  Mode kModeCheck: the recursive residual passed the test, so this
 // pass computes the true residual instead.
 // ---------------------------------------------------------------------------
+// Entry order of the symmetric pass: 0, w-1, 1, w-2, ...  A row's columns
+// ascend, so step t pairs its t-th lower entry with its t-th upper entry
+// counted from the end.  On a grid, the mirror that step t reads for lower
+// entry t (row i - W, i - W + 1 or i - 1) is the upper entry that its owner
+// row, 16 slices earlier at W = 1024, streams at its own step t: both
+// wavefronts touch the block at the same point of their lives, a fraction of
+// a microsecond apart, while it is still in L2.  (In ascending order the
+// mirror is read first and the owner reads it at the end of its loop, after
+// the XCD has streamed several L2 capacities.)
+__device__ __forceinline__ int sym_order(int k, int w) { return (k & 1) ? w - 1 - (k >> 1) : (k >> 1); }
+
 template <int WAVES, int UNROLL, bool BF, bool SYM>
 __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
                                                              const InstState* __restrict__ st) {
@@ -672,9 +683,12 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
         const int* mirp = A.mir + (size_t)sb * 64 + lane;
         const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
         for (int k0 = 0; k0 < w; k0 += UNROLL) {
-            int c[UNROLL];
+            int c[UNROLL], kk[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) c[u] = (k0 + u < w) ? colp[(k0 + u) * 64] : -1;
+            for (int u = 0; u < UNROLL; ++u) {
+                kk[u] = SYM ? sym_order(k0 + u, w) : k0 + u;
+                c[u] = (k0 + u < w) ? colp[kk[u] * 64] : -1;
+            }
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
                 // BF: padding entries hold zero blocks and gather the row's
@@ -685,13 +699,13 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
                     const F3 pj = ld3(z, cc) + ld3(pold, cc) * beta;
                     if (SYM && cc < row) {
                         // lower block (row, cc) = transpose of (cc, row), which
-                        // row cc's lane streamed moments ago (L2-resident)
-                        const float* bt = A.val + mirp[(k0 + u) * 64];
+                        // row cc's lane streamed at the same step of its loop
+                        const float* bt = A.val + mirp[kk[u] * 64];
                         a0 += bt[0] * pj.x + bt[192] * pj.y + bt[384] * pj.z;
                         a1 += bt[64] * pj.x + bt[256] * pj.y + bt[448] * pj.z;
                         a2 += bt[128] * pj.x + bt[320] * pj.y + bt[512] * pj.z;
                     } else {
-                        const float* bk = vp + (k0 + u) * 9 * 64;
+                        const float* bk = vp + kk[u] * 9 * 64;
                         a0 += bk[0] * pj.x + bk[64] * pj.y + bk[128] * pj.z;
                         a1 += bk[192] * pj.x + bk[256] * pj.y + bk[320] * pj.z;
                         a2 += bk[384] * pj.x + bk[448] * pj.y + bk[512] * pj.z;
