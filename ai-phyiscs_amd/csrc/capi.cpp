@@ -581,32 +581,42 @@ int chysx_cloth_set_autotune(chysx_cloth* sim, int on) {
     });
 }
 
-int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg8) {
+int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg) {
     return guard([&] {
         need(sim, "sim");
-        need(cfg8, "cfg");
+        need(cfg, "cfg");
         const LaunchConfig& c = const_cast<chysx_cloth*>(sim)->sim.problem().launch_config();
-        const int v[8] = {c.assemble_waves, c.residual_waves, c.spmv_waves,       c.spmv_unroll,
-                          c.update_waves,   c.finalize_waves, c.spmv_branch_free, c.spmv_symmetric};
-        std::memcpy(cfg8, v, sizeof(v));
+        const int v[CHYSX_LAUNCH_CONFIG_LEN] = {c.assemble_waves,   c.residual_waves, c.spmv_waves,
+                                                c.spmv_unroll,      c.update_waves,   c.finalize_waves,
+                                                c.spmv_branch_free, c.spmv_symmetric, c.spmv_packed};
+        std::memcpy(cfg, v, sizeof(v));
     });
 }
 
-int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg8) {
+int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg) {
     return guard([&] {
         need(sim, "sim");
-        need(cfg8, "cfg");
+        need(cfg, "cfg");
         sim->sim.ensure_topology();
         LaunchConfig c;
-        c.assemble_waves = cfg8[0];
-        c.residual_waves = cfg8[1];
-        c.spmv_waves = cfg8[2];
-        c.spmv_unroll = cfg8[3];
-        c.update_waves = cfg8[4];
-        c.finalize_waves = cfg8[5];
-        c.spmv_branch_free = cfg8[6];
-        c.spmv_symmetric = cfg8[7];
+        c.assemble_waves = cfg[0];
+        c.residual_waves = cfg[1];
+        c.spmv_waves = cfg[2];
+        c.spmv_unroll = cfg[3];
+        c.update_waves = cfg[4];
+        c.finalize_waves = cfg[5];
+        c.spmv_branch_free = cfg[6];
+        c.spmv_symmetric = cfg[7];
+        c.spmv_packed = cfg[8];
         sim->sim.problem().set_launch_config(c);
+    });
+}
+
+int chysx_cloth_block_floats(const chysx_cloth* sim, int* floats) {
+    return guard([&] {
+        need(sim, "sim");
+        need(floats, "floats");
+        *floats = const_cast<chysx_cloth*>(sim)->sim.problem().block_floats();
     });
 }
 

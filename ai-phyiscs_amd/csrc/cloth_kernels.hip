@@ -166,9 +166,21 @@ __global__ void area_mass_kernel(const int* __restrict__ vt_off, const int* __re
 //             the same quantity as (m/dt^2)(x_tilde - x_n) - grad without the
 //             FP32 cancellation of x_tilde - x_n.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void add_block(float* __restrict__ acc, int k, const float (&H)[9], float sgn) {
+// Adds sgn H into accumulator entry k (block stride bs floats).  bs == 6
+// keeps the upper triangle (xx, xy, xz, yy, yz, zz) of a symmetric H.
+__device__ __forceinline__ void add_block(float* __restrict__ acc, int k, const float (&H)[9], float sgn, int bs) {
+    if (bs == 6) {
+        float* a = acc + k * 6 * 64;
+        a[0] += sgn * H[0];
+        a[64] += sgn * H[1];
+        a[128] += sgn * H[2];
+        a[192] += sgn * H[4];
+        a[256] += sgn * H[5];
+        a[320] += sgn * H[8];
+    } else {
 #pragma unroll
-    for (int c = 0; c < 9; ++c) acc[(k * 9 + c) * 64] += sgn * H[c];
+        for (int c = 0; c < 9; ++c) acc[(k * 9 + c) * 64] += sgn * H[c];
+    }
 }
 
 // One Baraff-Witkin triangle (stretch or shear, by its D_m^-1) seen from its
@@ -176,7 +188,7 @@ __device__ __forceinline__ void add_block(float* __restrict__ acc, int k, const 
 // repeated vertex) or the row accumulator (triangle_gradient_kernel and
 // triangle_hessian_scatter_kernel, triangle_stretch_constraint.cu:153-311).
 __device__ __forceinline__ void tri_element(const float4 Dm, float keff, int a, F3 x0, F3 x1, F3 x2, int packed,
-                                            F3& grad, float (&D)[9], float* __restrict__ acc) {
+                                            F3& grad, float (&D)[9], float* __restrict__ acc, int bs) {
     const F3 ea = x1 - x0, eb = x2 - x0;
     const F3 wu = ea * Dm.x + eb * Dm.z;
     const F3 wv = ea * Dm.y + eb * Dm.w;
@@ -226,7 +238,7 @@ __device__ __forceinline__ void tri_element(const float4 Dm, float keff, int a, 
 #pragma unroll
             for (int c = 0; c < 9; ++c) D[c] += H[c];
         } else {
-            add_block(acc, kb, H, 1.0f);
+            add_block(acc, kb, H, 1.0f, bs);
         }
     }
 }
@@ -341,11 +353,11 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
     const InstParams pr = params[sd.inst];
     const int sb = A.slice_ptr[s];
     const int deg = A.row_deg[row];
-    float* __restrict__ vrow = A.val + (size_t)sb * 9 * 64 + lane;
+    const int bs = A.bs;
+    float* __restrict__ vrow = A.val + (size_t)sb * bs * 64 + lane;
     float* __restrict__ acc = LDS ? lds_acc + (size_t)(threadIdx.x >> 6) * maxw * 9 * 64 + lane : vrow;
     for (int k = 0; k < deg; ++k)
-#pragma unroll
-        for (int c = 0; c < 9; ++c) acc[(k * 9 + c) * 64] = 0.0f;
+        for (int c = 0; c < bs; ++c) acc[(k * bs + c) * 64] = 0.0f;
 
     const F3 xi = ld3(e.pos, p);
     const F3 vi = ld3(e.vel, p);
@@ -408,7 +420,7 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
 #pragma unroll
                 for (int c = 0; c < 9; ++c) D[c] -= H[c];
             } else {
-                add_block(acc, ko, H, -1.0f);
+                add_block(acc, ko, H, -1.0f, A.bs);
             }
         }
     }
@@ -420,9 +432,9 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
         const F3 x1 = ld3(e.pos, m.tri_v[3 * ti + 1]);
         const F3 x2 = ld3(e.pos, m.tri_v[3 * ti + 2]);
         if (pr.tri_stretch)
-            tri_element(m.tri_dminv[ti], m.tri_area[ti] * m.tri_k[ti], a, x0, x1, x2, ent.y, grad, D, acc);
+            tri_element(m.tri_dminv[ti], m.tri_area[ti] * m.tri_k[ti], a, x0, x1, x2, ent.y, grad, D, acc, bs);
         if (pr.tri_shear)
-            tri_element(m.sh_dminv[ti], m.sh_area[ti] * m.sh_k[ti], a, x0, x1, x2, ent.y, grad, D, acc);
+            tri_element(m.sh_dminv[ti], m.sh_area[ti] * m.sh_k[ti], a, x0, x1, x2, ent.y, grad, D, acc, bs);
     }
     // bending dihedrals: gradient k (theta - theta_rest) a_i, Gauss-Newton
     // blocks k a_i a_j^T (bending_constraint.cu:175-246)
@@ -453,7 +465,7 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
 #pragma unroll
                 for (int c = 0; c < 9; ++c) D[c] += H[c];
             } else {
-                add_block(acc, kk[bb], H, 1.0f);
+                add_block(acc, kk[bb], H, 1.0f, A.bs);
             }
         }
     }
@@ -461,8 +473,7 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
         const int* colr = A.col + (size_t)sb * 64 + lane;
         for (int k = 0; k < deg; ++k) {
             if (A.sym && colr[k * 64] < row) continue;  // read through the upper mirror
-#pragma unroll
-            for (int c = 0; c < 9; ++c) vrow[(k * 9 + c) * 64] = acc[(k * 9 + c) * 64];
+            for (int c = 0; c < bs; ++c) vrow[(k * bs + c) * 64] = acc[(k * bs + c) * 64];
         }
     }
     // static-shape contact at x_n (detect_kernel + scatter_gradient_kernel,
@@ -585,12 +596,12 @@ __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVector
     const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
     const int* colp = A.col + (size_t)sb * 64 + lane;
     const int* mirp = A.mir + (size_t)sb * 64 + lane;
-    const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+    const float* vp = A.val + (size_t)sb * A.bs * 64 + lane;
     for (int k = 0; k < w; ++k) {
         const int c = colp[k * 64];
         if (c >= 0) {
             float m[9];
-            load_block(A, vp + k * 9 * 64, mirp + k * 64, c, row, m);
+            load_block(A, vp + k * A.bs * 64, mirp + k * 64, c, row, m);
             const double y0 = x[3 * c], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
             a0 += double(m[0]) * y0 + double(m[1]) * y1 + double(m[2]) * y2;
             a1 += double(m[3]) * y0 + double(m[4]) * y1 + double(m[5]) * y2;
@@ -649,7 +660,7 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_residual_kernel(LaunchGeom g, 
 // the XCD has streamed several L2 capacities.)
 __device__ __forceinline__ int sym_order(int k, int w) { return (k & 1) ? w - 1 - (k >> 1) : (k >> 1); }
 
-template <int WAVES, int UNROLL, bool BF, bool SYM>
+template <int WAVES, int UNROLL, bool BF, bool SYM, bool PK>
 __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
                                                              const InstState* __restrict__ st) {
     const int s = my_slice<WAVES>(g.nslices);
@@ -681,7 +692,8 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;
-        const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+        constexpr int BS = PK ? 6 : 9;
+        const float* vp = A.val + (size_t)sb * BS * 64 + lane;
         for (int k0 = 0; k0 < w; k0 += UNROLL) {
             int c[UNROLL], kk[UNROLL];
 #pragma unroll
@@ -697,10 +709,19 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
                 if (live) {
                     const int cc = (c[u] >= 0) ? c[u] : row;
                     const F3 pj = ld3(z, cc) + ld3(pold, cc) * beta;
-                    if (SYM && cc < row) {
+                    if (PK) {
+                        // symmetric blocks: the lower block (row, cc) is its
+                        // mirror (cc, row) itself, which row cc's lane
+                        // streamed at the same step of its loop
+                        const float* b = (cc < row) ? A.val + val_offset(mirp[kk[u] * 64], 6) : vp + kk[u] * 6 * 64;
+                        const float b0 = b[0], b1 = b[64], b2 = b[128], b3 = b[192], b4 = b[256], b5 = b[320];
+                        a0 += b0 * pj.x + b1 * pj.y + b2 * pj.z;
+                        a1 += b1 * pj.x + b3 * pj.y + b4 * pj.z;
+                        a2 += b2 * pj.x + b4 * pj.y + b5 * pj.z;
+                    } else if (SYM && cc < row) {
                         // lower block (row, cc) = transpose of (cc, row), which
                         // row cc's lane streamed at the same step of its loop
-                        const float* bt = A.val + mirp[kk[u] * 64];
+                        const float* bt = A.val + val_offset(mirp[kk[u] * 64], 9);
                         a0 += bt[0] * pj.x + bt[192] * pj.y + bt[384] * pj.z;
                         a1 += bt[64] * pj.x + bt[256] * pj.y + bt[448] * pj.z;
                         a2 += bt[128] * pj.x + bt[320] * pj.y + bt[512] * pj.z;
@@ -1031,12 +1052,12 @@ void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVecto
     CHYSX_LAUNCH_CHECK();
 }
 namespace {
-template <bool BF, bool SYM>
+template <bool BF, bool SYM, bool PK>
 void pcg_spmv_dispatch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
                        int unroll, hipStream_t s) {
-#define L1(W) pcg_spmv_kernel<W, 1, BF, SYM><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-#define L2(W) pcg_spmv_kernel<W, 2, BF, SYM><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-#define L4(W) pcg_spmv_kernel<W, 4, BF, SYM><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define L1(W) pcg_spmv_kernel<W, 1, BF, SYM, PK><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define L2(W) pcg_spmv_kernel<W, 2, BF, SYM, PK><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
+#define L4(W) pcg_spmv_kernel<W, 4, BF, SYM, PK><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
     if (unroll == 1) {
         CHYSX_WAVES_SWITCH(waves, L1)
     } else if (unroll == 2) {
@@ -1055,13 +1076,16 @@ void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& 
     if (g.nslices <= 0) return;
     if (!valid_unroll(unroll)) throw std::invalid_argument("unsupported unroll");
     if (symmetric != A.sym) throw std::invalid_argument("symmetric SpMV: launch flag and matrix mode differ");
+    if (A.bs != 9 && !(A.bs == 6 && A.sym)) throw std::invalid_argument("packed blocks need the symmetric SpMV");
     if (symmetric && A.mir == nullptr) throw std::invalid_argument("symmetric SpMV needs the mirror offsets");
     if (branch_free) {
-        if (symmetric) pcg_spmv_dispatch<true, true>(g, A, v, st, waves, unroll, s);
-        else pcg_spmv_dispatch<true, false>(g, A, v, st, waves, unroll, s);
+        if (A.bs == 6) pcg_spmv_dispatch<true, true, true>(g, A, v, st, waves, unroll, s);
+        else if (symmetric) pcg_spmv_dispatch<true, true, false>(g, A, v, st, waves, unroll, s);
+        else pcg_spmv_dispatch<true, false, false>(g, A, v, st, waves, unroll, s);
     } else {
-        if (symmetric) pcg_spmv_dispatch<false, true>(g, A, v, st, waves, unroll, s);
-        else pcg_spmv_dispatch<false, false>(g, A, v, st, waves, unroll, s);
+        if (A.bs == 6) pcg_spmv_dispatch<false, true, true>(g, A, v, st, waves, unroll, s);
+        else if (symmetric) pcg_spmv_dispatch<false, true, false>(g, A, v, st, waves, unroll, s);
+        else pcg_spmv_dispatch<false, false, false>(g, A, v, st, waves, unroll, s);
     }
     CHYSX_LAUNCH_CHECK();
 }

@@ -42,9 +42,10 @@ struct DevMesh {
 };
 
 // Wave-sliced ELL matrix: entry (s, k, lane) of component c lives at
-// val[((slice_ptr[s] + k) * 9 + c) * 64 + lane]; diag[(s * 9 + c) * 64 + lane];
+// val[((slice_ptr[s] + k) * bs + c) * 64 + lane]; diag[(s * 9 + c) * 64 + lane];
 // symmetric inverse diagonal minv[(s * 6 + c) * 64 + lane] with c over
-// (xx, xy, xz, yy, yz, zz).
+// (xx, xy, xz, yy, yz, zz).  bs = 9 stores a block row-major; bs = 6 stores a
+// symmetric block as (xx, xy, xz, yy, yz, zz) and needs sym.
 struct DevMatrix {
     const int* slice_ptr;
     const int* col;
@@ -52,12 +53,20 @@ struct DevMatrix {
     float* val;
     float* diag;
     float* minv;
-    // For an entry (i, j) with j < i: float offset of component 0 of the
-    // block (j, i) in val (its transpose, stored by row j); -1 otherwise.
+    // For an entry (i, j) with j < i: entry index (slice_ptr[s'] + k') * 64 +
+    // lane' of the block (j, i) stored by row j (its transpose); -1 otherwise.
     const int* mir;
     // 1: lower blocks are read through mir (assembly writes upper blocks only)
     int sym;
+    // floats per stored off-diagonal block: 9, or 6 when every off-diagonal
+    // block is symmetric (no bending) and sym is set
+    int bs;
 };
+
+// Float offset in val of component 0 of entry index e (slot * 64 + lane).
+__host__ __device__ inline size_t val_offset(int e, int bs) {
+    return size_t(e & ~63) * size_t(bs) + size_t(e & 63);
+}
 
 // Solver vectors in padded row space, 3 components per row (AoS).
 struct DevVectors {

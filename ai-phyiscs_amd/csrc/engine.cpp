@@ -337,7 +337,7 @@ void Problem::build(const std::vector<const Instance*>& insts) {
                         break;
                     }
                 if (kk < 0) throw std::logic_error("pattern is not symmetric");
-                mir[e] = int((size_t(slice_ptr_h_[size_t(sj)]) + kk) * 9 * kSlice + lj);
+                mir[e] = int((size_t(slice_ptr_h_[size_t(sj)]) + kk) * kSlice + lj);
             }
         }
     }
@@ -485,7 +485,10 @@ void Problem::update_pin_targets(const Instance& inst, int k, hipStream_t s) {
 
 DevMatrix Problem::matrix() const {
     return DevMatrix{slice_ptr_.data(), col_.data(),  row_deg_.data(), val_.data(),
-                     diag_.data(),      minv_.data(), mir_.data(),     cfg_.spmv_symmetric};
+                     diag_.data(),      minv_.data(), mir_.data(),     cfg_.spmv_symmetric, block_floats()};
+}
+int Problem::block_floats() const {
+    return (cfg_.spmv_symmetric && cfg_.spmv_packed && dih_v_.size() == 0) ? 6 : 9;
 }
 DevVectors Problem::vectors() const {
     return DevVectors{rhs_.data(), r_.data(), z_.data(), ap_.data(), p0_.data(), p1_.data(), x_.data(),
@@ -540,7 +543,8 @@ void Problem::launch_pcg_iteration(hipStream_t s) {
 void Problem::set_launch_config(const LaunchConfig& c) {
     if (!valid_waves(c.assemble_waves) || !valid_waves(c.residual_waves) || !valid_waves(c.spmv_waves) ||
         !valid_waves(c.update_waves) || !valid_waves(c.finalize_waves) || !valid_unroll(c.spmv_unroll) ||
-        (c.spmv_branch_free != 0 && c.spmv_branch_free != 1) || (c.spmv_symmetric != 0 && c.spmv_symmetric != 1))
+        (c.spmv_branch_free != 0 && c.spmv_branch_free != 1) || (c.spmv_symmetric != 0 && c.spmv_symmetric != 1) ||
+        (c.spmv_packed != 0 && c.spmv_packed != 1))
         throw std::invalid_argument("launch config: waves must be 1, 2, 4, 8 or 16 and unroll 1, 2 or 4");
     cfg_ = c;
     destroy_graph();
@@ -857,20 +861,23 @@ void Problem::copy_values(int k, float* out) const {
     const Pattern& P = pats_[size_t(k)];
     const int r0 = inst_row0_[size_t(k)];
     const bool sym = cfg_.spmv_symmetric != 0;
+    const int bs = block_floats();
+    // packed upper triangle (xx, xy, xz, yy, yz, zz) -> row-major 3x3
+    static const int kUnpack[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
     for (int i = 0; i < P.n; ++i) {
         const int row = r0 + i, s = row / kSlice, lane = row % kSlice;
         const int sb = slice_ptr_h_[size_t(s)];
         for (int q = P.row_offsets[size_t(i)]; q < P.row_offsets[size_t(i) + 1]; ++q) {
             const int kk = q - P.row_offsets[size_t(i)];
-            const size_t e = (size_t(sb) + kk) * kSlice + lane;
-            if (sym && P.cols[size_t(q)] < i) {
-                // the operator's lower block: transpose of the upper mirror
-                const size_t m = size_t(mir_h_[e]);
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b) out[9 * size_t(q) + 3 * a + b] = v[m + size_t(3 * b + a) * kSlice];
-            } else {
-                for (int c = 0; c < 9; ++c) out[9 * size_t(q) + c] = v[((size_t(sb) + kk) * 9 + c) * kSlice + lane];
-            }
+            const int e = int((size_t(sb) + kk) * kSlice + lane);
+            // the operator's lower block: transpose of the upper mirror
+            const bool mirror = sym && P.cols[size_t(q)] < i;
+            const size_t base = val_offset(mirror ? mir_h_[size_t(e)] : e, bs);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const int c = mirror ? 3 * b + a : 3 * a + b;
+                    out[9 * size_t(q) + 3 * a + b] = v[base + size_t(bs == 6 ? kUnpack[c] : c) * kSlice];
+                }
         }
     }
 }

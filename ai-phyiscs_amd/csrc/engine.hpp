@@ -105,6 +105,10 @@ struct LaunchConfig {
     int spmv_symmetric = 1;
     int update_waves = 4;
     int finalize_waves = 4;
+    // 1: with spmv_symmetric and no bending elements (every off-diagonal
+    // block then symmetric), off-diagonal blocks are stored as their upper
+    // triangle (6 floats).  The operator is bitwise the same as with 9.
+    int spmv_packed = 1;
 };
 
 struct TuneSample {
@@ -177,6 +181,8 @@ public:
     // Kernel-level access for benchmarks.
     LaunchGeom geom() const { return LaunchGeom{slices_.data(), int(slices_h_.size())}; }
     DevMatrix matrix() const;
+    // Floats per stored off-diagonal block: 6 (symmetric, packed) or 9.
+    int block_floats() const;
     DevVectors vectors() const;
     DevMesh mesh() const;
     InstState* states() const { return state_.data(); }

@@ -4,7 +4,8 @@
 // multiplies by A.  With A.sym set the lower blocks (row, c), c < row, are
 // read as transposes of their upper mirrors (c, row); assembly then writes
 // only the upper blocks, and every product (iterations, true residual, warm
-// start) uses the same symmetric operator.
+// start) uses the same symmetric operator.  With A.bs == 6 every stored
+// block is symmetric and kept as its upper triangle.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,8 +18,20 @@ namespace chysx_amd {
 // slice, stride 64 floats), row-major into m.
 __device__ __forceinline__ void load_block(const DevMatrix& A, const float* __restrict__ vp_k,
                                            const int* __restrict__ mir_k, int c, int row, float (&m)[9]) {
-    if (A.sym && c < row) {
-        const float* bt = A.val + *mir_k;
+    if (A.bs == 6) {
+        // symmetric block: the mirror (c, row) equals its transpose
+        const float* b = (A.sym && c < row) ? A.val + val_offset(*mir_k, 6) : vp_k;
+        m[0] = b[0];
+        m[1] = b[64];
+        m[2] = b[128];
+        m[3] = m[1];
+        m[4] = b[192];
+        m[5] = b[256];
+        m[6] = m[2];
+        m[7] = m[5];
+        m[8] = b[320];
+    } else if (A.sym && c < row) {
+        const float* bt = A.val + val_offset(*mir_k, 9);
         m[0] = bt[0];
         m[1] = bt[192];
         m[2] = bt[384];

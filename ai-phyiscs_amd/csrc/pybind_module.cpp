@@ -425,7 +425,7 @@ PYBIND11_MODULE(_chysx_native, m) {
             py::arg("on"))
         .def("launch_config",
              [](const Sim& s) {
-                 int c[8];
+                 int c[CHYSX_LAUNCH_CONFIG_LEN];
                  check(chysx_cloth_launch_config(s.p(), c));
                  py::dict d;
                  d["assemble_waves"] = c[0];
@@ -436,21 +436,28 @@ PYBIND11_MODULE(_chysx_native, m) {
                  d["finalize_waves"] = c[5];
                  d["spmv_branch_free"] = c[6];
                  d["spmv_symmetric"] = c[7];
+                 d["spmv_packed"] = c[8];
                  return d;
              })
         .def(
             "set_launch_config",
             [](Sim& s, py::dict d) {
-                int c[8];
+                int c[CHYSX_LAUNCH_CONFIG_LEN];
                 check(chysx_cloth_launch_config(s.p(), c));
-                const char* keys[8] = {"assemble_waves", "residual_waves",   "spmv_waves",    "spmv_unroll",
-                                       "update_waves",   "finalize_waves",   "spmv_branch_free",
-                                       "spmv_symmetric"};
-                for (int i = 0; i < 8; ++i)
+                const char* keys[CHYSX_LAUNCH_CONFIG_LEN] = {
+                    "assemble_waves", "residual_waves",   "spmv_waves",     "spmv_unroll", "update_waves",
+                    "finalize_waves", "spmv_branch_free", "spmv_symmetric", "spmv_packed"};
+                for (int i = 0; i < CHYSX_LAUNCH_CONFIG_LEN; ++i)
                     if (d.contains(keys[i])) c[i] = d[keys[i]].cast<int>();
                 check(chysx_cloth_set_launch_config(s.p(), c));
             },
             py::arg("config"))
+        .def("block_floats",
+             [](const Sim& s) {
+                 int f = 0;
+                 check(chysx_cloth_block_floats(s.p(), &f));
+                 return f;
+             })
         .def("tune_log",
              [](const Sim& s) {
                  int n = 0;

@@ -819,12 +819,12 @@ __global__ __launch_bounds__(256) void ws_gram_kernel(LaunchGeom g, DevMatrix A,
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;
-        const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+        const float* vp = A.val + (size_t)sb * A.bs * 64 + lane;
         for (int k = 0; k < w; ++k) {
             const int c = colp[k * 64];
             if (c < 0) continue;
             float mf[9];
-            load_block(A, vp + k * 9 * 64, mirp + k * 64, c, row, mf);
+            load_block(A, vp + k * A.bs * 64, mirp + k * 64, c, row, mf);
 #pragma unroll
             for (int q = 0; q < 9; ++q) m[q] = double(mf[q]);
 #pragma unroll
@@ -1001,12 +1001,12 @@ __global__ __launch_bounds__(256) void ws_base_kernel(LaunchGeom g, DevMatrix A,
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;
-        const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+        const float* vp = A.val + (size_t)sb * A.bs * 64 + lane;
         for (int k = 0; k < w; ++k) {
             const int c = colp[k * 64];
             if (c >= 0) {
                 float m[9];
-                load_block(A, vp + k * 9 * 64, mirp + k * 64, c, row, m);
+                load_block(A, vp + k * A.bs * 64, mirp + k * 64, c, row, m);
                 const double y0 = x[3 * c], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
                 a0 += double(m[0]) * y0 + double(m[1]) * y1 + double(m[2]) * y2;
                 a1 += double(m[3]) * y0 + double(m[4]) * y1 + double(m[5]) * y2;
@@ -1164,12 +1164,12 @@ __global__ __launch_bounds__(256) void ws_dir_spmv_kernel(LaunchGeom g, DevMatri
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;
-        const float* vp = A.val + (size_t)sb * 9 * 64 + lane;
+        const float* vp = A.val + (size_t)sb * A.bs * 64 + lane;
         for (int k = 0; k < w; ++k) {
             const int c = colp[k * 64];
             if (c >= 0) {
                 float m[9];
-                load_block(A, vp + k * 9 * 64, mirp + k * 64, c, row, m);
+                load_block(A, vp + k * A.bs * 64, mirp + k * 64, c, row, m);
                 const double y0 = d[3 * c], y1 = d[3 * c + 1], y2 = d[3 * c + 2];
                 a0 += double(m[0]) * y0 + double(m[1]) * y1 + double(m[2]) * y2;
                 a1 += double(m[3]) * y0 + double(m[4]) * y1 + double(m[5]) * y2;

@@ -35,6 +35,12 @@ def kernel_bytes(n_rows: int, nnz: int):
         "spmv": 40.0 * nnz + 64.0 * n_rows + 4.0,
         # PCG SpMV: values+cols 40/block; diag 36, z 12, p_prev 12 reads, p 12, Ap 12 writes per row
         "pcg_spmv": 40.0 * nnz + 84.0 * n_rows,
+        # the same pass on symmetric storage: the nnz/2 upper blocks are
+        # streamed (36 B), every entry's column (4 B) and each lower entry's
+        # mirror index (4 B); lower blocks re-read their mirrors from L2
+        "pcg_spmv_symmetric": 18.0 * nnz + 4.0 * nnz + 2.0 * nnz + 84.0 * n_rows,
+        # ... and with packed symmetric blocks (24 B each)
+        "pcg_spmv_symmetric_packed": 12.0 * nnz + 4.0 * nnz + 2.0 * nnz + 84.0 * n_rows,
         # PCG update: p 12, Ap 12, x 24, r 12, M^-1 24 reads; x 24, r 12, z 12 writes per row
         "pcg_update": 132.0 * n_rows,
         # reference formulation of one PCG iteration: B_spmv + 180 per row
@@ -315,29 +321,46 @@ def main():
     kb = kernel_bytes(n, nnz)
     # Kernel timings with HIP events on the launch stream (null stream).
     stream = torch.cuda.current_stream().cuda_stream
-    t_pcg_spmv = g.sim.time_kernel(1, args.kernel_reps, s.dt, stream)
-    t_pcg_update = g.sim.time_kernel(2, args.kernel_reps, s.dt, stream)
-    t_spmv = g.sim.time_kernel(0, args.kernel_reps, s.dt, stream)
-    t_iter = g.sim.time_kernel(4, args.kernel_reps, s.dt, stream)
-    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9  # noqa: E731
-    # The other SpMV storage mode, same matrix and shapes (speed only).
     cfg0 = g.sim.launch_config()
-    g.sim.set_launch_config(dict(spmv_symmetric=1 - cfg0["spmv_symmetric"]))
-    t_alt = g.sim.time_kernel(1, args.kernel_reps, s.dt, stream)
-    t_alt_iter = g.sim.time_kernel(4, args.kernel_reps, s.dt, stream)
-    g.sim.set_launch_config(dict(spmv_symmetric=cfg0["spmv_symmetric"]))
+    mode0 = ("general" if not cfg0["spmv_symmetric"] else
+             "symmetric_packed" if g.sim.block_floats() == 6 else "symmetric")
+    t_pcg_update = g.sim.time_kernel(2, args.kernel_reps, s.dt, stream)
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9  # noqa: E731
+    # The SpMV pass and a whole iteration on each matrix storage (same
+    # matrix values, re-assembled in that storage; speed only).
+    storage = {}
+    for name, sym, packed in [("general", 0, 0), ("symmetric", 1, 0), ("symmetric_packed", 1, 1)]:
+        g.sim.set_launch_config(dict(spmv_symmetric=sym, spmv_packed=packed))
+        g.sim.time_kernel(3, 1, s.dt, stream)  # one assembly into this storage
+        t_sp = g.sim.time_kernel(1, args.kernel_reps, s.dt, stream)
+        t_it = g.sim.time_kernel(4, args.kernel_reps, s.dt, stream)
+        key = "pcg_spmv" if name == "general" else f"pcg_spmv_{name}"
+        storage[name] = {"block_floats": g.sim.block_floats(), "layout_bytes": kb[key], "ms": round(t_sp, 5),
+                         "GB/s_layout": round(gbs(kb[key], t_sp), 1),
+                         "frac_layout": round(gbs(kb[key], t_sp) / PEAK_HBM_GBS, 4),
+                         "GB/s_ref_formulation": round(gbs(kb["pcg_spmv"], t_sp), 1),
+                         "iteration_ms": round(t_it, 5)}
+        if name == "general":
+            t_spmv = g.sim.time_kernel(0, args.kernel_reps, s.dt, stream)  # plain SpMV reads general storage
+    g.sim.set_launch_config(dict(spmv_symmetric=cfg0["spmv_symmetric"], spmv_packed=cfg0["spmv_packed"]))
+    g.sim.time_kernel(3, 1, s.dt, stream)
+    t_pcg_spmv = storage[mode0]["ms"]
+    t_iter = storage[mode0]["iteration_ms"]
+    key0 = "pcg_spmv" if mode0 == "general" else f"pcg_spmv_{mode0}"
 
     traffic = read_traffic(args.size)
     roofline = {
         "bound": "hbm",
-        "kernel": "pcg_spmv_kernel (p = z + beta p_prev; Ap = A p; p.Ap)",
-        "achieved": round(gbs(kb["pcg_spmv"], t_pcg_spmv), 1),
+        "kernel": f"pcg_spmv_kernel, {mode0} storage (p = z + beta p_prev; Ap = A p; p.Ap)",
+        "achieved": round(gbs(kb[key0], t_pcg_spmv), 1),
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
-        "frac": round(gbs(kb["pcg_spmv"], t_pcg_spmv) / PEAK_HBM_GBS, 4),
+        "frac": round(gbs(kb[key0], t_pcg_spmv) / PEAK_HBM_GBS, 4),
         "traffic": traffic.get("pcg_spmv_kernel"),
-        "bytes_per_launch": kb["pcg_spmv"],
+        "bytes_per_launch": kb[key0],
         "ms_per_launch": round(t_pcg_spmv, 5),
+        "ref_formulation_bytes": kb["pcg_spmv"],
+        "ref_formulation_GB/s": round(gbs(kb["pcg_spmv"], t_pcg_spmv), 1),
     }
     extra_kernels = {
         "spmv_plain": {"bytes": kb["spmv"], "ms": round(t_spmv, 5),
@@ -347,9 +370,7 @@ def main():
                        "GB/s": round(gbs(kb["pcg_update"], t_pcg_update), 1)},
         "pcg_iteration": {"ref_bytes": kb["pcg_iteration_ref"], "ms": round(t_iter, 5),
                           "GB/s_ref_formulation": round(gbs(kb["pcg_iteration_ref"], t_iter), 1)},
-        ("pcg_spmv_symmetric" if not cfg0["spmv_symmetric"] else "pcg_spmv_general"): {
-            "ms": round(t_alt, 5), "GB/s_algorithmic": round(gbs(kb["pcg_spmv"], t_alt), 1),
-            "frac": round(gbs(kb["pcg_spmv"], t_alt) / PEAK_HBM_GBS, 4), "iteration_ms": round(t_alt_iter, 5)},
+        "pcg_spmv_storage": storage,
     }
 
     ws_cmp = None

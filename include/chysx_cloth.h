@@ -219,18 +219,24 @@ int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts
  * its reduction, update pass, its reduction).
  * The solver state is left unusable for warm starts; call before a step. */
 int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uintptr_t stream, double* ms);
-/* Kernel launch shapes (speed only: results never depend on them).
- * cfg[8] = {assemble waves, residual waves, SpMV-pass waves, SpMV unroll,
- * update-pass waves, finalize waves, SpMV branch-free padding (0/1), SpMV
- * symmetric (0/1: lower blocks read as transposes of their upper mirrors;
- * changes results at FP32 rounding level, never chosen by the autotuner)};
- * waves in {1,2,4,8,16} wavefronts of 64 lanes per workgroup, unroll in
- * {1,2,4}.  By default an autotuner times the
- * SpMV and update shapes on the problem's own matrix when its topology is
- * built (problems of 32768 rows or more). */
+/* Kernel launch shapes and matrix storage.
+ * cfg[CHYSX_LAUNCH_CONFIG_LEN] = {assemble waves, residual waves, SpMV-pass
+ * waves, SpMV unroll, update-pass waves, finalize waves, SpMV branch-free
+ * padding (0/1), SpMV symmetric (0/1: lower blocks read as transposes of
+ * their upper mirrors; changes results at FP32 rounding level, never chosen
+ * by the autotuner), packed blocks (0/1: with the symmetric SpMV and no
+ * bending elements, off-diagonal blocks stored as their upper triangle; the
+ * operator and results are bitwise those of 9-float blocks)}; waves in
+ * {1,2,4,8,16} wavefronts of 64 lanes per workgroup, unroll in {1,2,4}.
+ * Shapes never change results.  By default an autotuner times the SpMV and
+ * update shapes on the problem's own matrix when its topology is built
+ * (problems of 32768 rows or more). */
+#define CHYSX_LAUNCH_CONFIG_LEN 9
 int chysx_cloth_set_autotune(chysx_cloth* sim, int on);
 int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg);
 int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg);
+/* Floats per stored off-diagonal block in effect: 6 (packed) or 9. */
+int chysx_cloth_block_floats(const chysx_cloth* sim, int* floats);
 /* Autotuner samples: count in/out; per sample {kernel (0 SpMV, 1 update,
  * 2 branch-free SpMV),
  * waves, unroll} and mean ms. */

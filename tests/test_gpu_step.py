@@ -287,3 +287,38 @@ def test_symmetric_spmv_mode_matches_default(cuda):
         assert bsr_residual(diag, ro, co, vals, rhs, g.sim.get_dx64()) <= 1.0001e-6
         runs.append(g.positions())
     assert rel(runs[1], runs[0].astype(np.float64)) <= 1e-6
+
+
+def test_packed_blocks_bitwise_equal_to_full_blocks(cuda):
+    """Spring and stretch blocks are symmetric by construction (the element
+    formulas fill (b, a) from (a, b)), so the packed upper-triangle storage
+    holds the same operator: positions, iterations and dumped systems agree
+    bitwise with 9-float symmetric storage.  With bending (a_i a_j^T blocks
+    are not symmetric) storage stays at 9 floats."""
+    s = scenes.config2(w=72, steps=6)
+    runs = []
+    for packed in [0, 1]:
+        g = make_sim(s, tol=1e-6)
+        g.sim.set_autotune(False)
+        g.sim.set_launch_config(dict(spmv_symmetric=1, spmv_packed=packed))
+        assert g.sim.launch_config()["spmv_packed"] == packed
+        iters = []
+        for _ in range(6):
+            g.sim.step(s.dt)
+            st = g.sim.last_solve_stats()
+            assert st["status"] == CONVERGED and st["relres"] <= 1e-6
+            iters.append(st["iters"])
+        assert g.sim.block_floats() == (6 if packed else 9)
+        runs.append((g.positions(), g.velocities(), g.sim.get_dx64(), iters, g.sim.debug_dump_last_solve()))
+    (p0, v0, d0, i0, s0), (p1, v1, d1, i1, s1) = runs
+    assert i0 == i1
+    assert np.array_equal(p0, p1) and np.array_equal(v0, v1) and np.array_equal(d0, d1)
+    for a, b in zip(s0, s1):
+        assert np.array_equal(a, b)
+
+    d = scenes.drape(w=20)
+    g = make_sim(d, tol=1e-6, springs_active=False, force_active=False)
+    g.sim.set_launch_config(dict(spmv_symmetric=1, spmv_packed=1))
+    g.sim.step(d.dt)
+    assert g.sim.block_floats() == 9
+    assert g.sim.last_solve_stats()["status"] == CONVERGED
