@@ -548,9 +548,19 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
         }
     }
     st3(v.rhs, row, rr);
-    float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+    if (bs == 6) {  // symmetric diagonal blocks (no bending): upper triangle
+        float* dg = A.diag + (size_t)s * 6 * 64 + lane;
+        dg[0] = D[0];
+        dg[64] = D[1];
+        dg[128] = D[2];
+        dg[192] = D[4];
+        dg[256] = D[5];
+        dg[320] = D[8];
+    } else {
+        float* dg = A.diag + (size_t)s * 9 * 64 + lane;
 #pragma unroll
-    for (int c = 0; c < 9; ++c) dg[c * 64] = D[c];
+        for (int c = 0; c < 9; ++c) dg[c * 64] = D[c];
+    }
     // Inverse of the symmetric diagonal block by adjugate / determinant, no
     // guard (math::inverse, matrix.cuh:219-235; residual_m_inv_kernel,
     // pcg_solver.cu:29-51).
@@ -588,11 +598,12 @@ __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVector
                                              int row, bool report_only, double (&part)[4]) {
     const int s = row >> 6, lane = row & 63;
     const double* x = v.x;
-    const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+    float dm[9];
+    load_diag(A, s, lane, dm);
     const double x0 = x[3 * row], x1 = x[3 * row + 1], x2 = x[3 * row + 2];
-    double a0 = double(dg[0]) * x0 + double(dg[64]) * x1 + double(dg[128]) * x2;
-    double a1 = double(dg[192]) * x0 + double(dg[256]) * x1 + double(dg[320]) * x2;
-    double a2 = double(dg[384]) * x0 + double(dg[448]) * x1 + double(dg[512]) * x2;
+    double a0 = double(dm[0]) * x0 + double(dm[1]) * x1 + double(dm[2]) * x2;
+    double a1 = double(dm[3]) * x0 + double(dm[4]) * x1 + double(dm[5]) * x2;
+    double a2 = double(dm[6]) * x0 + double(dm[7]) * x1 + double(dm[8]) * x2;
     const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
     const int* colp = A.col + (size_t)sb * 64 + lane;
     const int* mirp = A.mir + (size_t)sb * 64 + lane;
@@ -685,10 +696,19 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
     double part[1] = {0.0};
     if (valid) {
         const F3 pi = ld3(z, row) + ld3(pold, row) * beta;
-        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
-        float a0 = dg[0] * pi.x + dg[64] * pi.y + dg[128] * pi.z;
-        float a1 = dg[192] * pi.x + dg[256] * pi.y + dg[320] * pi.z;
-        float a2 = dg[384] * pi.x + dg[448] * pi.y + dg[512] * pi.z;
+        float a0, a1, a2;
+        if (PK) {
+            const float* dg = A.diag + (size_t)s * 6 * 64 + lane;
+            const float d0 = dg[0], d1 = dg[64], d2 = dg[128], d3 = dg[192], d4 = dg[256], d5 = dg[320];
+            a0 = d0 * pi.x + d1 * pi.y + d2 * pi.z;
+            a1 = d1 * pi.x + d3 * pi.y + d4 * pi.z;
+            a2 = d2 * pi.x + d4 * pi.y + d5 * pi.z;
+        } else {
+            const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+            a0 = dg[0] * pi.x + dg[64] * pi.y + dg[128] * pi.z;
+            a1 = dg[192] * pi.x + dg[256] * pi.y + dg[320] * pi.z;
+            a2 = dg[384] * pi.x + dg[448] * pi.y + dg[512] * pi.z;
+        }
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;

@@ -45,7 +45,8 @@ struct DevMesh {
 // val[((slice_ptr[s] + k) * bs + c) * 64 + lane]; diag[(s * 9 + c) * 64 + lane];
 // symmetric inverse diagonal minv[(s * 6 + c) * 64 + lane] with c over
 // (xx, xy, xz, yy, yz, zz).  bs = 9 stores a block row-major; bs = 6 stores a
-// symmetric block as (xx, xy, xz, yy, yz, zz) and needs sym.
+// symmetric block as (xx, xy, xz, yy, yz, zz) and needs sym; the diagonal
+// blocks then use the same 6-float form, diag[(s * 6 + c) * 64 + lane].
 struct DevMatrix {
     const int* slice_ptr;
     const int* col;

@@ -851,9 +851,12 @@ std::vector<T> download(const T* dev, size_t n) {
 void Problem::copy_diag(int k, float* out) const {
     const std::vector<float> d = download(diag_.data(), diag_.size());
     const int r0 = inst_row0_[size_t(k)];
+    const int bs = block_floats();  // 6: symmetric diagonal blocks kept as their upper triangle
+    static const int kUnpack[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
     for (int i = 0; i < inst_n_[size_t(k)]; ++i) {
         const int row = r0 + i, s = row / kSlice, lane = row % kSlice;
-        for (int c = 0; c < 9; ++c) out[9 * i + c] = d[(size_t(s) * 9 + c) * kSlice + lane];
+        for (int c = 0; c < 9; ++c)
+            out[9 * i + c] = d[(size_t(s) * bs + (bs == 6 ? kUnpack[c] : c)) * kSlice + lane];
     }
 }
 void Problem::copy_values(int k, float* out) const {

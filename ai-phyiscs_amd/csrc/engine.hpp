@@ -105,8 +105,8 @@ struct LaunchConfig {
     int spmv_symmetric = 1;
     int update_waves = 4;
     int finalize_waves = 4;
-    // 1: with spmv_symmetric and no bending elements (every off-diagonal
-    // block then symmetric), off-diagonal blocks are stored as their upper
+    // 1: with spmv_symmetric and no bending elements (every block, diagonal
+    // or not, then symmetric bitwise), blocks are stored as their upper
     // triangle (6 floats).  The operator is bitwise the same as with 9.
     int spmv_packed = 1;
 };

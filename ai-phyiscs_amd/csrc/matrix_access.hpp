@@ -47,4 +47,26 @@ __device__ __forceinline__ void load_block(const DevMatrix& A, const float* __re
     }
 }
 
+// The diagonal block of row (s, lane), row-major into m.  With A.bs == 6
+// (no bending) the diagonal blocks are symmetric too and stored as their
+// upper triangle at diag[(s * 6 + c) * 64 + lane].
+__device__ __forceinline__ void load_diag(const DevMatrix& A, int s, int lane, float (&m)[9]) {
+    if (A.bs == 6) {
+        const float* d = A.diag + (size_t)s * 6 * 64 + lane;
+        m[0] = d[0];
+        m[1] = d[64];
+        m[2] = d[128];
+        m[3] = m[1];
+        m[4] = d[192];
+        m[5] = d[256];
+        m[6] = m[2];
+        m[7] = m[5];
+        m[8] = d[320];
+    } else {
+        const float* d = A.diag + (size_t)s * 9 * 64 + lane;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m[q] = d[q * 64];
+    }
+}
+
 }  // namespace chysx_amd

@@ -804,10 +804,11 @@ __global__ __launch_bounds__(256) void ws_gram_kernel(LaunchGeom g, DevMatrix A,
         }
     double b0 = 0.0, b1 = 0.0, b2 = 0.0;
     if (lane < sd.nvalid) {
-        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+        float dm[9];
+        load_diag(A, s, lane, dm);
         double m[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) m[q] = double(dg[64 * q]);
+        for (int q = 0; q < 9; ++q) m[q] = double(dm[q]);
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             ld3f(h.v[j], row, vr[j][0], vr[j][1], vr[j][2]);
@@ -991,13 +992,14 @@ __global__ __launch_bounds__(256) void ws_base_kernel(LaunchGeom g, DevMatrix A,
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     double x0 = 0.0, x1 = 0.0, x2 = 0.0;
     if (lane < sd.nvalid) {
-        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+        float dm[9];
+        load_diag(A, s, lane, dm);
         x0 = x[3 * row];
         x1 = x[3 * row + 1];
         x2 = x[3 * row + 2];
-        double a0 = double(dg[0]) * x0 + double(dg[64]) * x1 + double(dg[128]) * x2;
-        double a1 = double(dg[192]) * x0 + double(dg[256]) * x1 + double(dg[320]) * x2;
-        double a2 = double(dg[384]) * x0 + double(dg[448]) * x1 + double(dg[512]) * x2;
+        double a0 = double(dm[0]) * x0 + double(dm[1]) * x1 + double(dm[2]) * x2;
+        double a1 = double(dm[3]) * x0 + double(dm[4]) * x1 + double(dm[5]) * x2;
+        double a2 = double(dm[6]) * x0 + double(dm[7]) * x1 + double(dm[8]) * x2;
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;
@@ -1080,8 +1082,9 @@ __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatri
             const float w = e.inv_mass[p];
             mass = (w > 1.0e-12f) ? (1.0f / w) : 1.0e8f;
         }
-        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
-        const float tr = (dg[0] + dg[256] + dg[512]) * (1.0f / 3.0f);
+        float dm[9];
+        load_diag(A, s, lane, dm);
+        const float tr = (dm[0] + dm[4] + dm[8]) * (1.0f / 3.0f);
         f[13] = (tr > 0.0f) ? fminf(mass / (dt * dt) / tr, 1.0f) : 0.0f;
         f[14] = 1.0f;
         float px, py, pz, rx, ry, rz;
@@ -1154,13 +1157,14 @@ __global__ __launch_bounds__(256) void ws_dir_spmv_kernel(LaunchGeom g, DevMatri
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0;
     if (lane < sd.nvalid) {
-        const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
+        float dm[9];
+        load_diag(A, s, lane, dm);
         d0 = d[3 * row];
         d1 = d[3 * row + 1];
         d2 = d[3 * row + 2];
-        a0 = double(dg[0]) * d0 + double(dg[64]) * d1 + double(dg[128]) * d2;
-        a1 = double(dg[192]) * d0 + double(dg[256]) * d1 + double(dg[320]) * d2;
-        a2 = double(dg[384]) * d0 + double(dg[448]) * d1 + double(dg[512]) * d2;
+        a0 = double(dm[0]) * d0 + double(dm[1]) * d1 + double(dm[2]) * d2;
+        a1 = double(dm[3]) * d0 + double(dm[4]) * d1 + double(dm[5]) * d2;
+        a2 = double(dm[6]) * d0 + double(dm[7]) * d1 + double(dm[8]) * d2;
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;

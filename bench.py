@@ -39,8 +39,8 @@ def kernel_bytes(n_rows: int, nnz: int):
         # streamed (36 B), every entry's column (4 B) and each lower entry's
         # mirror index (4 B); lower blocks re-read their mirrors from L2
         "pcg_spmv_symmetric": 18.0 * nnz + 4.0 * nnz + 2.0 * nnz + 84.0 * n_rows,
-        # ... and with packed symmetric blocks (24 B each)
-        "pcg_spmv_symmetric_packed": 12.0 * nnz + 4.0 * nnz + 2.0 * nnz + 84.0 * n_rows,
+        # ... and with packed symmetric blocks (24 B each, diagonal blocks too)
+        "pcg_spmv_symmetric_packed": 12.0 * nnz + 4.0 * nnz + 2.0 * nnz + 72.0 * n_rows,
         # PCG update: p 12, Ap 12, x 24, r 12, M^-1 24 reads; x 24, r 12, z 12 writes per row
         "pcg_update": 132.0 * n_rows,
         # reference formulation of one PCG iteration: B_spmv + 180 per row

@@ -225,8 +225,8 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
  * padding (0/1), SpMV symmetric (0/1: lower blocks read as transposes of
  * their upper mirrors; changes results at FP32 rounding level, never chosen
  * by the autotuner), packed blocks (0/1: with the symmetric SpMV and no
- * bending elements, off-diagonal blocks stored as their upper triangle; the
- * operator and results are bitwise those of 9-float blocks)}; waves in
+ * bending elements, diagonal and off-diagonal blocks stored as their upper
+ * triangle; the operator and results are bitwise those of 9-float blocks)}; waves in
  * {1,2,4,8,16} wavefronts of 64 lanes per workgroup, unroll in {1,2,4}.
  * Shapes never change results.  By default an autotuner times the SpMV and
  * update shapes on the problem's own matrix when its topology is built
