@@ -241,7 +241,8 @@ def batch_warm_start_comparison(args, sc, make_batch):
 
 def read_traffic(size):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-    (recorded in profiles/traffic_cloth<size>.md as `kernel=bytes` lines)."""
+    (recorded in profiles/traffic_cloth<size>.md as
+    `traffic_bytes_per_launch <kernel> <bytes>` lines)."""
     path = os.path.join(ROOT, "profiles", f"traffic_cloth{size}.md")
     out = {}
     if os.path.exists(path):
@@ -356,7 +357,8 @@ def main():
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
         "frac": round(gbs(kb[key0], t_pcg_spmv) / PEAK_HBM_GBS, 4),
-        "traffic": traffic.get("pcg_spmv_kernel"),
+        "traffic": traffic.get({"symmetric_packed": "pcg_spmv_kernel", "symmetric": "pcg_spmv_kernel_symmetric",
+                                "general": "pcg_spmv_kernel_general"}[mode0]),
         "bytes_per_launch": kb[key0],
         "ms_per_launch": round(t_pcg_spmv, 5),
         "ref_formulation_bytes": kb["pcg_spmv"],
