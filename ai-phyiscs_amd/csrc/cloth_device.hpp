@@ -28,6 +28,7 @@ enum PcgStatus : int {
 enum PcgMode : int {
     kModeIter = 0,   // p = z + beta p_prev, A p, p . A p
     kModeCheck = 1,  // the recursive residual passed the test: true residual b - A x
+    kModeFlush = 2,  // x += alpha_pend p (the deferred term) before a check or a stop
 };
 
 // Reduction phases of pcg_reduce_kernel.
@@ -57,6 +58,16 @@ struct alignas(16) InstState {
     int parity;      // which p buffer holds the current direction
     int slice0;      // first slice of this instance (set at build)
     int nslices;     // slices of this instance (set at build)
+    // Deferred iterate update: an update pass with `defer` set leaves
+    // x += alpha p to the next update pass, which adds both terms while both
+    // directions are still in the p buffers (x is read and written every
+    // other iteration).  pend: a term alpha_pend p is owed; flush_to: what a
+    // flush pass leads to (kModeCheck, or kStopped as a status).
+    double alpha_pend;
+    int pend;
+    int defer;
+    int flush_to;
+    int pad;
 };
 
 // Per-instance step parameters and solver options.

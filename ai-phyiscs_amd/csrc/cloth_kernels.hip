@@ -689,6 +689,19 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
         wave_partials<4>(part, v.partials, s, g.nslices);
         return;
     }
+    if (S.mode == kModeFlush) {
+        // the owed term of a deferred update; the latest direction is the
+        // current p_prev buffer
+        if (valid) {
+            const F3 pk = ld3(parity ? v.p1 : v.p0, row);
+            const double ap = S.alpha_pend;
+            double* x = v.x + 3 * row;
+            x[0] += ap * double(pk.x);
+            x[1] += ap * double(pk.y);
+            x[2] += ap * double(pk.z);
+        }
+        return;
+    }
     const float beta = float(S.beta);
     const float* __restrict__ pold = parity ? v.p1 : v.p0;
     float* __restrict__ pnew = parity ? v.p0 : v.p1;
@@ -765,6 +778,8 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
 // PCG update pass: alpha = rho / sigma (guarded), x += alpha p (FP64),
 // r -= alpha Ap, z = M^-1 r, partials r . z and r . r.  Restates
 // alpha_jacobi_dot_partial_kernel (pcg_solver.cu:406-434) with an FP64 iterate.
+// Every other pass defers x += alpha p; the next pass adds both terms, with
+// the older direction still in the other p buffer.
 // ---------------------------------------------------------------------------
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void pcg_update_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
@@ -777,16 +792,26 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_update_kernel(LaunchGeom g, De
     const double sig = S.sigma, rho = S.rho;
     const double alpha = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;
     const float af = float(alpha);
-    const float* __restrict__ p = S.parity ? v.p0 : v.p1;  // written by the SpMV pass
+    const float* __restrict__ p = S.parity ? v.p0 : v.p1;      // written by the SpMV pass
+    const float* __restrict__ pprev = S.parity ? v.p1 : v.p0;  // the previous direction
+    const bool pend = S.pend != 0, defer = !pend && S.defer != 0;
+    const double ap = S.alpha_pend;
     const int lane = threadIdx.x & 63;
     double part[2] = {0.0, 0.0};
     if (lane < sd.nvalid) {
         const int row = s * 64 + lane;
-        const F3 pi = ld3(p, row);
         double* x = v.x + 3 * row;
-        x[0] += alpha * double(pi.x);
-        x[1] += alpha * double(pi.y);
-        x[2] += alpha * double(pi.z);
+        if (pend) {  // the owed term and this one
+            const F3 pi = ld3(p, row), pp = ld3(pprev, row);
+            x[0] += ap * double(pp.x) + alpha * double(pi.x);
+            x[1] += ap * double(pp.y) + alpha * double(pi.y);
+            x[2] += ap * double(pp.z) + alpha * double(pi.z);
+        } else if (!defer) {
+            const F3 pi = ld3(p, row);
+            x[0] += alpha * double(pi.x);
+            x[1] += alpha * double(pi.y);
+            x[2] += alpha * double(pi.z);
+        }
         const F3 ri = ld3(v.r, row) - ld3(v.ap, row) * af;
         const F3 zi = minv_mul(A, s, lane, ri);
         st3(v.r, row, ri);
@@ -903,12 +928,26 @@ __global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* _
             S.skip = 0;
             S.beta = 0.0;
             S.rho = tot[2];
+            S.pend = 0;
+            S.alpha_pend = 0.0;
+            S.defer = 1;
             S.status = (tol2 > 0.0 && tot[0] <= tol2 * S.bb) ? kConverged
                        : (pr.max_iter <= 0 ? kStopped : kRunning);
             break;
         }
         case kPhaseSigma: {
-            if (S.mode == kModeCheck) {
+            if (S.mode == kModeFlush) {  // x is whole again; no update pass this round
+                S.pend = 0;
+                S.alpha_pend = 0.0;
+                S.defer = 1;
+                S.skip = 1;
+                if (S.flush_to == kModeCheck) {
+                    S.mode = kModeCheck;
+                } else {
+                    S.mode = kModeIter;
+                    S.status = kStopped;
+                }
+            } else if (S.mode == kModeCheck) {
                 S.rr = tot[0];
                 S.relres = sqrt(tot[0] / S.bb);
                 if (tot[0] <= tol2 * S.bb) {
@@ -919,6 +958,7 @@ __global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* _
                     S.beta = 0.0;
                     S.mode = kModeIter;
                     S.skip = 1;
+                    S.defer = 1;
                     if (S.iters >= pr.max_iter) S.status = kStopped;
                 }
             } else {
@@ -928,13 +968,29 @@ __global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* _
         }
         case kPhaseRho: {
             const double rho = S.rho, rho_new = tot[0];
+            if (S.pend) {  // this update pass added the owed term
+                S.pend = 0;
+            } else if (S.defer) {  // this update pass left x += alpha p to the next
+                const double sig = S.sigma;
+                S.alpha_pend = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;  // the pass's alpha
+                S.pend = 1;
+            }
+            S.defer = S.pend ? 0 : 1;
             S.beta = (rho > kGuard || rho < -kGuard) ? rho_new / rho : 0.0;
             S.rho = rho_new;
             S.rr = tot[1];
             S.parity ^= 1;
             S.iters += 1;
-            if (tol2 > 0.0 && tot[1] <= tol2 * S.bb) S.mode = kModeCheck;
-            else if (S.iters >= pr.max_iter) S.status = kStopped;
+            const bool check = tol2 > 0.0 && tot[1] <= tol2 * S.bb;
+            const bool stop = !check && S.iters >= pr.max_iter;
+            if ((check || stop) && S.pend) {  // complete x first
+                S.mode = kModeFlush;
+                S.flush_to = check ? int(kModeCheck) : int(kStopped);
+            } else if (check) {
+                S.mode = kModeCheck;
+            } else if (stop) {
+                S.status = kStopped;
+            }
             break;
         }
         case kPhaseFinal: {

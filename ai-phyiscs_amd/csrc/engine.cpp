@@ -560,6 +560,9 @@ __global__ void prime_states_kernel(InstState* st, int n) {
         st[i].sigma = 1.0;
         st[i].rho = 1.0;
         st[i].beta = 0.5;
+        st[i].pend = 0;
+        st[i].defer = 0;
+        st[i].alpha_pend = 0.0;
     }
 }
 }  // namespace
