@@ -551,10 +551,9 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
                 switch (which) {
                     case 0: launch_spmv(g, A, v.p0, v.ap, c.spmv_waves, s); break;
                     case 1:
-                        launch_pcg_spmv(g, A, v, P.states(), c.spmv_waves, c.spmv_unroll, s, c.spmv_branch_free,
-                                        c.spmv_symmetric);
+                        launch_pcg_spmv(g, A, v, P.states(), c.spmv_unroll, s, c.spmv_branch_free, c.spmv_symmetric);
                         break;
-                    case 2: launch_pcg_update(g, A, v, P.states(), c.update_waves, s); break;
+                    case 2: launch_pcg_update(g, A, v, P.states(), s); break;
                     case 3: launch_assemble(g, P.mesh(), A, v, ext, P.params(), dt, c.assemble_waves, P.max_width(), s); break;
                     case 4: P.launch_pcg_iteration(s); break;
                 }

@@ -9,7 +9,9 @@ namespace chysx_amd {
 
 // A slice is 64 consecutive rows of one instance (rows 64 s .. 64 s + 63 of
 // the padded row space), processed by one 64-lane wavefront, one lane per
-// block row.  Instances are padded to whole slices.
+// block row.  Instances are padded to whole groups of kGroupSlices slices
+// (one PCG workgroup each); slices past an instance's rows have nvalid 0.
+constexpr int kGroupSlices = 16;
 struct SliceDesc {
     int inst;    // instance id
     int part0;   // particle index (external buffers) of the slice's first row
@@ -34,10 +36,18 @@ enum PcgMode : int {
 // Reduction phases of pcg_reduce_kernel.
 enum PcgPhase : int {
     kPhaseStart = 0,  // after the initial residual: |r|^2, |b|^2, r . z
-    kPhaseSigma = 1,  // after the SpMV pass: p . A p (or the check's |r|^2, r . z)
-    kPhaseRho = 2,    // after the update pass: r . z, |r|^2
+    kPhaseIter = 1,   // after an iteration's update pass: p . A p, r . z, |r|^2
+                      // (a check round: the true |r|^2, r . z; a flush round: none)
     kPhaseFinal = 3,  // after the report residual of a stopped solve: |r|^2
 };
+
+// Inner-product partials: one FP64 value per 16-slice group and slot,
+// partials[slot * ngroups + group].  Slots 0-2: |r|^2, |b|^2, r . z of a
+// residual pass (start, check, report) or r . z, |r|^2 of an update pass;
+// slot kSlotSigma: p . A p of an SpMV pass, apart because the update pass
+// reads it while it writes slots 0-1.
+constexpr int kSlotSigma = 4;
+constexpr int kPartialSlots = 5;
 
 // Per-instance solver state, FP64 scalars (inner products, alpha, beta and
 // the convergence test).  Lives in device memory; written only by
@@ -57,7 +67,9 @@ struct alignas(16) InstState {
     int restarts;
     int parity;      // which p buffer holds the current direction
     int slice0;      // first slice of this instance (set at build)
-    int nslices;     // slices of this instance (set at build)
+    int nslices;     // slices holding this instance's rows (set at build)
+    int group0;      // first 16-slice group of this instance (set at build)
+    int ngroups;     // its groups (set at build)
     // Deferred iterate update: an update pass with `defer` set leaves
     // x += alpha p to the next update pass, which adds both terms while both
     // directions are still in the p buffers (x is read and written every
@@ -67,7 +79,7 @@ struct alignas(16) InstState {
     int pend;
     int defer;
     int flush_to;
-    int pad;
+    int pad[3];
 };
 
 // Per-instance step parameters and solver options.

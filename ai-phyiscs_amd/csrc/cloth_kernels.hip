@@ -10,10 +10,13 @@
 // a contiguous eighth of the slices, keeping neighbour gathers (rows +-1,
 // +-W) inside one XCD's L2.
 //
-// Inner products: each wave writes its FP64 partial (wave butterfly sum) to
-// its slice's slot; pcg_reduce_kernel sums an instance's slices in slice
-// order.  The reduction order therefore depends neither on the launch shape
-// nor on scheduling, and every result is bitwise reproducible.
+// Inner products: the PCG passes run one workgroup of 16 wavefronts per
+// 16-slice group; each wave sums its lanes (butterfly), the group adds its
+// 16 waves in order and writes one FP64 partial per slot.  The consumers
+// (the update pass for p . Ap, pcg_reduce_kernel for the rest) add an
+// instance's group partials in one fixed order.  The reduction order
+// therefore depends neither on the launch shape nor on scheduling, and every
+// result is bitwise reproducible.
 
 #include <hip/hip_runtime.h>
 
@@ -40,7 +43,9 @@ __device__ __forceinline__ int my_slice(int nslices) {
     const int per = (nwg + 7) >> 3;
     const int b = blockIdx.x;
     const int wg = (b & 7) * per + (b >> 3);
-    return wg * WAVES + int(threadIdx.x >> 6);
+    // wave-uniform: in a scalar register, so the slice descriptor and the
+    // solver state are fetched with scalar loads and branched on uniformly
+    return __builtin_amdgcn_readfirstlane(wg * WAVES + int(threadIdx.x >> 6));
 }
 inline int grid_for(int nslices, int waves) {
     const int nwg = (nslices + waves - 1) / waves;
@@ -67,13 +72,13 @@ struct S6 {
     float m[6];
 };
 
-// Sum over the 64 lanes of a wave (fixed butterfly order); lane 0 writes the
-// slice's partial slots, slot j of every slice contiguous
-// (partials[j * nslices + slice]) so a reduction reads only the slots it
-// needs, coalesced.  Every lane of the wave must call it.
+// Workgroup of kGroupSlices wavefronts (one 16-slice group): wave sums, then
+// the 16 waves in order; thread j < N writes slot slot[j] of `group`.  Every
+// thread of the workgroup must call it.
 template <int N>
-__device__ __forceinline__ void wave_partials(double (&v)[N], double* __restrict__ partials, int slice,
-                                              int nslices) {
+__device__ __forceinline__ void group_partials(double (&v)[N], const int (&slot)[N], double* __restrict__ partials,
+                                               int group, int ngroups) {
+    __shared__ double sh[kGroupSlices][N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         double a = v[j];
@@ -83,7 +88,13 @@ __device__ __forceinline__ void wave_partials(double (&v)[N], double* __restrict
     }
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) partials[(size_t)j * nslices + slice] = v[j];
+        for (int j = 0; j < N; ++j) sh[threadIdx.x >> 6][j] = v[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        double t = 0.0;
+        for (int w = 0; w < kGroupSlices; ++w) t += sh[w][threadIdx.x];
+        partials[(size_t)slot[threadIdx.x] * ngroups + group] = t;
     }
 }
 
@@ -589,11 +600,70 @@ __device__ __forceinline__ F3 minv_mul(const DevMatrix& A, int s, int lane, F3 r
     return f3(m0 * r.x + m1 * r.y + m2 * r.z, m1 * r.x + m3 * r.y + m4 * r.z, m2 * r.x + m4 * r.y + m5 * r.z);
 }
 
+constexpr int kReduceThreads = 64 * kGroupSlices;
+
+// Sums slots slot[0..NS) of an instance's group partials (groups g0 ..
+// g0 + ng - 1, `ld` groups per slot).  Thread t adds groups t, t + T, t + 2T,
+// ... in order; the waves' sums are then combined in wave order, so the
+// result is one fixed sum whatever KU is, in every kernel that calls it with
+// kReduceThreads threads.  All NS x KU loads of a round are issued before any
+// of them is added.  The total of slot slot[j] lands in tot[slot[j]] on
+// thread 0 only.
+template <int NS, int KU>
+__device__ __forceinline__ void block_sum(const double* __restrict__ partials, int ld, int s0, int ns,
+                                          const int (&slot)[NS], double (&tot)[kPartialSlots]) {
+    double acc[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) acc[j] = 0.0;
+    auto round = [&](int q0) {
+        double t[NS][KU];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const double* pj = partials + (size_t)slot[j] * ld + s0;
+#pragma unroll
+            for (int u = 0; u < KU; ++u) {
+                // unconditional, clamped load (no branch): out-of-range
+                // entries re-read the last partial and are not added
+                const int q = q0 + u * kReduceThreads;
+                t[j][u] = pj[q < ns ? q : ns - 1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+#pragma unroll
+            for (int u = 0; u < KU; ++u)
+                if (q0 + u * kReduceThreads < ns) acc[j] += t[j][u];
+    };
+    // The first round is straight-line code: a loop header here would make
+    // the compiler wait for every load the caller has in flight.
+    if (ns > 0) {
+        round(int(threadIdx.x));
+        for (int q0 = int(threadIdx.x) + KU * kReduceThreads; q0 < ns; q0 += KU * kReduceThreads) round(q0);
+    }
+    __shared__ double sh[kReduceThreads / 64][NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) sh[threadIdx.x >> 6][j] = acc[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kReduceThreads / 64; ++w)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) tot[slot[j]] += sh[w][j];
+    }
+    __syncthreads();  // sh may be refilled by a second call
+}
+
+// ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 // True residual of one row: b - A x with FP64 products and sums (x is FP64).
 // Returns it and, unless `report_only`, stores r, z = M^-1 r and p = z.
 // Partials: [0] |r|^2, [1] |b|^2, [2] r . z.
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVectors& v, float* __restrict__ pcur,
                                              int row, bool report_only, double (&part)[4]) {
     const int s = row >> 6, lane = row & 63;
@@ -635,19 +705,28 @@ __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVector
 
 // Residual pass.  report == 0: start a solve (every instance); report == 1:
 // record the true residual of stopped instances only.
-template <int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void pcg_residual_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                                 const InstState* __restrict__ st, int report) {
-    const int s = my_slice<WAVES>(g.nslices);
+// Returns and early exits below are taken by whole workgroups: a group's 16
+// slices belong to one instance, and nslices is a multiple of 16.
+__global__ __launch_bounds__(64 * kGroupSlices) void pcg_residual_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+                                                                        const InstState* __restrict__ st,
+                                                                        int report) {
+    const int s = my_slice<kGroupSlices>(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const InstState& S = st[sd.inst];
     if (report && S.status != kStopped) return;
     float* pcur = S.parity ? v.p1 : v.p0;
     const int lane = threadIdx.x & 63;
-    double part[4] = {0.0, 0.0, 0.0, 0.0};
-    if (lane < sd.nvalid) residual_row(A, v, pcur, s * 64 + lane, report != 0, part);
-    wave_partials<4>(part, v.partials, s, g.nslices);
+    double part[3] = {0.0, 0.0, 0.0};
+    if (lane < sd.nvalid) {
+        double q[4] = {0.0, 0.0, 0.0, 0.0};
+        residual_row(A, v, pcur, s * 64 + lane, report != 0, q);
+        part[0] = q[0];
+        part[1] = q[1];
+        part[2] = q[2];
+    }
+    const int sl[3] = {0, 1, 2};
+    group_partials<3>(part, sl, v.partials, s / kGroupSlices, g.ngroups);
 }
 
 // ---------------------------------------------------------------------------
@@ -671,10 +750,10 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_residual_kernel(LaunchGeom g, 
 // the XCD has streamed several L2 capacities.)
 __device__ __forceinline__ int sym_order(int k, int w) { return (k & 1) ? w - 1 - (k >> 1) : (k >> 1); }
 
-template <int WAVES, int UNROLL, bool BF, bool SYM, bool PK>
-__global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                             const InstState* __restrict__ st) {
-    const int s = my_slice<WAVES>(g.nslices);
+template <int UNROLL, bool BF, bool SYM, bool PK>
+__global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+                                                                    const InstState* __restrict__ st) {
+    const int s = my_slice<kGroupSlices>(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const InstState& S = st[sd.inst];
@@ -684,9 +763,16 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
     const bool valid = lane < sd.nvalid;
     const int row = s * 64 + lane;
     if (S.mode == kModeCheck) {
-        double part[4] = {0.0, 0.0, 0.0, 0.0};
-        if (valid) residual_row(A, v, parity ? v.p1 : v.p0, row, false, part);
-        wave_partials<4>(part, v.partials, s, g.nslices);
+        double part[3] = {0.0, 0.0, 0.0};
+        if (valid) {
+            double q[4] = {0.0, 0.0, 0.0, 0.0};
+            residual_row(A, v, parity ? v.p1 : v.p0, row, false, q);
+            part[0] = q[0];
+            part[1] = q[1];
+            part[2] = q[2];
+        }
+        const int sl[3] = {0, 1, 2};
+        group_partials<3>(part, sl, v.partials, s / kGroupSlices, g.ngroups);
         return;
     }
     if (S.mode == kModeFlush) {
@@ -771,7 +857,8 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
         st3(pnew, row, pi);
         part[0] = double(pi.x) * a0 + double(pi.y) * a1 + double(pi.z) * a2;
     }
-    wave_partials<1>(part, v.partials, s, g.nslices);
+    const int sl[1] = {kSlotSigma};
+    group_partials<1>(part, sl, v.partials, s / kGroupSlices, g.ngroups);
 }
 
 // ---------------------------------------------------------------------------
@@ -781,139 +868,124 @@ __global__ __launch_bounds__(64 * WAVES) void pcg_spmv_kernel(LaunchGeom g, DevM
 // Every other pass defers x += alpha p; the next pass adds both terms, with
 // the older direction still in the other p buffer.
 // ---------------------------------------------------------------------------
-template <int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void pcg_update_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                               const InstState* __restrict__ st) {
-    const int s = my_slice<WAVES>(g.nslices);
-    if (s >= g.nslices) return;
-    const SliceDesc sd = g.slices[s];
-    const InstState& S = st[sd.inst];
-    if (S.status != kRunning || S.skip) return;
-    const double sig = S.sigma, rho = S.rho;
-    const double alpha = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;
-    const float af = float(alpha);
+// X: 0 defers x += alpha p; 1 adds the owed term and this one; 2 adds this
+// one.  Straight-line per variant, so every operand load (including x) is
+// issued before the p . Ap sum and waited for once.
+template <int X>
+__device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v,
+                                           const InstState& S, int s, const SliceDesc& sd) {
     const float* __restrict__ p = S.parity ? v.p0 : v.p1;      // written by the SpMV pass
     const float* __restrict__ pprev = S.parity ? v.p1 : v.p0;  // the previous direction
-    const bool pend = S.pend != 0, defer = !pend && S.defer != 0;
-    const double ap = S.alpha_pend;
     const int lane = threadIdx.x & 63;
+    const int row = s * 64 + lane;
+    // Every lane loads (rows past an instance's end are allocated padding);
+    // only valid lanes store.
+    const F3 r0 = ld3(v.r, row), api = ld3(v.ap, row);
+    float m[6];
+    {
+        const float* mv = A.minv + (size_t)s * 6 * 64 + lane;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) m[c] = mv[c * 64];
+    }
+    F3 pi = f3(0.0f, 0.0f, 0.0f), pp = pi;
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    double* x = v.x + 3 * row;
+    if (X != 0) {
+        pi = ld3(p, row);
+        x0 = x[0];
+        x1 = x[1];
+        x2 = x[2];
+    }
+    if (X == 1) pp = ld3(pprev, row);
+    // sigma = p . Ap of this instance, summed exactly as pcg_reduce_kernel sums it
+    __shared__ double sig_sh;
+    {
+        double tot[kPartialSlots] = {};
+        const int sl[1] = {kSlotSigma};
+        block_sum<1, 4>(v.partials, g.ngroups, S.group0, S.ngroups, sl, tot);
+        if (threadIdx.x == 0) sig_sh = tot[kSlotSigma];
+        __syncthreads();
+    }
+    const double sig = sig_sh, rho = S.rho;
+    const double alpha = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;
+    const float af = float(alpha);
     double part[2] = {0.0, 0.0};
     if (lane < sd.nvalid) {
-        const int row = s * 64 + lane;
-        double* x = v.x + 3 * row;
-        if (pend) {  // the owed term and this one
-            const F3 pi = ld3(p, row), pp = ld3(pprev, row);
-            x[0] += ap * double(pp.x) + alpha * double(pi.x);
-            x[1] += ap * double(pp.y) + alpha * double(pi.y);
-            x[2] += ap * double(pp.z) + alpha * double(pi.z);
-        } else if (!defer) {
-            const F3 pi = ld3(p, row);
-            x[0] += alpha * double(pi.x);
-            x[1] += alpha * double(pi.y);
-            x[2] += alpha * double(pi.z);
+        if (X == 1) {  // the owed term and this one
+            const double ap = S.alpha_pend;
+            x[0] = x0 + (ap * double(pp.x) + alpha * double(pi.x));
+            x[1] = x1 + (ap * double(pp.y) + alpha * double(pi.y));
+            x[2] = x2 + (ap * double(pp.z) + alpha * double(pi.z));
+        } else if (X == 2) {
+            x[0] = x0 + alpha * double(pi.x);
+            x[1] = x1 + alpha * double(pi.y);
+            x[2] = x2 + alpha * double(pi.z);
         }
-        const F3 ri = ld3(v.r, row) - ld3(v.ap, row) * af;
-        const F3 zi = minv_mul(A, s, lane, ri);
+        const F3 ri = r0 - api * af;
+        const F3 zi = f3(m[0] * ri.x + m[1] * ri.y + m[2] * ri.z, m[1] * ri.x + m[3] * ri.y + m[4] * ri.z,
+                         m[2] * ri.x + m[4] * ri.y + m[5] * ri.z);
         st3(v.r, row, ri);
         st3(v.z, row, zi);
         part[0] = double(ri.x) * zi.x + double(ri.y) * zi.y + double(ri.z) * zi.z;
         part[1] = double(ri.x) * ri.x + double(ri.y) * ri.y + double(ri.z) * ri.z;
     }
-    wave_partials<2>(part, v.partials, s, g.nslices);
+    const int sl[2] = {0, 1};
+    group_partials<2>(part, sl, v.partials, s / kGroupSlices, g.ngroups);
+}
+
+__global__ __launch_bounds__(64 * kGroupSlices) void pcg_update_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+                                                                      const InstState* __restrict__ st) {
+    const int s = my_slice<kGroupSlices>(g.nslices);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const InstState& S = st[sd.inst];
+    if (S.status != kRunning || S.mode != kModeIter) return;  // check and flush rounds update nothing
+    if (S.pend) update_rows<1>(g, A, v, S, s, sd);
+    else if (S.defer) update_rows<0>(g, A, v, S, s, sd);
+    else update_rows<2>(g, A, v, S, s, sd);
 }
 
 // ---------------------------------------------------------------------------
 // Per-instance reduction: one workgroup per instance sums that instance's
-// slice partials in slice order (deterministic) and advances its solver state.
+// group partials in a fixed order (deterministic) and advances its solver
+// state.  One reduction per iteration: the update pass computes alpha from
+// the SpMV pass's partials itself.
 // Scalar logic restates reduce_partial_kernel / reduce_rho_beta_kernel
 // (pcg_solver.cu:374-384, 436-454) with the 1e-37 guards, plus the stopping
 // rule: true relative residual sqrt(|b-Ax|^2 / max(|b|^2,1e-30)) <= tol
 // (pcg_solver.cu:595-641, 916-923), or exactly max_iter iterations when
 // tol == 0 (pcg_solver.cu:542-592).
 // ---------------------------------------------------------------------------
-constexpr int kReduceThreads = 1024;
-
-// Sums slots slot[0..NS) of an instance's slice partials.  Thread t adds
-// slices t, t + T, t + 2T, ... in order; the waves' sums are then combined
-// in wave order, so the result is one fixed sum whatever KU is.  All NS x KU
-// loads of a round are issued before any of them is added.  The total of
-// slot slot[j] lands in tot[slot[j]] on thread 0 only.
-template <int NS, int KU>
-__device__ __forceinline__ void block_sum(const double* __restrict__ partials, int nslices, int s0, int ns,
-                                          const int (&slot)[NS], double (&tot)[4]) {
-    double acc[NS];
-#pragma unroll
-    for (int j = 0; j < NS; ++j) acc[j] = 0.0;
-    for (int q0 = threadIdx.x; q0 < ns; q0 += KU * kReduceThreads) {
-        double t[NS][KU];
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const double* pj = partials + (size_t)slot[j] * nslices + s0;
-#pragma unroll
-            for (int u = 0; u < KU; ++u) {
-                const int q = q0 + u * kReduceThreads;
-                t[j][u] = (q < ns) ? pj[q] : 0.0;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < NS; ++j)
-#pragma unroll
-            for (int u = 0; u < KU; ++u)
-                if (q0 + u * kReduceThreads < ns) acc[j] += t[j][u];
-    }
-    __shared__ double sh[kReduceThreads / 64][NS];
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int j = 0; j < NS; ++j) sh[threadIdx.x >> 6][j] = acc[j];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 0; w < kReduceThreads / 64; ++w)
-#pragma unroll
-            for (int j = 0; j < NS; ++j) tot[slot[j]] += sh[w][j];
-    }
-    __syncthreads();  // sh may be refilled by a second call
-}
-
-// single_ns > 0: one instance owning slices [0, single_ns) (the partial
+// single_ng > 0: one instance owning groups [0, single_ng) (the partial
 // loads then need no state read first); 0: bounds from the instance state.
 // The phase's partials are loaded before the state is inspected, so the two
 // loads overlap; an instance that is not live discards them.
 __global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* __restrict__ st,
                                                                     const InstParams* __restrict__ params,
                                                                     const double* __restrict__ partials,
-                                                                    int nslices, int phase, int single_ns) {
+                                                                    int ngroups, int phase, int single_ng) {
     InstState& S = st[blockIdx.x];
     const InstParams& pr = params[blockIdx.x];
-    const int s0 = single_ns > 0 ? 0 : S.slice0;
-    const int ns = single_ns > 0 ? single_ns : S.nslices;
-    // Slots: start r.r, b.b, r.z; sigma p.Ap (check mode: r.r and r.z of the
-    // true residual); rho r.z, r.r; final r.r.
-    double tot[4] = {0.0, 0.0, 0.0, 0.0};
+    const int g0 = single_ng > 0 ? 0 : S.group0;
+    const int ng = single_ng > 0 ? single_ng : S.ngroups;
+    // Slots: start |r|^2, |b|^2, r . z; iteration p . Ap, r . z, |r|^2 (a
+    // check round: the true |r|^2 and r . z); final |r|^2.
+    double tot[kPartialSlots] = {};
     if (phase == kPhaseStart) {
         const int sl[3] = {0, 1, 2};
-        block_sum<3, 8>(partials, nslices, s0, ns, sl, tot);
-    } else if (phase == kPhaseRho) {
-        const int sl[2] = {0, 1};
-        block_sum<2, 16>(partials, nslices, s0, ns, sl, tot);
+        block_sum<3, 4>(partials, ngroups, g0, ng, sl, tot);
+    } else if (phase == kPhaseIter) {
+        const int sl[3] = {kSlotSigma, 0, 1};
+        block_sum<3, 4>(partials, ngroups, g0, ng, sl, tot);
     } else {
         const int sl[1] = {0};
-        block_sum<1, 16>(partials, nslices, s0, ns, sl, tot);
+        block_sum<1, 4>(partials, ngroups, g0, ng, sl, tot);
     }
-    if ((phase == kPhaseSigma || phase == kPhaseRho) && S.status != kRunning) return;
+    if (phase == kPhaseIter && S.status != kRunning) return;
     if (phase == kPhaseFinal && S.status != kStopped) return;
-    if (phase == kPhaseRho && S.skip) {
-        if (threadIdx.x == 0) S.skip = 0;
-        return;
-    }
-    if (phase == kPhaseSigma && S.mode == kModeCheck) {
+    if (phase == kPhaseIter && S.mode == kModeCheck) {
         const int sl[1] = {2};
-        block_sum<1, 16>(partials, nslices, s0, ns, sl, tot);
+        block_sum<1, 4>(partials, ngroups, g0, ng, sl, tot);
     }
     if (threadIdx.x != 0) return;
     const double tol2 = pr.tol2;
@@ -935,12 +1007,11 @@ __global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* _
                        : (pr.max_iter <= 0 ? kStopped : kRunning);
             break;
         }
-        case kPhaseSigma: {
-            if (S.mode == kModeFlush) {  // x is whole again; no update pass this round
+        case kPhaseIter: {
+            if (S.mode == kModeFlush) {  // x is whole again
                 S.pend = 0;
                 S.alpha_pend = 0.0;
                 S.defer = 1;
-                S.skip = 1;
                 if (S.flush_to == kModeCheck) {
                     S.mode = kModeCheck;
                 } else {
@@ -957,39 +1028,35 @@ __global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* _
                     S.rho = tot[2];
                     S.beta = 0.0;
                     S.mode = kModeIter;
-                    S.skip = 1;
                     S.defer = 1;
                     if (S.iters >= pr.max_iter) S.status = kStopped;
                 }
             } else {
-                S.sigma = tot[0];
-            }
-            break;
-        }
-        case kPhaseRho: {
-            const double rho = S.rho, rho_new = tot[0];
-            if (S.pend) {  // this update pass added the owed term
-                S.pend = 0;
-            } else if (S.defer) {  // this update pass left x += alpha p to the next
-                const double sig = S.sigma;
-                S.alpha_pend = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;  // the pass's alpha
-                S.pend = 1;
-            }
-            S.defer = S.pend ? 0 : 1;
-            S.beta = (rho > kGuard || rho < -kGuard) ? rho_new / rho : 0.0;
-            S.rho = rho_new;
-            S.rr = tot[1];
-            S.parity ^= 1;
-            S.iters += 1;
-            const bool check = tol2 > 0.0 && tot[1] <= tol2 * S.bb;
-            const bool stop = !check && S.iters >= pr.max_iter;
-            if ((check || stop) && S.pend) {  // complete x first
-                S.mode = kModeFlush;
-                S.flush_to = check ? int(kModeCheck) : int(kStopped);
-            } else if (check) {
-                S.mode = kModeCheck;
-            } else if (stop) {
-                S.status = kStopped;
+                // sigma as the update pass summed it; alpha as it computed it
+                const double sig = tot[kSlotSigma], rho = S.rho, rho_new = tot[0];
+                S.sigma = sig;
+                if (S.pend) {  // this update pass added the owed term
+                    S.pend = 0;
+                } else if (S.defer) {  // this update pass left x += alpha p to the next
+                    S.alpha_pend = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;
+                    S.pend = 1;
+                }
+                S.defer = S.pend ? 0 : 1;
+                S.beta = (rho > kGuard || rho < -kGuard) ? rho_new / rho : 0.0;
+                S.rho = rho_new;
+                S.rr = tot[1];
+                S.parity ^= 1;
+                S.iters += 1;
+                const bool check = tol2 > 0.0 && tot[1] <= tol2 * S.bb;
+                const bool stop = !check && S.iters >= pr.max_iter;
+                if ((check || stop) && S.pend) {  // complete x first
+                    S.mode = kModeFlush;
+                    S.flush_to = check ? int(kModeCheck) : int(kStopped);
+                } else if (check) {
+                    S.mode = kModeCheck;
+                } else if (stop) {
+                    S.status = kStopped;
+                }
             }
             break;
         }
@@ -1120,64 +1187,56 @@ void launch_rest_angles(const int4* dih, const float* pos, float* rest, int n, h
     CHYSX_LAUNCH_CHECK();
 }
 void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
-                         int report, int waves, hipStream_t s) {
+                         int report, hipStream_t s) {
     if (g.nslices <= 0) return;
-#define L(W) pcg_residual_kernel<W><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st, report)
-    CHYSX_WAVES_SWITCH(waves, L)
-#undef L
+    if (g.nslices % kGroupSlices != 0 || g.ngroups * kGroupSlices != g.nslices)
+        throw std::invalid_argument("PCG passes: slices must form whole 16-slice groups");
+    pcg_residual_kernel<<<grid_for(g.nslices, kGroupSlices), 64 * kGroupSlices, 0, s>>>(g, A, v, st, report);
     CHYSX_LAUNCH_CHECK();
 }
 namespace {
 template <bool BF, bool SYM, bool PK>
-void pcg_spmv_dispatch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
-                       int unroll, hipStream_t s) {
-#define L1(W) pcg_spmv_kernel<W, 1, BF, SYM, PK><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-#define L2(W) pcg_spmv_kernel<W, 2, BF, SYM, PK><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-#define L4(W) pcg_spmv_kernel<W, 4, BF, SYM, PK><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-    if (unroll == 1) {
-        CHYSX_WAVES_SWITCH(waves, L1)
-    } else if (unroll == 2) {
-        CHYSX_WAVES_SWITCH(waves, L2)
-    } else {
-        CHYSX_WAVES_SWITCH(waves, L4)
-    }
-#undef L1
-#undef L2
-#undef L4
+void pcg_spmv_dispatch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int unroll,
+                       hipStream_t s) {
+    const int grid = grid_for(g.nslices, kGroupSlices), block = 64 * kGroupSlices;
+    if (unroll == 1) pcg_spmv_kernel<1, BF, SYM, PK><<<grid, block, 0, s>>>(g, A, v, st);
+    else if (unroll == 2) pcg_spmv_kernel<2, BF, SYM, PK><<<grid, block, 0, s>>>(g, A, v, st);
+    else pcg_spmv_kernel<4, BF, SYM, PK><<<grid, block, 0, s>>>(g, A, v, st);
 }
 }  // namespace
 
-void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
-                     int unroll, hipStream_t s, int branch_free, int symmetric) {
+void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int unroll,
+                     hipStream_t s, int branch_free, int symmetric) {
     if (g.nslices <= 0) return;
     if (!valid_unroll(unroll)) throw std::invalid_argument("unsupported unroll");
+    if (g.nslices % kGroupSlices != 0 || g.ngroups * kGroupSlices != g.nslices)
+        throw std::invalid_argument("PCG passes: slices must form whole 16-slice groups");
     if (symmetric != A.sym) throw std::invalid_argument("symmetric SpMV: launch flag and matrix mode differ");
     if (A.bs != 9 && !(A.bs == 6 && A.sym)) throw std::invalid_argument("packed blocks need the symmetric SpMV");
-    if (symmetric && A.mir == nullptr) throw std::invalid_argument("symmetric SpMV needs the mirror offsets");
     if (branch_free) {
-        if (A.bs == 6) pcg_spmv_dispatch<true, true, true>(g, A, v, st, waves, unroll, s);
-        else if (symmetric) pcg_spmv_dispatch<true, true, false>(g, A, v, st, waves, unroll, s);
-        else pcg_spmv_dispatch<true, false, false>(g, A, v, st, waves, unroll, s);
+        if (A.bs == 6) pcg_spmv_dispatch<true, true, true>(g, A, v, st, unroll, s);
+        else if (symmetric) pcg_spmv_dispatch<true, true, false>(g, A, v, st, unroll, s);
+        else pcg_spmv_dispatch<true, false, false>(g, A, v, st, unroll, s);
     } else {
-        if (A.bs == 6) pcg_spmv_dispatch<false, true, true>(g, A, v, st, waves, unroll, s);
-        else if (symmetric) pcg_spmv_dispatch<false, true, false>(g, A, v, st, waves, unroll, s);
-        else pcg_spmv_dispatch<false, false, false>(g, A, v, st, waves, unroll, s);
+        if (A.bs == 6) pcg_spmv_dispatch<false, true, true>(g, A, v, st, unroll, s);
+        else if (symmetric) pcg_spmv_dispatch<false, true, false>(g, A, v, st, unroll, s);
+        else pcg_spmv_dispatch<false, false, false>(g, A, v, st, unroll, s);
     }
     CHYSX_LAUNCH_CHECK();
 }
-void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
+void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
                        hipStream_t s) {
     if (g.nslices <= 0) return;
-#define L(W) pcg_update_kernel<W><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, v, st)
-    CHYSX_WAVES_SWITCH(waves, L)
-#undef L
+    if (g.nslices % kGroupSlices != 0 || g.ngroups * kGroupSlices != g.nslices)
+        throw std::invalid_argument("PCG passes: slices must form whole 16-slice groups");
+    pcg_update_kernel<<<grid_for(g.nslices, kGroupSlices), 64 * kGroupSlices, 0, s>>>(g, A, v, st);
     CHYSX_LAUNCH_CHECK();
 }
-void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int nslices,
+void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
                        int phase, hipStream_t s) {
     if (ninst <= 0) return;
-    pcg_reduce_kernel<<<ninst, kReduceThreads, 0, s>>>(st, params, partials, nslices, phase,
-                                                       ninst == 1 ? nslices : 0);
+    pcg_reduce_kernel<<<ninst, kReduceThreads, 0, s>>>(st, params, partials, ngroups, phase,
+                                                       ninst == 1 ? ngroups : 0);
     CHYSX_LAUNCH_CHECK();
 }
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,

@@ -91,7 +91,8 @@ struct DevExt {
 
 struct LaunchGeom {
     const SliceDesc* slices;
-    int nslices;
+    int nslices;  // a multiple of kGroupSlices
+    int ngroups;  // nslices / kGroupSlices
 };
 
 // Workgroup shapes: `waves` in {1, 2, 4, 8, 16} (64 x waves lanes), SpMV
@@ -107,18 +108,21 @@ void launch_assemble(const LaunchGeom& g, const DevMesh& m, const DevMatrix& A, 
                      const DevExt& e, const InstParams* params, float dt, int waves, int maxw, hipStream_t s);
 // Rest dihedral angles (bending_rest_angle_kernel).
 void launch_rest_angles(const int4* dih, const float* pos, float* rest, int n, hipStream_t s);
+// The PCG passes run one workgroup of kGroupSlices wavefronts per 16-slice
+// group and write one partial per group and slot.
 // Residual pass: report == 0 starts a solve (true residual, r, z, p = z);
 // report == 1 only measures the true residual of stopped instances.
 void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
-                         int report, int waves, hipStream_t s);
-// SpMV pass (direction update, A p, p . A p; or the true-residual check).
-void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
-                     int unroll, hipStream_t s, int branch_free = 0, int symmetric = 0);
-// Update pass (x, r, z; r . z, r . r).
-void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int waves,
+                         int report, hipStream_t s);
+// SpMV pass (direction update, A p, p . A p; the true-residual check; or the
+// flush of a deferred iterate term).
+void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int unroll,
+                     hipStream_t s, int branch_free = 0, int symmetric = 0);
+// Update pass (alpha from the SpMV pass's partials; x, r, z; r . z, r . r).
+void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
                        hipStream_t s);
 // One workgroup per instance: finish the reductions of `phase` (PcgPhase).
-void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int nslices,
+void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
                        int phase, hipStream_t s);
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
                      int waves, hipStream_t s);

@@ -206,7 +206,7 @@ void Problem::build(const std::vector<const Instance*>& insts) {
         dih_slots_[size_t(k)] = build_dihedral_slots(pats_[size_t(k)], dih, D);
         lay[size_t(k)] = build_device_layout(pats_[size_t(k)], tris, T, tri_slots_[size_t(k)], spr, E,
                                              spr_slots_[size_t(k)], P ? I.pin_idx.data() : nullptr, P, dih, D,
-                                             &dih_slots_[size_t(k)]);
+                                             &dih_slots_[size_t(k)], kGroupSlices);
         inst_row0_[size_t(k)] = row0;
         inst_n_[size_t(k)] = I.n;
         inst_slice0_[size_t(k)] = slice0;
@@ -349,9 +349,12 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     for (int k = 0; k < ninst_; ++k) {
         const Instance& I = *insts[size_t(k)];
         st[size_t(k)].slice0 = inst_slice0_[size_t(k)];
-        st[size_t(k)].nslices = inst_nslices_[size_t(k)];
+        st[size_t(k)].nslices = (I.n + kSlice - 1) / kSlice;  // slices holding rows
+        st[size_t(k)].group0 = inst_slice0_[size_t(k)] / kGroupSlices;
+        st[size_t(k)].ngroups = inst_nslices_[size_t(k)] / kGroupSlices;
         for (int q = 0; q < inst_nslices_[size_t(k)]; ++q)
-            slices_h_.push_back(SliceDesc{k, I.part0 + kSlice * q, std::min(kSlice, I.n - kSlice * q), 0});
+            slices_h_.push_back(
+                SliceDesc{k, I.part0 + kSlice * q, std::max(0, std::min(kSlice, I.n - kSlice * q)), 0});
         st[size_t(k)].status = kStopped;
     }
     if (int(slices_h_.size()) != nslices) throw std::logic_error("slice count mismatch");
@@ -402,7 +405,7 @@ void Problem::build(const std::vector<const Instance*>& insts) {
         x_.resize(v3);
         x_.zero(s);
     }
-    partials_.resize(std::max<size_t>(size_t(4) * slices_h_.size(), 4));
+    partials_.resize(std::max<size_t>(size_t(kPartialSlots) * (slices_h_.size() / kGroupSlices), kPartialSlots));
     partials_.zero(s);
     // warm-start buffers (allocated on first use)
     hist_.release();
@@ -529,15 +532,17 @@ void Problem::launch_assemble(float dt, const DevExt& ext, hipStream_t s) {
                                max_width_, s);
 }
 void Problem::launch_pcg_start(hipStream_t s) {
-    launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 0, cfg_.residual_waves, s);
-    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, int(slices_h_.size()), kPhaseStart, s);
+    const LaunchGeom g = geom();
+    launch_pcg_residual(g, matrix(), vectors(), state_.data(), 0, s);
+    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, g.ngroups, kPhaseStart, s);
 }
+// One iteration: SpMV pass, update pass (it sums p . Ap itself), reduction.
 void Problem::launch_pcg_iteration(hipStream_t s) {
-    launch_pcg_spmv(geom(), matrix(), vectors(), state_.data(), cfg_.spmv_waves, cfg_.spmv_unroll, s,
-                    cfg_.spmv_branch_free, cfg_.spmv_symmetric);
-    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, int(slices_h_.size()), kPhaseSigma, s);
-    launch_pcg_update(geom(), matrix(), vectors(), state_.data(), cfg_.update_waves, s);
-    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, int(slices_h_.size()), kPhaseRho, s);
+    const LaunchGeom g = geom();
+    launch_pcg_spmv(g, matrix(), vectors(), state_.data(), cfg_.spmv_unroll, s, cfg_.spmv_branch_free,
+                    cfg_.spmv_symmetric);
+    launch_pcg_update(g, matrix(), vectors(), state_.data(), s);
+    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, g.ngroups, kPhaseIter, s);
 }
 
 void Problem::set_launch_config(const LaunchConfig& c) {
@@ -546,6 +551,10 @@ void Problem::set_launch_config(const LaunchConfig& c) {
         (c.spmv_branch_free != 0 && c.spmv_branch_free != 1) || (c.spmv_symmetric != 0 && c.spmv_symmetric != 1) ||
         (c.spmv_packed != 0 && c.spmv_packed != 1))
         throw std::invalid_argument("launch config: waves must be 1, 2, 4, 8 or 16 and unroll 1, 2 or 4");
+    if (c.residual_waves != 16 || c.spmv_waves != 16 || c.update_waves != 16)
+        throw std::invalid_argument(
+            "launch config: the PCG passes run 16 waves per workgroup (one inner-product partial per 16-slice "
+            "group); residual_waves, spmv_waves and update_waves must be 16");
     cfg_ = c;
     destroy_graph();
 }
@@ -572,9 +581,9 @@ void Problem::prime_states(InstState* st, hipStream_t s) const {
     check_hip(hipGetLastError(), "prime_states_kernel");
 }
 
-// Autotuner: times every SpMV-pass shape (waves x unroll) and update-pass
-// shape on this problem's own matrix layout, on scratch vectors and a scratch
-// solver state, and keeps the fastest.  Launch shapes never change results.
+// Autotuner: times every SpMV-pass variant (unroll x branch-free padding) on
+// this problem's own matrix layout, on scratch vectors and a scratch solver
+// state, and keeps the fastest.  Launch shapes never change results.
 void Problem::autotune(hipStream_t s) {
     if (!built_ || n_pad_ < kAutotuneMinRows) return;
     const size_t v3 = size_t(n_pad_) * 3;
@@ -607,32 +616,20 @@ void Problem::autotune(hipStream_t s) {
         check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
         return double(ms) / kAutotuneReps;
     };
-    const int waves_list[] = {1, 2, 4, 8, 16};
     const int unroll_list[] = {1, 2, 4};
     double best = 1e30;
     tune_log_.clear();
     for (int bf = 0; bf < 2; ++bf)
-        for (int w : waves_list)
-            for (int u : unroll_list) {
-                const double t = time_it(
-                    [&] { launch_pcg_spmv(geom(), matrix(), v, sst.data(), w, u, s, bf, cfg_.spmv_symmetric); });
-                tune_log_.push_back({bf ? 2 : 0, w, u, t});
-                if (t < best) {
-                    best = t;
-                    cfg_.spmv_waves = w;
-                    cfg_.spmv_unroll = u;
-                    cfg_.spmv_branch_free = bf;
-                }
+        for (int u : unroll_list) {
+            const double t =
+                time_it([&] { launch_pcg_spmv(geom(), matrix(), v, sst.data(), u, s, bf, cfg_.spmv_symmetric); });
+            tune_log_.push_back({bf ? 2 : 0, kGroupSlices, u, t});
+            if (t < best) {
+                best = t;
+                cfg_.spmv_unroll = u;
+                cfg_.spmv_branch_free = bf;
             }
-    best = 1e30;
-    for (int w : waves_list) {
-        const double t = time_it([&] { launch_pcg_update(geom(), matrix(), v, sst.data(), w, s); });
-        tune_log_.push_back({1, w, 0, t});
-        if (t < best) {
-            best = t;
-            cfg_.update_waves = w;
         }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(autotune)");
@@ -795,8 +792,8 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
         max_it = std::max(max_it, host_states_[k].iters);
     }
     if (any_stopped) {
-        launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 1, cfg_.residual_waves, s);
-        launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, int(slices_h_.size()), kPhaseFinal, s);
+        launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 1, s);
+        launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, geom().ngroups, kPhaseFinal, s);
         read_states(s);
     }
     if (ws_ran_ && ws_train_) {

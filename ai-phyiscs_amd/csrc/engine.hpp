@@ -92,18 +92,20 @@ struct Instance {
 };
 
 // Workgroup shapes of the step's kernels (speed only; results do not depend
-// on them).
+// on them).  The PCG passes (residual, SpMV, update) always run 16-wave
+// workgroups, one per 16-slice group of inner-product partials: their waves
+// fields are fixed at 16.
 struct LaunchConfig {
     int assemble_waves = 4;
-    int residual_waves = 4;
-    int spmv_waves = 4;
+    int residual_waves = 16;
+    int spmv_waves = 16;
     int spmv_unroll = 1;
     int spmv_branch_free = 0;  // 1: padding entries gather the row's own vector (zero blocks) instead of a branch
     // 1: the SpMV pass reads each lower block as the transpose of its upper
     // mirror (streams half the off-diagonal values; changes results at FP32
     // rounding level, so it is a fixed choice, never picked by the autotuner)
     int spmv_symmetric = 1;
-    int update_waves = 4;
+    int update_waves = 16;
     int finalize_waves = 4;
     // 1: with spmv_symmetric and no bending elements (every block, diagonal
     // or not, then symmetric bitwise), blocks are stored as their upper
@@ -179,7 +181,9 @@ public:
     void zero_dx(hipStream_t s);
 
     // Kernel-level access for benchmarks.
-    LaunchGeom geom() const { return LaunchGeom{slices_.data(), int(slices_h_.size())}; }
+    LaunchGeom geom() const {
+        return LaunchGeom{slices_.data(), int(slices_h_.size()), int(slices_h_.size()) / kGroupSlices};
+    }
     DevMatrix matrix() const;
     // Floats per stored off-diagonal block: 6 (symmetric, packed) or 9.
     int block_floats() const;

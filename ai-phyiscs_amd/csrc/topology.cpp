@@ -196,11 +196,12 @@ std::vector<int> build_dihedral_slots(const Pattern& p, const int* dih, int ndih
 DeviceLayout build_device_layout(const Pattern& p, const int* tris, int ntri, const std::vector<int>& tri_slots,
                                  const int* springs, int nspring, const std::vector<int>& spr_slots,
                                  const int* pins, int npin, const int* dih, int ndih,
-                                 const std::vector<int>* dih_slots) {
+                                 const std::vector<int>* dih_slots, int slice_group) {
     DeviceLayout L;
     const int n = p.n;
     L.n = n;
-    L.n_pad = ((n + kSlice - 1) / kSlice) * kSlice;
+    const int unit = kSlice * std::max(slice_group, 1);
+    L.n_pad = ((n + unit - 1) / unit) * unit;
     L.nslices = L.n_pad / kSlice;
     L.row_deg.assign(size_t(L.n_pad), 0);
     for (int r = 0; r < n; ++r) {

@@ -70,7 +70,7 @@ std::vector<int> build_dihedral_slots(const Pattern& p, const int* dih, int ndih
 // and in a fixed order.
 struct DeviceLayout {
     int n = 0;              // rows (particles)
-    int n_pad = 0;          // rows padded to kSlice
+    int n_pad = 0;          // rows padded to kSlice * slice_group
     int nslices = 0;
     int max_width = 0;
     std::vector<int> slice_ptr;   // nslices + 1 (in entries of 64 lanes)
@@ -96,7 +96,7 @@ struct DeviceLayout {
 DeviceLayout build_device_layout(const Pattern& p, const int* tris, int ntri, const std::vector<int>& tri_slots,
                                  const int* springs, int nspring, const std::vector<int>& spr_slots,
                                  const int* pins, int npin, const int* dih = nullptr, int ndih = 0,
-                                 const std::vector<int>* dih_slots = nullptr);
+                                 const std::vector<int>* dih_slots = nullptr, int slice_group = 1);
 
 // Per-vertex triangle incidence (for the mass redistribution helper):
 // for each vertex, the triangles touching it, in triangle order.

@@ -145,15 +145,19 @@ def test_cfg2_full_size_resync_and_residual(cuda):
     assert bsr_residual(diag, ro, co, vals, rhs, g.sim.get_dx64()) <= 1.0001e-6
 
 
-def test_reference_mode_runs_fixed_iterations(cuda):
+@pytest.mark.parametrize("iters", [50, 49, 1])
+def test_reference_mode_runs_fixed_iterations(cuda, iters):
+    """Exactly max_iter iterations.  An odd count ends on an update pass that
+    deferred its x += alpha p, so the solve flushes that term before it
+    stops; the iterate must still match the oracle's."""
     s = scenes.config1()
-    g = make_sim(s, tol=0.0, max_iter=50, springs_active=False, force_active=False)
+    g = make_sim(s, tol=0.0, max_iter=iters, springs_active=False, force_active=False)
     oc, pos, vel, im, f = oracle_from_scene(s, precision=64, springs_active=False, force_active=False,
-                                            tol=0.0, max_iter=50)
+                                            tol=0.0, max_iter=iters)
     for _ in range(5):
         g.sim.step(s.dt)
         st = g.sim.last_solve_stats()
-        assert st["iters"] == 50 and st["status"] == STOPPED
+        assert st["iters"] == iters and st["status"] == STOPPED
     oc.step(pos, vel, im, f, s.dt, 5)
     assert rel(g.positions(), pos) <= 1e-5
 
@@ -225,15 +229,19 @@ def test_damping_and_uniform_force_closed_form(cuda):
 
 
 def test_launch_shapes_do_not_change_results(cuda):
-    """Workgroup shapes and SpMV unrolling are speed knobs only: every inner
-    product is summed per 64-row slice in slice order."""
+    """Workgroup shapes and SpMV variants are speed knobs only: every inner
+    product is summed per 16-slice group, then over groups in one order.  The
+    PCG passes' own workgroup shape is fixed (16 waves, one group each)."""
     s = scenes.config2(w=72, steps=5)
+    g = make_sim(s, tol=1e-6)
+    with pytest.raises(ValueError):
+        g.sim.set_launch_config(dict(spmv_waves=4))
     runs = []
-    for cfg in [dict(spmv_waves=1, spmv_unroll=1, update_waves=16, assemble_waves=2),
-                dict(spmv_waves=16, spmv_unroll=4, update_waves=1, assemble_waves=8),
-                dict(spmv_waves=4, spmv_unroll=2, update_waves=4, residual_waves=1, finalize_waves=16),
-                dict(spmv_waves=8, spmv_unroll=4, spmv_branch_free=1),
-                dict(spmv_waves=2, spmv_unroll=1, spmv_branch_free=1)]:
+    for cfg in [dict(spmv_unroll=1, assemble_waves=2),
+                dict(spmv_unroll=4, assemble_waves=8),
+                dict(spmv_unroll=2, finalize_waves=16),
+                dict(spmv_unroll=4, spmv_branch_free=1),
+                dict(spmv_unroll=1, spmv_branch_free=1, assemble_waves=1)]:
         g = make_sim(s, tol=1e-6)
         g.sim.set_autotune(False)
         g.sim.set_launch_config(cfg)
