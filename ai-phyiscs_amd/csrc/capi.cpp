@@ -587,7 +587,8 @@ int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg) {
         const LaunchConfig& c = const_cast<chysx_cloth*>(sim)->sim.problem().launch_config();
         const int v[CHYSX_LAUNCH_CONFIG_LEN] = {c.assemble_waves,   c.residual_waves, c.spmv_waves,
                                                 c.spmv_unroll,      c.update_waves,   c.finalize_waves,
-                                                c.spmv_branch_free, c.spmv_symmetric, c.spmv_packed};
+                                                c.spmv_branch_free, c.spmv_symmetric, c.spmv_packed,
+                                                c.spmv_index16};
         std::memcpy(cfg, v, sizeof(v));
     });
 }
@@ -607,7 +608,16 @@ int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg) {
         c.spmv_branch_free = cfg[6];
         c.spmv_symmetric = cfg[7];
         c.spmv_packed = cfg[8];
+        c.spmv_index16 = cfg[9];
         sim->sim.problem().set_launch_config(c);
+    });
+}
+
+int chysx_cloth_index_bits(const chysx_cloth* sim, int* bits) {
+    return guard([&] {
+        need(sim, "sim");
+        need(bits, "bits");
+        *bits = const_cast<chysx_cloth*>(sim)->sim.problem().index_bits();
     });
 }
 

@@ -750,7 +750,7 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_residual_kernel(LaunchG
 // the XCD has streamed several L2 capacities.)
 __device__ __forceinline__ int sym_order(int k, int w) { return (k & 1) ? w - 1 - (k >> 1) : (k >> 1); }
 
-template <int UNROLL, bool BF, bool SYM, bool PK>
+template <int UNROLL, bool BF, bool SYM, bool PK, bool I16>
 __global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
                                                                     const InstState* __restrict__ st) {
     const int s = my_slice<kGroupSlices>(g.nslices);
@@ -811,14 +811,24 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom 
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;
+        const int16_t* colp16 = A.col16 + (size_t)sb * 64 + lane;
+        const int16_t* mirp16 = A.mir16 + (size_t)sb * 64 + lane;
+        const int e0 = sb * 64 + lane;  // entry index of the row's entry 0
         constexpr int BS = PK ? 6 : 9;
         const float* vp = A.val + (size_t)sb * BS * 64 + lane;
+        // mirror entry index of entry kk (a lower entry)
+        auto mirror = [&](int kk) { return I16 ? e0 + kk * 64 + int(mirp16[kk * 64]) : mirp[kk * 64]; };
         for (int k0 = 0; k0 < w; k0 += UNROLL) {
             int c[UNROLL], kk[UNROLL];
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
                 kk[u] = SYM ? sym_order(k0 + u, w) : k0 + u;
-                c[u] = (k0 + u < w) ? colp[kk[u] * 64] : -1;
+                if (I16) {
+                    const int d = (k0 + u < w) ? int(colp16[kk[u] * 64]) : kNoCol16;
+                    c[u] = (d == kNoCol16) ? -1 : row + d;
+                } else {
+                    c[u] = (k0 + u < w) ? colp[kk[u] * 64] : -1;
+                }
             }
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
@@ -832,7 +842,7 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom 
                         // symmetric blocks: the lower block (row, cc) is its
                         // mirror (cc, row) itself, which row cc's lane
                         // streamed at the same step of its loop
-                        const float* b = (cc < row) ? A.val + val_offset(mirp[kk[u] * 64], 6) : vp + kk[u] * 6 * 64;
+                        const float* b = (cc < row) ? A.val + val_offset(mirror(kk[u]), 6) : vp + kk[u] * 6 * 64;
                         const float b0 = b[0], b1 = b[64], b2 = b[128], b3 = b[192], b4 = b[256], b5 = b[320];
                         a0 += b0 * pj.x + b1 * pj.y + b2 * pj.z;
                         a1 += b1 * pj.x + b3 * pj.y + b4 * pj.z;
@@ -840,7 +850,7 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom 
                     } else if (SYM && cc < row) {
                         // lower block (row, cc) = transpose of (cc, row), which
                         // row cc's lane streamed at the same step of its loop
-                        const float* bt = A.val + val_offset(mirp[kk[u] * 64], 9);
+                        const float* bt = A.val + val_offset(mirror(kk[u]), 9);
                         a0 += bt[0] * pj.x + bt[192] * pj.y + bt[384] * pj.z;
                         a1 += bt[64] * pj.x + bt[256] * pj.y + bt[448] * pj.z;
                         a2 += bt[128] * pj.x + bt[320] * pj.y + bt[512] * pj.z;
@@ -1195,13 +1205,19 @@ void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVecto
     CHYSX_LAUNCH_CHECK();
 }
 namespace {
+template <bool BF, bool SYM, bool PK, bool I16>
+void pcg_spmv_launch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int unroll,
+                     hipStream_t s) {
+    const int grid = grid_for(g.nslices, kGroupSlices), block = 64 * kGroupSlices;
+    if (unroll == 1) pcg_spmv_kernel<1, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st);
+    else if (unroll == 2) pcg_spmv_kernel<2, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st);
+    else pcg_spmv_kernel<4, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st);
+}
 template <bool BF, bool SYM, bool PK>
 void pcg_spmv_dispatch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int unroll,
                        hipStream_t s) {
-    const int grid = grid_for(g.nslices, kGroupSlices), block = 64 * kGroupSlices;
-    if (unroll == 1) pcg_spmv_kernel<1, BF, SYM, PK><<<grid, block, 0, s>>>(g, A, v, st);
-    else if (unroll == 2) pcg_spmv_kernel<2, BF, SYM, PK><<<grid, block, 0, s>>>(g, A, v, st);
-    else pcg_spmv_kernel<4, BF, SYM, PK><<<grid, block, 0, s>>>(g, A, v, st);
+    if (A.col16 != nullptr && A.mir16 != nullptr) pcg_spmv_launch<BF, SYM, PK, true>(g, A, v, st, unroll, s);
+    else pcg_spmv_launch<BF, SYM, PK, false>(g, A, v, st, unroll, s);
 }
 }  // namespace
 

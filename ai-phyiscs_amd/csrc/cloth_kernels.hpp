@@ -62,7 +62,13 @@ struct DevMatrix {
     // floats per stored off-diagonal block: 9, or 6 when every off-diagonal
     // block is symmetric (no bending) and sym is set
     int bs;
+    // The same columns and mirror indices as 16-bit offsets, when every one
+    // fits (null otherwise): col16 = col - row (kNoCol16 for padding), mir16 =
+    // mir - own entry index.  Read by the SpMV pass only.
+    const int16_t* col16;
+    const int16_t* mir16;
 };
+constexpr int kNoCol16 = -32768;
 
 // Float offset in val of component 0 of entry index e (slot * 64 + lane).
 __host__ __device__ inline size_t val_offset(int e, int bs) {

@@ -49,6 +49,7 @@ void DeviceArray<T>::release() {
     n_ = 0;
 }
 template class DeviceArray<int>;
+template class DeviceArray<int16_t>;
 template class DeviceArray<int2>;
 template class DeviceArray<float>;
 template class DeviceArray<float4>;
@@ -388,6 +389,32 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     slice_ptr_.assign(slice_ptr_h_.data(), slice_ptr_h_.size(), s);
     col_.assign(sell_col.data(), sell_col.size(), s);
     mir_.assign(mir.data(), mir.size(), s);
+    {
+        // 16-bit offsets for the SpMV pass when the mesh's bandwidth allows
+        std::vector<int16_t> c16(sell_col.size()), m16(mir.size(), 0);
+        bool fits = true;
+        for (int sl = 0; sl < nslices && fits; ++sl) {
+            const int w = slice_ptr_h_[size_t(sl) + 1] - slice_ptr_h_[size_t(sl)];
+            for (int k = 0; k < w && fits; ++k)
+                for (int lane = 0; lane < kSlice; ++lane) {
+                    const size_t e = (size_t(slice_ptr_h_[size_t(sl)]) + k) * kSlice + lane;
+                    const int row = sl * kSlice + lane, c = sell_col[e];
+                    const long long dc = (c < 0) ? kNoCol16 : (long long)c - row;
+                    const long long dm = (mir[e] < 0) ? 0 : (long long)mir[e] - (long long)e;
+                    if (c >= 0 && (dc <= kNoCol16 || dc > 32767)) fits = false;
+                    if (dm < -32768 || dm > 32767) fits = false;
+                    c16[e] = int16_t(dc);
+                    m16[e] = int16_t(dm);
+                }
+        }
+        if (fits) {
+            col16_.assign(c16.data(), c16.size(), s);
+            mir16_.assign(m16.data(), m16.size(), s);
+        } else {
+            col16_.release();
+            mir16_.release();
+        }
+    }
     mir_h_ = std::move(mir);
     row_deg_.assign(row_deg_h_.data(), row_deg_h_.size(), s);
     val_.resize(size_t(entries) * kSlice * 9);
@@ -487,8 +514,18 @@ void Problem::update_pin_targets(const Instance& inst, int k, hipStream_t s) {
 }
 
 DevMatrix Problem::matrix() const {
-    return DevMatrix{slice_ptr_.data(), col_.data(),  row_deg_.data(), val_.data(),
-                     diag_.data(),      minv_.data(), mir_.data(),     cfg_.spmv_symmetric, block_floats()};
+    const bool i16 = cfg_.spmv_index16 && col16_.size() > 0;
+    return DevMatrix{slice_ptr_.data(),
+                     col_.data(),
+                     row_deg_.data(),
+                     val_.data(),
+                     diag_.data(),
+                     minv_.data(),
+                     mir_.data(),
+                     cfg_.spmv_symmetric,
+                     block_floats(),
+                     i16 ? col16_.data() : nullptr,
+                     i16 ? mir16_.data() : nullptr};
 }
 int Problem::block_floats() const {
     return (cfg_.spmv_symmetric && cfg_.spmv_packed && dih_v_.size() == 0) ? 6 : 9;
@@ -549,7 +586,7 @@ void Problem::set_launch_config(const LaunchConfig& c) {
     if (!valid_waves(c.assemble_waves) || !valid_waves(c.residual_waves) || !valid_waves(c.spmv_waves) ||
         !valid_waves(c.update_waves) || !valid_waves(c.finalize_waves) || !valid_unroll(c.spmv_unroll) ||
         (c.spmv_branch_free != 0 && c.spmv_branch_free != 1) || (c.spmv_symmetric != 0 && c.spmv_symmetric != 1) ||
-        (c.spmv_packed != 0 && c.spmv_packed != 1))
+        (c.spmv_packed != 0 && c.spmv_packed != 1) || (c.spmv_index16 != 0 && c.spmv_index16 != 1))
         throw std::invalid_argument("launch config: waves must be 1, 2, 4, 8 or 16 and unroll 1, 2 or 4");
     if (c.residual_waves != 16 || c.spmv_waves != 16 || c.update_waves != 16)
         throw std::invalid_argument(

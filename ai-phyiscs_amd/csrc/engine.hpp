@@ -111,6 +111,9 @@ struct LaunchConfig {
     // or not, then symmetric bitwise), blocks are stored as their upper
     // triangle (6 floats).  The operator is bitwise the same as with 9.
     int spmv_packed = 1;
+    // 1: the SpMV pass reads 16-bit column and mirror offsets when every
+    // offset of the mesh fits (speed only: the same indices).
+    int spmv_index16 = 1;
 };
 
 struct TuneSample {
@@ -187,6 +190,8 @@ public:
     DevMatrix matrix() const;
     // Floats per stored off-diagonal block: 6 (symmetric, packed) or 9.
     int block_floats() const;
+    // Bits of the column / mirror offsets the SpMV pass reads: 16 or 32.
+    int index_bits() const { return (cfg_.spmv_index16 && col16_.size() > 0) ? 16 : 32; }
     DevVectors vectors() const;
     DevMesh mesh() const;
     InstState* states() const { return state_.data(); }
@@ -273,6 +278,7 @@ private:
     // matrix
     DeviceArray<int> slice_ptr_, col_, row_deg_, mir_;
     std::vector<int> mir_h_;
+    DeviceArray<int16_t> col16_, mir16_;  // empty when an offset does not fit in 16 bits
     DeviceArray<float> val_, diag_, minv_;
     // vectors
     DeviceArray<float> rhs_, r_, z_, ap_, p0_, p1_;

@@ -437,6 +437,7 @@ PYBIND11_MODULE(_chysx_native, m) {
                  d["spmv_branch_free"] = c[6];
                  d["spmv_symmetric"] = c[7];
                  d["spmv_packed"] = c[8];
+                 d["spmv_index16"] = c[9];
                  return d;
              })
         .def(
@@ -446,12 +447,18 @@ PYBIND11_MODULE(_chysx_native, m) {
                 check(chysx_cloth_launch_config(s.p(), c));
                 const char* keys[CHYSX_LAUNCH_CONFIG_LEN] = {
                     "assemble_waves", "residual_waves",   "spmv_waves",     "spmv_unroll", "update_waves",
-                    "finalize_waves", "spmv_branch_free", "spmv_symmetric", "spmv_packed"};
+                    "finalize_waves", "spmv_branch_free", "spmv_symmetric", "spmv_packed", "spmv_index16"};
                 for (int i = 0; i < CHYSX_LAUNCH_CONFIG_LEN; ++i)
                     if (d.contains(keys[i])) c[i] = d[keys[i]].cast<int>();
                 check(chysx_cloth_set_launch_config(s.p(), c));
             },
             py::arg("config"))
+        .def("index_bits",
+             [](const Sim& s) {
+                 int b = 0;
+                 check(chysx_cloth_index_bits(s.p(), &b));
+                 return b;
+             })
         .def("block_floats",
              [](const Sim& s) {
                  int f = 0;

@@ -35,17 +35,27 @@ def kernel_bytes(n_rows: int, nnz: int):
         "spmv": 40.0 * nnz + 64.0 * n_rows + 4.0,
         # PCG SpMV: values+cols 40/block; diag 36, z 12, p_prev 12 reads, p 12, Ap 12 writes per row
         "pcg_spmv": 40.0 * nnz + 84.0 * n_rows,
-        # the same pass on symmetric storage: the nnz/2 upper blocks are
-        # streamed (36 B), every entry's column (4 B) and each lower entry's
-        # mirror index (4 B); lower blocks re-read their mirrors from L2
-        "pcg_spmv_symmetric": 18.0 * nnz + 4.0 * nnz + 2.0 * nnz + 84.0 * n_rows,
-        # ... and with packed symmetric blocks (24 B each, diagonal blocks too)
-        "pcg_spmv_symmetric_packed": 12.0 * nnz + 4.0 * nnz + 2.0 * nnz + 72.0 * n_rows,
         # PCG update: p 12, Ap 12, x 24, r 12, M^-1 24 reads; x 24, r 12, z 12 writes per row
         "pcg_update": 132.0 * n_rows,
         # reference formulation of one PCG iteration: B_spmv + 180 per row
         "pcg_iteration_ref": 40.0 * nnz + 244.0 * n_rows + 4.0,
     }
+
+
+def spmv_layout_bytes(n_rows: int, nnz: int, storage: str, index_bits: int) -> float:
+    """Bytes the SpMV pass must move on a storage: general streams every block
+    (36 B); the symmetric storages stream the nnz/2 upper blocks (36 B, or
+    24 B packed) and re-read lower blocks through their mirrors (L2), with a
+    column index per entry and a mirror index per lower entry (4 B, or 2 B as
+    16-bit offsets); diagonal 36 B (24 B packed); z, p_prev read and p, Ap
+    written, 12 B each."""
+    ib = index_bits / 8.0
+    vectors = 48.0 * n_rows
+    if storage == "general":
+        return 36.0 * nnz + ib * nnz + 36.0 * n_rows + vectors
+    if storage == "symmetric":
+        return 18.0 * nnz + ib * nnz + ib * nnz / 2 + 36.0 * n_rows + vectors
+    return 12.0 * nnz + ib * nnz + ib * nnz / 2 + 24.0 * n_rows + vectors
 
 
 def bench_batch(args):
@@ -335,10 +345,11 @@ def main():
         g.sim.time_kernel(3, 1, s.dt, stream)  # one assembly into this storage
         t_sp = g.sim.time_kernel(1, args.kernel_reps, s.dt, stream)
         t_it = g.sim.time_kernel(4, args.kernel_reps, s.dt, stream)
-        key = "pcg_spmv" if name == "general" else f"pcg_spmv_{name}"
-        storage[name] = {"block_floats": g.sim.block_floats(), "layout_bytes": kb[key], "ms": round(t_sp, 5),
-                         "GB/s_layout": round(gbs(kb[key], t_sp), 1),
-                         "frac_layout": round(gbs(kb[key], t_sp) / PEAK_HBM_GBS, 4),
+        lb = spmv_layout_bytes(n, nnz, name, g.sim.index_bits())
+        storage[name] = {"block_floats": g.sim.block_floats(), "index_bits": g.sim.index_bits(),
+                         "layout_bytes": lb, "ms": round(t_sp, 5),
+                         "GB/s_layout": round(gbs(lb, t_sp), 1),
+                         "frac_layout": round(gbs(lb, t_sp) / PEAK_HBM_GBS, 4),
                          "GB/s_ref_formulation": round(gbs(kb["pcg_spmv"], t_sp), 1),
                          "iteration_ms": round(t_it, 5)}
         if name == "general":
@@ -347,19 +358,20 @@ def main():
     g.sim.time_kernel(3, 1, s.dt, stream)
     t_pcg_spmv = storage[mode0]["ms"]
     t_iter = storage[mode0]["iteration_ms"]
-    key0 = "pcg_spmv" if mode0 == "general" else f"pcg_spmv_{mode0}"
+    lb0 = storage[mode0]["layout_bytes"]
 
     traffic = read_traffic(args.size)
     roofline = {
         "bound": "hbm",
-        "kernel": f"pcg_spmv_kernel, {mode0} storage (p = z + beta p_prev; Ap = A p; p.Ap)",
-        "achieved": round(gbs(kb[key0], t_pcg_spmv), 1),
+        "kernel": f"pcg_spmv_kernel, {mode0} storage, {storage[mode0]['index_bits']}-bit indices "
+                  "(p = z + beta p_prev; Ap = A p; p.Ap)",
+        "achieved": round(gbs(lb0, t_pcg_spmv), 1),
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
-        "frac": round(gbs(kb[key0], t_pcg_spmv) / PEAK_HBM_GBS, 4),
+        "frac": round(gbs(lb0, t_pcg_spmv) / PEAK_HBM_GBS, 4),
         "traffic": traffic.get({"symmetric_packed": "pcg_spmv_kernel", "symmetric": "pcg_spmv_kernel_symmetric",
                                 "general": "pcg_spmv_kernel_general"}[mode0]),
-        "bytes_per_launch": kb[key0],
+        "bytes_per_launch": lb0,
         "ms_per_launch": round(t_pcg_spmv, 5),
         "ref_formulation_bytes": kb["pcg_spmv"],
         "ref_formulation_GB/s": round(gbs(kb["pcg_spmv"], t_pcg_spmv), 1),

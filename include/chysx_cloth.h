@@ -226,19 +226,23 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
  * their upper mirrors; changes results at FP32 rounding level, never chosen
  * by the autotuner), packed blocks (0/1: with the symmetric SpMV and no
  * bending elements, diagonal and off-diagonal blocks stored as their upper
- * triangle; the operator and results are bitwise those of 9-float blocks)}; waves in
+ * triangle; the operator and results are bitwise those of 9-float blocks),
+ * 16-bit indices (0/1: the SpMV pass reads 16-bit column and mirror offsets
+ * when every offset of the mesh fits; the same indices)}; waves in
  * {1,2,4,8,16} wavefronts of 64 lanes per workgroup, except the PCG passes
  * (residual, SpMV, update), which always run 16 (one workgroup per 16-slice
  * group of inner-product partials; other values are rejected); unroll in
  * {1,2,4}.  Shapes never change results.  By default an autotuner times the
  * SpMV variants (unroll x branch-free padding) on the problem's own matrix
  * when its topology is built (problems of 32768 rows or more). */
-#define CHYSX_LAUNCH_CONFIG_LEN 9
+#define CHYSX_LAUNCH_CONFIG_LEN 10
 int chysx_cloth_set_autotune(chysx_cloth* sim, int on);
 int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg);
 int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg);
 /* Floats per stored off-diagonal block in effect: 6 (packed) or 9. */
 int chysx_cloth_block_floats(const chysx_cloth* sim, int* floats);
+/* Bits of the column / mirror offsets the SpMV pass reads: 16 or 32. */
+int chysx_cloth_index_bits(const chysx_cloth* sim, int* bits);
 /* Autotuner samples: count in/out; per sample {kernel (0 SpMV, 2 branch-free
  * SpMV), waves, unroll} and mean ms. */
 int chysx_cloth_tune_log(const chysx_cloth* sim, int* count, int* kernel_waves_unroll, double* ms);

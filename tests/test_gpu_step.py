@@ -330,3 +330,43 @@ def test_packed_blocks_bitwise_equal_to_full_blocks(cuda):
     g.sim.step(d.dt)
     assert g.sim.block_floats() == 9
     assert g.sim.last_solve_stats()["status"] == CONVERGED
+
+
+def test_index16_bitwise_equal_and_fallback(cuda):
+    """16-bit column / mirror offsets are the same indices: bitwise-equal
+    results.  A mesh whose numbering puts neighbours more than 32767 rows
+    apart falls back to 32-bit indices and still solves."""
+    s = scenes.config2(w=72, steps=4)
+    runs = []
+    for i16 in [0, 1]:
+        g = make_sim(s, tol=1e-6)
+        g.sim.set_autotune(False)
+        g.sim.set_launch_config(dict(spmv_index16=i16))
+        for _ in range(4):
+            g.sim.step(s.dt)
+            assert g.sim.last_solve_stats()["status"] == CONVERGED
+        assert g.sim.index_bits() == (16 if i16 else 32)
+        runs.append((g.positions(), g.sim.get_dx64()))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+    # 200 x 200 wind cloth with a random particle numbering: neighbours far apart
+    base = scenes.config3(w=200, steps=2)
+    perm = np.random.default_rng(3).permutation(base.n)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(base.n)
+    sc = scenes.Scene(**{**base.__dict__})
+    sc.pos = np.ascontiguousarray(base.pos[perm])
+    sc.vel = np.ascontiguousarray(base.vel[perm])
+    sc.inv_mass = np.ascontiguousarray(base.inv_mass[perm])
+    sc.force = np.ascontiguousarray(base.force[perm])
+    sc.tris = inv[base.tris].astype(np.int32)
+    sc.pin_idx = inv[base.pin_idx].astype(np.int32)
+    g = make_sim(sc, tol=1e-6)
+    ref = make_sim(base, tol=1e-6)
+    for _ in range(2):
+        g.sim.step(sc.dt)
+        ref.sim.step(base.dt)
+        assert g.sim.last_solve_stats()["status"] == CONVERGED
+    assert g.sim.index_bits() == 32 and ref.sim.index_bits() == 16
+    # the same cloth, numbered differently: the same motion to rounding
+    assert rel(g.positions()[inv], ref.positions().astype(np.float64)) <= 1e-5
