@@ -366,7 +366,7 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
     const int deg = A.row_deg[row];
     const int bs = A.bs;
     float* __restrict__ vrow = A.val + (size_t)sb * bs * 64 + lane;
-    float* __restrict__ acc = LDS ? lds_acc + (size_t)(threadIdx.x >> 6) * maxw * 9 * 64 + lane : vrow;
+    float* __restrict__ acc = LDS ? lds_acc + (size_t)(threadIdx.x >> 6) * maxw * bs * 64 + lane : vrow;
     for (int k = 0; k < deg; ++k)
         for (int c = 0; c < bs; ++c) acc[(k * bs + c) * 64] = 0.0f;
 
@@ -1174,10 +1174,10 @@ void launch_area_mass(const int* vt_off, const int* vt_ids, const int* tri_v, co
 void launch_assemble(const LaunchGeom& g, const DevMesh& m, const DevMatrix& A, const DevVectors& v,
                      const DevExt& e, const InstParams* params, float dt, int waves, int maxw, hipStream_t s) {
     if (g.nslices <= 0) return;
-    // LDS accumulator: waves * maxw blocks of 9 x 64 floats per workgroup,
+    // LDS accumulator: waves * maxw blocks of bs x 64 floats per workgroup,
     // kept within 64 KiB by lowering the waves per workgroup; rows wider
-    // than 28 blocks accumulate in the matrix itself.
-    const size_t slab = size_t(std::max(maxw, 1)) * 9 * 64 * sizeof(float);
+    // than 28 blocks (42 packed) accumulate in the matrix itself.
+    const size_t slab = size_t(std::max(maxw, 1)) * size_t(A.bs) * 64 * sizeof(float);
     if (slab * 1 <= kAssembleLdsBytes) {
         while (waves > 1 && slab * size_t(waves) > kAssembleLdsBytes) waves >>= 1;
         const size_t bytes = slab * size_t(waves);

@@ -811,9 +811,10 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
     // The projection and the network's inputs need two previous solutions.
     if (ws_on && hist_n_ >= 2) warm_start_stage(dt, ext, s);
     launch_pcg_start(s);
-    // Launch about as many iterations as the previous step needed before the
-    // first poll; chunks after convergence exit at once.
-    int chunks = std::max(1, (last_iters_ + 1) / kChunkIters);
+    // Launch as many rounds as the previous step used (its iterations plus a
+    // flush and a check round) before the first poll; rounds after
+    // convergence exit at once, and each extra poll costs a host round trip.
+    int chunks = std::max(1, (last_iters_ + 2 + kChunkIters - 1) / kChunkIters);
     int guard = 0;
     for (;;) {
         for (int c = 0; c < chunks; ++c) launch_pcg_chunk(s);
