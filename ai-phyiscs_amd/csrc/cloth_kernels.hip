@@ -882,21 +882,12 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom 
 // one.  Straight-line per variant, so every operand load (including x) is
 // issued before the p . Ap sum and waited for once.
 template <int X>
-__device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v,
-                                           const InstState& S, int s, const SliceDesc& sd) {
+__device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVectors& v, const InstState& S, int s,
+                                           const SliceDesc& sd, F3 r0, F3 api, const float (&m)[6]) {
     const float* __restrict__ p = S.parity ? v.p0 : v.p1;      // written by the SpMV pass
     const float* __restrict__ pprev = S.parity ? v.p1 : v.p0;  // the previous direction
     const int lane = threadIdx.x & 63;
     const int row = s * 64 + lane;
-    // Every lane loads (rows past an instance's end are allocated padding);
-    // only valid lanes store.
-    const F3 r0 = ld3(v.r, row), api = ld3(v.ap, row);
-    float m[6];
-    {
-        const float* mv = A.minv + (size_t)s * 6 * 64 + lane;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) m[c] = mv[c * 64];
-    }
     F3 pi = f3(0.0f, 0.0f, 0.0f), pp = pi;
     double x0 = 0.0, x1 = 0.0, x2 = 0.0;
     double* x = v.x + 3 * row;
@@ -947,12 +938,24 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_update_kernel(LaunchGeo
                                                                       const InstState* __restrict__ st) {
     const int s = my_slice<kGroupSlices>(g.nslices);
     if (s >= g.nslices) return;
+    // The operands every variant needs are loaded before the solver state is
+    // read, so the two latencies overlap.  Every lane loads (rows past an
+    // instance's end are allocated padding); only valid lanes store.
+    const int lane = threadIdx.x & 63;
+    const int row = s * 64 + lane;
+    const F3 r0 = ld3(v.r, row), api = ld3(v.ap, row);
+    float m[6];
+    {
+        const float* mv = A.minv + (size_t)s * 6 * 64 + lane;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) m[c] = mv[c * 64];
+    }
     const SliceDesc sd = g.slices[s];
     const InstState& S = st[sd.inst];
     if (S.status != kRunning || S.mode != kModeIter) return;  // check and flush rounds update nothing
-    if (S.pend) update_rows<1>(g, A, v, S, s, sd);
-    else if (S.defer) update_rows<0>(g, A, v, S, s, sd);
-    else update_rows<2>(g, A, v, S, s, sd);
+    if (S.pend) update_rows<1>(g, v, S, s, sd, r0, api, m);
+    else if (S.defer) update_rows<0>(g, v, S, s, sd, r0, api, m);
+    else update_rows<2>(g, v, S, s, sd, r0, api, m);
 }
 
 // ---------------------------------------------------------------------------
