@@ -592,8 +592,18 @@ void Problem::set_launch_config(const LaunchConfig& c) {
         throw std::invalid_argument(
             "launch config: the PCG passes run 16 waves per workgroup (one inner-product partial per 16-slice "
             "group); residual_waves, spmv_waves and update_waves must be 16");
+    const int bs_old = block_floats(), sym_old = cfg_.spmv_symmetric;
     cfg_ = c;
     destroy_graph();
+    if (built_ && (block_floats() != bs_old || cfg_.spmv_symmetric != sym_old)) {
+        // A new storage reuses the value and diagonal buffers with another
+        // layout.  Padding entries must read as zero blocks (the branch-free
+        // SpMV multiplies them), and assembly writes only real entries, so
+        // clear what the old layout left behind.
+        val_.zero(nullptr);
+        diag_.zero(nullptr);
+        check_hip(hipStreamSynchronize(nullptr), "hipStreamSynchronize(storage change)");
+    }
 }
 
 namespace {

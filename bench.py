@@ -392,19 +392,32 @@ def main():
         ws_cmp = warm_start_comparison(args, s, make_sim)
 
     cpu = None
+    fp_err = None
     if cpu_state is not None:
         from oracle.oracle import oracle_from_scene
         threads = min(16, os.cpu_count() or 1)
         oc, _, _, im, f = oracle_from_scene(s, precision=64, tol=args.tol, max_iter=20000, threads=threads)
         p0, v0, dx0 = cpu_state
+        p_start, v_start = p0.copy(), v0.copy()
         oc.build_topology()
         oc.set_dx(dx0)
-        secs, it = oc.time_steps(p0, v0, im, f, s.dt, args.cpu_steps)
+        secs, it = oc.time_steps(p0, v0, im, f, s.dt, args.cpu_steps)  # p0, v0 advance in place
         cpu = {"value": round(args.cpu_steps / secs, 5), "unit": "steps/s", "cores": threads, "kind": "port",
                "sample": f"FP64 oracle (C++ restatement of the reference step, OpenMP) stepping the "
                          f"{args.size}x{args.size} cloth {args.cpu_steps} step(s) from the state the GPU timed "
                          f"region starts from; {it} PCG iterations, {secs:.2f} s",
                "pcg_iters": it}
+        # FP32 GPU step vs the FP64 oracle from the same state (both to the same tolerance)
+        g.set_state(p_start, v_start)
+        g.sim.set_warm_start(dx0)
+        for _ in range(args.cpu_steps):
+            g.sim.step(s.dt)
+        gp = g.positions().astype(np.float64)
+        d_gpu, d_cpu = gp - p_start, p0 - p_start
+        fp_err = {"steps": args.cpu_steps, "tol": args.tol,
+                  "max_abs_pos_diff_m": float(np.abs(gp - p0).max()),
+                  "rel_pos_diff": float(np.linalg.norm(gp - p0) / np.linalg.norm(p0)),
+                  "rel_displacement_diff": float(np.linalg.norm(d_gpu - d_cpu) / max(np.linalg.norm(d_cpu), 1e-30))}
 
     out = {
         "metric": METRIC,
@@ -436,6 +449,7 @@ def main():
         "roofline": roofline,
         "kernels": extra_kernels,
         "cpu_baseline": cpu,
+        "fp32_vs_fp64_oracle": fp_err,
         "launch_config": g.sim.launch_config(),
         "autotune": [list(t) for t in g.sim.tune_log()],
     }

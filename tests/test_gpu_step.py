@@ -370,3 +370,24 @@ def test_index16_bitwise_equal_and_fallback(cuda):
     assert g.sim.index_bits() == 32 and ref.sim.index_bits() == 16
     # the same cloth, numbered differently: the same motion to rounding
     assert rel(g.positions()[inv], ref.positions().astype(np.float64)) <= 1e-5
+
+
+def test_storage_switches_keep_padding_zero(cuda):
+    """Switching between storages reuses the value buffers with another
+    layout; the branch-free SpMV multiplies padding entries, so they must be
+    zero after a switch.  A simulator that switches general -> packed ->
+    general -> packed matches one that never switches (rounding level)."""
+    s = scenes.config3(w=128, steps=6)
+    a = make_sim(s, tol=1e-6)
+    b = make_sim(s, tol=1e-6)
+    for g in (a, b):
+        g.sim.set_autotune(False)
+        g.sim.set_launch_config(dict(spmv_branch_free=1))
+    for mode in [(1, 1), (0, 0), (1, 1), (0, 0), (1, 1)]:
+        a.sim.set_launch_config(dict(spmv_symmetric=mode[0], spmv_packed=mode[1]))
+        a.sim.step(s.dt)
+        b.sim.step(s.dt)
+        for g in (a, b):
+            st = g.sim.last_solve_stats()
+            assert st["status"] == CONVERGED and st["relres"] <= 1e-6, (mode, st)
+    assert rel(a.positions(), b.positions().astype(np.float64)) <= 1e-6
