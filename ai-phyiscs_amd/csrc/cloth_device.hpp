@@ -50,9 +50,10 @@ constexpr int kSlotSigma = 4;
 constexpr int kPartialSlots = 5;
 
 // Per-instance solver state, FP64 scalars (inner products, alpha, beta and
-// the convergence test).  Lives in device memory; written only by
-// pcg_reduce_kernel (one workgroup per instance) and read by the row kernels
-// that follow it, so no kernel needs cross-workgroup synchronisation.
+// the convergence test).  Lives in device memory; written by
+// pcg_reduce_kernel (one wavefront per instance) and read by the row kernels
+// that follow it; the update pass writes only `sigma` and `alpha`, which no
+// row kernel reads.  No kernel needs cross-workgroup synchronisation.
 struct alignas(16) InstState {
     double rho;      // r . z
     double sigma;    // p . A p
@@ -79,7 +80,10 @@ struct alignas(16) InstState {
     int pend;
     int defer;
     int flush_to;
-    int pad[3];
+    int pad;
+    // alpha of the latest update pass, written there by the workgroup of the
+    // instance's first group (the update pass itself never reads it)
+    double alpha;
 };
 
 // Per-instance step parameters and solver options.

@@ -605,8 +605,8 @@ constexpr int kReduceThreads = 64 * kGroupSlices;
 // Sums slots slot[0..NS) of an instance's group partials (groups g0 ..
 // g0 + ng - 1, `ld` groups per slot).  Thread t adds groups t, t + T, t + 2T,
 // ... in order; the waves' sums are then combined in wave order, so the
-// result is one fixed sum whatever KU is, in every kernel that calls it with
-// kReduceThreads threads.  All NS x KU loads of a round are issued before any
+// result is one fixed sum whatever KU is (every workgroup of the update pass
+// computes the same alpha).  All NS x KU loads of a round are issued before any
 // of them is added.  The total of slot slot[j] lands in tot[slot[j]] on
 // thread 0 only.
 template <int NS, int KU>
@@ -657,6 +657,34 @@ __device__ __forceinline__ void block_sum(const double* __restrict__ partials, i
             for (int j = 0; j < NS; ++j) tot[slot[j]] += sh[w][j];
     }
     __syncthreads();  // sh may be refilled by a second call
+}
+
+// Sum of ng group partials p[0 .. ng) by one wavefront in a fixed order:
+// lane l adds groups l, l + 64, l + 128, ..., then the lanes combine by a
+// fixed butterfly; every lane gets the total.  The first 1024 groups are
+// loaded by straight-line code (clamped, unconditional loads).
+__device__ __forceinline__ double wave_group_sum(const double* __restrict__ p, int ng) {
+    constexpr int KU = 16;
+    const int lane = threadIdx.x & 63;
+    double a = 0.0;
+    auto round = [&](int q0) {
+        double t[KU];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const int q = q0 + u * 64;
+            t[u] = p[q < ng ? q : ng - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < KU; ++u)
+            if (q0 + u * 64 < ng) a += t[u];
+    };
+    if (ng > 0) {
+        round(lane);
+        for (int q0 = lane + KU * 64; q0 < ng; q0 += KU * 64) round(q0);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    return a;
 }
 
 // ---------------------------------------------------------------------------
@@ -882,7 +910,7 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom 
 // one.  Straight-line per variant, so every operand load (including x) is
 // issued before the p . Ap sum and waited for once.
 template <int X>
-__device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVectors& v, const InstState& S, int s,
+__device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVectors& v, InstState& S, int s,
                                            const SliceDesc& sd, F3 r0, F3 api, const float (&m)[6]) {
     const float* __restrict__ p = S.parity ? v.p0 : v.p1;      // written by the SpMV pass
     const float* __restrict__ pprev = S.parity ? v.p1 : v.p0;  // the previous direction
@@ -910,6 +938,10 @@ __device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVector
     const double sig = sig_sh, rho = S.rho;
     const double alpha = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;
     const float af = float(alpha);
+    if (threadIdx.x == 0 && s / kGroupSlices == S.group0) {  // one writer per instance
+        S.sigma = sig;
+        S.alpha = alpha;
+    }
     double part[2] = {0.0, 0.0};
     if (lane < sd.nvalid) {
         if (X == 1) {  // the owed term and this one
@@ -935,7 +967,7 @@ __device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVector
 }
 
 __global__ __launch_bounds__(64 * kGroupSlices) void pcg_update_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                                      const InstState* __restrict__ st) {
+                                                                      InstState* __restrict__ st) {
     const int s = my_slice<kGroupSlices>(g.nslices);
     if (s >= g.nslices) return;
     // The operands every variant needs are loaded before the solver state is
@@ -951,7 +983,7 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_update_kernel(LaunchGeo
         for (int c = 0; c < 6; ++c) m[c] = mv[c * 64];
     }
     const SliceDesc sd = g.slices[s];
-    const InstState& S = st[sd.inst];
+    InstState& S = st[sd.inst];
     if (S.status != kRunning || S.mode != kModeIter) return;  // check and flush rounds update nothing
     if (S.pend) update_rows<1>(g, v, S, s, sd, r0, api, m);
     else if (S.defer) update_rows<0>(g, v, S, s, sd, r0, api, m);
@@ -959,10 +991,10 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_update_kernel(LaunchGeo
 }
 
 // ---------------------------------------------------------------------------
-// Per-instance reduction: one workgroup per instance sums that instance's
+// Per-instance reduction: one wavefront per instance sums that instance's
 // group partials in a fixed order (deterministic) and advances its solver
 // state.  One reduction per iteration: the update pass computes alpha from
-// the SpMV pass's partials itself.
+// the SpMV pass's partials itself and leaves it (and p . Ap) in the state.
 // Scalar logic restates reduce_partial_kernel / reduce_rho_beta_kernel
 // (pcg_solver.cu:374-384, 436-454) with the 1e-37 guards, plus the stopping
 // rule: true relative residual sqrt(|b-Ax|^2 / max(|b|^2,1e-30)) <= tol
@@ -973,33 +1005,24 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_update_kernel(LaunchGeo
 // loads then need no state read first); 0: bounds from the instance state.
 // The phase's partials are loaded before the state is inspected, so the two
 // loads overlap; an instance that is not live discards them.
-__global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* __restrict__ st,
-                                                                    const InstParams* __restrict__ params,
-                                                                    const double* __restrict__ partials,
-                                                                    int ngroups, int phase, int single_ng) {
+__global__ __launch_bounds__(64) void pcg_reduce_kernel(InstState* __restrict__ st,
+                                                        const InstParams* __restrict__ params,
+                                                        const double* __restrict__ partials, int ngroups, int phase,
+                                                        int single_ng) {
     InstState& S = st[blockIdx.x];
     const InstParams& pr = params[blockIdx.x];
     const int g0 = single_ng > 0 ? 0 : S.group0;
     const int ng = single_ng > 0 ? single_ng : S.ngroups;
-    // Slots: start |r|^2, |b|^2, r . z; iteration p . Ap, r . z, |r|^2 (a
-    // check round: the true |r|^2 and r . z); final |r|^2.
+    // Slots: start |r|^2, |b|^2, r . z; iteration r . z, |r|^2 (a check
+    // round: the true |r|^2 and r . z); final |r|^2.
+    const double* base = partials + g0;
     double tot[kPartialSlots] = {};
-    if (phase == kPhaseStart) {
-        const int sl[3] = {0, 1, 2};
-        block_sum<3, 4>(partials, ngroups, g0, ng, sl, tot);
-    } else if (phase == kPhaseIter) {
-        const int sl[3] = {kSlotSigma, 0, 1};
-        block_sum<3, 4>(partials, ngroups, g0, ng, sl, tot);
-    } else {
-        const int sl[1] = {0};
-        block_sum<1, 4>(partials, ngroups, g0, ng, sl, tot);
-    }
+    tot[0] = wave_group_sum(base, ng);
+    if (phase == kPhaseStart || phase == kPhaseIter) tot[1] = wave_group_sum(base + (size_t)ngroups, ng);
+    if (phase == kPhaseStart) tot[2] = wave_group_sum(base + (size_t)2 * ngroups, ng);
     if (phase == kPhaseIter && S.status != kRunning) return;
     if (phase == kPhaseFinal && S.status != kStopped) return;
-    if (phase == kPhaseIter && S.mode == kModeCheck) {
-        const int sl[1] = {2};
-        block_sum<1, 4>(partials, ngroups, g0, ng, sl, tot);
-    }
+    if (phase == kPhaseIter && S.mode == kModeCheck) tot[2] = wave_group_sum(base + (size_t)2 * ngroups, ng);
     if (threadIdx.x != 0) return;
     const double tol2 = pr.tol2;
     switch (phase) {
@@ -1045,13 +1068,11 @@ __global__ __launch_bounds__(kReduceThreads) void pcg_reduce_kernel(InstState* _
                     if (S.iters >= pr.max_iter) S.status = kStopped;
                 }
             } else {
-                // sigma as the update pass summed it; alpha as it computed it
-                const double sig = tot[kSlotSigma], rho = S.rho, rho_new = tot[0];
-                S.sigma = sig;
+                const double rho = S.rho, rho_new = tot[0];
                 if (S.pend) {  // this update pass added the owed term
                     S.pend = 0;
                 } else if (S.defer) {  // this update pass left x += alpha p to the next
-                    S.alpha_pend = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;
+                    S.alpha_pend = S.alpha;  // the alpha that pass used
                     S.pend = 1;
                 }
                 S.defer = S.pend ? 0 : 1;
@@ -1243,7 +1264,7 @@ void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& 
     }
     CHYSX_LAUNCH_CHECK();
 }
-void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
                        hipStream_t s) {
     if (g.nslices <= 0) return;
     if (g.nslices % kGroupSlices != 0 || g.ngroups * kGroupSlices != g.nslices)
@@ -1254,7 +1275,7 @@ void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors
 void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
                        int phase, hipStream_t s) {
     if (ninst <= 0) return;
-    pcg_reduce_kernel<<<ninst, kReduceThreads, 0, s>>>(st, params, partials, ngroups, phase,
+    pcg_reduce_kernel<<<ninst, 64, 0, s>>>(st, params, partials, ngroups, phase,
                                                        ninst == 1 ? ngroups : 0);
     CHYSX_LAUNCH_CHECK();
 }

@@ -125,7 +125,7 @@ void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVecto
 void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int unroll,
                      hipStream_t s, int branch_free = 0, int symmetric = 0);
 // Update pass (alpha from the SpMV pass's partials; x, r, z; r . z, r . r).
-void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
                        hipStream_t s);
 // One workgroup per instance: finish the reductions of `phase` (PcgPhase).
 void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
