@@ -104,15 +104,18 @@ def train(pairs: int = 50000, cloths: int = 16, width: int = 256, episode: int =
 MODES = ("previous", "learned", "projection", "projection_learned")
 
 
-def compare(sim_factory, steps: int, warmup: int, theta: np.ndarray, modes=MODES):
+def compare(sim_factory, steps: int, warmup: int, theta: np.ndarray, modes=MODES, accept: str = "energy"):
     """Run the same workload with each warm start; returns per-mode mean
-    iterations, steps/s, share of steps using each stage and final positions."""
+    iterations, steps/s, share of steps using each stage and final positions.
+    `accept`: the network direction's acceptance rule ('energy' or
+    'residual', see ClothSimulator.ws_set_acceptance)."""
     import torch
     out = {}
     for mode in modes:
         g = sim_factory()
         g.sim.ws_set_params(theta)
         g.sim.set_warm_start_mode(mode)
+        g.sim.ws_set_acceptance(accept)
         for _ in range(warmup):
             g.sim.step(g.scene.dt)
         torch.cuda.synchronize()
@@ -160,6 +163,7 @@ def _main(argv=None):
     e.add_argument("--scene", choices=["wind", "train"], default="wind")
     e.add_argument("--steps", type=int, default=60)
     e.add_argument("--warmup", type=int, default=10)
+    e.add_argument("--accept", choices=["energy", "residual"], default="energy")
     a = ap.parse_args(argv)
     if a.cmd == "train":
         theta, meta = train(a.pairs, a.cloths, a.width, a.episode, a.lr, a.lr_final, a.seed,
@@ -172,9 +176,10 @@ def _main(argv=None):
             scene = sc.config3(w=a.width)
         else:
             scene = sc.config5_train(count=1, w=a.width, seed=12345)[0]
-        r = compare(lambda: make_sim(scene, tol=1e-6), a.steps, a.warmup, theta)
+        r = compare(lambda: make_sim(scene, tol=1e-6), a.steps, a.warmup, theta, accept=a.accept)
         out = {m: {k: round(v, 4) for k, v in r[m].items() if k != "pos"} for m in r}
         out["scene"] = scene.name
+        out["accept"] = a.accept
         for m in MODES[1:]:
             out[m]["iter_ratio_vs_previous"] = round(r["previous"]["iters"] / max(r[m]["iters"], 1e-9), 4)
             out[m]["time_ratio_vs_previous"] = round(r[m]["steps_per_s"] / r["previous"]["steps_per_s"], 4)

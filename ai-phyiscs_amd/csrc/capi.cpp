@@ -717,6 +717,12 @@ int chysx_cloth_ws_set_training(chysx_cloth* sim, int on, float lr) {
         cloth_prob(sim).set_ws_training(on != 0, lr);
     });
 }
+int chysx_cloth_ws_set_acceptance(chysx_cloth* sim, int residual) {
+    return guard([&] {
+        need(sim, "sim");
+        cloth_prob(sim).set_ws_accept_residual(residual != 0);
+    });
+}
 int chysx_cloth_ws_set_history(chysx_cloth* sim, int k) {
     return guard([&] {
         need(sim, "sim");
@@ -759,6 +765,12 @@ int chysx_batch_ws_set_training(chysx_batch* b, int on, float lr) {
     return guard([&] {
         need(b, "batch");
         batch_prob(b).set_ws_training(on != 0, lr);
+    });
+}
+int chysx_batch_ws_set_acceptance(chysx_batch* b, int residual) {
+    return guard([&] {
+        need(b, "batch");
+        batch_prob(b).set_ws_accept_residual(residual != 0);
     });
 }
 int chysx_batch_ws_set_history(chysx_batch* b, int k) {

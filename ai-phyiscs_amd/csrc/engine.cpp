@@ -807,7 +807,7 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
                        N.features(), N.edge_features(), s);
     N.forward(ws_graph(), s, ws_train_);
     launch_ws_direction(geom(), matrix(), N.out(), ws_inst_.data(), rb_.data(), d_.data(), ad_.data(), red, s);
-    launch_ws_finish(red, 4, state_.data(), ws_inst_.data(), ninst_, kWsPhaseLine, 0, s);
+    launch_ws_finish(red, 4, state_.data(), ws_inst_.data(), ninst_, kWsPhaseLine, ws_accept_residual_ ? 1 : 0, s);
     if (learned) launch_ws_line_apply(geom(), ws_inst_.data(), d_.data(), x_.data(), s);
     ws_ran_ = true;
 }

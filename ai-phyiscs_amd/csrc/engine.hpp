@@ -222,6 +222,11 @@ public:
     // only in the learned mode; after the solve the line-search loss against
     // the converged dx gives one Adam step.
     void set_ws_training(bool on, float lr) { ws_train_ = on; ws_lr_ = lr; }
+    // Acceptance of the network's direction: false (default) when it lowers
+    // the energy (always, unless alpha == 0); true only when it also lowers
+    // the predicted residual.
+    void set_ws_accept_residual(bool on) { ws_accept_residual_ = on; }
+    bool ws_accept_residual() const { return ws_accept_residual_; }
     WarmStartNet& net();
     double last_ws_loss() const { return ws_loss_; }
 
@@ -293,6 +298,7 @@ private:
     // warm start
     int ws_mode_ = kWarmPrevious;
     bool ws_train_ = false;
+    bool ws_accept_residual_ = false;
     float ws_lr_ = 1e-3f;
     double ws_loss_ = 0.0;
     bool ws_ran_ = false;    // network stage ran this step

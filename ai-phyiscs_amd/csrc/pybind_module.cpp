@@ -509,6 +509,14 @@ PYBIND11_MODULE(_chysx_native, m) {
             "ws_set_training",
             [](Sim& s, bool on, float lr) { check(chysx_cloth_ws_set_training(s.p(), on ? 1 : 0, lr)); },
             py::arg("on"), py::arg("lr") = 1e-3f)
+        .def(
+            "ws_set_acceptance",
+            [](Sim& s, const std::string& rule) {
+                if (rule != "energy" && rule != "residual")
+                    throw py::value_error("ws_set_acceptance: 'energy' or 'residual'");
+                check(chysx_cloth_ws_set_acceptance(s.p(), rule == "residual" ? 1 : 0));
+            },
+            py::arg("rule"))
         .def("ws_stats",
              [](Sim& s) {
                  double o[CHYSX_WS_STATS_LEN];
@@ -601,6 +609,14 @@ PYBIND11_MODULE(_chysx_native, m) {
             "ws_set_training",
             [](Batch& b, bool on, float lr) { check(chysx_batch_ws_set_training(b.h, on ? 1 : 0, lr)); },
             py::arg("on"), py::arg("lr") = 1e-3f)
+        .def(
+            "ws_set_acceptance",
+            [](Batch& b, const std::string& rule) {
+                if (rule != "energy" && rule != "residual")
+                    throw py::value_error("ws_set_acceptance: 'energy' or 'residual'");
+                check(chysx_batch_ws_set_acceptance(b.h, rule == "residual" ? 1 : 0));
+            },
+            py::arg("rule"))
         .def(
             "ws_stats",
             [](Batch& b, int k) {

@@ -940,11 +940,13 @@ __global__ __launch_bounds__(128) void ws_finish_kernel(WsReduce red, int nv, co
         const double al = (w > 0.0 && isfinite(u / w)) ? u / w : 0.0;
         const double rr = W.rr_base - 2.0 * al * q + al * al * a2;
         W.rr_cand = fmax(rr, 0.0);
-        // Accept when the energy x^T A x / 2 - b^T x drops, i.e. the residual
-        // measured in the A^-1 norm (the A-norm of the error, what PCG
-        // minimises): the exact line search lowers it by u^2 / (2 w) > 0
-        // whenever d is not A-orthogonal to the error.
-        W.use_cand = (al != 0.0) ? 1 : 0;
+        // hk == 0 (default): accept when the energy x^T A x / 2 - b^T x
+        // drops, i.e. the residual measured in the A^-1 norm (the A-norm of
+        // the error, what PCG minimises): the exact line search lowers it by
+        // u^2 / (2 w) > 0 whenever d is not A-orthogonal to the error.
+        // hk == 1: accept only when the predicted residual |r_b - alpha A d|
+        // is below the base guess's (otherwise keep the base guess).
+        W.use_cand = (al != 0.0 && (hk == 0 || W.rr_cand < W.rr_base)) ? 1 : 0;
         W.alpha = W.use_cand ? al : 0.0;
     } else {  // kWsPhaseEnergy
         const double E = tot[0], u = W.u, w = W.w;

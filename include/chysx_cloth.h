@@ -272,6 +272,12 @@ int chysx_cloth_ws_set_params(chysx_cloth* sim, const float* theta, int n);
 int chysx_cloth_ws_get_params(chysx_cloth* sim, float* theta, int n);
 int chysx_cloth_ws_init_params(chysx_cloth* sim, uint64_t seed);
 int chysx_cloth_ws_set_training(chysx_cloth* sim, int on, float lr);
+/* Acceptance of the network's line-searched direction: residual == 0
+ * (default) keeps it whenever it lowers the energy x^T A x / 2 - b^T x (the
+ * A-norm error PCG minimises; the exact line search always does unless
+ * alpha = 0); residual == 1 keeps it only when the predicted residual
+ * |b - A x| also drops, else falls back to the base guess. */
+int chysx_cloth_ws_set_acceptance(chysx_cloth* sim, int residual);
 /* Last step, out[0..6]: projection used (1/0), learned direction used (1/0),
  * projection directions kept, relative residuals |b - A x0| / |b| of dx_prev,
  * of the projection and of the learned guess (0 when that stage did not run),
@@ -283,6 +289,7 @@ int chysx_batch_ws_set_params(chysx_batch* b, const float* theta, int n);
 int chysx_batch_ws_get_params(chysx_batch* b, float* theta, int n);
 int chysx_batch_ws_init_params(chysx_batch* b, uint64_t seed);
 int chysx_batch_ws_set_training(chysx_batch* b, int on, float lr);
+int chysx_batch_ws_set_acceptance(chysx_batch* b, int residual);
 int chysx_batch_ws_stats(chysx_batch* b, int index, double* out, int n);
 /* Projection solve on the host (tests): c = G^+ g over the eigen-directions
  * of the symmetric k x k G (row-major) above 1e-12 of the largest; *kept =

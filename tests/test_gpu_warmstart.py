@@ -212,6 +212,33 @@ def test_learned_direction_applied_and_steps_stay_exact(cuda):
     assert np.mean(used) > 0.5
 
 
+def test_residual_acceptance_falls_back_when_residual_rises(cuda):
+    """With the residual rule the network's direction is kept only when the
+    predicted residual drops below the previous step's guess; otherwise the
+    solve starts from dx_prev.  The trained network, 40 steps of a wind cloth:
+    every kept direction lowered the residual, steps fell back, and the
+    solves all converge."""
+    s = scenes.config3(w=64, steps=40)
+    from chysx import warmstart
+    theta, _ = warmstart.load_params()
+    g = make_sim(s, tol=1e-6)
+    g.sim.ws_set_params(theta)
+    g.sim.set_warm_start_mode("learned")
+    with pytest.raises(ValueError):
+        g.sim.ws_set_acceptance("sometimes")
+    g.sim.ws_set_acceptance("residual")
+    used = []
+    for k in range(40):
+        g.sim.step(s.dt)
+        assert g.sim.last_solve_stats()["converged"]
+        st = g.sim.ws_stats()
+        if k >= 2:
+            used.append(st["used"])
+            if st["used"]:
+                assert st["r0_learned"] < st["r0_previous"], (k, st)
+    assert np.mean(used) < 1.0
+
+
 def test_online_training_reduces_loss(cuda):
     """Online training from dx_prev: the line-search loss (share of the A-norm
     error of dx_prev the direction leaves) starts near 1 for a fresh network
