@@ -68,6 +68,20 @@ __device__ __forceinline__ void st3(float* p, int i, F3 v) {
 }
 
 // Symmetric 3x3 (xx, xy, xz, yy, yz, zz).
+// a + (m0 p.x + m1 p.y + m2 p.z) as one fixed chain of FMAs.  Written out so
+// that every kernel variant (unroll, padding mode, storage) rounds the same
+// way whatever the compiler schedules.
+__device__ __forceinline__ float fma3(float a, float m0, float m1, float m2, F3 p) {
+    return fmaf(m2, p.z, fmaf(m1, p.y, fmaf(m0, p.x, a)));
+}
+__device__ __forceinline__ double fma3d(double a, double m0, double m1, double m2, double x0, double x1, double x2) {
+    return fma(m2, x2, fma(m1, x1, fma(m0, x0, a)));
+}
+// z + beta p_prev, component-wise FMAs
+__device__ __forceinline__ F3 dir3(F3 z, F3 pp, float beta) {
+    return F3{fmaf(pp.x, beta, z.x), fmaf(pp.y, beta, z.y), fmaf(pp.z, beta, z.z)};
+}
+
 struct S6 {
     float m[6];
 };
@@ -597,7 +611,7 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
 __device__ __forceinline__ F3 minv_mul(const DevMatrix& A, int s, int lane, F3 r) {
     const float* mv = A.minv + (size_t)s * 6 * 64 + lane;
     const float m0 = mv[0], m1 = mv[64], m2 = mv[128], m3 = mv[192], m4 = mv[256], m5 = mv[320];
-    return f3(m0 * r.x + m1 * r.y + m2 * r.z, m1 * r.x + m3 * r.y + m4 * r.z, m2 * r.x + m4 * r.y + m5 * r.z);
+    return f3(fma3(0.0f, m0, m1, m2, r), fma3(0.0f, m1, m3, m4, r), fma3(0.0f, m2, m4, m5, r));
 }
 
 constexpr int kReduceThreads = 64 * kGroupSlices;
@@ -699,9 +713,9 @@ __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVector
     float dm[9];
     load_diag(A, s, lane, dm);
     const double x0 = x[3 * row], x1 = x[3 * row + 1], x2 = x[3 * row + 2];
-    double a0 = double(dm[0]) * x0 + double(dm[1]) * x1 + double(dm[2]) * x2;
-    double a1 = double(dm[3]) * x0 + double(dm[4]) * x1 + double(dm[5]) * x2;
-    double a2 = double(dm[6]) * x0 + double(dm[7]) * x1 + double(dm[8]) * x2;
+    double a0 = fma3d(0.0, dm[0], dm[1], dm[2], x0, x1, x2);
+    double a1 = fma3d(0.0, dm[3], dm[4], dm[5], x0, x1, x2);
+    double a2 = fma3d(0.0, dm[6], dm[7], dm[8], x0, x1, x2);
     const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
     const int* colp = A.col + (size_t)sb * 64 + lane;
     const int* mirp = A.mir + (size_t)sb * 64 + lane;
@@ -712,22 +726,22 @@ __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVector
             float m[9];
             load_block(A, vp + k * A.bs * 64, mirp + k * 64, c, row, m);
             const double y0 = x[3 * c], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
-            a0 += double(m[0]) * y0 + double(m[1]) * y1 + double(m[2]) * y2;
-            a1 += double(m[3]) * y0 + double(m[4]) * y1 + double(m[5]) * y2;
-            a2 += double(m[6]) * y0 + double(m[7]) * y1 + double(m[8]) * y2;
+            a0 = fma3d(a0, m[0], m[1], m[2], y0, y1, y2);
+            a1 = fma3d(a1, m[3], m[4], m[5], y0, y1, y2);
+            a2 = fma3d(a2, m[6], m[7], m[8], y0, y1, y2);
         }
     }
     const F3 bf = ld3(v.rhs, row);
     const double r0 = double(bf.x) - a0, r1 = double(bf.y) - a1, r2 = double(bf.z) - a2;
-    part[0] = r0 * r0 + r1 * r1 + r2 * r2;
-    part[1] = double(bf.x) * bf.x + double(bf.y) * bf.y + double(bf.z) * bf.z;
+    part[0] = fma3d(0.0, r0, r1, r2, r0, r1, r2);
+    part[1] = fma3d(0.0, bf.x, bf.y, bf.z, bf.x, bf.y, bf.z);
     if (!report_only) {
         const F3 rf = f3(float(r0), float(r1), float(r2));
         const F3 zf = minv_mul(A, s, lane, rf);
         st3(v.r, row, rf);
         st3(v.z, row, zf);
         st3(pcur, row, zf);
-        part[2] = double(rf.x) * zf.x + double(rf.y) * zf.y + double(rf.z) * zf.z;
+        part[2] = fma3d(0.0, rf.x, rf.y, rf.z, zf.x, zf.y, zf.z);
     }
 }
 
@@ -822,19 +836,19 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom 
     const float* __restrict__ z = v.z;
     double part[1] = {0.0};
     if (valid) {
-        const F3 pi = ld3(z, row) + ld3(pold, row) * beta;
+        const F3 pi = dir3(ld3(z, row), ld3(pold, row), beta);
         float a0, a1, a2;
         if (PK) {
             const float* dg = A.diag + (size_t)s * 6 * 64 + lane;
             const float d0 = dg[0], d1 = dg[64], d2 = dg[128], d3 = dg[192], d4 = dg[256], d5 = dg[320];
-            a0 = d0 * pi.x + d1 * pi.y + d2 * pi.z;
-            a1 = d1 * pi.x + d3 * pi.y + d4 * pi.z;
-            a2 = d2 * pi.x + d4 * pi.y + d5 * pi.z;
+            a0 = fma3(0.0f, d0, d1, d2, pi);
+            a1 = fma3(0.0f, d1, d3, d4, pi);
+            a2 = fma3(0.0f, d2, d4, d5, pi);
         } else {
             const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
-            a0 = dg[0] * pi.x + dg[64] * pi.y + dg[128] * pi.z;
-            a1 = dg[192] * pi.x + dg[256] * pi.y + dg[320] * pi.z;
-            a2 = dg[384] * pi.x + dg[448] * pi.y + dg[512] * pi.z;
+            a0 = fma3(0.0f, dg[0], dg[64], dg[128], pi);
+            a1 = fma3(0.0f, dg[192], dg[256], dg[320], pi);
+            a2 = fma3(0.0f, dg[384], dg[448], dg[512], pi);
         }
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
         const int* colp = A.col + (size_t)sb * 64 + lane;
@@ -865,35 +879,35 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom 
                 const bool live = BF ? (k0 + u < w) : (c[u] >= 0);
                 if (live) {
                     const int cc = (c[u] >= 0) ? c[u] : row;
-                    const F3 pj = ld3(z, cc) + ld3(pold, cc) * beta;
+                    const F3 pj = dir3(ld3(z, cc), ld3(pold, cc), beta);
                     if (PK) {
                         // symmetric blocks: the lower block (row, cc) is its
                         // mirror (cc, row) itself, which row cc's lane
                         // streamed at the same step of its loop
                         const float* b = (cc < row) ? A.val + val_offset(mirror(kk[u]), 6) : vp + kk[u] * 6 * 64;
                         const float b0 = b[0], b1 = b[64], b2 = b[128], b3 = b[192], b4 = b[256], b5 = b[320];
-                        a0 += b0 * pj.x + b1 * pj.y + b2 * pj.z;
-                        a1 += b1 * pj.x + b3 * pj.y + b4 * pj.z;
-                        a2 += b2 * pj.x + b4 * pj.y + b5 * pj.z;
+                        a0 = fma3(a0, b0, b1, b2, pj);
+                        a1 = fma3(a1, b1, b3, b4, pj);
+                        a2 = fma3(a2, b2, b4, b5, pj);
                     } else if (SYM && cc < row) {
                         // lower block (row, cc) = transpose of (cc, row), which
                         // row cc's lane streamed at the same step of its loop
                         const float* bt = A.val + val_offset(mirror(kk[u]), 9);
-                        a0 += bt[0] * pj.x + bt[192] * pj.y + bt[384] * pj.z;
-                        a1 += bt[64] * pj.x + bt[256] * pj.y + bt[448] * pj.z;
-                        a2 += bt[128] * pj.x + bt[320] * pj.y + bt[512] * pj.z;
+                        a0 = fma3(a0, bt[0], bt[192], bt[384], pj);
+                        a1 = fma3(a1, bt[64], bt[256], bt[448], pj);
+                        a2 = fma3(a2, bt[128], bt[320], bt[512], pj);
                     } else {
                         const float* bk = vp + kk[u] * 9 * 64;
-                        a0 += bk[0] * pj.x + bk[64] * pj.y + bk[128] * pj.z;
-                        a1 += bk[192] * pj.x + bk[256] * pj.y + bk[320] * pj.z;
-                        a2 += bk[384] * pj.x + bk[448] * pj.y + bk[512] * pj.z;
+                        a0 = fma3(a0, bk[0], bk[64], bk[128], pj);
+                        a1 = fma3(a1, bk[192], bk[256], bk[320], pj);
+                        a2 = fma3(a2, bk[384], bk[448], bk[512], pj);
                     }
                 }
             }
         }
         st3(v.ap, row, f3(a0, a1, a2));
         st3(pnew, row, pi);
-        part[0] = double(pi.x) * a0 + double(pi.y) * a1 + double(pi.z) * a2;
+        part[0] = fma3d(0.0, double(pi.x), double(pi.y), double(pi.z), a0, a1, a2);
     }
     const int sl[1] = {kSlotSigma};
     group_partials<1>(part, sl, v.partials, s / kGroupSlices, g.ngroups);
