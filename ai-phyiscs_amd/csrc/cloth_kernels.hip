@@ -797,6 +797,7 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom 
                                                                     const InstState* __restrict__ st) {
     const int s = my_slice<kGroupSlices>(g.nslices);
     if (s >= g.nslices) return;
+    if (g.gactive != nullptr && g.gactive[s / kGroupSlices] == 0) return;  // a finished cloth
     const SliceDesc sd = g.slices[s];
     const InstState& S = st[sd.inst];
     if (S.status != kRunning) return;
@@ -984,6 +985,7 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_update_kernel(LaunchGeo
                                                                       InstState* __restrict__ st) {
     const int s = my_slice<kGroupSlices>(g.nslices);
     if (s >= g.nslices) return;
+    if (g.gactive != nullptr && g.gactive[s / kGroupSlices] == 0) return;  // a finished cloth
     // The operands every variant needs are loaded before the solver state is
     // read, so the two latencies overlap.  Every lane loads (rows past an
     // instance's end are allocated padding); only valid lanes store.
@@ -1022,7 +1024,7 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_update_kernel(LaunchGeo
 __global__ __launch_bounds__(64) void pcg_reduce_kernel(InstState* __restrict__ st,
                                                         const InstParams* __restrict__ params,
                                                         const double* __restrict__ partials, int ngroups, int phase,
-                                                        int single_ng) {
+                                                        int single_ng, int* __restrict__ gactive) {
     InstState& S = st[blockIdx.x];
     const InstParams& pr = params[blockIdx.x];
     const int g0 = single_ng > 0 ? 0 : S.group0;
@@ -1037,7 +1039,13 @@ __global__ __launch_bounds__(64) void pcg_reduce_kernel(InstState* __restrict__ 
     if (phase == kPhaseIter && S.status != kRunning) return;
     if (phase == kPhaseFinal && S.status != kStopped) return;
     if (phase == kPhaseIter && S.mode == kModeCheck) tot[2] = wave_group_sum(base + (size_t)2 * ngroups, ng);
+    if (phase == kPhaseStart && gactive != nullptr) {
+        // the instance iterates from here on unless it is done already
+        const bool run = !(pr.tol2 > 0.0 && tot[0] <= pr.tol2 * fmax(tot[1], 1.0e-30)) && pr.max_iter > 0;
+        for (int q = int(threadIdx.x); q < ng; q += 64) gactive[g0 + q] = run ? 1 : 0;
+    }
     if (threadIdx.x != 0) return;
+    const int status_in = S.status;
     const double tol2 = pr.tol2;
     switch (phase) {
         case kPhaseStart: {
@@ -1114,6 +1122,9 @@ __global__ __launch_bounds__(64) void pcg_reduce_kernel(InstState* __restrict__ 
             break;
         }
     }
+    // a cloth that just finished: its groups leave the iteration passes
+    if (phase == kPhaseIter && gactive != nullptr && status_in == kRunning && S.status != kRunning)
+        for (int q = 0; q < ng; ++q) gactive[g0 + q] = 0;
 }
 
 // finalize_step_kernel (cloth_simulator.cu:222-244): x = x_n + dx,
@@ -1287,10 +1298,10 @@ void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors
     CHYSX_LAUNCH_CHECK();
 }
 void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
-                       int phase, hipStream_t s) {
+                       int phase, hipStream_t s, int* gactive) {
     if (ninst <= 0) return;
-    pcg_reduce_kernel<<<ninst, 64, 0, s>>>(st, params, partials, ngroups, phase,
-                                                       ninst == 1 ? ngroups : 0);
+    pcg_reduce_kernel<<<ninst, 64, 0, s>>>(st, params, partials, ngroups, phase, ninst == 1 ? ngroups : 0,
+                                           gactive);
     CHYSX_LAUNCH_CHECK();
 }
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,

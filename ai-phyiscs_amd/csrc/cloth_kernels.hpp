@@ -99,6 +99,10 @@ struct LaunchGeom {
     const SliceDesc* slices;
     int nslices;  // a multiple of kGroupSlices
     int ngroups;  // nslices / kGroupSlices
+    // Batches: 1 for the groups of instances still iterating, else 0 (kept
+    // by pcg_reduce_kernel), so the iteration passes drop a finished cloth's
+    // workgroups after one load.  Null for a single instance.
+    int* gactive;
 };
 
 // Workgroup shapes: `waves` in {1, 2, 4, 8, 16} (64 x waves lanes), SpMV
@@ -128,8 +132,10 @@ void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& 
 void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
                        hipStream_t s);
 // One workgroup per instance: finish the reductions of `phase` (PcgPhase).
+// gactive (batches; may be null): per-group 1/0 flags of running instances,
+// set at kPhaseStart and cleared when an instance finishes.
 void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
-                       int phase, hipStream_t s);
+                       int phase, hipStream_t s, int* gactive = nullptr);
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
                      int waves, hipStream_t s);
 void launch_area_mass(const int* vt_off, const int* vt_ids, const int* tri_v, const float* pos, float density_over_3,

@@ -433,6 +433,8 @@ void Problem::build(const std::vector<const Instance*>& insts) {
         x_.zero(s);
     }
     partials_.resize(std::max<size_t>(size_t(kPartialSlots) * (slices_h_.size() / kGroupSlices), kPartialSlots));
+    gactive_.resize(std::max<size_t>(slices_h_.size() / kGroupSlices, 1));
+    gactive_.zero(s);
     partials_.zero(s);
     // warm-start buffers (allocated on first use)
     hist_.release();
@@ -571,7 +573,7 @@ void Problem::launch_assemble(float dt, const DevExt& ext, hipStream_t s) {
 void Problem::launch_pcg_start(hipStream_t s) {
     const LaunchGeom g = geom();
     launch_pcg_residual(g, matrix(), vectors(), state_.data(), 0, s);
-    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, g.ngroups, kPhaseStart, s);
+    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, g.ngroups, kPhaseStart, s, g.gactive);
 }
 // One iteration: SpMV pass, update pass (it sums p . Ap itself), reduction.
 void Problem::launch_pcg_iteration(hipStream_t s) {
@@ -579,7 +581,7 @@ void Problem::launch_pcg_iteration(hipStream_t s) {
     launch_pcg_spmv(g, matrix(), vectors(), state_.data(), cfg_.spmv_unroll, s, cfg_.spmv_branch_free,
                     cfg_.spmv_symmetric);
     launch_pcg_update(g, matrix(), vectors(), state_.data(), s);
-    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, g.ngroups, kPhaseIter, s);
+    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, g.ngroups, kPhaseIter, s, g.gactive);
 }
 
 void Problem::set_launch_config(const LaunchConfig& c) {
@@ -626,6 +628,9 @@ __global__ void prime_states_kernel(InstState* st, int n) {
 void Problem::prime_states(InstState* st, hipStream_t s) const {
     hipLaunchKernelGGL(prime_states_kernel, dim3((ninst_ + 63) / 64), dim3(64), 0, s, st, ninst_);
     check_hip(hipGetLastError(), "prime_states_kernel");
+    if (ninst_ > 1 && gactive_.size() > 0)  // every cloth iterating
+        check_hip(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(gactive_.data()), 1, gactive_.size(), s),
+                  "hipMemsetD32Async(active groups)");
 }
 
 // Autotuner: times every SpMV-pass variant (unroll x branch-free padding) on

@@ -185,7 +185,8 @@ public:
 
     // Kernel-level access for benchmarks.
     LaunchGeom geom() const {
-        return LaunchGeom{slices_.data(), int(slices_h_.size()), int(slices_h_.size()) / kGroupSlices};
+        return LaunchGeom{slices_.data(), int(slices_h_.size()), int(slices_h_.size()) / kGroupSlices,
+                          ninst_ > 1 ? gactive_.data() : nullptr};
     }
     DevMatrix matrix() const;
     // Floats per stored off-diagonal block: 6 (symmetric, packed) or 9.
@@ -288,6 +289,7 @@ private:
     // vectors
     DeviceArray<float> rhs_, r_, z_, ap_, p0_, p1_;
     DeviceArray<double> x_, partials_;
+    DeviceArray<int> gactive_;  // batches: 1 per group of a running instance
 
     InstState* host_states_ = nullptr;  // pinned
     std::vector<SolveStats> stats_;
