@@ -63,7 +63,6 @@ struct alignas(16) InstState {
     double relres;   // last true relative residual
     int status;      // PcgStatus
     int mode;        // PcgMode
-    int skip;        // 1: this round's update pass is skipped (after a restart)
     int iters;
     int restarts;
     int parity;      // which p buffer holds the current direction

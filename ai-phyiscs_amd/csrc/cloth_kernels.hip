@@ -1055,7 +1055,6 @@ __global__ __launch_bounds__(64) void pcg_reduce_kernel(InstState* __restrict__ 
             S.iters = 0;
             S.restarts = 0;
             S.mode = kModeIter;
-            S.skip = 0;
             S.beta = 0.0;
             S.rho = tot[2];
             S.pend = 0;

@@ -614,7 +614,6 @@ __global__ void prime_states_kernel(InstState* st, int n) {
     if (i < n) {
         st[i].status = kRunning;
         st[i].mode = kModeIter;
-        st[i].skip = 0;
         st[i].sigma = 1.0;
         st[i].rho = 1.0;
         st[i].beta = 0.5;
