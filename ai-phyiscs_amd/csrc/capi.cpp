@@ -815,7 +815,9 @@ struct NetGraph {
             if (c < 0 || c >= n) throw std::out_of_range("wsnet: column out of range");
         const DeviceLayout L = build_device_layout(p, nullptr, 0, {}, nullptr, 0, {}, nullptr, 0);
         std::vector<SliceDesc> sd(static_cast<size_t>(L.nslices));
-        for (int q = 0; q < L.nslices; ++q) sd[size_t(q)] = SliceDesc{0, 64 * q, std::min(64, n - 64 * q), 0};
+        // nlower: the slice width (every entry may be lower), a bound that holds for any pattern
+        for (int q = 0; q < L.nslices; ++q)
+            sd[size_t(q)] = SliceDesc{0, 64 * q, std::min(64, n - 64 * q), L.slice_ptr[size_t(q) + 1] - L.slice_ptr[size_t(q)]};
         slices.assign(sd.data(), sd.size());
         slice_ptr.assign(L.slice_ptr.data(), L.slice_ptr.size());
         col.assign(L.sell_col.data(), L.sell_col.size());

@@ -16,7 +16,8 @@ struct SliceDesc {
     int inst;    // instance id
     int part0;   // particle index (external buffers) of the slice's first row
     int nvalid;  // rows of the slice that exist (<= 64)
-    int pad;
+    int nlower;  // one past the last entry position holding a lower block (column < row) in
+                 // any lane: entries k >= nlower are upper or padding in every lane
 };
 
 // PCG status of one instance.

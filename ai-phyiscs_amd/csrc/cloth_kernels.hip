@@ -837,74 +837,108 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom 
     const float* __restrict__ z = v.z;
     double part[1] = {0.0};
     if (valid) {
-        const F3 pi = dir3(ld3(z, row), ld3(pold, row), beta);
-        float a0, a1, a2;
-        if (PK) {
-            const float* dg = A.diag + (size_t)s * 6 * 64 + lane;
-            const float d0 = dg[0], d1 = dg[64], d2 = dg[128], d3 = dg[192], d4 = dg[256], d5 = dg[320];
-            a0 = fma3(0.0f, d0, d1, d2, pi);
-            a1 = fma3(0.0f, d1, d3, d4, pi);
-            a2 = fma3(0.0f, d2, d4, d5, pi);
-        } else {
-            const float* dg = A.diag + (size_t)s * 9 * 64 + lane;
-            a0 = fma3(0.0f, dg[0], dg[64], dg[128], pi);
-            a1 = fma3(0.0f, dg[192], dg[256], dg[320], pi);
-            a2 = fma3(0.0f, dg[384], dg[448], dg[512], pi);
-        }
+        constexpr int BS = PK ? 6 : 9;
         const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
+        const int nlow = SYM ? sd.nlower : 0;  // entries >= nlow hold no lower block in any lane
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;
         const int16_t* colp16 = A.col16 + (size_t)sb * 64 + lane;
         const int16_t* mirp16 = A.mir16 + (size_t)sb * 64 + lane;
         const int e0 = sb * 64 + lane;  // entry index of the row's entry 0
-        constexpr int BS = PK ? 6 : 9;
         const float* vp = A.val + (size_t)sb * BS * 64 + lane;
-        // mirror entry index of entry kk (a lower entry)
-        auto mirror = [&](int kk) { return I16 ? e0 + kk * 64 + int(mirp16[kk * 64]) : mirp[kk * 64]; };
-        for (int k0 = 0; k0 < w; k0 += UNROLL) {
-            int c[UNROLL], kk[UNROLL];
+        // A batch of UNROLL entries runs in three phases: (1) the column and
+        // mirror offsets of every entry, (2) every gather and block load,
+        // (3) the FMAs.  No load of phase 2 depends on another load of the
+        // batch, so a batch costs two memory round trips.  Whether an entry
+        // needs its mirror offset (kk < nlow) is wave-uniform and known
+        // before any vector load returns.
+        int kk[UNROLL], cix[UNROLL], mix[UNROLL];
+        F3 zj[UNROLL], qj[UNROLL];
+        float blk[UNROLL][BS];
+        auto load_offsets = [&](int k0) {
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
-                kk[u] = SYM ? sym_order(k0 + u, w) : k0 + u;
-                if (I16) {
-                    const int d = (k0 + u < w) ? int(colp16[kk[u] * 64]) : kNoCol16;
-                    c[u] = (d == kNoCol16) ? -1 : row + d;
-                } else {
-                    c[u] = (k0 + u < w) ? colp[kk[u] * 64] : -1;
+                const int k = k0 + u;
+                kk[u] = SYM ? sym_order(k, w) : k;
+                cix[u] = I16 ? int(kNoCol16) : -1;
+                mix[u] = 0;
+                if (k < w) {  // wave-uniform
+                    cix[u] = I16 ? int(colp16[kk[u] * 64]) : colp[kk[u] * 64];
+                    if (SYM && kk[u] < nlow) mix[u] = I16 ? int(mirp16[kk[u] * 64]) : mirp[kk[u] * 64];
                 }
             }
+        };
+        auto column = [&](int u) { return I16 ? (cix[u] == kNoCol16 ? -1 : row + cix[u]) : cix[u]; };
+        // BF: padding entries hold zero blocks and gather the row's own
+        // vectors instead of branching (k0 + u < w is wave-uniform).
+        auto load_operands = [&](int k0) {
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
-                // BF: padding entries hold zero blocks and gather the row's
-                // own vector instead of branching (k0 + u < w is wave-uniform).
-                const bool live = BF ? (k0 + u < w) : (c[u] >= 0);
-                if (live) {
-                    const int cc = (c[u] >= 0) ? c[u] : row;
-                    const F3 pj = dir3(ld3(z, cc), ld3(pold, cc), beta);
-                    if (PK) {
-                        // symmetric blocks: the lower block (row, cc) is its
-                        // mirror (cc, row) itself, which row cc's lane
-                        // streamed at the same step of its loop
-                        const float* b = (cc < row) ? A.val + val_offset(mirror(kk[u]), 6) : vp + kk[u] * 6 * 64;
-                        const float b0 = b[0], b1 = b[64], b2 = b[128], b3 = b[192], b4 = b[256], b5 = b[320];
-                        a0 = fma3(a0, b0, b1, b2, pj);
-                        a1 = fma3(a1, b1, b3, b4, pj);
-                        a2 = fma3(a2, b2, b4, b5, pj);
-                    } else if (SYM && cc < row) {
-                        // lower block (row, cc) = transpose of (cc, row), which
-                        // row cc's lane streamed at the same step of its loop
-                        const float* bt = A.val + val_offset(mirror(kk[u]), 9);
-                        a0 = fma3(a0, bt[0], bt[192], bt[384], pj);
-                        a1 = fma3(a1, bt[64], bt[256], bt[448], pj);
-                        a2 = fma3(a2, bt[128], bt[320], bt[512], pj);
+                const int c = column(u);
+                if (BF ? (k0 + u < w) : (c >= 0)) {
+                    const int cc = (c >= 0) ? c : row;
+                    zj[u] = ld3(z, cc);
+                    qj[u] = ld3(pold, cc);
+                    // the lower block (row, cc) is read from its mirror (cc, row),
+                    // which row cc's lane streams at the same step of its loop
+                    const float* b = (SYM && cc < row)
+                                         ? A.val + val_offset(I16 ? e0 + kk[u] * 64 + mix[u] : mix[u], BS)
+                                         : vp + kk[u] * BS * 64;
+#pragma unroll
+                    for (int q = 0; q < BS; ++q) blk[u][q] = b[q * 64];
+                }
+            }
+        };
+        float a0, a1, a2;
+        auto accumulate = [&](int k0) {
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                const int c = column(u);
+                if (BF ? (k0 + u < w) : (c >= 0)) {
+                    const int cc = (c >= 0) ? c : row;
+                    const F3 pj = dir3(zj[u], qj[u], beta);
+                    const float* m = blk[u];
+                    if (PK) {  // symmetric block, upper triangle
+                        a0 = fma3(a0, m[0], m[1], m[2], pj);
+                        a1 = fma3(a1, m[1], m[3], m[4], pj);
+                        a2 = fma3(a2, m[2], m[4], m[5], pj);
+                    } else if (SYM && cc < row) {  // transpose of the mirror
+                        a0 = fma3(a0, m[0], m[3], m[6], pj);
+                        a1 = fma3(a1, m[1], m[4], m[7], pj);
+                        a2 = fma3(a2, m[2], m[5], m[8], pj);
                     } else {
-                        const float* bk = vp + kk[u] * 9 * 64;
-                        a0 = fma3(a0, bk[0], bk[64], bk[128], pj);
-                        a1 = fma3(a1, bk[192], bk[256], bk[320], pj);
-                        a2 = fma3(a2, bk[384], bk[448], bk[512], pj);
+                        a0 = fma3(a0, m[0], m[1], m[2], pj);
+                        a1 = fma3(a1, m[3], m[4], m[5], pj);
+                        a2 = fma3(a2, m[6], m[7], m[8], pj);
                     }
                 }
             }
+        };
+        // The first batch is issued together with the row's own loads.
+        load_offsets(0);
+        const F3 zi = ld3(z, row), qi = ld3(pold, row);
+        float dg[BS];
+        {
+            const float* dp = A.diag + (size_t)s * BS * 64 + lane;
+#pragma unroll
+            for (int q = 0; q < BS; ++q) dg[q] = dp[q * 64];
+        }
+        load_operands(0);
+        const F3 pi = dir3(zi, qi, beta);
+        if (PK) {
+            a0 = fma3(0.0f, dg[0], dg[1], dg[2], pi);
+            a1 = fma3(0.0f, dg[1], dg[3], dg[4], pi);
+            a2 = fma3(0.0f, dg[2], dg[4], dg[5], pi);
+        } else {
+            a0 = fma3(0.0f, dg[0], dg[1], dg[2], pi);
+            a1 = fma3(0.0f, dg[3], dg[4], dg[5], pi);
+            a2 = fma3(0.0f, dg[6], dg[7], dg[8], pi);
+        }
+        accumulate(0);
+        for (int k0 = UNROLL; k0 < w; k0 += UNROLL) {
+            load_offsets(k0);
+            load_operands(k0);
+            accumulate(k0);
         }
         st3(v.ap, row, f3(a0, a1, a2));
         st3(pnew, row, pi);

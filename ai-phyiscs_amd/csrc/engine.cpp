@@ -320,6 +320,7 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     // Mirror offsets of the lower entries: block (i, j), j < i, is the
     // transpose of block (j, i) stored by row j.
     std::vector<int> mir(sell_col.size(), -1);
+    std::vector<int> nlower(size_t(nslices), 0);
     if (size_t(entries) * 9 * kSlice > size_t(INT32_MAX)) throw std::invalid_argument("matrix too large");
     for (int sl = 0; sl < nslices; ++sl) {
         const int w = slice_ptr_h_[size_t(sl) + 1] - slice_ptr_h_[size_t(sl)];
@@ -329,6 +330,7 @@ void Problem::build(const std::vector<const Instance*>& insts) {
                 const size_t e = (size_t(slice_ptr_h_[size_t(sl)]) + k) * kSlice + lane;
                 const int j = sell_col[e];
                 if (j < 0 || j >= i) continue;
+                nlower[size_t(sl)] = std::max(nlower[size_t(sl)], k + 1);  // one past the last lower position
                 const int sj = j / kSlice, lj = j % kSlice;
                 const int wj = slice_ptr_h_[size_t(sj) + 1] - slice_ptr_h_[size_t(sj)];
                 int kk = -1;
@@ -354,8 +356,8 @@ void Problem::build(const std::vector<const Instance*>& insts) {
         st[size_t(k)].group0 = inst_slice0_[size_t(k)] / kGroupSlices;
         st[size_t(k)].ngroups = inst_nslices_[size_t(k)] / kGroupSlices;
         for (int q = 0; q < inst_nslices_[size_t(k)]; ++q)
-            slices_h_.push_back(
-                SliceDesc{k, I.part0 + kSlice * q, std::max(0, std::min(kSlice, I.n - kSlice * q)), 0});
+            slices_h_.push_back(SliceDesc{k, I.part0 + kSlice * q, std::max(0, std::min(kSlice, I.n - kSlice * q)),
+                                          nlower[size_t(inst_slice0_[size_t(k)] + q)]});
         st[size_t(k)].status = kStopped;
     }
     if (int(slices_h_.size()) != nslices) throw std::logic_error("slice count mismatch");
