@@ -546,16 +546,25 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
         hipEvent_t e0, e1;
         check_hip(hipEventCreate(&e0), "hipEventCreate");
         check_hip(hipEventCreate(&e1), "hipEventCreate");
+        // 1: the SpMV pass as the solve runs it (advancing the state between the
+        // two buffers); 4: one iteration, timed as a whole chunk (its passes and
+        // closing advance) over its iteration count.
+        int pass = 0;
         auto run = [&](int n) {
             for (int i = 0; i < n; ++i) {
                 switch (which) {
                     case 0: launch_spmv(g, A, v.p0, v.ap, c.spmv_waves, s); break;
-                    case 1:
-                        launch_pcg_spmv(g, A, v, P.states(), c.spmv_unroll, s, c.spmv_branch_free, c.spmv_symmetric);
+                    case 1: {
+                        InstState* in = (pass & 1) ? P.states_b() : P.states();
+                        InstState* out = (pass & 1) ? P.states() : P.states_b();
+                        ++pass;
+                        launch_pcg_spmv(g, A, v, in, out, P.params(), c.spmv_unroll, s, c.spmv_branch_free,
+                                        c.spmv_symmetric);
                         break;
+                    }
                     case 2: launch_pcg_update(g, A, v, P.states(), s); break;
                     case 3: launch_assemble(g, P.mesh(), A, v, ext, P.params(), dt, c.assemble_waves, P.max_width(), s); break;
-                    case 4: P.launch_pcg_iteration(s); break;
+                    case 4: P.launch_pcg_chunk_body(s); break;
                 }
             }
         };
@@ -568,7 +577,7 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
         check_hip(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        *ms = double(t) / reps;
+        *ms = double(t) / reps / (which == 4 ? Problem::kChunkIters : 1);
         P.upload_params({&S.instance()}, s);
     });
 }

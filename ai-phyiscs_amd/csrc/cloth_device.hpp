@@ -34,27 +34,37 @@ enum PcgMode : int {
     kModeFlush = 2,  // x += alpha_pend p (the deferred term) before a check or a stop
 };
 
-// Reduction phases of pcg_reduce_kernel.
+// Reduction phases of pcg_reduce_kernel.  (The per-iteration reduction,
+// r . z and |r|^2 of an update pass, or the true |r|^2 and r . z of a check
+// pass, is pcg_advance: run by the next SpMV pass itself, or by
+// pcg_advance_kernel at the end of a chunk of iterations.)
 enum PcgPhase : int {
     kPhaseStart = 0,  // after the initial residual: |r|^2, |b|^2, r . z
-    kPhaseIter = 1,   // after an iteration's update pass: p . A p, r . z, |r|^2
-                      // (a check round: the true |r|^2, r . z; a flush round: none)
     kPhaseFinal = 3,  // after the report residual of a stopped solve: |r|^2
 };
 
 // Inner-product partials: one FP64 value per 16-slice group and slot,
 // partials[slot * ngroups + group].  Slots 0-2: |r|^2, |b|^2, r . z of a
-// residual pass (start, check, report) or r . z, |r|^2 of an update pass;
-// slot kSlotSigma: p . A p of an SpMV pass, apart because the update pass
-// reads it while it writes slots 0-1.
+// residual pass (start, report), or r . z, |r|^2 of an update pass (slots
+// 0-1).  A check pass (the SpMV pass in kModeCheck) writes the true |r|^2
+// and r . z to kSlotCheckRR and kSlotCheckRZ; slot kSlotSigma: p . A p of an
+// SpMV pass.  No pass writes a slot that the same pass reads: the SpMV pass
+// reads slots 0-1 (or the check slots) for its advance while it writes the
+// check slots (or sigma), and the update pass reads sigma while it writes
+// slots 0-1.
+constexpr int kSlotCheckRZ = 2;
+constexpr int kSlotCheckRR = 3;
 constexpr int kSlotSigma = 4;
 constexpr int kPartialSlots = 5;
 
 // Per-instance solver state, FP64 scalars (inner products, alpha, beta and
-// the convergence test).  Lives in device memory; written by
-// pcg_reduce_kernel (one wavefront per instance) and read by the row kernels
-// that follow it; the update pass writes only `sigma` and `alpha`, which no
-// row kernel reads.  No kernel needs cross-workgroup synchronisation.
+// the convergence test).  Lives in device memory, in two buffers during the
+// iterations: each SpMV pass reads one, advances the state by the previous
+// iteration's partials (every workgroup computes the same result), and the
+// workgroup of the instance's first group writes it to the other, which the
+// update pass and the next SpMV pass read.  The update pass writes only
+// `sigma` and `alpha`, which no other workgroup of that pass reads.  No kernel
+// needs cross-workgroup synchronisation.
 struct alignas(16) InstState {
     double rho;      // r . z
     double sigma;    // p . A p
@@ -80,7 +90,7 @@ struct alignas(16) InstState {
     int pend;
     int defer;
     int flush_to;
-    int pad;
+    int owed;  // 1: the partials of the latest SpMV/update pair are not folded into the state yet
     // alpha of the latest update pass, written there by the workgroup of the
     // instance's first group (the update pass itself never reads it)
     double alpha;

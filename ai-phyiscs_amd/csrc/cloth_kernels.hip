@@ -792,30 +792,160 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_residual_kernel(LaunchG
 // the XCD has streamed several L2 capacities.)
 __device__ __forceinline__ int sym_order(int k, int w) { return (k & 1) ? w - 1 - (k >> 1) : (k >> 1); }
 
+// ---------------------------------------------------------------------------
+// Advance of an instance's solver state by one SpMV/update pair, given the
+// sums of the partials that pair wrote (Iter: r . z, |r|^2 in tot[0], tot[1];
+// Check: the true |r|^2 and r . z in tot[kSlotCheckRR], tot[kSlotCheckRZ];
+// Flush: none).  Scalar logic restates
+// reduce_rho_beta_kernel (pcg_solver.cu:436-454) with its 1e-37 guard, plus
+// the stopping rule: true relative residual sqrt(|b-Ax|^2 / max(|b|^2,1e-30))
+// <= tol (pcg_solver.cu:595-641, 916-923), or exactly max_iter iterations
+// when tol == 0 (pcg_solver.cu:542-592).  Runs in the next SpMV pass (every
+// workgroup, identical inputs and order) or in pcg_advance_kernel.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void pcg_advance(InstState& S, const InstParams& pr,
+                                            const double (&tot)[kPartialSlots]) {
+    const double tol2 = pr.tol2;
+    if (S.mode == kModeFlush) {  // x is whole again
+        S.pend = 0;
+        S.alpha_pend = 0.0;
+        S.defer = 1;
+        if (S.flush_to == kModeCheck) {
+            S.mode = kModeCheck;
+        } else {
+            S.mode = kModeIter;
+            S.status = kStopped;
+        }
+    } else if (S.mode == kModeCheck) {
+        const double rr = tot[kSlotCheckRR];
+        S.rr = rr;
+        S.relres = sqrt(rr / S.bb);
+        if (rr <= tol2 * S.bb) {
+            S.status = kConverged;
+        } else {  // residual replacement: restart the recursion from b - A x
+            S.restarts += 1;
+            S.rho = tot[kSlotCheckRZ];
+            S.beta = 0.0;
+            S.mode = kModeIter;
+            S.defer = 1;
+            if (S.iters >= pr.max_iter) S.status = kStopped;
+        }
+    } else {
+        const double rho = S.rho, rho_new = tot[0];
+        if (S.pend) {  // this update pass added the owed term
+            S.pend = 0;
+        } else if (S.defer) {  // this update pass left x += alpha p to the next
+            S.alpha_pend = S.alpha;  // the alpha that pass used
+            S.pend = 1;
+        }
+        S.defer = S.pend ? 0 : 1;
+        S.beta = (rho > kGuard || rho < -kGuard) ? rho_new / rho : 0.0;
+        S.rho = rho_new;
+        S.rr = tot[1];
+        S.parity ^= 1;
+        S.iters += 1;
+        const bool check = tol2 > 0.0 && tot[1] <= tol2 * S.bb;
+        const bool stop = !check && S.iters >= pr.max_iter;
+        if ((check || stop) && S.pend) {  // complete x first
+            S.mode = kModeFlush;
+            S.flush_to = check ? int(kModeCheck) : int(kStopped);
+        } else if (check) {
+            S.mode = kModeCheck;
+        } else if (stop) {
+            S.status = kStopped;
+        }
+    }
+}
+
+// Sums the partials an owed advance needs (by the whole workgroup, in the
+// fixed order of block_sum; the totals land on thread 0).
+__device__ __forceinline__ void advance_sums(const double* __restrict__ partials, int ngroups, const InstState& S,
+                                             double (&tot)[kPartialSlots]) {
+    // constant slot pairs: a slot chosen at run time costs the SpMV pass
+    // registers (and occupancy)
+    if (S.mode == kModeIter) {
+        const int sl[2] = {0, 1};
+        block_sum<2, 1>(partials, ngroups, S.group0, S.ngroups, sl, tot);
+    } else if (S.mode == kModeCheck) {
+        const int sl[2] = {kSlotCheckRR, kSlotCheckRZ};
+        block_sum<2, 1>(partials, ngroups, S.group0, S.ngroups, sl, tot);
+    }
+}
+
+// The state of a workgroup's instance after the owed advance: every
+// workgroup sums the previous pair's partials in the same fixed order, so all
+// compute the same state; the workgroup of the instance's first group writes
+// it to st_out, for the update pass and the next SpMV pass (also when the
+// instance is not running, so both buffers agree once it has stopped).
+__device__ __forceinline__ int uniform_int(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double uniform_double(double x) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+    const unsigned lo = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(b))));
+    const unsigned hi = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(b >> 32))));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+__device__ __forceinline__ void advance_in_pass(const LaunchGeom& g, const DevVectors& v, InstState& S,
+                                                const InstParams* __restrict__ params, InstState* __restrict__ st_out,
+                                                int inst, int grp) {
+    const int status_in = S.status;
+    if (S.status == kRunning && S.owed) {
+        __shared__ InstState adv;
+        double tot[kPartialSlots] = {};
+        advance_sums(v.partials, g.ngroups, S, tot);
+        if (threadIdx.x == 0) {
+            pcg_advance(S, params[inst], tot);
+            adv = S;
+        }
+        __syncthreads();
+        S = adv;
+        S.status = uniform_int(S.status);
+        S.mode = uniform_int(S.mode);
+        S.parity = uniform_int(S.parity);
+        S.group0 = uniform_int(S.group0);
+        S.ngroups = uniform_int(S.ngroups);
+        S.beta = uniform_double(S.beta);
+        S.alpha_pend = uniform_double(S.alpha_pend);
+    }
+    S.owed = S.status == kRunning ? 1 : 0;  // this pass and its update pass owe the next advance
+    if (grp != S.group0) return;
+    if (threadIdx.x == 0) st_out[inst] = S;
+    // a cloth that just finished: its groups leave the iteration passes
+    if (g.gactive != nullptr && status_in == kRunning && S.status != kRunning)
+        for (int q = int(threadIdx.x); q < S.ngroups; q += kReduceThreads) g.gactive[S.group0 + q] = 0;
+}
+
+// st_out null: no advance (the state in st is current; for timing and the
+// autotuner); otherwise the pass advances st into st_out first.
 template <int UNROLL, bool BF, bool SYM, bool PK, bool I16>
 __global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                                    const InstState* __restrict__ st) {
+                                                                    const InstState* __restrict__ st,
+                                                                    InstState* __restrict__ st_out,
+                                                                    const InstParams* __restrict__ params) {
     const int s = my_slice<kGroupSlices>(g.nslices);
     if (s >= g.nslices) return;
-    if (g.gactive != nullptr && g.gactive[s / kGroupSlices] == 0) return;  // a finished cloth
+    const int grp = s / kGroupSlices;
     const SliceDesc sd = g.slices[s];
-    const InstState& S = st[sd.inst];
+    // a finished cloth (its first workgroup still copies the state)
+    if (g.gactive != nullptr && g.gactive[grp] == 0 && (st_out == nullptr || grp != st[sd.inst].group0)) return;
+    InstState S = st[sd.inst];
+    if (st_out != nullptr) advance_in_pass(g, v, S, params, st_out, sd.inst, grp);
     if (S.status != kRunning) return;
     const int parity = S.parity;
     const int lane = threadIdx.x & 63;
     const bool valid = lane < sd.nvalid;
     const int row = s * 64 + lane;
     if (S.mode == kModeCheck) {
-        double part[3] = {0.0, 0.0, 0.0};
+        // true |r|^2 and r . z, into slots no workgroup of this pass reads
+        // (the advance above read the update pass's slots 0-1)
+        double part[2] = {0.0, 0.0};
         if (valid) {
             double q[4] = {0.0, 0.0, 0.0, 0.0};
             residual_row(A, v, parity ? v.p1 : v.p0, row, false, q);
             part[0] = q[0];
-            part[1] = q[1];
-            part[2] = q[2];
+            part[1] = q[2];
         }
-        const int sl[3] = {0, 1, 2};
-        group_partials<3>(part, sl, v.partials, s / kGroupSlices, g.ngroups);
+        const int sl[2] = {kSlotCheckRR, kSlotCheckRZ};
+        group_partials<2>(part, sl, v.partials, s / kGroupSlices, g.ngroups);
         return;
     }
     if (S.mode == kModeFlush) {
@@ -1040,16 +1170,31 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_update_kernel(LaunchGeo
     else update_rows<2>(g, v, S, s, sd, r0, api, m);
 }
 
+// Stand-alone advance at the end of a chunk of iterations (one workgroup of
+// kReduceThreads per instance, the same sums as the SpMV pass), so the state
+// the host polls is up to date.  In place; clears `owed`.
+__global__ __launch_bounds__(kReduceThreads) void pcg_advance_kernel(InstState* __restrict__ st,
+                                                                     const InstParams* __restrict__ params,
+                                                                     const double* __restrict__ partials,
+                                                                     int ngroups, int* __restrict__ gactive) {
+    InstState S = st[blockIdx.x];
+    if (S.status != kRunning || !S.owed) return;
+    double tot[kPartialSlots] = {};
+    advance_sums(partials, ngroups, S, tot);
+    if (threadIdx.x != 0) return;
+    pcg_advance(S, params[blockIdx.x], tot);
+    S.owed = 0;
+    st[blockIdx.x] = S;
+    // a cloth that just finished: its groups leave the iteration passes
+    if (gactive != nullptr && S.status != kRunning)
+        for (int q = 0; q < S.ngroups; ++q) gactive[S.group0 + q] = 0;
+}
+
 // ---------------------------------------------------------------------------
-// Per-instance reduction: one wavefront per instance sums that instance's
-// group partials in a fixed order (deterministic) and advances its solver
-// state.  One reduction per iteration: the update pass computes alpha from
-// the SpMV pass's partials itself and leaves it (and p . Ap) in the state.
-// Scalar logic restates reduce_partial_kernel / reduce_rho_beta_kernel
-// (pcg_solver.cu:374-384, 436-454) with the 1e-37 guards, plus the stopping
-// rule: true relative residual sqrt(|b-Ax|^2 / max(|b|^2,1e-30)) <= tol
-// (pcg_solver.cu:595-641, 916-923), or exactly max_iter iterations when
-// tol == 0 (pcg_solver.cu:542-592).
+// Start and final reductions: one wavefront per instance sums that
+// instance's group partials in a fixed order (deterministic).  Restates
+// reduce_partial_kernel (pcg_solver.cu:374-384) for the initial r . z, and
+// the true relative residual of the stopping rule.
 // ---------------------------------------------------------------------------
 // single_ng > 0: one instance owning groups [0, single_ng) (the partial
 // loads then need no state read first); 0: bounds from the instance state.
@@ -1063,101 +1208,40 @@ __global__ __launch_bounds__(64) void pcg_reduce_kernel(InstState* __restrict__ 
     const InstParams& pr = params[blockIdx.x];
     const int g0 = single_ng > 0 ? 0 : S.group0;
     const int ng = single_ng > 0 ? single_ng : S.ngroups;
-    // Slots: start |r|^2, |b|^2, r . z; iteration r . z, |r|^2 (a check
-    // round: the true |r|^2 and r . z); final |r|^2.
+    // Slots: start |r|^2, |b|^2, r . z; final |r|^2.
     const double* base = partials + g0;
     double tot[kPartialSlots] = {};
     tot[0] = wave_group_sum(base, ng);
-    if (phase == kPhaseStart || phase == kPhaseIter) tot[1] = wave_group_sum(base + (size_t)ngroups, ng);
-    if (phase == kPhaseStart) tot[2] = wave_group_sum(base + (size_t)2 * ngroups, ng);
-    if (phase == kPhaseIter && S.status != kRunning) return;
+    if (phase == kPhaseStart) {
+        tot[1] = wave_group_sum(base + (size_t)ngroups, ng);
+        tot[2] = wave_group_sum(base + (size_t)2 * ngroups, ng);
+    }
     if (phase == kPhaseFinal && S.status != kStopped) return;
-    if (phase == kPhaseIter && S.mode == kModeCheck) tot[2] = wave_group_sum(base + (size_t)2 * ngroups, ng);
     if (phase == kPhaseStart && gactive != nullptr) {
         // the instance iterates from here on unless it is done already
         const bool run = !(pr.tol2 > 0.0 && tot[0] <= pr.tol2 * fmax(tot[1], 1.0e-30)) && pr.max_iter > 0;
         for (int q = int(threadIdx.x); q < ng; q += 64) gactive[g0 + q] = run ? 1 : 0;
     }
     if (threadIdx.x != 0) return;
-    const int status_in = S.status;
     const double tol2 = pr.tol2;
-    switch (phase) {
-        case kPhaseStart: {
-            S.bb = fmax(tot[1], 1.0e-30);
-            S.rr = tot[0];
-            S.relres = sqrt(tot[0] / S.bb);
-            S.iters = 0;
-            S.restarts = 0;
-            S.mode = kModeIter;
-            S.beta = 0.0;
-            S.rho = tot[2];
-            S.pend = 0;
-            S.alpha_pend = 0.0;
-            S.defer = 1;
-            S.status = (tol2 > 0.0 && tot[0] <= tol2 * S.bb) ? kConverged
-                       : (pr.max_iter <= 0 ? kStopped : kRunning);
-            break;
-        }
-        case kPhaseIter: {
-            if (S.mode == kModeFlush) {  // x is whole again
-                S.pend = 0;
-                S.alpha_pend = 0.0;
-                S.defer = 1;
-                if (S.flush_to == kModeCheck) {
-                    S.mode = kModeCheck;
-                } else {
-                    S.mode = kModeIter;
-                    S.status = kStopped;
-                }
-            } else if (S.mode == kModeCheck) {
-                S.rr = tot[0];
-                S.relres = sqrt(tot[0] / S.bb);
-                if (tot[0] <= tol2 * S.bb) {
-                    S.status = kConverged;
-                } else {  // residual replacement: restart the recursion from b - A x
-                    S.restarts += 1;
-                    S.rho = tot[2];
-                    S.beta = 0.0;
-                    S.mode = kModeIter;
-                    S.defer = 1;
-                    if (S.iters >= pr.max_iter) S.status = kStopped;
-                }
-            } else {
-                const double rho = S.rho, rho_new = tot[0];
-                if (S.pend) {  // this update pass added the owed term
-                    S.pend = 0;
-                } else if (S.defer) {  // this update pass left x += alpha p to the next
-                    S.alpha_pend = S.alpha;  // the alpha that pass used
-                    S.pend = 1;
-                }
-                S.defer = S.pend ? 0 : 1;
-                S.beta = (rho > kGuard || rho < -kGuard) ? rho_new / rho : 0.0;
-                S.rho = rho_new;
-                S.rr = tot[1];
-                S.parity ^= 1;
-                S.iters += 1;
-                const bool check = tol2 > 0.0 && tot[1] <= tol2 * S.bb;
-                const bool stop = !check && S.iters >= pr.max_iter;
-                if ((check || stop) && S.pend) {  // complete x first
-                    S.mode = kModeFlush;
-                    S.flush_to = check ? int(kModeCheck) : int(kStopped);
-                } else if (check) {
-                    S.mode = kModeCheck;
-                } else if (stop) {
-                    S.status = kStopped;
-                }
-            }
-            break;
-        }
-        case kPhaseFinal: {
-            S.rr = tot[0];
-            S.relres = sqrt(tot[0] / S.bb);
-            break;
-        }
+    if (phase == kPhaseStart) {
+        S.bb = fmax(tot[1], 1.0e-30);
+        S.rr = tot[0];
+        S.relres = sqrt(tot[0] / S.bb);
+        S.iters = 0;
+        S.restarts = 0;
+        S.mode = kModeIter;
+        S.beta = 0.0;
+        S.rho = tot[2];
+        S.pend = 0;
+        S.alpha_pend = 0.0;
+        S.defer = 1;
+        S.owed = 0;
+        S.status = (tol2 > 0.0 && tot[0] <= tol2 * S.bb) ? kConverged : (pr.max_iter <= 0 ? kStopped : kRunning);
+    } else {
+        S.rr = tot[0];
+        S.relres = sqrt(tot[0] / S.bb);
     }
-    // a cloth that just finished: its groups leave the iteration passes
-    if (phase == kPhaseIter && gactive != nullptr && status_in == kRunning && S.status != kRunning)
-        for (int q = 0; q < ng; ++q) gactive[g0 + q] = 0;
 }
 
 // finalize_step_kernel (cloth_simulator.cu:222-244): x = x_n + dx,
@@ -1288,37 +1372,42 @@ void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVecto
 }
 namespace {
 template <bool BF, bool SYM, bool PK, bool I16>
-void pcg_spmv_launch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int unroll,
-                     hipStream_t s) {
+void pcg_spmv_launch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                     InstState* st_out, const InstParams* params, int unroll, hipStream_t s) {
     const int grid = grid_for(g.nslices, kGroupSlices), block = 64 * kGroupSlices;
-    if (unroll == 1) pcg_spmv_kernel<1, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st);
-    else if (unroll == 2) pcg_spmv_kernel<2, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st);
-    else pcg_spmv_kernel<4, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st);
+    if (unroll == 1) pcg_spmv_kernel<1, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st, st_out, params);
+    else if (unroll == 2) pcg_spmv_kernel<2, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st, st_out, params);
+    else pcg_spmv_kernel<4, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st, st_out, params);
 }
 template <bool BF, bool SYM, bool PK>
-void pcg_spmv_dispatch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int unroll,
-                       hipStream_t s) {
-    if (A.col16 != nullptr && A.mir16 != nullptr) pcg_spmv_launch<BF, SYM, PK, true>(g, A, v, st, unroll, s);
-    else pcg_spmv_launch<BF, SYM, PK, false>(g, A, v, st, unroll, s);
+void pcg_spmv_dispatch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                       InstState* st_out, const InstParams* params, int unroll, hipStream_t s) {
+    if (A.col16 != nullptr && A.mir16 != nullptr)
+        pcg_spmv_launch<BF, SYM, PK, true>(g, A, v, st, st_out, params, unroll, s);
+    else
+        pcg_spmv_launch<BF, SYM, PK, false>(g, A, v, st, st_out, params, unroll, s);
 }
 }  // namespace
 
-void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int unroll,
-                     hipStream_t s, int branch_free, int symmetric) {
+void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                     InstState* st_out, const InstParams* params, int unroll, hipStream_t s, int branch_free,
+                     int symmetric) {
     if (g.nslices <= 0) return;
     if (!valid_unroll(unroll)) throw std::invalid_argument("unsupported unroll");
     if (g.nslices % kGroupSlices != 0 || g.ngroups * kGroupSlices != g.nslices)
         throw std::invalid_argument("PCG passes: slices must form whole 16-slice groups");
     if (symmetric != A.sym) throw std::invalid_argument("symmetric SpMV: launch flag and matrix mode differ");
     if (A.bs != 9 && !(A.bs == 6 && A.sym)) throw std::invalid_argument("packed blocks need the symmetric SpMV");
+    if (st_out != nullptr && (params == nullptr || st_out == st))
+        throw std::invalid_argument("SpMV pass: the advanced state needs the parameters and a second buffer");
     if (branch_free) {
-        if (A.bs == 6) pcg_spmv_dispatch<true, true, true>(g, A, v, st, unroll, s);
-        else if (symmetric) pcg_spmv_dispatch<true, true, false>(g, A, v, st, unroll, s);
-        else pcg_spmv_dispatch<true, false, false>(g, A, v, st, unroll, s);
+        if (A.bs == 6) pcg_spmv_dispatch<true, true, true>(g, A, v, st, st_out, params, unroll, s);
+        else if (symmetric) pcg_spmv_dispatch<true, true, false>(g, A, v, st, st_out, params, unroll, s);
+        else pcg_spmv_dispatch<true, false, false>(g, A, v, st, st_out, params, unroll, s);
     } else {
-        if (A.bs == 6) pcg_spmv_dispatch<false, true, true>(g, A, v, st, unroll, s);
-        else if (symmetric) pcg_spmv_dispatch<false, true, false>(g, A, v, st, unroll, s);
-        else pcg_spmv_dispatch<false, false, false>(g, A, v, st, unroll, s);
+        if (A.bs == 6) pcg_spmv_dispatch<false, true, true>(g, A, v, st, st_out, params, unroll, s);
+        else if (symmetric) pcg_spmv_dispatch<false, true, false>(g, A, v, st, st_out, params, unroll, s);
+        else pcg_spmv_dispatch<false, false, false>(g, A, v, st, st_out, params, unroll, s);
     }
     CHYSX_LAUNCH_CHECK();
 }
@@ -1330,9 +1419,16 @@ void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors
     pcg_update_kernel<<<grid_for(g.nslices, kGroupSlices), 64 * kGroupSlices, 0, s>>>(g, A, v, st);
     CHYSX_LAUNCH_CHECK();
 }
+void launch_pcg_advance(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
+                        hipStream_t s, int* gactive) {
+    if (ninst <= 0) return;
+    pcg_advance_kernel<<<ninst, kReduceThreads, 0, s>>>(st, params, partials, ngroups, gactive);
+    CHYSX_LAUNCH_CHECK();
+}
 void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
                        int phase, hipStream_t s, int* gactive) {
     if (ninst <= 0) return;
+    if (phase != kPhaseStart && phase != kPhaseFinal) throw std::invalid_argument("unknown reduction phase");
     pcg_reduce_kernel<<<ninst, 64, 0, s>>>(st, params, partials, ngroups, phase, ninst == 1 ? ngroups : 0,
                                            gactive);
     CHYSX_LAUNCH_CHECK();

@@ -100,8 +100,8 @@ struct LaunchGeom {
     int nslices;  // a multiple of kGroupSlices
     int ngroups;  // nslices / kGroupSlices
     // Batches: 1 for the groups of instances still iterating, else 0 (kept
-    // by pcg_reduce_kernel), so the iteration passes drop a finished cloth's
-    // workgroups after one load.  Null for a single instance.
+    // by the start reduction and the advances), so the iteration passes drop
+    // a finished cloth's workgroups after one load.  Null for a single instance.
     int* gactive;
 };
 
@@ -125,17 +125,26 @@ void launch_rest_angles(const int4* dih, const float* pos, float* rest, int n, h
 void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
                          int report, hipStream_t s);
 // SpMV pass (direction update, A p, p . A p; the true-residual check; or the
-// flush of a deferred iterate term).
-void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st, int unroll,
-                     hipStream_t s, int branch_free = 0, int symmetric = 0);
+// flush of a deferred iterate term).  With st_out (not st): the pass first
+// advances the state in st by the previous iteration's partials (when owed)
+// and writes it to st_out, which its update pass and the next SpMV pass read.
+// Without: the state in st is current (timing, autotuner).
+void launch_pcg_spmv(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                     InstState* st_out, const InstParams* params, int unroll, hipStream_t s, int branch_free = 0,
+                     int symmetric = 0);
 // Update pass (alpha from the SpMV pass's partials; x, r, z; r . z, r . r).
 void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, InstState* st,
                        hipStream_t s);
-// One workgroup per instance: finish the reductions of `phase` (PcgPhase).
+// One wavefront per instance: the start or final reduction (PcgPhase).
 // gactive (batches; may be null): per-group 1/0 flags of running instances,
-// set at kPhaseStart and cleared when an instance finishes.
+// set at kPhaseStart and cleared when an instance finishes (by the SpMV pass
+// or launch_pcg_advance).
 void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
                        int phase, hipStream_t s, int* gactive = nullptr);
+// One workgroup per instance: the owed advance of the state in st, in place
+// (the end of a chunk of iterations, so the polled state is current).
+void launch_pcg_advance(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
+                        hipStream_t s, int* gactive = nullptr);
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
                      int waves, hipStream_t s);
 void launch_area_mass(const int* vt_off, const int* vt_ids, const int* tri_v, const float* pos, float density_over_3,

@@ -366,6 +366,7 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     hipStream_t s = nullptr;
     slices_.assign(slices_h_.data(), slices_h_.size(), s);
     state_.assign(st.data(), st.size(), s);
+    state_b_.assign(st.data(), st.size(), s);
     tri_v_.assign(tri_v.data(), tri_v.size(), s);
     tri_dminv_.assign(reinterpret_cast<const float4*>(tri_dminv.data()), tri_dminv.size() / 4, s);
     tri_area_.assign(tri_area.data(), tri_area.size(), s);
@@ -577,13 +578,20 @@ void Problem::launch_pcg_start(hipStream_t s) {
     launch_pcg_residual(g, matrix(), vectors(), state_.data(), 0, s);
     launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, g.ngroups, kPhaseStart, s, g.gactive);
 }
-// One iteration: SpMV pass, update pass (it sums p . Ap itself), reduction.
-void Problem::launch_pcg_iteration(hipStream_t s) {
+// One iteration: the SpMV pass (it first folds the previous iteration's
+// partials into the state) and the update pass (it sums p . Ap itself).
+void Problem::launch_pcg_iteration(hipStream_t s, int i) {
     const LaunchGeom g = geom();
-    launch_pcg_spmv(g, matrix(), vectors(), state_.data(), cfg_.spmv_unroll, s, cfg_.spmv_branch_free,
+    InstState* in = (i & 1) ? state_b_.data() : state_.data();
+    InstState* out = (i & 1) ? state_.data() : state_b_.data();
+    launch_pcg_spmv(g, matrix(), vectors(), in, out, params_.data(), cfg_.spmv_unroll, s, cfg_.spmv_branch_free,
                     cfg_.spmv_symmetric);
-    launch_pcg_update(g, matrix(), vectors(), state_.data(), s);
-    launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, g.ngroups, kPhaseIter, s, g.gactive);
+    launch_pcg_update(g, matrix(), vectors(), out, s);
+}
+void Problem::launch_pcg_chunk_body(hipStream_t s) {
+    static_assert(kChunkIters % 2 == 0, "a chunk must end in the first state buffer");
+    for (int i = 0; i < kChunkIters; ++i) launch_pcg_iteration(s, i);
+    launch_pcg_advance(state_.data(), params_.data(), partials_.data(), ninst_, geom().ngroups, s, geom().gactive);
 }
 
 void Problem::set_launch_config(const LaunchConfig& c) {
@@ -621,6 +629,7 @@ __global__ void prime_states_kernel(InstState* st, int n) {
         st[i].beta = 0.5;
         st[i].pend = 0;
         st[i].defer = 0;
+        st[i].owed = 0;
         st[i].alpha_pend = 0.0;
     }
 }
@@ -674,8 +683,8 @@ void Problem::autotune(hipStream_t s) {
     tune_log_.clear();
     for (int bf = 0; bf < 2; ++bf)
         for (int u : unroll_list) {
-            const double t =
-                time_it([&] { launch_pcg_spmv(geom(), matrix(), v, sst.data(), u, s, bf, cfg_.spmv_symmetric); });
+            const double t = time_it(
+                [&] { launch_pcg_spmv(geom(), matrix(), v, sst.data(), nullptr, nullptr, u, s, bf, cfg_.spmv_symmetric); });
             tune_log_.push_back({bf ? 2 : 0, kGroupSlices, u, t});
             if (t < best) {
                 best = t;
@@ -693,7 +702,7 @@ void Problem::capture_chunk_graph() {
     hipGraph_t graph = nullptr;
     check_hip(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
-        for (int i = 0; i < kChunkIters; ++i) launch_pcg_iteration(cap_stream_);
+        launch_pcg_chunk_body(cap_stream_);
     } catch (...) {
         (void)hipStreamEndCapture(cap_stream_, &graph);
         if (graph) (void)hipGraphDestroy(graph);

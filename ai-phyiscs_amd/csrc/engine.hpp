@@ -195,7 +195,11 @@ public:
     int index_bits() const { return (cfg_.spmv_index16 && col16_.size() > 0) ? 16 : 32; }
     DevVectors vectors() const;
     DevMesh mesh() const;
+    // The solver state; a second buffer carries it between the passes of a
+    // chunk (iteration i reads buffer i % 2 and writes the other; a chunk has
+    // an even number of iterations, so it ends in states()).
     InstState* states() const { return state_.data(); }
+    InstState* states_b() const { return state_b_.data(); }
     const InstParams* params() const { return params_.data(); }
     // Bytes of the algorithmic SpMV traffic (reference formulation):
     // 40 per off-diagonal block + 64 per row + 4.
@@ -207,8 +211,12 @@ public:
     // Launch pieces (used by step and by the benchmark).
     void launch_assemble(float dt, const DevExt& ext, hipStream_t s);
     void launch_pcg_start(hipStream_t s);
-    void launch_pcg_iteration(hipStream_t s);  // SpMV pass, reduce, update pass, reduce
-    void launch_pcg_chunk(hipStream_t s);  // kChunk iterations (graph when possible)
+    // Iteration i of a chunk: SpMV pass (advancing the state from buffer i % 2
+    // into the other) and update pass.
+    void launch_pcg_iteration(hipStream_t s, int i);
+    // kChunkIters iterations and the closing advance, launched directly.
+    void launch_pcg_chunk_body(hipStream_t s);
+    void launch_pcg_chunk(hipStream_t s);  // the same, as a replayed graph
     void launch_finalize(float dt, const DevExt& ext, hipStream_t s);
     void read_states(hipStream_t s);        // blocking
     bool all_done() const;
@@ -267,7 +275,7 @@ private:
     std::vector<int> row_deg_h_;
 
     DeviceArray<SliceDesc> slices_;
-    DeviceArray<InstState> state_;
+    DeviceArray<InstState> state_, state_b_;
     DeviceArray<InstParams> params_;
     // mesh
     DeviceArray<int> tri_v_, spr_v_, pin_v_;

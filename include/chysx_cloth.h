@@ -214,9 +214,11 @@ int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts
 /* ---- kernel-level benchmark hooks -------------------------------------- */
 /* Time `reps` launches of one kernel of the last assembled system on `stream`
  * with HIP events recorded on that stream; returns the mean ms per launch.
- * which: 0 = plain SpMV y = A x, 1 = PCG SpMV (direction + A p + p.Ap),
- * 2 = PCG update, 3 = assembly, 4 = one full PCG iteration (SpMV pass,
- * its reduction, update pass, its reduction).
+ * which: 0 = plain SpMV y = A x, 1 = PCG SpMV pass as the solve runs it
+ * (advance of the solver state by the previous iteration's partials,
+ * direction + A p + p.Ap), 2 = PCG update, 3 = assembly, 4 = one full PCG
+ * iteration (SpMV pass and update pass, plus its share of the advance that
+ * closes each chunk of iterations; timed over whole chunks).
  * The solver state is left unusable for warm starts; call before a step. */
 int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uintptr_t stream, double* ms);
 /* Kernel launch shapes and matrix storage.
