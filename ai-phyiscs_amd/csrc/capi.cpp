@@ -527,7 +527,7 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
         need(sim, "sim");
         need(ms, "ms");
         if (reps <= 0) throw std::invalid_argument("time_kernel: reps must be positive");
-        if (which < 0 || which > 4) throw std::invalid_argument("time_kernel: unknown kernel id");
+        if (which < 0 || which > 5) throw std::invalid_argument("time_kernel: unknown kernel id");
         ClothSim& S = sim->sim;
         if (!S.problem().built()) throw std::runtime_error("time_kernel: step() once before timing");
         Problem& P = S.problem();
@@ -548,7 +548,8 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
         check_hip(hipEventCreate(&e1), "hipEventCreate");
         // 1: the SpMV pass as the solve runs it (advancing the state between the
         // two buffers); 4: one iteration, timed as a whole chunk (its passes and
-        // closing advance) over its iteration count.
+        // closing advance) over its iteration count; 5: the SpMV pass without
+        // the advance.
         int pass = 0;
         auto run = [&](int n) {
             for (int i = 0; i < n; ++i) {
@@ -565,6 +566,10 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
                     case 2: launch_pcg_update(g, A, v, P.states(), s); break;
                     case 3: launch_assemble(g, P.mesh(), A, v, ext, P.params(), dt, c.assemble_waves, P.max_width(), s); break;
                     case 4: P.launch_pcg_chunk_body(s); break;
+                    case 5:
+                        launch_pcg_spmv(g, A, v, P.states(), nullptr, nullptr, c.spmv_unroll, s, c.spmv_branch_free,
+                                        c.spmv_symmetric);
+                        break;
                 }
             }
         };

@@ -13,8 +13,9 @@
 // Inner products: the PCG passes run one workgroup of 16 wavefronts per
 // 16-slice group; each wave sums its lanes (butterfly), the group adds its
 // 16 waves in order and writes one FP64 partial per slot.  The consumers
-// (the update pass for p . Ap, pcg_reduce_kernel for the rest) add an
-// instance's group partials in one fixed order.  The reduction order
+// (the update pass for p . Ap; the next SpMV pass, pcg_advance_kernel and
+// pcg_reduce_kernel for the rest) add an instance's group partials in one
+// fixed order.  The reduction order
 // therefore depends neither on the launch shape nor on scheduling, and every
 // result is bitwise reproducible.
 
@@ -1105,7 +1106,7 @@ __device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVector
         x2 = x[2];
     }
     if (X == 1) pp = ld3(pprev, row);
-    // sigma = p . Ap of this instance, summed exactly as pcg_reduce_kernel sums it
+    // sigma = p . Ap of this instance, in the fixed order of every per-iteration sum
     __shared__ double sig_sh;
     {
         double tot[kPartialSlots] = {};

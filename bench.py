@@ -344,6 +344,7 @@ def main():
         g.sim.set_launch_config(dict(spmv_symmetric=sym, spmv_packed=packed))
         g.sim.time_kernel(3, 1, s.dt, stream)  # one assembly into this storage
         t_sp = g.sim.time_kernel(1, args.kernel_reps, s.dt, stream)
+        t_na = g.sim.time_kernel(5, args.kernel_reps, s.dt, stream)
         t_it = g.sim.time_kernel(4, args.kernel_reps, s.dt, stream)
         lb = spmv_layout_bytes(n, nnz, name, g.sim.index_bits())
         storage[name] = {"block_floats": g.sim.block_floats(), "index_bits": g.sim.index_bits(),
@@ -351,6 +352,9 @@ def main():
                          "GB/s_layout": round(gbs(lb, t_sp), 1),
                          "frac_layout": round(gbs(lb, t_sp) / PEAK_HBM_GBS, 4),
                          "GB/s_ref_formulation": round(gbs(kb["pcg_spmv"], t_sp), 1),
+                         # the same pass without the solver-state advance it runs first
+                         "ms_no_advance": round(t_na, 5),
+                         "frac_layout_no_advance": round(gbs(lb, t_na) / PEAK_HBM_GBS, 4),
                          "iteration_ms": round(t_it, 5)}
         if name == "general":
             t_spmv = g.sim.time_kernel(0, args.kernel_reps, s.dt, stream)  # plain SpMV reads general storage
@@ -364,7 +368,7 @@ def main():
     roofline = {
         "bound": "hbm",
         "kernel": f"pcg_spmv_kernel, {mode0} storage, {storage[mode0]['index_bits']}-bit indices "
-                  "(p = z + beta p_prev; Ap = A p; p.Ap)",
+                  "(solver-state advance; p = z + beta p_prev; Ap = A p; p.Ap)",
         "achieved": round(gbs(lb0, t_pcg_spmv), 1),
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",

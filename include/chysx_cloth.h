@@ -218,7 +218,8 @@ int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts
  * (advance of the solver state by the previous iteration's partials,
  * direction + A p + p.Ap), 2 = PCG update, 3 = assembly, 4 = one full PCG
  * iteration (SpMV pass and update pass, plus its share of the advance that
- * closes each chunk of iterations; timed over whole chunks).
+ * closes each chunk of iterations; timed over whole chunks), 5 = the PCG
+ * SpMV pass without the advance (the state is taken as current).
  * The solver state is left unusable for warm starts; call before a step. */
 int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uintptr_t stream, double* ms);
 /* Kernel launch shapes and matrix storage.
