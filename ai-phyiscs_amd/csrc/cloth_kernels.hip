@@ -620,13 +620,14 @@ constexpr int kReduceThreads = 64 * kGroupSlices;
 // Sums slots slot[0..NS) of an instance's group partials (groups g0 ..
 // g0 + ng - 1, `ld` groups per slot).  Thread t adds groups t, t + T, t + 2T,
 // ... in order; the waves' sums are then combined in wave order, so the
-// result is one fixed sum whatever KU is (every workgroup of the update pass
-// computes the same alpha).  All NS x KU loads of a round are issued before any
-// of them is added.  The total of slot slot[j] lands in tot[slot[j]] on
-// thread 0 only.
+// result is one fixed sum whatever KU is (every workgroup of a pass computes
+// the same alpha or state).  All NS x KU loads of a round are issued before
+// any of them is added.  The total of slot slot[j] lands in out[j] in every
+// thread: after one barrier each thread adds the 16 wave sums itself, in wave
+// order.  One call per kernel (the wave sums' LDS is not reused).
 template <int NS, int KU>
 __device__ __forceinline__ void block_sum(const double* __restrict__ partials, int ld, int s0, int ns,
-                                          const int (&slot)[NS], double (&tot)[kPartialSlots]) {
+                                          const int (&slot)[NS], double (&out)[NS]) {
     double acc[NS];
 #pragma unroll
     for (int j = 0; j < NS; ++j) acc[j] = 0.0;
@@ -666,12 +667,13 @@ __device__ __forceinline__ void block_sum(const double* __restrict__ partials, i
         for (int j = 0; j < NS; ++j) sh[threadIdx.x >> 6][j] = acc[j];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 0; w < kReduceThreads / 64; ++w)
 #pragma unroll
-            for (int j = 0; j < NS; ++j) tot[slot[j]] += sh[w][j];
+    for (int j = 0; j < NS; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kReduceThreads / 64; ++w) t += sh[w][j];
+        out[j] = t;
     }
-    __syncthreads();  // sh may be refilled by a second call
 }
 
 // Sum of ng group partials p[0 .. ng) by one wavefront in a fixed order:
@@ -795,17 +797,19 @@ __device__ __forceinline__ int sym_order(int k, int w) { return (k & 1) ? w - 1 
 
 // ---------------------------------------------------------------------------
 // Advance of an instance's solver state by one SpMV/update pair, given the
-// sums of the partials that pair wrote (Iter: r . z, |r|^2 in tot[0], tot[1];
-// Check: the true |r|^2 and r . z in tot[kSlotCheckRR], tot[kSlotCheckRZ];
-// Flush: none).  Scalar logic restates
+// sums of the partials that pair wrote: r . z and |r|^2 of the update pass
+// (Iter), or the true ones of the check pass (Check); none for a flush.  Scalar logic restates
 // reduce_rho_beta_kernel (pcg_solver.cu:436-454) with its 1e-37 guard, plus
 // the stopping rule: true relative residual sqrt(|b-Ax|^2 / max(|b|^2,1e-30))
 // <= tol (pcg_solver.cu:595-641, 916-923), or exactly max_iter iterations
 // when tol == 0 (pcg_solver.cu:542-592).  Runs in the next SpMV pass (every
 // workgroup, identical inputs and order) or in pcg_advance_kernel.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void pcg_advance(InstState& S, const InstParams& pr,
-                                            const double (&tot)[kPartialSlots]) {
+struct AdvanceSums {
+    double rz;  // r . z
+    double rr;  // |r|^2
+};
+__device__ __forceinline__ void pcg_advance(InstState& S, const InstParams& pr, const AdvanceSums& t) {
     const double tol2 = pr.tol2;
     if (S.mode == kModeFlush) {  // x is whole again
         S.pend = 0;
@@ -818,21 +822,21 @@ __device__ __forceinline__ void pcg_advance(InstState& S, const InstParams& pr,
             S.status = kStopped;
         }
     } else if (S.mode == kModeCheck) {
-        const double rr = tot[kSlotCheckRR];
+        const double rr = t.rr;
         S.rr = rr;
         S.relres = sqrt(rr / S.bb);
         if (rr <= tol2 * S.bb) {
             S.status = kConverged;
         } else {  // residual replacement: restart the recursion from b - A x
             S.restarts += 1;
-            S.rho = tot[kSlotCheckRZ];
+            S.rho = t.rz;
             S.beta = 0.0;
             S.mode = kModeIter;
             S.defer = 1;
             if (S.iters >= pr.max_iter) S.status = kStopped;
         }
     } else {
-        const double rho = S.rho, rho_new = tot[0];
+        const double rho = S.rho, rho_new = t.rz;
         if (S.pend) {  // this update pass added the owed term
             S.pend = 0;
         } else if (S.defer) {  // this update pass left x += alpha p to the next
@@ -842,10 +846,10 @@ __device__ __forceinline__ void pcg_advance(InstState& S, const InstParams& pr,
         S.defer = S.pend ? 0 : 1;
         S.beta = (rho > kGuard || rho < -kGuard) ? rho_new / rho : 0.0;
         S.rho = rho_new;
-        S.rr = tot[1];
+        S.rr = t.rr;
         S.parity ^= 1;
         S.iters += 1;
-        const bool check = tol2 > 0.0 && tot[1] <= tol2 * S.bb;
+        const bool check = tol2 > 0.0 && t.rr <= tol2 * S.bb;
         const bool stop = !check && S.iters >= pr.max_iter;
         if ((check || stop) && S.pend) {  // complete x first
             S.mode = kModeFlush;
@@ -859,18 +863,20 @@ __device__ __forceinline__ void pcg_advance(InstState& S, const InstParams& pr,
 }
 
 // Sums the partials an owed advance needs (by the whole workgroup, in the
-// fixed order of block_sum; the totals land on thread 0).
-__device__ __forceinline__ void advance_sums(const double* __restrict__ partials, int ngroups, const InstState& S,
-                                             double (&tot)[kPartialSlots]) {
+// fixed order of block_sum; every thread gets the totals).
+__device__ __forceinline__ AdvanceSums advance_sums(const double* __restrict__ partials, int ngroups,
+                                                    const InstState& S) {
     // constant slot pairs: a slot chosen at run time costs the SpMV pass
     // registers (and occupancy)
+    double t[2] = {0.0, 0.0};
     if (S.mode == kModeIter) {
         const int sl[2] = {0, 1};
-        block_sum<2, 1>(partials, ngroups, S.group0, S.ngroups, sl, tot);
+        block_sum<2, 1>(partials, ngroups, S.group0, S.ngroups, sl, t);
     } else if (S.mode == kModeCheck) {
-        const int sl[2] = {kSlotCheckRR, kSlotCheckRZ};
-        block_sum<2, 1>(partials, ngroups, S.group0, S.ngroups, sl, tot);
+        const int sl[2] = {kSlotCheckRZ, kSlotCheckRR};
+        block_sum<2, 1>(partials, ngroups, S.group0, S.ngroups, sl, t);
     }
+    return AdvanceSums{t[0], t[1]};
 }
 
 // The state of a workgroup's instance after the owed advance: every
@@ -890,20 +896,14 @@ __device__ __forceinline__ void advance_in_pass(const LaunchGeom& g, const DevVe
                                                 int inst, int grp) {
     const int status_in = S.status;
     if (S.status == kRunning && S.owed) {
-        __shared__ InstState adv;
-        double tot[kPartialSlots] = {};
-        advance_sums(v.partials, g.ngroups, S, tot);
-        if (threadIdx.x == 0) {
-            pcg_advance(S, params[inst], tot);
-            adv = S;
-        }
-        __syncthreads();
-        S = adv;
+        // every thread holds the same sums and applies the same step
+        AdvanceSums t = advance_sums(v.partials, g.ngroups, S);
+        t.rz = uniform_double(t.rz);
+        t.rr = uniform_double(t.rr);
+        pcg_advance(S, params[inst], t);
         S.status = uniform_int(S.status);
         S.mode = uniform_int(S.mode);
         S.parity = uniform_int(S.parity);
-        S.group0 = uniform_int(S.group0);
-        S.ngroups = uniform_int(S.ngroups);
         S.beta = uniform_double(S.beta);
         S.alpha_pend = uniform_double(S.alpha_pend);
     }
@@ -917,8 +917,13 @@ __device__ __forceinline__ void advance_in_pass(const LaunchGeom& g, const DevVe
 
 // st_out null: no advance (the state in st is current; for timing and the
 // autotuner); otherwise the pass advances st into st_out first.
+// Scalar registers are capped at 80: measured on MI355X, this 16-wave
+// workgroup at 85 scalar registers runs one workgroup per CU instead of two
+// (the SpMV pass 33.4 instead of 27.4 us at 1024^2), although the compiler's
+// occupancy estimate still reports 8 waves per SIMD.  At 80 the compiler
+// spills a few scalars into vector-register lanes.
 template <int UNROLL, bool BF, bool SYM, bool PK, bool I16>
-__global__ __launch_bounds__(64 * kGroupSlices) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
+__global__ __launch_bounds__(64 * kGroupSlices) __attribute__((amdgpu_num_sgpr(80))) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
                                                                     const InstState* __restrict__ st,
                                                                     InstState* __restrict__ st_out,
                                                                     const InstParams* __restrict__ params) {
@@ -1107,15 +1112,10 @@ __device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVector
     }
     if (X == 1) pp = ld3(pprev, row);
     // sigma = p . Ap of this instance, in the fixed order of every per-iteration sum
-    __shared__ double sig_sh;
-    {
-        double tot[kPartialSlots] = {};
-        const int sl[1] = {kSlotSigma};
-        block_sum<1, 4>(v.partials, g.ngroups, S.group0, S.ngroups, sl, tot);
-        if (threadIdx.x == 0) sig_sh = tot[kSlotSigma];
-        __syncthreads();
-    }
-    const double sig = sig_sh, rho = S.rho;
+    double sum[1];
+    const int sig_slot[1] = {kSlotSigma};
+    block_sum<1, 4>(v.partials, g.ngroups, S.group0, S.ngroups, sig_slot, sum);
+    const double sig = uniform_double(sum[0]), rho = S.rho;
     const double alpha = (sig > kGuard || sig < -kGuard) ? rho / sig : 0.0;
     const float af = float(alpha);
     if (threadIdx.x == 0 && s / kGroupSlices == S.group0) {  // one writer per instance
@@ -1180,10 +1180,9 @@ __global__ __launch_bounds__(kReduceThreads) void pcg_advance_kernel(InstState* 
                                                                      int ngroups, int* __restrict__ gactive) {
     InstState S = st[blockIdx.x];
     if (S.status != kRunning || !S.owed) return;
-    double tot[kPartialSlots] = {};
-    advance_sums(partials, ngroups, S, tot);
+    const AdvanceSums t = advance_sums(partials, ngroups, S);
     if (threadIdx.x != 0) return;
-    pcg_advance(S, params[blockIdx.x], tot);
+    pcg_advance(S, params[blockIdx.x], t);
     S.owed = 0;
     st[blockIdx.x] = S;
     // a cloth that just finished: its groups leave the iteration passes
