@@ -209,6 +209,13 @@ __device__ __forceinline__ void add_block(float* __restrict__ acc, int k, const 
     }
 }
 
+// One of three values, chosen by an index known only at run time, as a chain
+// of selects over values.  (A per-thread array indexed at run time would be
+// placed in scratch memory, which every launch of the kernel then sets up;
+// selects over the elements of such an array are folded back into an indexed
+// load, so the element sets here are named values, not arrays.)
+__device__ __forceinline__ float sel3(int i, float v0, float v1, float v2) { return i == 0 ? v0 : i == 1 ? v1 : v2; }
+
 // One Baraff-Witkin triangle (stretch or shear, by its D_m^-1) seen from its
 // vertex a: gradient into `grad`, the three blocks H_ab into D (b == a or a
 // repeated vertex) or the row accumulator (triangle_gradient_kernel and
@@ -225,9 +232,10 @@ __device__ __forceinline__ void tri_element(const float4 Dm, float keff, int a, 
     const float inv_lu = 1.0f / lu, inv_lv = 1.0f / lv;
     const float cu = lu - 1.0f, cv = lv - 1.0f;
     const F3 uh = wu * inv_lu, vh = wv * inv_lv;
-    const float dwu[3] = {-Dm.x - Dm.z, Dm.x, Dm.z};
-    const float dwv[3] = {-Dm.y - Dm.w, Dm.y, Dm.w};
-    grad = grad + (uh * (cu * dwu[a]) + vh * (cv * dwv[a])) * keff;
+    const float dwu0 = -Dm.x - Dm.z, dwu1 = Dm.x, dwu2 = Dm.z;
+    const float dwv0 = -Dm.y - Dm.w, dwv1 = Dm.y, dwv2 = Dm.w;
+    const float dwu_a = sel3(a, dwu0, dwu1, dwu2), dwv_a = sel3(a, dwv0, dwv1, dwv2);
+    grad = grad + (uh * (cu * dwu_a) + vh * (cv * dwv_a)) * keff;
     // H_ab = dwu_a dwu_b Mu + dwv_a dwv_b Mv with
     // Mu = keff uh uh^T + [cu > 0] keff cu / lu (I - uh uh^T), Mv likewise.
     const float su = (cu > 0.0f) ? keff * cu * inv_lu : 0.0f;
@@ -248,7 +256,7 @@ __device__ __forceinline__ void tri_element(const float4 Dm, float keff, int a, 
     Mv[5] = qv * vh.z * vh.z + sv;
 #pragma unroll
     for (int bb = 0; bb < 3; ++bb) {
-        const float cu_ab = dwu[a] * dwu[bb], cv_ab = dwv[a] * dwv[bb];
+        const float cu_ab = dwu_a * sel3(bb, dwu0, dwu1, dwu2), cv_ab = dwv_a * sel3(bb, dwv0, dwv1, dwv2);
         float H[9];
         H[0] = cu_ab * Mu[0] + cv_ab * Mv[0];
         H[1] = cu_ab * Mu[1] + cv_ab * Mv[1];
@@ -274,8 +282,19 @@ __device__ __forceinline__ void tri_element(const float4 Dm, float keff, int a, 
 // height (floor 1e-12).
 struct DihFrame {
     float theta;
-    F3 a[4];
+    F3 a0, a1, a2, a3;  // gradient of theta with respect to vertex 0..3
     bool valid;
+    // a_i for an index known at compile time (after unrolling)
+    __device__ __forceinline__ F3 a(int i) const { return i == 0 ? a0 : i == 1 ? a1 : i == 2 ? a2 : a3; }
+    // a_i for an index known only at run time, as a sum with one-hot weights
+    // (exact for a valid frame, whose vectors are finite): a select here is
+    // folded into a load at a run-time offset, which puts the frame in
+    // scratch memory.
+    __device__ __forceinline__ F3 a_dyn(int i) const {
+        const float w0 = i == 0 ? 1.0f : 0.0f, w1 = i == 1 ? 1.0f : 0.0f;
+        const float w2 = i == 2 ? 1.0f : 0.0f, w3 = i == 3 ? 1.0f : 0.0f;
+        return a0 * w0 + a1 * w1 + a2 * w2 + a3 * w3;
+    }
 };
 __device__ __forceinline__ F3 cross(F3 a, F3 b) {
     return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
@@ -285,6 +304,7 @@ __device__ __forceinline__ DihFrame dihedral_frame(F3 v0, F3 v1, F3 v2, F3 v3) {
     DihFrame F;
     F.valid = false;
     F.theta = 0.0f;
+    F.a0 = F.a1 = F.a2 = F.a3 = f3(0.0f, 0.0f, 0.0f);
     const F3 e10 = v1 - v0, e20 = v2 - v0, e30 = v3 - v0;
     const float l = sqrtf(dot(e10, e10));
     if (l < kFloor) return F;
@@ -304,10 +324,10 @@ __device__ __forceinline__ DihFrame dihedral_frame(F3 v0, F3 v1, F3 v2, F3 v3) {
     const float h1 = sqrtf(dot(p1, p1)), h2 = sqrtf(dot(p2, p2));
     if (h1 < kFloor || h2 < kFloor) return F;
     const float ih1 = 1.0f / h1, ih2 = 1.0f / h2;
-    const float t1[4] = {(om1 - 1.0f) * ih1, -om1 * ih1, ih1, 0.0f};
-    const float t2[4] = {(om2 - 1.0f) * ih2, -om2 * ih2, 0.0f, ih2};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) F.a[i] = n1 * t1[i] + n2 * t2[i];
+    F.a0 = n1 * ((om1 - 1.0f) * ih1) + n2 * ((om2 - 1.0f) * ih2);
+    F.a1 = n1 * (-om1 * ih1) + n2 * (-om2 * ih2);
+    F.a2 = n1 * ih1 + n2 * 0.0f;
+    F.a3 = n1 * 0.0f + n2 * ih2;
     F.valid = true;
     return F;
 }
@@ -471,12 +491,13 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
         const DihFrame F = dihedral_frame(ld3(e.pos, dv.x), ld3(e.pos, dv.y), ld3(e.pos, dv.z), ld3(e.pos, dv.w));
         if (!F.valid) continue;
         const float kb = pr.bend_k;
-        grad = grad + F.a[a] * (kb * (F.theta - m.dih_rest[di]));
-        const F3 ka = F.a[a] * kb;
+        const F3 fa = F.a_dyn(a);
+        grad = grad + fa * (kb * (F.theta - m.dih_rest[di]));
+        const F3 ka = fa * kb;
         const int kk[4] = {ent.y & 0xFFFF, (ent.y >> 16) & 0xFFFF, ent.z & 0xFFFF, (ent.z >> 16) & 0xFFFF};
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
-            const F3 ab = F.a[bb];
+            const F3 ab = F.a(bb);
             float H[9];
             H[0] = ka.x * ab.x;
             H[1] = ka.x * ab.y;
