@@ -168,8 +168,10 @@ Problem::~Problem() {
 }
 
 void Problem::destroy_graph() {
-    if (chunk_exec_) (void)hipGraphExecDestroy(chunk_exec_);
-    chunk_exec_ = nullptr;
+    for (hipGraphExec_t& g : chunk_exec_) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 void Problem::build(const std::vector<const Instance*>& insts) {
@@ -697,25 +699,30 @@ void Problem::autotune(hipStream_t s) {
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(autotune)");
     destroy_graph();
 }
-void Problem::capture_chunk_graph() {
+void Problem::capture_chunk_graph(int level) {
     if (!cap_stream_) check_hip(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking), "hipStreamCreate");
     hipGraph_t graph = nullptr;
     check_hip(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
-        launch_pcg_chunk_body(cap_stream_);
+        for (int c = 0; c < (1 << level); ++c) launch_pcg_chunk_body(cap_stream_);
     } catch (...) {
         (void)hipStreamEndCapture(cap_stream_, &graph);
         if (graph) (void)hipGraphDestroy(graph);
         throw;
     }
     check_hip(hipStreamEndCapture(cap_stream_, &graph), "hipStreamEndCapture");
-    hipError_t e = hipGraphInstantiate(&chunk_exec_, graph, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&chunk_exec_[level], graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     check_hip(e, "hipGraphInstantiate");
 }
-void Problem::launch_pcg_chunk(hipStream_t s) {
-    if (!chunk_exec_) capture_chunk_graph();
-    check_hip(hipGraphLaunch(chunk_exec_, s), "hipGraphLaunch(pcg chunk)");
+void Problem::launch_pcg_chunks(hipStream_t s, int count) {
+    for (int level = kChunkGraphLevels - 1; level >= 0 && count > 0; --level) {
+        while (count >= (1 << level)) {
+            if (!chunk_exec_[level]) capture_chunk_graph(level);
+            check_hip(hipGraphLaunch(chunk_exec_[level], s), "hipGraphLaunch(pcg chunks)");
+            count -= 1 << level;
+        }
+    }
 }
 void Problem::launch_finalize(float dt, const DevExt& ext, hipStream_t s) {
     chysx_amd::launch_finalize(geom(), vectors(), ext, params_.data(), dt, cfg_.finalize_waves, s);
@@ -842,7 +849,7 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
     int chunks = std::max(1, (last_iters_ + 2 + kChunkIters - 1) / kChunkIters);
     int guard = 0;
     for (;;) {
-        for (int c = 0; c < chunks; ++c) launch_pcg_chunk(s);
+        launch_pcg_chunks(s, chunks);
         read_states(s);
         if (all_done()) break;
         chunks = 1;

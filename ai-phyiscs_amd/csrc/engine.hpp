@@ -216,7 +216,10 @@ public:
     void launch_pcg_iteration(hipStream_t s, int i);
     // kChunkIters iterations and the closing advance, launched directly.
     void launch_pcg_chunk_body(hipStream_t s);
-    void launch_pcg_chunk(hipStream_t s);  // the same, as a replayed graph
+    // `count` chunks as replayed graphs of 8, 4, 2 and 1 chunks (largest
+    // first): each graph launch costs a gap of several microseconds, the
+    // chunks inside one graph follow each other directly.
+    void launch_pcg_chunks(hipStream_t s, int count);
     void launch_finalize(float dt, const DevExt& ext, hipStream_t s);
     void read_states(hipStream_t s);        // blocking
     bool all_done() const;
@@ -253,7 +256,7 @@ public:
     void prime_states(InstState* st, hipStream_t s) const;
 
 private:
-    void capture_chunk_graph();
+    void capture_chunk_graph(int level);  // a graph of 2^level chunks
     void destroy_graph();
     void warm_start_stage(float dt, const DevExt& ext, hipStream_t s);
     void push_history(hipStream_t s);
@@ -324,7 +327,8 @@ private:
     DeviceArray<WsInst> ws_inst_;
     std::vector<WsInst> ws_host_;
     hipStream_t cap_stream_ = nullptr;
-    hipGraphExec_t chunk_exec_ = nullptr;
+    static constexpr int kChunkGraphLevels = 4;  // graphs of 1, 2, 4 and 8 chunks
+    hipGraphExec_t chunk_exec_[kChunkGraphLevels] = {};
 };
 
 // The reference's ClothSimulator behaviour on one instance.
