@@ -462,6 +462,8 @@ void Problem::build(const std::vector<const Instance*>& insts) {
     hist_n_ = 0;
     hist_head_ = 0;
     params_.resize(size_t(std::max(ninst_, 1)));
+    params_h_.clear();
+    shapes_h_.clear();
     if (host_states_) (void)hipHostFree(host_states_);
     host_states_ = nullptr;
     check_hip(hipHostMalloc(reinterpret_cast<void**>(&host_states_), sizeof(InstState) * std::max(ninst_, 1)),
@@ -505,10 +507,18 @@ void Problem::upload_params(const std::vector<const Instance*>& insts, hipStream
         q.bend_k = I.has_bending ? I.bend_k : 0.0f;
     }
     if (p.empty()) return;
+    // Unchanged since the last upload (bytewise; p is value-initialised, so its
+    // padding is zero): the device already holds these values.
+    if (p.size() == params_h_.size() && shapes.size() == shapes_h_.size() &&
+        std::memcmp(p.data(), params_h_.data(), sizeof(InstParams) * p.size()) == 0 &&
+        (shapes.empty() || std::memcmp(shapes.data(), shapes_h_.data(), sizeof(float4) * shapes.size()) == 0))
+        return;
     if (!shapes.empty()) shapes_.assign(shapes.data(), shapes.size(), s);
     check_hip(hipMemcpyAsync(params_.data(), p.data(), sizeof(InstParams) * p.size(), hipMemcpyHostToDevice, s),
               "hipMemcpyAsync(params)");
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(params)");
+    params_h_ = std::move(p);
+    shapes_h_ = std::move(shapes);
 }
 
 void Problem::update_pin_targets(const Instance& inst, int k, hipStream_t s) {

@@ -280,6 +280,10 @@ private:
     DeviceArray<SliceDesc> slices_;
     DeviceArray<InstState> state_, state_b_;
     DeviceArray<InstParams> params_;
+    // What params_ and shapes_ hold on the device (empty: unknown), so an
+    // unchanged upload is skipped with its stream synchronisation.
+    std::vector<InstParams> params_h_;
+    std::vector<float4> shapes_h_;
     // mesh
     DeviceArray<int> tri_v_, spr_v_, pin_v_;
     DeviceArray<float4> tri_dminv_, sh_dminv_;
