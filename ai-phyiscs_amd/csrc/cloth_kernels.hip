@@ -796,14 +796,17 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_residual_kernel(LaunchG
 }
 
 // ---------------------------------------------------------------------------
-// PCG SpMV pass.  Mode kModeIter: p = z + beta p_prev (beta = 0 right after a
+// PCG SpMV pass.  It first advances the solver state by the previous
+// iteration's partials (advance_in_pass), then works in the mode that state
+// gives.  Mode kModeIter: p = z + beta p_prev (beta = 0 right after a
 // (re)start), Ap = A p, partial p . Ap.  The direction update of the
 // reference's update_p_kernel (pcg_solver.cu:456-464) is fused here: every
 // gathered column recomputes z + beta p_prev and the row's own p is written
 // to the other p buffer.  The SpMV restates spmv_kernel (block_csr.cu:54-83)
-// on the wave-sliced ELL layout; UNROLL columns are loaded ahead of their
-// gathers.  Mode kModeCheck: the recursive residual passed the test, so this
-// pass computes the true residual instead.
+// on the wave-sliced ELL layout, in batches of UNROLL entries whose loads are
+// issued together.  Mode kModeCheck: the recursive residual passed the test,
+// so this pass computes the true residual instead.  Mode kModeFlush: the
+// owed iterate term is added.
 // ---------------------------------------------------------------------------
 // Entry order of the symmetric pass: 0, w-1, 1, w-2, ...  A row's columns
 // ascend, so step t pairs its t-th lower entry with its t-th upper entry
