@@ -284,8 +284,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (the cloth step "
-              "would shard by row bands with a one-row halo per PCG iteration).", file=sys.stderr)
+        print("bench.py measures one GPU only: the cloth step has no multi-GPU path (it would "
+              "shard by row bands with a one-row halo per PCG iteration).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
