@@ -730,6 +730,11 @@ __device__ __forceinline__ double wave_group_sum(const double* __restrict__ p, i
 // True residual of one row: b - A x with FP64 products and sums (x is FP64).
 // Returns it and, unless `report_only`, stores r, z = M^-1 r and p = z.
 // Partials: [0] |r|^2, [1] |b|^2, [2] r . z.
+// Entries per load batch of residual_row: two keeps both the residual kernel
+// and the SpMV kernel (whose check mode runs residual_row) at 64 VGPRs or
+// fewer, 8 waves per SIMD.
+constexpr int kResidualBatch = 2;
+
 __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVectors& v, float* __restrict__ pcur,
                                              int row, bool report_only, double (&part)[4]) {
     const int s = row >> 6, lane = row & 63;
@@ -744,15 +749,40 @@ __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVector
     const int* colp = A.col + (size_t)sb * 64 + lane;
     const int* mirp = A.mir + (size_t)sb * 64 + lane;
     const float* vp = A.val + (size_t)sb * A.bs * 64 + lane;
-    for (int k = 0; k < w; ++k) {
-        const int c = colp[k * 64];
-        if (c >= 0) {
-            float m[9];
-            load_block(A, vp + k * A.bs * 64, mirp + k * 64, c, row, m);
-            const double y0 = x[3 * c], y1 = x[3 * c + 1], y2 = x[3 * c + 2];
-            a0 = fma3d(a0, m[0], m[1], m[2], y0, y1, y2);
-            a1 = fma3d(a1, m[3], m[4], m[5], y0, y1, y2);
-            a2 = fma3d(a2, m[6], m[7], m[8], y0, y1, y2);
+    // Batches of entries in three phases (the column and mirror indices, then
+    // every block and x gather, then the FP64 FMAs in entry order, so the sums
+    // are those of one entry at a time), so a batch costs two memory round
+    // trips instead of three per entry.
+    constexpr int RB = kResidualBatch;
+    for (int k0 = 0; k0 < w; k0 += RB) {
+        int c[RB], mi[RB];
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+            c[u] = -1;
+            mi[u] = 0;
+            if (k0 + u < w) {  // wave-uniform
+                c[u] = colp[(k0 + u) * 64];
+                mi[u] = mirp[(k0 + u) * 64];
+            }
+        }
+        float m[RB][9];
+        double y[RB][3];
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+            if (c[u] >= 0) {
+                load_block_at(A, vp + (k0 + u) * A.bs * 64, mi[u], c[u], row, m[u]);
+                y[u][0] = x[3 * c[u]];
+                y[u][1] = x[3 * c[u] + 1];
+                y[u][2] = x[3 * c[u] + 2];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+            if (c[u] >= 0) {
+                a0 = fma3d(a0, m[u][0], m[u][1], m[u][2], y[u][0], y[u][1], y[u][2]);
+                a1 = fma3d(a1, m[u][3], m[u][4], m[u][5], y[u][0], y[u][1], y[u][2]);
+                a2 = fma3d(a2, m[u][6], m[u][7], m[u][8], y[u][0], y[u][1], y[u][2]);
+            }
         }
     }
     const F3 bf = ld3(v.rhs, row);

@@ -15,12 +15,15 @@
 namespace chysx_amd {
 
 // The 3x3 block of entry k of `row` (vp_k = that entry's component 0 in its
-// slice, stride 64 floats), row-major into m.
-__device__ __forceinline__ void load_block(const DevMatrix& A, const float* __restrict__ vp_k,
-                                           const int* __restrict__ mir_k, int c, int row, float (&m)[9]) {
+// slice, stride 64 floats), row-major into m, with the entry's mirror index
+// `mir` already loaded (it is used only for a lower entry of the symmetric
+// storage), so a caller can issue the index loads of several entries before
+// any block load.
+__device__ __forceinline__ void load_block_at(const DevMatrix& A, const float* __restrict__ vp_k, int mir, int c,
+                                              int row, float (&m)[9]) {
     if (A.bs == 6) {
         // symmetric block: the mirror (c, row) equals its transpose
-        const float* b = (A.sym && c < row) ? A.val + val_offset(*mir_k, 6) : vp_k;
+        const float* b = (A.sym && c < row) ? A.val + val_offset(mir, 6) : vp_k;
         m[0] = b[0];
         m[1] = b[64];
         m[2] = b[128];
@@ -31,7 +34,7 @@ __device__ __forceinline__ void load_block(const DevMatrix& A, const float* __re
         m[7] = m[5];
         m[8] = b[320];
     } else if (A.sym && c < row) {
-        const float* bt = A.val + val_offset(*mir_k, 9);
+        const float* bt = A.val + val_offset(mir, 9);
         m[0] = bt[0];
         m[1] = bt[192];
         m[2] = bt[384];
@@ -45,6 +48,14 @@ __device__ __forceinline__ void load_block(const DevMatrix& A, const float* __re
 #pragma unroll
         for (int q = 0; q < 9; ++q) m[q] = vp_k[q * 64];
     }
+}
+
+// The same, reading the mirror index (mir_k = the entry's slot in A.mir)
+// only for a lower entry of the symmetric storage.
+__device__ __forceinline__ void load_block(const DevMatrix& A, const float* __restrict__ vp_k,
+                                           const int* __restrict__ mir_k, int c, int row, float (&m)[9]) {
+    const int mir = (A.sym && c < row) ? *mir_k : 0;
+    load_block_at(A, vp_k, mir, c, row, m);
 }
 
 // The diagonal block of row (s, lane), row-major into m.  With A.bs == 6
