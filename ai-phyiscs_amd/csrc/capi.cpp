@@ -669,6 +669,8 @@ int chysx_batch_set_autotune(chysx_batch* b, int on) {
 }
 
 // ---- learned warm start ------------------------------------------------------
+}  // extern "C"
+
 namespace {
 Problem& cloth_prob(chysx_cloth* sim) { return sim->sim.problem(); }
 Problem& batch_prob(chysx_batch* b) { return b->batch.problem(); }
@@ -694,6 +696,8 @@ void get_ws_params(Problem& P, float* theta, int n) {
     P.net().get_params(theta);
 }
 }  // namespace
+
+extern "C" {
 
 int chysx_ws_num_params(int* out) {
     return guard([&] {
