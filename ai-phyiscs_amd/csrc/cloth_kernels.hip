@@ -735,8 +735,10 @@ __device__ __forceinline__ double wave_group_sum(const double* __restrict__ p, i
 // fewer, 8 waves per SIMD.
 constexpr int kResidualBatch = 2;
 
+// nlow: the slice's nlower (entries at positions >= nlow hold no lower block
+// in any lane, so their mirror index is not read).
 __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVectors& v, float* __restrict__ pcur,
-                                             int row, bool report_only, double (&part)[4]) {
+                                             int row, int nlow, bool report_only, double (&part)[4]) {
     const int s = row >> 6, lane = row & 63;
     const double* x = v.x;
     float dm[9];
@@ -762,7 +764,7 @@ __device__ __forceinline__ void residual_row(const DevMatrix& A, const DevVector
             mi[u] = 0;
             if (k0 + u < w) {  // wave-uniform
                 c[u] = colp[(k0 + u) * 64];
-                mi[u] = mirp[(k0 + u) * 64];
+                if (A.sym && k0 + u < nlow) mi[u] = mirp[(k0 + u) * 64];  // wave-uniform
             }
         }
         float m[RB][9];
@@ -816,7 +818,7 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_residual_kernel(LaunchG
     double part[3] = {0.0, 0.0, 0.0};
     if (lane < sd.nvalid) {
         double q[4] = {0.0, 0.0, 0.0, 0.0};
-        residual_row(A, v, pcur, s * 64 + lane, report != 0, q);
+        residual_row(A, v, pcur, s * 64 + lane, sd.nlower, report != 0, q);
         part[0] = q[0];
         part[1] = q[1];
         part[2] = q[2];
@@ -1000,7 +1002,7 @@ __global__ __launch_bounds__(64 * kGroupSlices) __attribute__((amdgpu_num_sgpr(8
         double part[2] = {0.0, 0.0};
         if (valid) {
             double q[4] = {0.0, 0.0, 0.0, 0.0};
-            residual_row(A, v, parity ? v.p1 : v.p0, row, false, q);
+            residual_row(A, v, parity ? v.p1 : v.p0, row, sd.nlower, false, q);
             part[0] = q[0];
             part[1] = q[2];
         }
