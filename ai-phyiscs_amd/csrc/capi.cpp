@@ -835,7 +835,8 @@ struct NetGraph {
         std::vector<SliceDesc> sd(static_cast<size_t>(L.nslices));
         // nlower: the slice width (every entry may be lower), a bound that holds for any pattern
         for (int q = 0; q < L.nslices; ++q)
-            sd[size_t(q)] = SliceDesc{0, 64 * q, std::min(64, n - 64 * q), L.slice_ptr[size_t(q) + 1] - L.slice_ptr[size_t(q)]};
+            sd[size_t(q)] = SliceDesc{0, 64 * q, std::min(64, n - 64 * q), L.slice_ptr[size_t(q) + 1] - L.slice_ptr[size_t(q)],
+                                      0, (L.nslices + kGroupSlices - 1) / kGroupSlices};
         slices.assign(sd.data(), sd.size());
         slice_ptr.assign(L.slice_ptr.data(), L.slice_ptr.size());
         col.assign(L.sell_col.data(), L.sell_col.size());

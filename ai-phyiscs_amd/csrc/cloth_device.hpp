@@ -18,6 +18,8 @@ struct SliceDesc {
     int nvalid;  // rows of the slice that exist (<= 64)
     int nlower;  // one past the last entry position holding a lower block (column < row) in
                  // any lane: entries k >= nlower are upper or padding in every lane
+    int group0;  // the instance's first 16-slice group and its group count (as in its
+    int ngroups; // InstState), so a pass can load group partials before the state arrives
 };
 
 // PCG status of one instance.

@@ -646,25 +646,40 @@ constexpr int kReduceThreads = 64 * kGroupSlices;
 // any of them is added.  The total of slot slot[j] lands in out[j] in every
 // thread: after one barrier each thread adds the 16 wave sums itself, in wave
 // order.  One call per kernel (the wave sums' LDS is not reused).
+//
+// block_sum_issue issues the first round's loads on its own and
+// block_sum_finish does the rest, so a caller can issue other loads while
+// the first round is in flight; block_sum is the two back to back.
 template <int NS, int KU>
-__device__ __forceinline__ void block_sum(const double* __restrict__ partials, int ld, int s0, int ns,
-                                          const int (&slot)[NS], double (&out)[NS]) {
+__device__ __forceinline__ void block_sum_round(const double* __restrict__ partials, int ld, int s0, int ns,
+                                                const int (&slot)[NS], int q0, double (&t)[NS][KU]) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        const double* pj = partials + (size_t)slot[j] * ld + s0;
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            // unconditional, clamped load (no branch): out-of-range
+            // entries re-read the last partial and are not added
+            const int q = q0 + u * kReduceThreads;
+            t[j][u] = pj[q < ns ? q : ns - 1];
+        }
+    }
+}
+
+template <int NS, int KU>
+__device__ __forceinline__ void block_sum_issue(const double* __restrict__ partials, int ld, int s0, int ns,
+                                                const int (&slot)[NS], double (&t)[NS][KU]) {
+    if (ns > 0) block_sum_round<NS, KU>(partials, ld, s0, ns, slot, int(threadIdx.x), t);
+}
+
+template <int NS, int KU>
+__device__ __forceinline__ void block_sum_finish(const double* __restrict__ partials, int ld, int s0, int ns,
+                                                 const int (&slot)[NS], const double (&t0)[NS][KU],
+                                                 double (&out)[NS]) {
     double acc[NS];
 #pragma unroll
     for (int j = 0; j < NS; ++j) acc[j] = 0.0;
-    auto round = [&](int q0) {
-        double t[NS][KU];
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const double* pj = partials + (size_t)slot[j] * ld + s0;
-#pragma unroll
-            for (int u = 0; u < KU; ++u) {
-                // unconditional, clamped load (no branch): out-of-range
-                // entries re-read the last partial and are not added
-                const int q = q0 + u * kReduceThreads;
-                t[j][u] = pj[q < ns ? q : ns - 1];
-            }
-        }
+    auto add = [&](int q0, const double (&t)[NS][KU]) {
 #pragma unroll
         for (int j = 0; j < NS; ++j)
 #pragma unroll
@@ -674,8 +689,12 @@ __device__ __forceinline__ void block_sum(const double* __restrict__ partials, i
     // The first round is straight-line code: a loop header here would make
     // the compiler wait for every load the caller has in flight.
     if (ns > 0) {
-        round(int(threadIdx.x));
-        for (int q0 = int(threadIdx.x) + KU * kReduceThreads; q0 < ns; q0 += KU * kReduceThreads) round(q0);
+        add(int(threadIdx.x), t0);
+        for (int q0 = int(threadIdx.x) + KU * kReduceThreads; q0 < ns; q0 += KU * kReduceThreads) {
+            double t[NS][KU];
+            block_sum_round<NS, KU>(partials, ld, s0, ns, slot, q0, t);
+            add(q0, t);
+        }
     }
     __shared__ double sh[kReduceThreads / 64][NS];
 #pragma unroll
@@ -695,6 +714,14 @@ __device__ __forceinline__ void block_sum(const double* __restrict__ partials, i
         for (int w = 0; w < kReduceThreads / 64; ++w) t += sh[w][j];
         out[j] = t;
     }
+}
+
+template <int NS, int KU>
+__device__ __forceinline__ void block_sum(const double* __restrict__ partials, int ld, int s0, int ns,
+                                          const int (&slot)[NS], double (&out)[NS]) {
+    double t[NS][KU];
+    block_sum_issue<NS, KU>(partials, ld, s0, ns, slot, t);
+    block_sum_finish<NS, KU>(partials, ld, s0, ns, slot, t, out);
 }
 
 // Sum of ng group partials p[0 .. ng) by one wavefront in a fixed order:
@@ -1150,10 +1177,13 @@ __global__ __launch_bounds__(64 * kGroupSlices) __attribute__((amdgpu_num_sgpr(8
 // X: 0 defers x += alpha p; 1 adds the owed term and this one; 2 adds this
 // one.  Straight-line per variant, so every operand load (including x) is
 // issued before the p . Ap sum and waited for once.
+// sig_t: the first round of sigma's partial loads (block_sum_issue), issued
+// by update_group before the solver state was read.
+constexpr int kSigmaKU = 2;
 template <int X>
 __device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVectors& v, InstState& S, int s,
                                            const SliceDesc& sd, F3 r0, F3 api, const float (&m)[6],
-                                           double& sig_cache, int& sig_inst) {
+                                           const double (&sig_t)[1][kSigmaKU], double& sig_cache, int& sig_inst) {
     const float* __restrict__ p = S.parity ? v.p0 : v.p1;      // written by the SpMV pass
     const float* __restrict__ pprev = S.parity ? v.p1 : v.p0;  // the previous direction
     const int lane = threadIdx.x & 63;
@@ -1173,7 +1203,7 @@ __device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVector
     if (sig_inst != sd.inst) {
         double sum[1];
         const int sig_slot[1] = {kSlotSigma};
-        block_sum<1, 4>(v.partials, g.ngroups, S.group0, S.ngroups, sig_slot, sum);
+        block_sum_finish<1, kSigmaKU>(v.partials, g.ngroups, sd.group0, sd.ngroups, sig_slot, sig_t, sum);
         sig_cache = uniform_double(sum[0]);
         sig_inst = sd.inst;
     }
@@ -1226,11 +1256,19 @@ __device__ __forceinline__ void update_group(const LaunchGeom& g, const DevMatri
         for (int c = 0; c < 6; ++c) m[c] = mv[c * 64];
     }
     const SliceDesc sd = g.slices[s];
+    // sigma's partial loads go out before the state is read too (the group
+    // range comes with the slice descriptor); update_rows issues its
+    // state-dependent loads before it waits for them
+    double sig_t[1][kSigmaKU];
+    if (sig_inst != sd.inst) {
+        const int sig_slot[1] = {kSlotSigma};
+        block_sum_issue<1, kSigmaKU>(v.partials, g.ngroups, sd.group0, sd.ngroups, sig_slot, sig_t);
+    }
     InstState& S = st[sd.inst];
     if (S.status != kRunning || S.mode != kModeIter) return;  // check and flush rounds update nothing
-    if (S.pend) update_rows<1>(g, v, S, s, sd, r0, api, m, sig_cache, sig_inst);
-    else if (S.defer) update_rows<0>(g, v, S, s, sd, r0, api, m, sig_cache, sig_inst);
-    else update_rows<2>(g, v, S, s, sd, r0, api, m, sig_cache, sig_inst);
+    if (S.pend) update_rows<1>(g, v, S, s, sd, r0, api, m, sig_t, sig_cache, sig_inst);
+    else if (S.defer) update_rows<0>(g, v, S, s, sd, r0, api, m, sig_t, sig_cache, sig_inst);
+    else update_rows<2>(g, v, S, s, sd, r0, api, m, sig_t, sig_cache, sig_inst);
 }
 
 // With `pairs` (one instance), each workgroup updates two groups, i and

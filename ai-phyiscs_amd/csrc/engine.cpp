@@ -359,7 +359,8 @@ void Problem::build(const std::vector<const Instance*>& insts) {
         st[size_t(k)].ngroups = inst_nslices_[size_t(k)] / kGroupSlices;
         for (int q = 0; q < inst_nslices_[size_t(k)]; ++q)
             slices_h_.push_back(SliceDesc{k, I.part0 + kSlice * q, std::max(0, std::min(kSlice, I.n - kSlice * q)),
-                                          nlower[size_t(inst_slice0_[size_t(k)] + q)]});
+                                          nlower[size_t(inst_slice0_[size_t(k)] + q)], st[size_t(k)].group0,
+                                          st[size_t(k)].ngroups});
         st[size_t(k)].status = kStopped;
     }
     if (int(slices_h_.size()) != nslices) throw std::logic_error("slice count mismatch");
