@@ -214,6 +214,44 @@ def drape(w: int = 21, size: float = 0.8, seed: Optional[int] = None) -> Scene:
                  shear_k=500.0, bend_k=0.01)
 
 
+def degenerate(w: int = 5) -> Scene:
+    """A W x W grid cloth (1 m, top edge pinned, springs + stretch k = 1e3,
+    m = 0.01 kg) with four extra particles hung from its first rows by
+    degenerate triangles, one per guard of the reference's element code:
+
+    * ``p1``, ``p2``: a millimetre triangle (0, p1, p2) with edge 2^-10 m and
+      height 2^-11 m, whose rest determinant lab * v_c = 2^-21 is below 1e-6,
+      so the rest frame falls back to the identity with a non-zero area
+      (triangle_stretch_constraint.cu:95-99);
+    * ``q``: on the segment (1, 2), a zero-area triangle (determinant 0);
+    * ``r``: at the position of particle W+1 and joined to it by a spring of
+      rest length 0, so the spring length is clamped to 1e-6
+      (spring_constraint.cu:35) and the triangle (W+1, r, W+2) has two
+      coincident corners.
+
+    With W = 5 every coordinate is a short binary fraction, so the FP32 rest
+    frame (v_c^2 = |ac|^2 - u_c^2, as the reference computes it) is exact or
+    free of cancellation and takes the same branch as the FP64 oracle.  A
+    sliver with long edges would not: its v_c^2 cancels to rounding noise in
+    FP32, in the reference's kernel as here, and the branch taken depends on
+    that noise.
+    """
+    pos, tris = grid(w, 1.0)
+    h = 1.0 / (w - 1)
+    n0 = w * w
+    p1, p2, q, r = n0, n0 + 1, n0 + 2, n0 + 3
+    extra = np.array([[2.0 ** -10, 0.0, 0.0],
+                      [2.0 ** -11, 0.0, -(2.0 ** -11)],
+                      [1.5 * h, 0.0, 0.0],
+                      pos[w + 1]], np.float32)
+    pos = np.concatenate([pos, extra]).astype(np.float32)
+    tris = np.concatenate([tris, np.array([[0, p1, p2], [1, q, 2], [w + 1, r, w + 2]], np.int32)])
+    idx, tgt = _pins(w, "top_edge", pos)
+    return Scene(f"degenerate_{w}", pos, np.zeros_like(pos), np.full(pos.shape[0], 100.0, np.float32),
+                 tris, idx, tgt, 1.0e6, 1.0e3, 1.0e3, None, 1.0 / 60.0, 20, grid_w=w,
+                 meta={"small": (p1, p2), "collinear": q, "coincident": r, "coincident_with": w + 1})
+
+
 def freefall(w: int = 6) -> Scene:
     """Unpinned cloth with no elastic terms active beyond rest state: every
     particle follows x_n = x_0 + dt^2 g n(n+1)/2 exactly."""

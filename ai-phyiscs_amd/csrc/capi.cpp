@@ -10,6 +10,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 
@@ -286,6 +287,25 @@ int chysx_cloth_step(chysx_cloth* sim, float dt, uintptr_t stream) {
         sim->sim.step(dt, stream);
     });
 }
+int chysx_cloth_set_pin_projection(chysx_cloth* sim, int on) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.set_pin_projection(on != 0);
+    });
+}
+int chysx_cloth_pin_projection(const chysx_cloth* sim, int* out) {
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        *out = sim->sim.pin_projection() ? 1 : 0;
+    });
+}
+int chysx_cloth_set_blocking_step(chysx_cloth* sim, int on) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.problem().set_blocking_step(on != 0);
+    });
+}
 int chysx_cloth_num_particles(const chysx_cloth* sim, int* out) {
     return guard([&] {
         need(sim, "sim");
@@ -502,6 +522,18 @@ int chysx_batch_step(chysx_batch* b, float dt, uintptr_t stream) {
         b->batch.step(dt, stream);
     });
 }
+int chysx_batch_set_pin_projection(chysx_batch* b, int on) {
+    return guard([&] {
+        need(b, "batch");
+        b->batch.set_pin_projection(on != 0);
+    });
+}
+int chysx_batch_set_blocking_step(chysx_batch* b, int on) {
+    return guard([&] {
+        need(b, "batch");
+        b->batch.problem().set_blocking_step(on != 0);
+    });
+}
 int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts, int* status, double* relres) {
     return guard([&] {
         need(b, "batch");
@@ -511,6 +543,28 @@ int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts
         if (restarts) *restarts = s.restarts;
         if (status) *status = s.status;
         if (relres) *relres = s.relres;
+    });
+}
+
+int chysx_cloth_spmv(chysx_cloth* sim, uintptr_t x, uintptr_t y, uintptr_t stream) {
+    return guard([&] {
+        need(sim, "sim");
+        if (x == 0 || y == 0) throw std::invalid_argument("spmv: x and y must be non-null device pointers");
+        Problem& P = sim->sim.problem();
+        if (!P.built() || sim->sim.num_particles() == 0)
+            throw std::runtime_error("spmv: step() once to assemble a system first");
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        P.wait_pending(s);
+        launch_spmv(P.geom(), P.matrix(), reinterpret_cast<const float*>(x), reinterpret_cast<float*>(y),
+                    P.launch_config().spmv_waves, s);
+    });
+}
+
+int chysx_stream_bench(int kind, uint64_t bytes_per_buffer, int reps, uintptr_t stream, double* ms,
+                       double* bytes_moved) {
+    return guard([&] {
+        need(ms, "ms");
+        *ms = stream_bench(kind, size_t(bytes_per_buffer), reps, reinterpret_cast<hipStream_t>(stream), bytes_moved);
     });
 }
 
@@ -559,8 +613,15 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
                         InstState* in = (pass & 1) ? P.states_b() : P.states();
                         InstState* out = (pass & 1) ? P.states() : P.states_b();
                         ++pass;
-                        launch_pcg_spmv(g, A, v, in, out, P.params(), c.spmv_unroll, s, c.spmv_branch_free,
-                                        c.spmv_symmetric);
+                        if (P.split_advance()) {
+                            launch_pcg_advance_copy(in, out, P.params(), v.partials, P.ninst(), g.ngroups, s,
+                                                    g.gactive);
+                            launch_pcg_spmv(g, A, v, out, nullptr, nullptr, c.spmv_unroll, s, c.spmv_branch_free,
+                                            c.spmv_symmetric);
+                        } else {
+                            launch_pcg_spmv(g, A, v, in, out, P.params(), c.spmv_unroll, s, c.spmv_branch_free,
+                                            c.spmv_symmetric);
+                        }
                         break;
                     }
                     case 2: launch_pcg_update(g, A, v, P.states(), s); break;
@@ -602,7 +663,7 @@ int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg) {
         const int v[CHYSX_LAUNCH_CONFIG_LEN] = {c.assemble_waves,   c.residual_waves, c.spmv_waves,
                                                 c.spmv_unroll,      c.update_waves,   c.finalize_waves,
                                                 c.spmv_branch_free, c.spmv_symmetric, c.spmv_packed,
-                                                c.spmv_index16};
+                                                c.spmv_index16,     c.pcg_split_advance};
         std::memcpy(cfg, v, sizeof(v));
     });
 }
@@ -623,6 +684,7 @@ int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg) {
         c.spmv_symmetric = cfg[7];
         c.spmv_packed = cfg[8];
         c.spmv_index16 = cfg[9];
+        c.pcg_split_advance = cfg[10];
         sim->sim.problem().set_launch_config(c);
     });
 }
@@ -876,8 +938,27 @@ int chysx_wsnet_forward(chysx_wsnet* net, int n, const int32_t* row_offsets, con
         G.build(n, row_offsets, cols);
         net->net.reserve(G.rows, false);
         net->net.set_inputs(F, E, n, nullptr);
-        net->net.forward(G.g, nullptr, false);
+        net->net.forward(G.g, nullptr, false, false);
         net->net.copy_out(out, n, nullptr);
+    });
+}
+int chysx_wsnet_forward_direction(chysx_wsnet* net, int n, const int32_t* row_offsets, const int32_t* cols,
+                                  const float* F, const float* E, float* out) {
+    return guard([&] {
+        need(net, "net");
+        need(row_offsets, "row_offsets");
+        need(F, "F");
+        need(E, "E");
+        need(out, "out");
+        NetGraph G;
+        G.build(n, row_offsets, cols);
+        net->net.reserve(G.rows, false);
+        net->net.set_inputs(F, E, n, nullptr);
+        net->net.forward(G.g, nullptr, false, true);
+        std::vector<float> full(size_t(n) * WarmStartNet::kOut);
+        net->net.copy_out(full.data(), n, nullptr);
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) out[3 * size_t(i) + c] = full[size_t(i) * WarmStartNet::kOut + c];
     });
 }
 int chysx_wsnet_loss_grad(chysx_wsnet* net, int n, const int32_t* row_offsets, const int32_t* cols, const float* F,

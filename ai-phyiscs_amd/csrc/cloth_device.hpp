@@ -114,6 +114,9 @@ struct alignas(16) InstParams {
     // instance's planes (1 float4 each) and boxes (4 float4 each) in the shape table
     float contact_h, contact_k, contact_mu, contact_eps;
     int plane0, nplane, box0, nbox;
+    // 1: pins are constraints (extension): a pinned particle's displacement
+    // is fixed at target - x_n and eliminated from the system
+    int pin_proj;
 };
 
 }  // namespace chysx_amd

@@ -192,6 +192,17 @@ __global__ void area_mass_kernel(const int* __restrict__ vt_off, const int* __re
 //             the same quantity as (m/dt^2)(x_tilde - x_n) - grad without the
 //             FP32 cancellation of x_tilde - x_n.
 // ---------------------------------------------------------------------------
+// B d for accumulator entry k (block stride bs floats; bs == 6 holds the
+// upper triangle xx, xy, xz, yy, yz, zz of a symmetric block).
+__device__ __forceinline__ F3 block_mul(const float* __restrict__ acc, int k, int bs, F3 d) {
+    const float* a = acc + k * bs * 64;
+    if (bs == 6)
+        return f3(a[0] * d.x + a[64] * d.y + a[128] * d.z, a[64] * d.x + a[192] * d.y + a[256] * d.z,
+                  a[128] * d.x + a[256] * d.y + a[320] * d.z);
+    return f3(a[0] * d.x + a[64] * d.y + a[128] * d.z, a[192] * d.x + a[256] * d.y + a[320] * d.z,
+              a[384] * d.x + a[448] * d.y + a[512] * d.z);
+}
+
 // Adds sgn H into accumulator entry k (block stride bs floats).  bs == 6
 // keeps the upper triangle (xx, xy, xz, yy, yz, zz) of a symmetric H.
 __device__ __forceinline__ void add_block(float* __restrict__ acc, int k, const float (&H)[9], float sgn, int bs) {
@@ -424,8 +435,8 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
     float D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     F3 grad = f3(0.0f, 0.0f, 0.0f);
 
-    // pins
-    for (int q = m.pin_inc_off[row]; q < m.pin_inc_off[row + 1]; ++q) {
+    // pins (penalties; with pin projection they are constraints, applied below)
+    for (int q = m.pin_inc_off[row]; q < (pr.pin_proj ? m.pin_inc_off[row] : m.pin_inc_off[row + 1]); ++q) {
         const int c = m.pin_inc[q];
         const F3 tg = ld3(m.pin_tgt, c);
         grad = grad + (xi - tg) * pr.pin_k;
@@ -516,6 +527,37 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
             }
         }
     }
+    // Pin projection (extension; no reference counterpart, whose pins are
+    // penalties, pin_constraint.h:5-15): a pinned particle's displacement is
+    // d = target - x_n.  A free row moves its blocks to pinned columns into
+    // the right-hand side (rhs -= B d) and drops them; a pinned row drops
+    // every block and becomes I dx = d, with the iterate started at d, so its
+    // residual is 0 throughout.  Both halves of a coupling are dropped (the
+    // pinned row's own block and the free row's), so the symmetric storages'
+    // mirrors agree.
+    bool pinned = false;
+    F3 dpin = f3(0.0f, 0.0f, 0.0f), pcorr = f3(0.0f, 0.0f, 0.0f);
+    if (pr.pin_proj) {
+        const int q0 = m.pin_inc_off[row];
+        if (m.pin_inc_off[row + 1] > q0) {
+            pinned = true;
+            dpin = ld3(m.pin_tgt, m.pin_inc[q0]) - xi;
+        }
+        const int* colr = A.col + (size_t)sb * 64 + lane;
+        for (int k = 0; k < deg; ++k) {
+            const int c = colr[k * 64];
+            if (c < 0) continue;
+            const int qc = m.pin_inc_off[c];
+            const bool pc = m.pin_inc_off[c + 1] > qc;
+            if (!pinned && !pc) continue;
+            if (!pinned) {
+                const int pi = m.pin_inc[qc];
+                const F3 dc = ld3(m.pin_tgt, pi) - ld3(e.pos, m.pin_v[pi]);
+                pcorr = pcorr + block_mul(acc, k, bs, dc);
+            }
+            for (int cc = 0; cc < bs; ++cc) acc[(k * bs + cc) * 64] = 0.0f;
+        }
+    }
     if (LDS) {
         const int* colr = A.col + (size_t)sb * 64 + lane;
         for (int k = 0; k < deg; ++k) {
@@ -593,6 +635,16 @@ __global__ __launch_bounds__(64 * WAVES) void assemble_kernel(LaunchGeom g, DevM
                 }
             }
         }
+    }
+    if (pinned) {
+        rr = dpin;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) D[c] = (c == 0 || c == 4 || c == 8) ? 1.0f : 0.0f;
+        v.x[3 * row] = double(dpin.x);
+        v.x[3 * row + 1] = double(dpin.y);
+        v.x[3 * row + 2] = double(dpin.z);
+    } else if (pr.pin_proj) {
+        rr = rr - pcorr;
     }
     st3(v.rhs, row, rr);
     if (bs == 6) {  // symmetric diagonal blocks (no bending): upper triangle
@@ -1312,6 +1364,29 @@ __global__ __launch_bounds__(kReduceThreads) void pcg_advance_kernel(InstState* 
         for (int q = 0; q < S.ngroups; ++q) gactive[S.group0 + q] = 0;
 }
 
+// The advance of advance_in_pass as a kernel of its own (one workgroup of
+// kReduceThreads per instance, the same sums in the same order and the same
+// scalar step): reads the state from st_in and writes the advanced state,
+// marked owed while the solve runs, to st_out, for an SpMV pass launched
+// without an advance and its update pass.
+__global__ __launch_bounds__(kReduceThreads) void pcg_advance_copy_kernel(const InstState* __restrict__ st_in,
+                                                                          InstState* __restrict__ st_out,
+                                                                          const InstParams* __restrict__ params,
+                                                                          const double* __restrict__ partials,
+                                                                          int ngroups, int* __restrict__ gactive) {
+    InstState S = st_in[blockIdx.x];
+    const int status_in = S.status;
+    if (S.status == kRunning && S.owed) {
+        const AdvanceSums t = advance_sums(partials, ngroups, S);
+        pcg_advance(S, params[blockIdx.x], t);
+    }
+    S.owed = S.status == kRunning ? 1 : 0;
+    if (threadIdx.x == 0) st_out[blockIdx.x] = S;
+    // a cloth that just finished: its groups leave the iteration passes
+    if (gactive != nullptr && status_in == kRunning && S.status != kRunning)
+        for (int q = int(threadIdx.x); q < S.ngroups; q += kReduceThreads) gactive[S.group0 + q] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // Start and final reductions: one wavefront per instance sums that
 // instance's group partials in a fixed order (deterministic).  Restates
@@ -1370,7 +1445,10 @@ __global__ __launch_bounds__(64) void pcg_reduce_kernel(InstState* __restrict__ 
 // v = dx / dt (times exp(-damping dt) when damping > 0).
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void finalize_kernel(LaunchGeom g, DevVectors v, DevExt e,
-                                                             const InstParams* __restrict__ params, float dt) {
+                                                             const InstParams* __restrict__ params, float dt,
+                                                             const int* __restrict__ pin_inc_off,
+                                                             const int* __restrict__ pin_inc,
+                                                             const float* __restrict__ pin_tgt) {
     const int s = my_slice<WAVES>(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
@@ -1378,12 +1456,17 @@ __global__ __launch_bounds__(64 * WAVES) void finalize_kernel(LaunchGeom g, DevV
     if (lane >= sd.nvalid) return;
     const int row = s * 64 + lane, p = sd.part0 + lane;
     const InstParams pr = params[sd.inst];
-    const F3 d = f3(float(v.x[3 * row]), float(v.x[3 * row + 1]), float(v.x[3 * row + 2]));
+    F3 d = f3(float(v.x[3 * row]), float(v.x[3 * row + 1]), float(v.x[3 * row + 2]));
     const F3 xn = ld3(e.pos, p);
+    F3 xnew = xn + d;
+    if (pr.pin_proj && pin_inc_off[row + 1] > pin_inc_off[row]) {  // a projected pin ends at its target
+        xnew = ld3(pin_tgt, pin_inc[pin_inc_off[row]]);
+        d = xnew - xn;
+    }
     const float inv_dt = 1.0f / dt;
     F3 vn = d * inv_dt;
     if (pr.damping > 0.0f) vn = vn * expf(-pr.damping * dt);
-    st3(e.pos, p, xn + d);
+    st3(e.pos, p, xnew);
     st3(e.vel, p, vn);
 }
 
@@ -1544,6 +1627,13 @@ void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors
     pcg_update_kernel<<<8 * ((nfirst + 7) / 8), 64 * kGroupSlices, 0, s>>>(g, A, v, st, pairs);
     CHYSX_LAUNCH_CHECK();
 }
+void launch_pcg_advance_copy(const InstState* st_in, InstState* st_out, const InstParams* params,
+                             const double* partials, int ninst, int ngroups, hipStream_t s, int* gactive) {
+    if (ninst <= 0) return;
+    if (st_in == st_out) throw std::invalid_argument("advance copy: input and output states must differ");
+    pcg_advance_copy_kernel<<<ninst, kReduceThreads, 0, s>>>(st_in, st_out, params, partials, ngroups, gactive);
+    CHYSX_LAUNCH_CHECK();
+}
 void launch_pcg_advance(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
                         hipStream_t s, int* gactive) {
     if (ninst <= 0) return;
@@ -1559,14 +1649,22 @@ void launch_pcg_reduce(InstState* st, const InstParams* params, const double* pa
     CHYSX_LAUNCH_CHECK();
 }
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
-                     int waves, hipStream_t s) {
+                     const DevMesh& m, int waves, hipStream_t s) {
     if (g.nslices <= 0) return;
-#define L(W) finalize_kernel<W><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, v, e, params, dt)
+#define L(W)                                                                                 \
+    finalize_kernel<W><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, v, e, params, dt, m.pin_inc_off, m.pin_inc, \
+                                                                  m.pin_tgt)
     CHYSX_WAVES_SWITCH(waves, L)
 #undef L
     CHYSX_LAUNCH_CHECK();
 }
 void launch_spmv(const LaunchGeom& g, const DevMatrix& A, const float* x, float* y, int waves, hipStream_t s) {
+    // spmv_kernel reads every block itself as 9 floats: the symmetric
+    // storages hold only upper blocks (6 floats when packed)
+    if (A.sym != 0 || A.bs != 9)
+        throw std::invalid_argument(
+            "plain SpMV reads the general storage (9-float blocks, both halves): set spmv_symmetric=0 and "
+            "spmv_packed=0, then step or assemble");
     if (g.nslices <= 0) return;
 #define L(W) spmv_kernel<W><<<grid_for(g.nslices, W), 64 * W, 0, s>>>(g, A, x, y)
     CHYSX_WAVES_SWITCH(waves, L)

@@ -141,15 +141,23 @@ void launch_pcg_update(const LaunchGeom& g, const DevMatrix& A, const DevVectors
 // or launch_pcg_advance).
 void launch_pcg_reduce(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
                        int phase, hipStream_t s, int* gactive = nullptr);
+// The advance of the previous SpMV/update pair from st_in into st_out (one
+// workgroup per instance), for an SpMV pass launched without its own advance.
+void launch_pcg_advance_copy(const InstState* st_in, InstState* st_out, const InstParams* params,
+                             const double* partials, int ninst, int ngroups, hipStream_t s, int* gactive);
 // One workgroup per instance: the owed advance of the state in st, in place
 // (the end of a chunk of iterations, so the polled state is current).
 void launch_pcg_advance(InstState* st, const InstParams* params, const double* partials, int ninst, int ngroups,
                         hipStream_t s, int* gactive = nullptr);
 void launch_finalize(const LaunchGeom& g, const DevVectors& v, const DevExt& e, const InstParams* params, float dt,
-                     int waves, hipStream_t s);
+                     const DevMesh& m, int waves, hipStream_t s);
 void launch_area_mass(const int* vt_off, const int* vt_ids, const int* tri_v, const float* pos, float density_over_3,
                       float* inv_mass_out, int n, hipStream_t s);
 // Plain SpMV y = A x (FP32 in, FP32 out) over the tiles, for benchmarks and tests.
 void launch_spmv(const LaunchGeom& g, const DevMatrix& A, const float* x, float* y, int waves, hipStream_t s);
+
+// HBM probe (stream_bench.hip): kind 0 copy, 1 triad, 2 / 3 the same with
+// non-temporal accesses; returns mean ms per launch, *moved = bytes per launch.
+double stream_bench(int kind, size_t bytes_per_buffer, int reps, hipStream_t s, double* moved);
 
 }  // namespace chysx_amd

@@ -162,9 +162,12 @@ void compute_rest_bending(Instance& inst, const float* pos, float k, hipStream_t
 Problem::Problem() = default;
 
 Problem::~Problem() {
+    // an enqueued step still uses the buffers and copies into host_states_
+    if (stats_pending_ && done_event_) (void)hipEventSynchronize(done_event_);
     destroy_graph();
     if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
     if (host_states_) (void)hipHostFree(host_states_);
+    if (done_event_) (void)hipEventDestroy(done_event_);
 }
 
 void Problem::destroy_graph() {
@@ -175,6 +178,10 @@ void Problem::destroy_graph() {
 }
 
 void Problem::build(const std::vector<const Instance*>& insts) {
+    if (stats_pending_) {  // an enqueued step still uses the buffers rebuilt below
+        check_hip(hipEventSynchronize(done_event_), "hipEventSynchronize(previous step)");
+        stats_pending_ = false;
+    }
     destroy_graph();
     built_ = false;
     ninst_ = int(insts.size());
@@ -477,6 +484,7 @@ void Problem::build(const std::vector<const Instance*>& insts) {
 }
 
 void Problem::upload_params(const std::vector<const Instance*>& insts, hipStream_t s) {
+    wait_pending(s);
     std::vector<InstParams> p(insts.size());
     std::vector<float4> shapes;
     for (size_t k = 0; k < insts.size(); ++k) {
@@ -506,6 +514,7 @@ void Problem::upload_params(const std::vector<const Instance*>& insts, hipStream
         q.tri_stretch = I.has_stretch ? 1 : 0;
         q.tri_shear = I.has_shear ? 1 : 0;
         q.bend_k = I.has_bending ? I.bend_k : 0.0f;
+        q.pin_proj = I.pin_projection ? 1 : 0;
     }
     if (p.empty()) return;
     // Unchanged since the last upload (bytewise; p is value-initialised, so its
@@ -522,9 +531,14 @@ void Problem::upload_params(const std::vector<const Instance*>& insts, hipStream
     shapes_h_ = std::move(shapes);
 }
 
+void Problem::wait_pending(hipStream_t s) const {
+    if (stats_pending_) check_hip(hipStreamWaitEvent(s, done_event_, 0), "hipStreamWaitEvent(enqueued step)");
+}
+
 void Problem::update_pin_targets(const Instance& inst, int k, hipStream_t s) {
     const int P = int(inst.pin_idx.size());
     if (P == 0) return;
+    wait_pending(s);
     check_hip(hipMemcpyAsync(pin_tgt_.data() + 3 * pin_offset_[size_t(k)], inst.pin_tgt.data(), sizeof(float) * 3 * P,
                              hipMemcpyHostToDevice, s),
               "hipMemcpyAsync(pin targets)");
@@ -597,8 +611,16 @@ void Problem::launch_pcg_iteration(hipStream_t s, int i) {
     const LaunchGeom g = geom();
     InstState* in = (i & 1) ? state_b_.data() : state_.data();
     InstState* out = (i & 1) ? state_.data() : state_b_.data();
-    launch_pcg_spmv(g, matrix(), vectors(), in, out, params_.data(), cfg_.spmv_unroll, s, cfg_.spmv_branch_free,
-                    cfg_.spmv_symmetric);
+    if (split_advance()) {
+        // the advance of the previous pair, from `in` into `out`, then the SpMV
+        // pass on the current state in `out`
+        launch_pcg_advance_copy(in, out, params_.data(), partials_.data(), ninst_, g.ngroups, s, g.gactive);
+        launch_pcg_spmv(g, matrix(), vectors(), out, nullptr, nullptr, cfg_.spmv_unroll, s, cfg_.spmv_branch_free,
+                        cfg_.spmv_symmetric);
+    } else {
+        launch_pcg_spmv(g, matrix(), vectors(), in, out, params_.data(), cfg_.spmv_unroll, s, cfg_.spmv_branch_free,
+                        cfg_.spmv_symmetric);
+    }
     launch_pcg_update(g, matrix(), vectors(), out, s);
 }
 void Problem::launch_pcg_chunk_body(hipStream_t s) {
@@ -611,7 +633,8 @@ void Problem::set_launch_config(const LaunchConfig& c) {
     if (!valid_waves(c.assemble_waves) || !valid_waves(c.residual_waves) || !valid_waves(c.spmv_waves) ||
         !valid_waves(c.update_waves) || !valid_waves(c.finalize_waves) || !valid_unroll(c.spmv_unroll) ||
         (c.spmv_branch_free != 0 && c.spmv_branch_free != 1) || (c.spmv_symmetric != 0 && c.spmv_symmetric != 1) ||
-        (c.spmv_packed != 0 && c.spmv_packed != 1) || (c.spmv_index16 != 0 && c.spmv_index16 != 1))
+        (c.spmv_packed != 0 && c.spmv_packed != 1) || (c.spmv_index16 != 0 && c.spmv_index16 != 1) ||
+        c.pcg_split_advance < -1 || c.pcg_split_advance > 1)
         throw std::invalid_argument("launch config: waves must be 1, 2, 4, 8 or 16 and unroll 1, 2 or 4");
     if (c.residual_waves != 16 || c.spmv_waves != 16 || c.update_waves != 16)
         throw std::invalid_argument(
@@ -736,7 +759,7 @@ void Problem::launch_pcg_chunks(hipStream_t s, int count) {
     }
 }
 void Problem::launch_finalize(float dt, const DevExt& ext, hipStream_t s) {
-    chysx_amd::launch_finalize(geom(), vectors(), ext, params_.data(), dt, cfg_.finalize_waves, s);
+    chysx_amd::launch_finalize(geom(), vectors(), ext, params_.data(), dt, mesh(), cfg_.finalize_waves, s);
 }
 void Problem::read_states(hipStream_t s) {
     check_hip(hipMemcpyAsync(host_states_, state_.data(), sizeof(InstState) * ninst_, hipMemcpyDeviceToHost, s),
@@ -838,7 +861,7 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
     launch_ws_finish(red, 3, state_.data(), ws_inst_.data(), ninst_, kWsPhaseBase, proj_ran_ ? 1 : 0, s);
     launch_ws_features(geom(), matrix(), v, ext, mesh(), rest_pos_.data(), h, xb_.data(), ws_inst_.data(), dt,
                        N.features(), N.edge_features(), s);
-    N.forward(ws_graph(), s, ws_train_);
+    N.forward(ws_graph(), s, ws_train_, !ws_train_);
     launch_ws_direction(geom(), matrix(), N.out(), ws_inst_.data(), rb_.data(), d_.data(), ad_.data(), red, s);
     launch_ws_finish(red, 4, state_.data(), ws_inst_.data(), ninst_, kWsPhaseLine, ws_accept_residual_ ? 1 : 0, s);
     if (learned) launch_ws_line_apply(geom(), ws_inst_.data(), d_.data(), x_.data(), s);
@@ -847,11 +870,37 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
 
 void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
     if (!built_ || ninst_ == 0) return;
+    wait_pending(s);
+    stats_pending_ = false;
     launch_assemble(dt, ext, s);
     ws_ran_ = false;
     proj_ran_ = false;
     const bool ws_on = ws_mode_ != kWarmPrevious || ws_train_;
-    // The projection and the network's inputs need two previous solutions.
+    // Every instance at a fixed iteration count (tol == 0): the number of
+    // rounds is known, so the whole solve is enqueued without a poll.
+    bool fixed = !ws_on && !params_h_.empty();
+    int fixed_iters = 0;
+    for (const InstParams& q : params_h_) {
+        fixed = fixed && q.tol2 == 0.0;
+        fixed_iters = std::max(fixed_iters, q.max_iter);
+    }
+    if (fixed && !blocking_) {
+        launch_pcg_start(s);
+        // max_iter iterations plus a flush round and the advance that stops
+        // the solve; rounds after the stop exit at once
+        launch_pcg_chunks(s, std::max(1, (fixed_iters + 2 + kChunkIters - 1) / kChunkIters));
+        // the true residual of every stopped solve, for the statistics
+        launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 1, s);
+        launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, geom().ngroups, kPhaseFinal, s);
+        launch_finalize(dt, ext, s);
+        check_hip(hipMemcpyAsync(host_states_, state_.data(), sizeof(InstState) * ninst_, hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync(states)");
+        if (!done_event_) check_hip(hipEventCreateWithFlags(&done_event_, hipEventDisableTiming), "hipEventCreate");
+        check_hip(hipEventRecord(done_event_, s), "hipEventRecord(step)");
+        stats_pending_ = true;
+        last_iters_ = fixed_iters;
+        return;
+    }
     if (ws_on && hist_n_ >= 2) warm_start_stage(dt, ext, s);
     launch_pcg_start(s);
     // Launch as many rounds as the previous step used (its iterations plus a
@@ -893,6 +942,16 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
                   "hipMemcpyAsync(warm start stats)");
         check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(warm start stats)");
     }
+    fill_stats();
+    if (ws_ran_ && ws_train_) {
+        double l = 0.0;
+        for (int k = 0; k < ninst_; ++k) l += ws_host_[size_t(k)].loss;
+        ws_loss_ = l / double(ninst_);
+    }
+    last_iters_ = max_it;
+}
+
+void Problem::fill_stats() const {
     for (int k = 0; k < ninst_; ++k) {
         SolveStats& q = stats_[size_t(k)];
         q.iters = host_states_[k].iters;
@@ -908,12 +967,16 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
         q.ws_r0_proj = proj_ran_ ? std::sqrt(W.rr_proj / wb) : 0.0;
         q.ws_r0_learned = ws_ran_ ? std::sqrt(W.rr_cand / wb) : 0.0;
     }
-    if (ws_ran_ && ws_train_) {
-        double l = 0.0;
-        for (int k = 0; k < ninst_; ++k) l += ws_host_[size_t(k)].loss;
-        ws_loss_ = l / double(ninst_);
-    }
-    last_iters_ = max_it;
+}
+
+void Problem::resolve_stats() const {
+    if (!stats_pending_) return;
+    check_hip(hipEventSynchronize(done_event_), "hipEventSynchronize(step)");
+    stats_pending_ = false;
+    for (int k = 0; k < ninst_; ++k)
+        if (host_states_[k].status == kRunning)
+            throw std::runtime_error("PCG: a fixed-iteration solve did not end within its enqueued rounds");
+    fill_stats();
 }
 
 // ---- reference-layout copies --------------------------------------------
@@ -982,6 +1045,7 @@ void Problem::copy_dx(int k, float* out) const {
 }
 void Problem::set_dx64(int k, const double* in, hipStream_t s) {
     const int r0 = inst_row0_[size_t(k)];
+    wait_pending(s);
     check_hip(hipMemcpyAsync(x_.data() + 3 * size_t(r0), in, sizeof(double) * 3 * inst_n_[size_t(k)],
                              hipMemcpyHostToDevice, s),
               "hipMemcpyAsync(dx)");
@@ -989,6 +1053,7 @@ void Problem::set_dx64(int k, const double* in, hipStream_t s) {
     reset_history();  // the caller's guess is used as given
 }
 void Problem::zero_dx(hipStream_t s) {
+    wait_pending(s);
     x_.zero(s);
     reset_history();
 }
@@ -1328,6 +1393,11 @@ void ClothBatch::step(float dt, std::uintptr_t stream) {
     DevExt e{reinterpret_cast<float*>(pos_), reinterpret_cast<float*>(vel_),
              reinterpret_cast<const float*>(inv_mass_), reinterpret_cast<const float*>(force_)};
     prob_.step(dt, e, s);
+}
+
+void ClothBatch::set_pin_projection(bool on) {
+    for (auto& I : insts_) I->pin_projection = on;
+    params_dirty_ = true;
 }
 
 SolveStats ClothBatch::stats(int k) const {

@@ -83,6 +83,8 @@ struct Instance {
     std::vector<int> pin_idx;
     std::vector<float> pin_tgt;
     float pin_k = 1.0e6f;
+    // pins as constraints (extension): displacement fixed at target - x_n
+    bool pin_projection = false;
     // options
     bool springs_active = false;
     bool force_active = false;
@@ -114,6 +116,13 @@ struct LaunchConfig {
     // 1: the SpMV pass reads 16-bit column and mirror offsets when every
     // offset of the mesh fits (speed only: the same indices).
     int spmv_index16 = 1;
+    // 1: each PCG iteration advances the solver state in a small kernel of
+    // its own (one workgroup per instance) before the SpMV pass, instead of
+    // in every SpMV workgroup.  The in-pass advance costs each generation of
+    // SpMV workgroups its start-up, so the kernel of its own wins on large
+    // problems (speed only: the same sums, in the same order, and the same
+    // scalar step).  -1: chosen from the problem's size at build time.
+    int pcg_split_advance = -1;
 };
 
 struct TuneSample {
@@ -161,11 +170,27 @@ public:
     void upload_params(const std::vector<const Instance*>& insts, hipStream_t s);
     void update_pin_targets(const Instance& inst, int inst_index, hipStream_t s);
 
-    // One step over all instances.  Blocks until every PCG solve has ended.
+    // One step over all instances.  With a tolerance on any instance, or a
+    // warm-start stage, it polls the solver states between chunks of
+    // iterations, so it returns once every solve has ended.  When every
+    // instance runs a fixed iteration count (tol == 0, the reference's mode)
+    // the count is known: the step enqueues it all on `s` and returns without
+    // waiting (unless set_blocking_step(true)); stats() then waits for it.
     void step(float dt, const DevExt& ext, hipStream_t s);
+    void set_blocking_step(bool on) { blocking_ = on; }
+    bool blocking_step() const { return blocking_; }
+    // True when the last step was enqueued without waiting and its results
+    // have not been read yet.
+    bool step_pending() const { return stats_pending_; }
+    // Orders stream `s` after an enqueued step (no-op otherwise); every
+    // writer of the problem's device buffers calls it first.
+    void wait_pending(hipStream_t s) const;
 
-    // Per-instance results of the last step.
-    std::vector<SolveStats> stats() const { return stats_; }
+    // Per-instance results of the last step (waits for an enqueued step).
+    std::vector<SolveStats> stats() const {
+        resolve_stats();
+        return stats_;
+    }
     int n_rows_padded() const { return n_pad_; }
     int ninst() const { return ninst_; }
     int nslices() const { return int(slices_h_.size()); }
@@ -245,6 +270,15 @@ public:
     static constexpr int kChunkIters = 8;
     static constexpr int kAutotuneMinRows = 1 << 15;
     static constexpr int kAutotuneReps = 8;
+    // Groups from which the solver state is advanced in a kernel of its own
+    // (pcg_split_advance = -1).  Each generation of resident SpMV workgroups
+    // repeats the in-pass advance's start-up; measured on MI355X the kernel of
+    // its own is as fast at 1024 groups (1024^2) and faster beyond.
+    static constexpr int kSplitAdvanceGroups = 512;
+    bool split_advance() const {
+        return cfg_.pcg_split_advance > 0 || (cfg_.pcg_split_advance < 0 && ninst_ == 1 &&
+                                              int(slices_h_.size()) / kGroupSlices >= kSplitAdvanceGroups);
+    }
 
     const LaunchConfig& launch_config() const { return cfg_; }
     void set_launch_config(const LaunchConfig& c);
@@ -257,6 +291,11 @@ public:
 
 private:
     void capture_chunk_graph(int level);  // a graph of 2^level chunks
+    // Copies host_states_ (and the warm-start records) into stats_.
+    void fill_stats() const;
+    // Waits for an enqueued step's states, checks that every solve ended, and
+    // fills stats_.
+    void resolve_stats() const;
     void destroy_graph();
     void warm_start_stage(float dt, const DevExt& ext, hipStream_t s);
     void push_history(hipStream_t s);
@@ -307,8 +346,11 @@ private:
     DeviceArray<int> gactive_;  // batches: 1 per group of a running instance
 
     InstState* host_states_ = nullptr;  // pinned
-    std::vector<SolveStats> stats_;
+    mutable std::vector<SolveStats> stats_;
     int last_iters_ = 0;
+    bool blocking_ = false;
+    mutable bool stats_pending_ = false;  // host_states_ is being copied for an enqueued step
+    hipEvent_t done_event_ = nullptr;     // recorded after that copy
 
     LaunchConfig cfg_;
     std::vector<TuneSample> tune_log_;
@@ -359,6 +401,8 @@ public:
     double pcg_tolerance() const { return inst_.tol; }
     void set_springs_active(bool on);
     void set_external_force_active(bool on) { inst_.force_active = on; params_dirty_ = true; }
+    void set_pin_projection(bool on) { inst_.pin_projection = on; params_dirty_ = true; }
+    bool pin_projection() const { return inst_.pin_projection; }
     void set_autotune(bool on) { autotune_ = on; }
     void step(float dt, std::uintptr_t stream);
 
@@ -421,6 +465,7 @@ public:
                               std::uintptr_t force) noexcept;
     void set_material(int k, const Material& m);
     void set_options(double tol, int max_iter, bool springs_active, bool force_active);
+    void set_pin_projection(bool on);
     void set_autotune(bool on) { autotune_ = on; }
     void step(float dt, std::uintptr_t stream);
     int count() const { return int(insts_.size()); }

@@ -413,6 +413,29 @@ PYBIND11_MODULE(_chysx_native, m) {
                  return out;
              })
         .def(
+            "set_pin_projection", [](Sim& s, bool on) { check(chysx_cloth_set_pin_projection(s.p(), on ? 1 : 0)); },
+            py::arg("on"),
+            "True: pins are constraints (each pinned particle ends every step exactly at its target); False "
+            "(default): the reference's penalty pins.")
+        .def("pin_projection",
+             [](const Sim& s) {
+                 int on = 0;
+                 check(chysx_cloth_pin_projection(s.p(), &on));
+                 return on != 0;
+             })
+        .def(
+            "set_blocking_step", [](Sim& s, bool on) { check(chysx_cloth_set_blocking_step(s.p(), on ? 1 : 0)); },
+            py::arg("on"),
+            "True: a fixed-iteration step (tolerance 0) waits until its solve has ended; False (default): it "
+            "returns once its work is enqueued on the stream.")
+        .def(
+            "spmv",
+            [](Sim& s, std::uintptr_t x, std::uintptr_t y, std::uintptr_t stream) {
+                check(chysx_cloth_spmv(s.p(), x, y, stream));
+            },
+            py::arg("x_ptr"), py::arg("y_ptr"), py::arg("cuda_stream") = 0,
+            "y = A x on the last assembled system (general storage only); device float3 pointers.")
+        .def(
             "time_kernel",
             [](Sim& s, int which, int reps, float dt, std::uintptr_t stream) {
                 double ms = 0.0;
@@ -438,6 +461,7 @@ PYBIND11_MODULE(_chysx_native, m) {
                  d["spmv_symmetric"] = c[7];
                  d["spmv_packed"] = c[8];
                  d["spmv_index16"] = c[9];
+                 d["pcg_split_advance"] = c[10];
                  return d;
              })
         .def(
@@ -447,7 +471,8 @@ PYBIND11_MODULE(_chysx_native, m) {
                 check(chysx_cloth_launch_config(s.p(), c));
                 const char* keys[CHYSX_LAUNCH_CONFIG_LEN] = {
                     "assemble_waves", "residual_waves",   "spmv_waves",     "spmv_unroll", "update_waves",
-                    "finalize_waves", "spmv_branch_free", "spmv_symmetric", "spmv_packed", "spmv_index16"};
+                    "finalize_waves", "spmv_branch_free", "spmv_symmetric", "spmv_packed", "spmv_index16",
+                    "pcg_split_advance"};
                 for (int i = 0; i < CHYSX_LAUNCH_CONFIG_LEN; ++i)
                     if (d.contains(keys[i])) c[i] = d[keys[i]].cast<int>();
                 check(chysx_cloth_set_launch_config(s.p(), c));
@@ -629,6 +654,12 @@ PYBIND11_MODULE(_chysx_native, m) {
             "step", [](Batch& b, float dt, std::uintptr_t stream) { check(chysx_batch_step(b.h, dt, stream)); },
             py::arg("dt"), py::arg("cuda_stream") = 0)
         .def(
+            "set_blocking_step", [](Batch& b, bool on) { check(chysx_batch_set_blocking_step(b.h, on ? 1 : 0)); },
+            py::arg("on"))
+        .def(
+            "set_pin_projection",
+            [](Batch& b, bool on) { check(chysx_batch_set_pin_projection(b.h, on ? 1 : 0)); }, py::arg("on"))
+        .def(
             "stats",
             [](const Batch& b, int k) {
                 int it = 0, rs = 0, st = 0;
@@ -693,6 +724,15 @@ PYBIND11_MODULE(_chysx_native, m) {
             return py::make_tuple(c, kept);
         },
         py::arg("G"), py::arg("g"));
+    m.def(
+        "stream_bench",
+        [](int kind, std::uint64_t bytes, int reps, std::uintptr_t stream) {
+            double ms = 0.0, moved = 0.0;
+            check(chysx_stream_bench(kind, bytes, reps, stream, &ms, &moved));
+            return py::make_tuple(ms, moved);
+        },
+        py::arg("kind"), py::arg("bytes_per_buffer"), py::arg("reps"), py::arg("cuda_stream") = 0,
+        "HBM probe: kind 0 copy, 1 triad, 2 / 3 non-temporal; returns (ms per launch, bytes per launch).");
     m.def("ws_num_params", []() {
         int n = 0;
         check(chysx_ws_num_params(&n));
@@ -700,23 +740,26 @@ PYBIND11_MODULE(_chysx_native, m) {
     });
     m.def(
         "wsnet_forward",
-        [](FltArr theta, IntArr ro, IntArr cols, FltArr F, FltArr E) {
+        [](FltArr theta, IntArr ro, IntArr cols, FltArr F, FltArr E, bool direction_only) {
             const int n = int(ro.size()) - 1;
             if (F.ndim() != 2 || F.shape(0) != n || F.shape(1) != 32 || E.ndim() != 2 || E.shape(0) != n ||
                 E.shape(1) != 32)
                 throw py::value_error("wsnet_forward: F and E must be (n, 32)");
             chysx_wsnet* net = nullptr;
             check(chysx_wsnet_create(&net));
-            py::array_t<float> out({n, 32});
+            py::array_t<float> out({n, direction_only ? 3 : 32});
             const int rc1 = chysx_wsnet_set_params(net, theta.data(), int(theta.size()));
             const int rc2 = rc1 ? rc1
-                                : chysx_wsnet_forward(net, n, ro.data(), cols.data(), F.data(), E.data(),
-                                                      out.mutable_data());
+                                : direction_only ? chysx_wsnet_forward_direction(net, n, ro.data(), cols.data(),
+                                                                                 F.data(), E.data(), out.mutable_data())
+                                                 : chysx_wsnet_forward(net, n, ro.data(), cols.data(), F.data(),
+                                                                       E.data(), out.mutable_data());
             chysx_wsnet_destroy(net);
             check(rc2);
             return out;
         },
-        py::arg("theta"), py::arg("row_offsets"), py::arg("cols"), py::arg("F"), py::arg("E"));
+        py::arg("theta"), py::arg("row_offsets"), py::arg("cols"), py::arg("F"), py::arg("E"),
+        py::arg("direction_only") = false);
     m.def(
         "wsnet_loss_grad",
         [](FltArr theta, IntArr ro, IntArr cols, FltArr F, FltArr E, FltArr target) {

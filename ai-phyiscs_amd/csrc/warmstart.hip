@@ -128,6 +128,146 @@ __global__ __launch_bounds__(256) void ws_gemm_kernel(GemmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Inference layer, fused: h' = relu([h | mean_{j in N(i)} h_j | e] W + b) + h
+// for the 64 rows of one slice per wave, the neighbour mean formed while the
+// MFMA operand is built (it never goes to memory).  The MFMA operands are
+// swapped (weights as A, rows as B), so each lane ends with four consecutive
+// output columns of one row and stores them as one 8-byte vector.  With LAST
+// the decoder follows in the epilogue: only the output columns the warm start
+// uses (0..2) are formed, from the BF16-rounded h' (as the decoder GEMM would
+// read it), by FP32 FMAs and a sum over the four lanes of a row; h' itself is
+// not stored.  The neighbour mean is bitwise that of ws_agg_kernel (same
+// order, same 1/deg, same BF16 rounding).
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+constexpr int kDirCols = 3;  // output columns the warm start reads
+
+struct LayerArgs {
+    int nslices;
+    const int* slice_ptr;
+    const int* col;
+    const int* row_deg;
+    const __bf16* h;     // rows x HID
+    const __bf16* e;     // rows x 32
+    const __bf16* w;     // (HID, XL): the layer's weights, (NOUT, K) layout
+    const float* bias;   // HID
+    __bf16* h_out;       // rows x HID (not LAST)
+    const __bf16* wd;    // (NOUTD, HID): the decoder's weights (LAST)
+    const float* bd;     // NOUTD (LAST)
+    float* out;          // rows x NOUTD, columns 0..kDirCols-1 written (LAST)
+};
+
+// Mean over the slice lane's neighbours of h[., c0 .. c0 + 7], as BF16.
+__device__ __forceinline__ bf16x8 neighbour_mean8(const LayerArgs& a, int s, int slane, int row, int c0) {
+    const int sb = a.slice_ptr[s], w = a.slice_ptr[s + 1] - sb;
+    const int* colp = a.col + (size_t)sb * 64 + slane;
+    float m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k0 = 0; k0 < w; k0 += 4) {
+        int c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = (k0 + u < w) ? colp[(k0 + u) * 64] : -1;
+        bf16x8 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (c[u] >= 0) v[u] = *reinterpret_cast<const bf16x8*>(a.h + (size_t)c[u] * HID + c0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (c[u] >= 0) {
+#pragma unroll
+                for (int t = 0; t < 8; ++t) m[t] += bf2f(v[u][t]);
+            }
+    }
+    const int deg = a.row_deg[row];
+    const float inv = deg > 0 ? 1.0f / float(deg) : 0.0f;
+    bf16x8 r;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) r[t] = f2bf(m[t] * inv);
+    return r;
+}
+
+template <bool LAST>
+__global__ __launch_bounds__(256) void ws_layer_kernel(LayerArgs a) {
+    constexpr int NT = HID / 16;
+    constexpr int MH = 2;  // row tiles of 16 per pass: the wave's 64 rows in two halves (fewer registers)
+    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    if (s >= a.nslices) return;
+    const int row0 = s * 64;
+    const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
+    for (int half = 0; half < 4 / MH; ++half) {
+        f32x4 acc[MH][NT];
+#pragma unroll
+        for (int mt = 0; mt < MH; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        // K = [h (2 chunks of 32) | mean (2 chunks) | e (1 chunk)], in that order
+        for (int kc = 0; kc < XL / 32; ++kc) {
+            bf16x8 xf[MH];
+#pragma unroll
+            for (int mt = 0; mt < MH; ++mt) {
+                const int slane = (half * MH + mt) * 16 + li, row = row0 + slane;
+                if (kc < 2)
+                    xf[mt] = *reinterpret_cast<const bf16x8*>(a.h + (size_t)row * HID + 32 * kc + 8 * lg);
+                else if (kc < 4)
+                    xf[mt] = neighbour_mean8(a, s, slane, row, 32 * (kc - 2) + 8 * lg);
+                else
+                    xf[mt] = *reinterpret_cast<const bf16x8*>(a.e + (size_t)row * 32 + 8 * lg);
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const bf16x8 wf =
+                    *reinterpret_cast<const bf16x8*>(a.w + (size_t)(nt * 16 + li) * XL + 32 * kc + 8 * lg);
+#pragma unroll
+                for (int mt = 0; mt < MH; ++mt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xf[mt], acc[mt][nt], 0, 0, 0);
+            }
+        }
+        // lane (li, lg): acc[mt][nt][r] = output (row0 + 16 (half MH + mt) + li, column 16 nt + 4 lg + r)
+#pragma unroll
+        for (int mt = 0; mt < MH; ++mt) {
+            const int row = row0 + (half * MH + mt) * 16 + li;
+            float dsum[kDirCols] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int n0 = nt * 16 + 4 * lg;
+                const bf16x4 res = *reinterpret_cast<const bf16x4*>(a.h + (size_t)row * HID + n0);
+                const float4 bb = *reinterpret_cast<const float4*>(a.bias + n0);
+                const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+                bf16x4 o;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    float v = fmaxf(acc[mt][nt][r] + bv[r], 0.0f);
+                    v += bf2f(res[r]);
+                    o[r] = f2bf(v);
+                }
+                if constexpr (LAST) {
+#pragma unroll
+                    for (int q = 0; q < kDirCols; ++q) {
+                        const bf16x4 wq = *reinterpret_cast<const bf16x4*>(a.wd + (size_t)q * HID + n0);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) dsum[q] = fmaf(bf2f(o[r]), bf2f(wq[r]), dsum[q]);
+                    }
+                } else {
+                    *reinterpret_cast<bf16x4*>(a.h_out + (size_t)row * HID + n0) = o;
+                }
+            }
+            if constexpr (LAST) {
+#pragma unroll
+                for (int q = 0; q < kDirCols; ++q) {
+                    float t = dsum[q];
+                    t += __shfl_xor(t, 16, 64);
+                    t += __shfl_xor(t, 32, 64);
+                    dsum[q] = t + a.bd[q];
+                }
+                if (lg == 0) {
+#pragma unroll
+                    for (int q = 0; q < kDirCols; ++q) a.out[(size_t)row * NOUTD + q] = dsum[q];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient: part[wg] = sum over this workgroup's 32-row chunks of
 // X^T dY (K x NOUT) and column sums of dY (bias gradient).  Chunks go to the
 // 4 waves of a workgroup round-robin; waves combine in order through LDS, so
@@ -540,7 +680,7 @@ void WarmStartNet::copy_grad(float* host, hipStream_t s) const {
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
-void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep) {
+void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep, bool direction_only) {
     const int rows = g.nslices * 64;
     if (rows > rows_) throw std::invalid_argument("warm start: reserve() rows first");
     if (keep && !training_) throw std::invalid_argument("warm start: reserve(training) before a kept forward");
@@ -558,26 +698,53 @@ void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep) {
     a.out_b = H_[0];
     a.pre_b = keep ? P_[0] : nullptr;
     run_gemm<HID / 16>(a, s);
-    for (int l = 0; l < NL; ++l) {
-        hipLaunchKernelGGL(ws_agg_kernel, dim3((g.nslices + 3) / 4), dim3(256), 0, s, g, H_[l], A_[l]);
-        check_hip(hipGetLastError(), "ws_agg_kernel");
-        GemmArgs b{};
-        b.rows = rows;
-        b.seg[0] = H_[l];
-        b.segw[0] = HID;
-        b.seg[1] = A_[l];
-        b.segw[1] = HID;
-        b.seg[2] = E_;
-        b.segw[2] = 32;
-        b.nseg = 3;
-        b.K = XL;
-        b.wnk = wnk_ + off_w(l);
-        b.bias = theta_ + off_b(l);
-        b.relu = 1;
-        b.resid = H_[l];
-        b.out_b = H_[l + 1];
-        b.pre_b = keep ? P_[l + 1] : nullptr;
-        run_gemm<HID / 16>(b, s);
+    if (!keep) {
+        // inference: each layer in one fused kernel; with direction_only the
+        // decoder runs in the last one's epilogue (output columns 0..2, the
+        // ones the warm start uses), otherwise as a GEMM below
+        for (int l = 0; l < NL; ++l) {
+            LayerArgs la{};
+            la.nslices = g.nslices;
+            la.slice_ptr = g.slice_ptr;
+            la.col = g.col;
+            la.row_deg = g.row_deg;
+            la.h = H_[l];
+            la.e = E_;
+            la.w = wnk_ + off_w(l);
+            la.bias = theta_ + off_b(l);
+            la.h_out = H_[l + 1];
+            la.wd = wnk_ + OFF_WD;
+            la.bd = theta_ + OFF_BD;
+            la.out = out_;
+            if (l + 1 < NL || !direction_only)
+                hipLaunchKernelGGL(ws_layer_kernel<false>, dim3(grid_rows(rows)), dim3(256), 0, s, la);
+            else
+                hipLaunchKernelGGL(ws_layer_kernel<true>, dim3(grid_rows(rows)), dim3(256), 0, s, la);
+            check_hip(hipGetLastError(), "ws_layer_kernel");
+        }
+        if (direction_only) return;
+    } else {
+        for (int l = 0; l < NL; ++l) {
+            hipLaunchKernelGGL(ws_agg_kernel, dim3((g.nslices + 3) / 4), dim3(256), 0, s, g, H_[l], A_[l]);
+            check_hip(hipGetLastError(), "ws_agg_kernel");
+            GemmArgs b{};
+            b.rows = rows;
+            b.seg[0] = H_[l];
+            b.segw[0] = HID;
+            b.seg[1] = A_[l];
+            b.segw[1] = HID;
+            b.seg[2] = E_;
+            b.segw[2] = 32;
+            b.nseg = 3;
+            b.K = XL;
+            b.wnk = wnk_ + off_w(l);
+            b.bias = theta_ + off_b(l);
+            b.relu = 1;
+            b.resid = H_[l];
+            b.out_b = H_[l + 1];
+            b.pre_b = keep ? P_[l + 1] : nullptr;
+            run_gemm<HID / 16>(b, s);
+        }
     }
     GemmArgs d{};
     d.rows = rows;

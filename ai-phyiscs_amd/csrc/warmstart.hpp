@@ -58,8 +58,12 @@ public:
     // Inputs written by the caller before forward().
     __bf16* features() { return F_; }
     __bf16* edge_features() { return E_; }
-    // Forward: out (rows x 32, FP32).  keep = store activations for backward.
-    void forward(const WsGraph& g, hipStream_t s, bool keep);
+    // Forward: out (rows x 32, FP32).  keep = store activations for backward
+    // (one kernel per aggregation and per GEMM).  Without keep each layer is
+    // one fused kernel (aggregation inside the GEMM's operand load); with
+    // direction_only the decoder is folded into the last layer and only the
+    // output columns 0..2 (the warm-start direction) are written.
+    void forward(const WsGraph& g, hipStream_t s, bool keep, bool direction_only = false);
     const float* out() const { return out_; }
     // Loss against target (rows x 3, FP32; rows with valid[row] == 0 or
     // beyond a slice's valid count are ignored): mean over valid rows of the

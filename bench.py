@@ -58,6 +58,34 @@ def spmv_layout_bytes(n_rows: int, nnz: int, storage: str, index_bits: int) -> f
     return 12.0 * nnz + ib * nnz + ib * nnz / 2 + 24.0 * n_rows + vectors
 
 
+def batch_roofline(sc, iters, ms_per_step):
+    """HBM roofline of the batch step: every cloth's PCG iterations times the
+    bytes one iteration of that cloth moves on the default storage (packed
+    symmetric blocks, 16-bit offsets: SpMV pass 15 nnz + 72 N, update pass
+    72 N or 144 N alternately), plus each solve's start (residual pass 18 nnz
+    + 120 N) and the assembly (upper blocks 12 nnz, diagonal, inverse and
+    rhs 60 N written; positions, velocities, masses 28 N and element data
+    read), over the measured time per step."""
+    import numpy as np
+    from chysx import scenes
+    total = 0.0
+    for s, it in zip(sc, iters):
+        n = s.n
+        e = scenes.edges_of(s.tris).shape[0]
+        nnz = 2 * e
+        t = s.tris.shape[0]
+        per_iter = 15.0 * nnz + 72.0 * n + 108.0 * n
+        start = 18.0 * nnz + 120.0 * n
+        asm = 12.0 * nnz + 88.0 * n + t * (12 + 16 + 4 + 4) + e * (8 + 4 + 4) + 8.0 * 3 * t + 8.0 * 2 * e
+        total += it * per_iter + start + asm
+    achieved = total / (ms_per_step * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "whole batch step (assembly, PCG start, every cloth's iterations)",
+            "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "bytes_per_step": total, "ms_per_step": round(ms_per_step, 3),
+            "iters_weighted_by_rows": round(float(np.average(iters, weights=[s.n for s in sc])), 2)}
+
+
 def bench_batch(args):
     """512 cloths, W ~ U{16..128}, per-cloth k, mass and pins, stepped
     together: one launch per kernel, per-cloth PCG state and convergence."""
@@ -79,6 +107,7 @@ def bench_batch(args):
     if not args.no_cpu:
         cpu_state = (b.pos.cpu().numpy().astype(np.float64), b.vel.cpu().numpy().astype(np.float64))
     it_mean, it_max = [], []
+    it_sum = np.zeros(len(sc))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -87,11 +116,13 @@ def bench_batch(args):
         if not all(x["converged"] for x in st):
             raise RuntimeError("a cloth did not converge")
         its = [x["iters"] for x in st]
+        it_sum += np.array(its, np.float64)
         it_mean.append(np.mean(its))
         it_max.append(np.max(its))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     n_total = int(sum(s.n for s in sc))
+    roofline = batch_roofline(sc, it_sum / args.steps, 1e3 * elapsed / args.steps)
     ws_cmp = None
     if args.ws_steps > 0:
         del b
@@ -101,7 +132,7 @@ def bench_batch(args):
     cpu = None
     if cpu_state is not None:
         from oracle.oracle import oracle_from_scene
-        threads = min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads
         pos_all, vel_all = cpu_state
         secs_total, rows_done, its = 0.0, 0, 0
         order = np.random.default_rng(0).permutation(len(sc))[:48]
@@ -120,7 +151,8 @@ def bench_batch(args):
         est = secs_total * n_total / rows_done
         cpu = {"value": round(1.0 / est, 5), "unit": "steps/s", "cores": threads, "kind": "port",
                "sample": f"FP64 oracle stepping 48 randomly chosen cloths of the batch (one step each, cold start, "
-                         f"{rows_done} particles, {secs_total:.2f} s), scaled by particle count to all 512 cloths"}
+                         f"{rows_done} particles, {secs_total:.2f} s), scaled by particle count to all 512 cloths",
+               "host_cpu": host_cpu_model(), "host_nproc": os.cpu_count()}
     out = {
         "metric": METRIC,
         "value": round(args.steps / elapsed, 3),
@@ -141,7 +173,7 @@ def bench_batch(args):
         "pcg_iters_per_step_max": round(float(np.mean(it_max)), 2),
         "warm_start_comparison": ws_cmp,
         "setup_s": round(setup_s, 2),
-        "roofline": None,
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -249,6 +281,222 @@ def batch_warm_start_comparison(args, sc, make_batch):
     return out
 
 
+def gbs(b, ms):
+    return b / (ms * 1e-3) / 1e9
+
+
+def storage_of(g):
+    cfg = g.sim.launch_config()
+    return ("general" if not cfg["spmv_symmetric"] else
+            "symmetric_packed" if g.sim.block_floats() == 6 else "symmetric")
+
+
+def time_storages(g, s, n, nnz, args, stream, names):
+    """The SpMV pass (with and without the state advance) and a whole
+    iteration on each matrix storage in `names` (same matrix values,
+    re-assembled in that storage; speed only), and the plain SpMV on the
+    general storage.  Leaves the simulator in its own storage."""
+    cfg0 = g.sim.launch_config()
+    table = {"general": (0, 0), "symmetric": (1, 0), "symmetric_packed": (1, 1)}
+    storage, t_spmv = {}, None
+    for name in names:
+        sym, packed = table[name]
+        g.sim.set_launch_config(dict(spmv_symmetric=sym, spmv_packed=packed))
+        g.sim.time_kernel(3, 1, s.dt, stream)  # one assembly into this storage
+        t_sp = g.sim.time_kernel(1, args.kernel_reps, s.dt, stream)
+        t_na = g.sim.time_kernel(5, args.kernel_reps, s.dt, stream)
+        t_it = g.sim.time_kernel(4, args.kernel_reps, s.dt, stream)
+        lb = spmv_layout_bytes(n, nnz, name, g.sim.index_bits())
+        storage[name] = {"block_floats": g.sim.block_floats(), "index_bits": g.sim.index_bits(),
+                         "layout_bytes": lb, "ms": round(t_sp, 5),
+                         "GB/s_layout": round(gbs(lb, t_sp), 1),
+                         "frac_layout": round(gbs(lb, t_sp) / PEAK_HBM_GBS, 4),
+                         "GB/s_ref_formulation": round(gbs(kernel_bytes(n, nnz)["pcg_spmv"], t_sp), 1),
+                         # the same pass without the solver-state advance it runs first
+                         "ms_no_advance": round(t_na, 5),
+                         "frac_layout_no_advance": round(gbs(lb, t_na) / PEAK_HBM_GBS, 4),
+                         "iteration_ms": round(t_it, 5)}
+        if name == "general":
+            t_spmv = g.sim.time_kernel(0, args.kernel_reps, s.dt, stream)  # plain SpMV reads general storage
+    g.sim.set_launch_config(dict(spmv_symmetric=cfg0["spmv_symmetric"], spmv_packed=cfg0["spmv_packed"]))
+    g.sim.time_kernel(3, 1, s.dt, stream)
+    # where the solver state is advanced: inside every SpMV workgroup (0) or
+    # in a kernel of its own before the pass (1); -1 (default) picks by size
+    mode = storage_of(g)
+    adv = {}
+    for split in (0, 1):
+        g.sim.set_launch_config(dict(pcg_split_advance=split))
+        adv[f"split_{split}"] = {"spmv_pass_ms": round(g.sim.time_kernel(1, args.kernel_reps, s.dt, stream), 5),
+                                 "iteration_ms": round(g.sim.time_kernel(4, args.kernel_reps, s.dt, stream), 5)}
+    g.sim.set_launch_config(dict(pcg_split_advance=cfg0["pcg_split_advance"]))
+    storage[mode]["advance_variants"] = adv
+    return storage, t_spmv
+
+
+def spmv_roofline(storage, mode, kb, traffic):
+    st = storage[mode]
+    lb, ms = st["layout_bytes"], st["ms"]
+    return {
+        "bound": "hbm",
+        "kernel": f"pcg_spmv_kernel, {mode} storage, {st['index_bits']}-bit indices "
+                  "(solver-state advance; p = z + beta p_prev; Ap = A p; p.Ap)",
+        "achieved": round(gbs(lb, ms), 1),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(gbs(lb, ms) / PEAK_HBM_GBS, 4),
+        "traffic": traffic.get({"symmetric_packed": "pcg_spmv_kernel", "symmetric": "pcg_spmv_kernel_symmetric",
+                                "general": "pcg_spmv_kernel_general"}[mode]),
+        "bytes_per_launch": lb,
+        "ms_per_launch": round(ms, 5),
+        "ref_formulation_bytes": kb["pcg_spmv"],
+        "ref_formulation_GB/s": round(gbs(kb["pcg_spmv"], ms), 1),
+    }
+
+
+def hbm_kernels(args, make_sim):
+    """PCG kernels of the wind cloth at --hbm-size (default 2048^2: the packed
+    SpMV pass touches ~680 MB and an iteration ~930 MB, several times the
+    256 MB Infinity Cache), on the default storage and on the general one."""
+    import numpy as np
+    import torch
+    from chysx import scenes
+    w = args.hbm_size
+    s = scenes.config3(w=w, steps=2)
+    g = make_sim(s, tol=args.tol, max_iter=20000)
+    g.sim.step(s.dt)
+    g.sim.step(s.dt)
+    st = g.sim.last_solve_stats()
+    torch.cuda.synchronize()
+    _, ro, _, _, _, _ = g.sim.debug_dump_last_solve()
+    n, nnz = s.n, int(ro[-1])
+    kb = kernel_bytes(n, nnz)
+    stream = torch.cuda.current_stream().cuda_stream
+    mode = storage_of(g)
+    t_up = g.sim.time_kernel(2, args.kernel_reps, s.dt, stream)
+    storage, t_spmv = time_storages(g, s, n, nnz, args, stream, (mode, "general"))
+    iter_bytes = storage[mode]["layout_bytes"] + 0.5 * (72.0 + 144.0) * n  # SpMV pass + mean update pass
+    out = {
+        "workload": f"cloth_{w}x{w}_wind, system of its second step (PCG {st['iters']} iterations)",
+        "particles": n, "off_diag_blocks": nnz,
+        "pcg_spmv_storage": storage,
+        "spmv_plain": {"bytes": kb["spmv"], "ms": round(t_spmv, 5), "GB/s": round(gbs(kb["spmv"], t_spmv), 1),
+                       "frac": round(gbs(kb["spmv"], t_spmv) / PEAK_HBM_GBS, 4)},
+        "pcg_update_adding_two_terms": {"bytes": 144.0 * n, "ms": round(t_up, 5),
+                                        "GB/s": round(gbs(144.0 * n, t_up), 1),
+                                        "frac": round(gbs(144.0 * n, t_up) / PEAK_HBM_GBS, 4)},
+        "pcg_iteration": {"layout_bytes": iter_bytes, "ms": round(storage[mode]["iteration_ms"], 5),
+                          "GB/s_layout": round(gbs(iter_bytes, storage[mode]["iteration_ms"]), 1),
+                          "frac_layout": round(gbs(iter_bytes, storage[mode]["iteration_ms"]) / PEAK_HBM_GBS, 4),
+                          "ref_bytes": kb["pcg_iteration_ref"],
+                          "GB/s_ref_formulation": round(gbs(kb["pcg_iteration_ref"],
+                                                            storage[mode]["iteration_ms"]), 1)},
+        "spmv_pass_roofline": spmv_roofline(storage, mode, kb, read_traffic(w)),
+    }
+    out["spmv_pass_roofline"]["working_set"] = (f"{w}x{w}: {storage[mode]['layout_bytes'] / 1e6:.0f} MB per "
+                                                "pass, beyond the 256 MB Infinity Cache")
+    del g
+    return out
+
+
+def copy_ceiling(args):
+    """Hand-written copy and triad kernels over --copy-gib GiB buffers: the
+    bandwidth a streaming kernel reaches here, beside the 8 TB/s peak."""
+    import torch
+    from chysx import _chysx_native as nat
+    stream = torch.cuda.current_stream().cuda_stream
+    nbytes = int(args.copy_gib * (1 << 30))
+    out = {"bytes_per_buffer": nbytes}
+    best = 0.0
+    for kind, name in enumerate(["copy", "triad", "copy_nontemporal", "triad_nontemporal"]):
+        ms, moved = nat.stream_bench(kind, nbytes, 20, stream)
+        r = gbs(moved, ms)
+        out[name] = {"ms": round(ms, 5), "bytes": moved, "GB/s": round(r, 1), "frac": round(r / PEAK_HBM_GBS, 4)}
+        best = max(best, r)
+    out["best_GB/s"] = round(best, 1)
+    out["best_frac_of_peak"] = round(best / PEAK_HBM_GBS, 4)
+    return out
+
+
+def host_cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, s, g, cpu_state):
+    """The FP64 oracle (C++ restatement of the reference step, OpenMP) stepping
+    the state the GPU timed region starts from, on --cpu-threads host threads
+    and on 1 thread; then the GPU re-stepped from that state against the
+    oracle's result."""
+    import numpy as np
+    from oracle.oracle import oracle_from_scene
+    p0, v0, dx0 = cpu_state
+    runs = {}
+    for threads in (args.cpu_threads, 1):
+        oc, _, _, im, f = oracle_from_scene(s, precision=64, tol=args.tol, max_iter=20000, threads=threads)
+        oc.build_topology()
+        oc.set_dx(dx0)
+        p, v = p0.copy(), v0.copy()
+        secs, it = oc.time_steps(p, v, im, f, s.dt, args.cpu_steps)  # p, v advance in place
+        runs[threads] = (secs, it, p)
+    secs, it, p_cpu = runs[args.cpu_threads]
+    secs1, _, _ = runs[1]
+    cpu = {"value": round(args.cpu_steps / secs, 5), "unit": "steps/s", "cores": args.cpu_threads, "kind": "port",
+           "sample": f"FP64 oracle (C++ restatement of the reference step, OpenMP) stepping the "
+                     f"{args.size}x{args.size} cloth {args.cpu_steps} step(s) from the state the GPU timed "
+                     f"region starts from; {it} PCG iterations, {secs:.2f} s on {args.cpu_threads} threads",
+           "pcg_iters": it,
+           "one_thread": {"value": round(args.cpu_steps / secs1, 5), "unit": "steps/s", "cores": 1,
+                          "seconds": round(secs1, 3)},
+           "host_cpu": host_cpu_model(), "host_nproc": os.cpu_count(),
+           "threads_note": "the box's CPU share for one GPU is 16 threads, so the multi-thread run uses 16 of "
+                           "the host's cores, not all of them"}
+    # FP32 GPU step vs the FP64 oracle from the same state (both to the same tolerance)
+    g.set_state(p0, v0)
+    g.sim.set_warm_start(dx0)
+    for _ in range(args.cpu_steps):
+        g.sim.step(s.dt)
+    gp = g.positions().astype(np.float64)
+    d_gpu, d_cpu = gp - p0, p_cpu - p0
+    fp_err = {"steps": args.cpu_steps, "tol": args.tol,
+              "max_abs_pos_diff_m": float(np.abs(gp - p_cpu).max()),
+              "rel_pos_diff": float(np.linalg.norm(gp - p_cpu) / np.linalg.norm(p_cpu)),
+              "rel_displacement_diff": float(np.linalg.norm(d_gpu - d_cpu) / max(np.linalg.norm(d_cpu), 1e-30))}
+    return cpu, fp_err
+
+
+def full_run(args, make_sim):
+    """The whole 300-step scenario from rest: PCG iterations of every step,
+    so the timed window's iteration count can be set against the run's."""
+    import numpy as np
+    import torch
+    from chysx import scenes
+    s = scenes.config3(w=args.size)
+    g = make_sim(s, tol=args.tol, max_iter=20000)
+    its = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(s.steps):
+        g.sim.step(s.dt)
+        st = g.sim.last_solve_stats()
+        if not st["converged"]:
+            raise RuntimeError(f"PCG did not converge in the full run: {st}")
+        its.append(int(st["iters"]))
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    its = np.array(its)
+    w0, w1 = args.warmup, args.warmup + args.steps
+    return {"steps": int(s.steps), "steps_per_s_incl_setup_step": round(s.steps / el, 3),
+            "pcg_iters_mean": round(float(its.mean()), 2), "pcg_iters_min": int(its.min()),
+            "pcg_iters_max": int(its.max()),
+            "pcg_iters_timed_window_mean": round(float(its[w0:w1].mean()), 2) if w1 <= len(its) else None,
+            "pcg_iters_per_step": its.tolist()}
+
+
 def read_traffic(size):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
     (recorded in profiles/traffic_cloth<size>.md as
@@ -281,6 +529,23 @@ def main():
     ap.add_argument("--ws-weights", default=None, help="learned warm-start parameters (.npz)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (known bytes) for counter calibration")
+    ap.add_argument("--hbm-size", type=int, default=2048,
+                    help="grid width of the cloth whose PCG kernels give the HBM roofline (its matrix and "
+                         "vectors exceed the 256 MB Infinity Cache); 0 skips it")
+    ap.add_argument("--copy-gib", type=float, default=1.0,
+                    help="GiB per buffer of the hand-written copy / triad HBM probe (0 skips it)")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="host threads of the CPU baseline (the box's CPU share for one GPU); a 1-thread run "
+                         "is reported beside it")
+    ap.add_argument("--warm-start", default="previous",
+                    choices=["previous", "learned", "projection", "projection_learned"],
+                    help="warm start of the timed run (previous: the reference's)")
+    ap.add_argument("--split-advance", type=int, default=None, choices=[-1, 0, 1],
+                    help="where the PCG state is advanced: -1 by problem size (default), 0 inside every SpMV "
+                         "workgroup, 1 in a kernel of its own")
+    ap.add_argument("--full-run", action="store_true", default=True,
+                    help="also step the whole scenario (300 steps) from rest and report iterations per step")
+    ap.add_argument("--no-full-run", dest="full_run", action="store_false")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -300,6 +565,13 @@ def main():
     s = scenes.config3(w=args.size, steps=args.warmup + args.steps)
     t0 = time.perf_counter()
     g = make_sim(s, tol=args.tol, max_iter=20000)
+    if args.split_advance is not None:
+        g.sim.set_launch_config(dict(pcg_split_advance=args.split_advance))
+    if args.warm_start != "previous":
+        from chysx import warmstart as ws
+        if args.warm_start.endswith("learned"):
+            g.sim.ws_set_params(ws.load_params(args.ws_weights or ws.DEFAULT_WEIGHTS)[0])
+        g.sim.set_warm_start_mode(args.warm_start)
     g.sim.step(s.dt)  # builds topology + first solve
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
@@ -332,96 +604,50 @@ def main():
     kb = kernel_bytes(n, nnz)
     # Kernel timings with HIP events on the launch stream (null stream).
     stream = torch.cuda.current_stream().cuda_stream
-    cfg0 = g.sim.launch_config()
-    mode0 = ("general" if not cfg0["spmv_symmetric"] else
-             "symmetric_packed" if g.sim.block_floats() == 6 else "symmetric")
+    mode0 = storage_of(g)
     t_pcg_update = g.sim.time_kernel(2, args.kernel_reps, s.dt, stream)
-    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9  # noqa: E731
-    # The SpMV pass and a whole iteration on each matrix storage (same
-    # matrix values, re-assembled in that storage; speed only).
-    storage = {}
-    for name, sym, packed in [("general", 0, 0), ("symmetric", 1, 0), ("symmetric_packed", 1, 1)]:
-        g.sim.set_launch_config(dict(spmv_symmetric=sym, spmv_packed=packed))
-        g.sim.time_kernel(3, 1, s.dt, stream)  # one assembly into this storage
-        t_sp = g.sim.time_kernel(1, args.kernel_reps, s.dt, stream)
-        t_na = g.sim.time_kernel(5, args.kernel_reps, s.dt, stream)
-        t_it = g.sim.time_kernel(4, args.kernel_reps, s.dt, stream)
-        lb = spmv_layout_bytes(n, nnz, name, g.sim.index_bits())
-        storage[name] = {"block_floats": g.sim.block_floats(), "index_bits": g.sim.index_bits(),
-                         "layout_bytes": lb, "ms": round(t_sp, 5),
-                         "GB/s_layout": round(gbs(lb, t_sp), 1),
-                         "frac_layout": round(gbs(lb, t_sp) / PEAK_HBM_GBS, 4),
-                         "GB/s_ref_formulation": round(gbs(kb["pcg_spmv"], t_sp), 1),
-                         # the same pass without the solver-state advance it runs first
-                         "ms_no_advance": round(t_na, 5),
-                         "frac_layout_no_advance": round(gbs(lb, t_na) / PEAK_HBM_GBS, 4),
-                         "iteration_ms": round(t_it, 5)}
-        if name == "general":
-            t_spmv = g.sim.time_kernel(0, args.kernel_reps, s.dt, stream)  # plain SpMV reads general storage
-    g.sim.set_launch_config(dict(spmv_symmetric=cfg0["spmv_symmetric"], spmv_packed=cfg0["spmv_packed"]))
-    g.sim.time_kernel(3, 1, s.dt, stream)
-    t_pcg_spmv = storage[mode0]["ms"]
-    t_iter = storage[mode0]["iteration_ms"]
-    lb0 = storage[mode0]["layout_bytes"]
-
-    traffic = read_traffic(args.size)
-    roofline = {
-        "bound": "hbm",
-        "kernel": f"pcg_spmv_kernel, {mode0} storage, {storage[mode0]['index_bits']}-bit indices "
-                  "(solver-state advance; p = z + beta p_prev; Ap = A p; p.Ap)",
-        "achieved": round(gbs(lb0, t_pcg_spmv), 1),
-        "peak": PEAK_HBM_GBS,
-        "unit": "GB/s",
-        "frac": round(gbs(lb0, t_pcg_spmv) / PEAK_HBM_GBS, 4),
-        "traffic": traffic.get({"symmetric_packed": "pcg_spmv_kernel", "symmetric": "pcg_spmv_kernel_symmetric",
-                                "general": "pcg_spmv_kernel_general"}[mode0]),
-        "bytes_per_launch": lb0,
-        "ms_per_launch": round(t_pcg_spmv, 5),
-        "ref_formulation_bytes": kb["pcg_spmv"],
-        "ref_formulation_GB/s": round(gbs(kb["pcg_spmv"], t_pcg_spmv), 1),
-    }
+    storage, t_spmv = time_storages(g, s, n, nnz, args, stream, ("general", "symmetric", "symmetric_packed"))
     extra_kernels = {
+        "working_set": f"{args.size}x{args.size}: the SpMV pass touches {storage[mode0]['layout_bytes'] / 1e6:.0f} MB, "
+                       "an iteration about 1.4x that; within the 256 MB Infinity Cache, so these rates are "
+                       "effective (cache-resident), not HBM",
         "spmv_plain": {"bytes": kb["spmv"], "ms": round(t_spmv, 5),
                        "GB/s": round(gbs(kb["spmv"], t_spmv), 1),
                        "frac": round(gbs(kb["spmv"], t_spmv) / PEAK_HBM_GBS, 4)},
         "pcg_update": {"bytes": kb["pcg_update"], "ms": round(t_pcg_update, 5),
                        "GB/s": round(gbs(kb["pcg_update"], t_pcg_update), 1)},
-        "pcg_iteration": {"ref_bytes": kb["pcg_iteration_ref"], "ms": round(t_iter, 5),
-                          "GB/s_ref_formulation": round(gbs(kb["pcg_iteration_ref"], t_iter), 1)},
+        "pcg_iteration": {"ref_bytes": kb["pcg_iteration_ref"], "ms": round(storage[mode0]["iteration_ms"], 5),
+                          "GB/s_ref_formulation": round(gbs(kb["pcg_iteration_ref"],
+                                                            storage[mode0]["iteration_ms"]), 1)},
         "pcg_spmv_storage": storage,
+        "spmv_pass_roofline": spmv_roofline(storage, mode0, kb, read_traffic(args.size)),
     }
+    launch_cfg = g.sim.launch_config()
+    tune = [list(t) for t in g.sim.tune_log()]
+
+    cpu = None
+    fp_err = None
+    if cpu_state is not None:
+        cpu, fp_err = cpu_baseline(args, s, g, cpu_state)
+    del g
+
+    full = full_run(args, make_sim) if args.full_run else None
 
     ws_cmp = None
     if args.ws_steps > 0:
         ws_cmp = warm_start_comparison(args, s, make_sim)
 
-    cpu = None
-    fp_err = None
-    if cpu_state is not None:
-        from oracle.oracle import oracle_from_scene
-        threads = min(16, os.cpu_count() or 1)
-        oc, _, _, im, f = oracle_from_scene(s, precision=64, tol=args.tol, max_iter=20000, threads=threads)
-        p0, v0, dx0 = cpu_state
-        p_start, v_start = p0.copy(), v0.copy()
-        oc.build_topology()
-        oc.set_dx(dx0)
-        secs, it = oc.time_steps(p0, v0, im, f, s.dt, args.cpu_steps)  # p0, v0 advance in place
-        cpu = {"value": round(args.cpu_steps / secs, 5), "unit": "steps/s", "cores": threads, "kind": "port",
-               "sample": f"FP64 oracle (C++ restatement of the reference step, OpenMP) stepping the "
-                         f"{args.size}x{args.size} cloth {args.cpu_steps} step(s) from the state the GPU timed "
-                         f"region starts from; {it} PCG iterations, {secs:.2f} s",
-               "pcg_iters": it}
-        # FP32 GPU step vs the FP64 oracle from the same state (both to the same tolerance)
-        g.set_state(p_start, v_start)
-        g.sim.set_warm_start(dx0)
-        for _ in range(args.cpu_steps):
-            g.sim.step(s.dt)
-        gp = g.positions().astype(np.float64)
-        d_gpu, d_cpu = gp - p_start, p0 - p_start
-        fp_err = {"steps": args.cpu_steps, "tol": args.tol,
-                  "max_abs_pos_diff_m": float(np.abs(gp - p0).max()),
-                  "rel_pos_diff": float(np.linalg.norm(gp - p0) / np.linalg.norm(p0)),
-                  "rel_displacement_diff": float(np.linalg.norm(d_gpu - d_cpu) / max(np.linalg.norm(d_cpu), 1e-30))}
+    # The HBM roofline: the same kernels on a cloth whose matrix and vectors
+    # exceed the Infinity Cache.
+    hbm = hbm_kernels(args, make_sim) if args.hbm_size > 0 else None
+    copy = copy_ceiling(args) if args.copy_gib > 0 else None
+    if hbm is not None:
+        roofline = hbm["spmv_pass_roofline"]
+        if copy is not None:
+            roofline["copy_ceiling_GB/s"] = copy["best_GB/s"]
+            roofline["frac_of_copy_ceiling"] = round(roofline["achieved"] / copy["best_GB/s"], 4)
+    else:
+        roofline = extra_kernels["spmv_pass_roofline"]
 
     out = {
         "metric": METRIC,
@@ -442,7 +668,7 @@ def main():
                         f"springs + stretch k=1e3, m=0.01 kg, dt=1/60, PCG to {args.tol:g} true relative residual",
             "particles": n,
             "off_diag_blocks": nnz,
-            "warm_start": "previous-step dx (plain)",
+            "warm_start": "previous-step dx (plain)" if args.warm_start == "previous" else args.warm_start,
             "parallelism": "1 GPU",
         },
         "pcg_iters_per_step": round(float(np.mean(iters)), 2),
@@ -451,11 +677,14 @@ def main():
         "warm_start_comparison": ws_cmp,
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
+        "hbm_copy_ceiling": copy,
         "kernels": extra_kernels,
+        "kernels_hbm": hbm,
+        "full_run": full,
         "cpu_baseline": cpu,
         "fp32_vs_fp64_oracle": fp_err,
-        "launch_config": g.sim.launch_config(),
-        "autotune": [list(t) for t in g.sim.tune_log()],
+        "launch_config": launch_cfg,
+        "autotune": tune,
     }
     if args.calibrate_copy:
         a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)  # 256 MiB

@@ -134,8 +134,18 @@ int chysx_cloth_redistribute_mass_area_weighted(chysx_cloth* sim, float surface_
 int chysx_cloth_set_pcg_iterations(chysx_cloth* sim, int max_iter);
 int chysx_cloth_pcg_iterations(const chysx_cloth* sim, int* out);
 
-/* step (src/cloth/cloth_simulator.cu:814-1261; bindings.cpp:78-84).  Blocks
- * until the PCG solve has ended, then returns with pos / vel updated. */
+/* step (src/cloth/cloth_simulator.cu:814-1261; bindings.cpp:78-84).
+ * With a fixed iteration count (tolerance 0, the default and the reference's
+ * mode) every kernel, the iterations and the pos / vel update are enqueued on
+ * `stream` and the call returns without waiting, as the reference returns
+ * after its graph launch (pcg_solver.cu:882-894); pos / vel are updated when
+ * the stream reaches that point.  The solve statistics wait for it.  With a
+ * tolerance (chysx_cloth_set_pcg_tolerance > 0) or a warm-start stage the
+ * step synchronises `stream` once per poll of the solver state (after the
+ * iterations the previous step needed, then every 8) and returns when the
+ * solve has ended, with the pos / vel update enqueued behind it on `stream`.
+ * Consecutive steps may use different streams: each is ordered after the
+ * previous one. */
 int chysx_cloth_step(chysx_cloth* sim, float dt, uintptr_t stream);
 
 /* num_particles / num_off_diag_blocks of the last solve (cloth_simulator.h:632-633). */
@@ -164,10 +174,23 @@ int chysx_cloth_set_springs_active(chysx_cloth* sim, int on);
 /* Feed the external force buffer into the predictor (the reference passes
  * null, cloth_simulator.cu:892-894).  Default 0. */
 int chysx_cloth_set_external_force_active(chysx_cloth* sim, int on);
+/* Pins as constraints (no reference counterpart: the reference's pins are
+ * penalties of stiffness k, pin_constraint.h:5-15, pin_constraint.cu:61-92).
+ * on = 1: each pinned particle's displacement is fixed at target - x_n and
+ * eliminated from the system (its couplings move to the right-hand side of
+ * the free rows, its own row becomes the identity), so it ends every step
+ * exactly at its target; the pin stiffness is unused.  Default 0 (penalty). */
+int chysx_cloth_set_pin_projection(chysx_cloth* sim, int on);
+int chysx_cloth_pin_projection(const chysx_cloth* sim, int* out);
 /* Per-element stiffness (one value per spring in edge order / per triangle). */
 int chysx_cloth_set_spring_stiffnesses(chysx_cloth* sim, const float* k, int n);
 int chysx_cloth_set_fem_stretch_stiffnesses(chysx_cloth* sim, const float* k, int n);
-/* Iterations, restarts, status (CHYSX_PCG_*) and true relative residual of the last solve. */
+/* 1: a fixed-iteration step also waits until its solve has ended (the
+ * polled path); 0 (default): it returns once enqueued.  Results do not
+ * depend on it.  No reference counterpart (the reference never waits). */
+int chysx_cloth_set_blocking_step(chysx_cloth* sim, int on);
+/* Iterations, restarts, status (CHYSX_PCG_*) and true relative residual of
+ * the last solve; waits for an enqueued step. */
 int chysx_cloth_last_solve_stats(const chysx_cloth* sim, int* iters, int* restarts, int* status, double* relres);
 /* The FP64 warm start dx (n x 3) for the next step; zero resets it. */
 int chysx_cloth_set_warm_start(chysx_cloth* sim, const double* dx, int n);
@@ -208,7 +231,12 @@ int chysx_batch_set_external_buffers(chysx_batch* b, uintptr_t pos, uintptr_t ve
                                      uintptr_t force);
 int chysx_batch_set_material(chysx_batch* b, int index, const chysx_cloth_material* m);
 int chysx_batch_set_options(chysx_batch* b, double tol, int max_iter, int springs_active, int force_active);
+/* Same enqueue / poll behaviour as chysx_cloth_step: enqueued without waiting
+ * when every cloth runs a fixed iteration count. */
 int chysx_batch_step(chysx_batch* b, float dt, uintptr_t stream);
+int chysx_batch_set_blocking_step(chysx_batch* b, int on);
+/* Pin projection (as chysx_cloth_set_pin_projection) for every cloth of the batch. */
+int chysx_batch_set_pin_projection(chysx_batch* b, int on);
 int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts, int* status, double* relres);
 
 /* ---- kernel-level benchmark hooks -------------------------------------- */
@@ -231,14 +259,18 @@ int chysx_cloth_time_kernel(chysx_cloth* sim, int which, int reps, float dt, uin
  * bending elements, diagonal and off-diagonal blocks stored as their upper
  * triangle; the operator and results are bitwise those of 9-float blocks),
  * 16-bit indices (0/1: the SpMV pass reads 16-bit column and mirror offsets
- * when every offset of the mesh fits; the same indices)}; waves in
+ * when every offset of the mesh fits; the same indices), split advance
+ * (-1/0/1: each PCG iteration advances the solver state in a small kernel of
+ * its own instead of inside every SpMV workgroup; the same arithmetic; -1,
+ * the default, splits for a single cloth of 512 or more groups of 1024 rows)};
+ * waves in
  * {1,2,4,8,16} wavefronts of 64 lanes per workgroup, except the PCG passes
  * (residual, SpMV, update), which always run 16 (one workgroup per 16-slice
  * group of inner-product partials; other values are rejected); unroll in
  * {1,2,4}.  Shapes never change results.  By default an autotuner times the
  * SpMV variants (unroll x branch-free padding) on the problem's own matrix
  * when its topology is built (problems of 32768 rows or more). */
-#define CHYSX_LAUNCH_CONFIG_LEN 10
+#define CHYSX_LAUNCH_CONFIG_LEN 11
 int chysx_cloth_set_autotune(chysx_cloth* sim, int on);
 int chysx_cloth_launch_config(const chysx_cloth* sim, int* cfg);
 int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg);
@@ -307,9 +339,28 @@ int chysx_wsnet_destroy(chysx_wsnet* net);
 int chysx_wsnet_set_params(chysx_wsnet* net, const float* theta, int n);
 int chysx_wsnet_forward(chysx_wsnet* net, int n, const int32_t* row_offsets, const int32_t* cols, const float* F,
                         const float* E, float* out);
+/* The simulator's inference path: the decoder folded into the last layer,
+ * only the three output columns the warm-start direction uses; out n x 3. */
+int chysx_wsnet_forward_direction(chysx_wsnet* net, int n, const int32_t* row_offsets, const int32_t* cols,
+                                  const float* F, const float* E, float* out);
 int chysx_wsnet_loss_grad(chysx_wsnet* net, int n, const int32_t* row_offsets, const int32_t* cols, const float* F,
                           const float* E, const float* target, double* loss, float* grad);
 
+/* y = A x on the last assembled system with the plain one-lane-per-block-row
+ * kernel (BlockCSR3 spmv, src/sparse/block_csr.cu:54-83, 210-243; alpha 1,
+ * beta 0).  x, y: device float3 AoS of num_particles rows, y written.  It
+ * reads the general storage (9-float blocks, both halves stored); with the
+ * symmetric or packed storage in effect it fails with CHYSX_EINVAL. */
+int chysx_cloth_spmv(chysx_cloth* sim, uintptr_t x, uintptr_t y, uintptr_t stream);
+/* HBM bandwidth probe (no reference counterpart): a hand-written copy
+ * (kind 0: b = a) or triad (kind 1: a = b + s c) over device buffers of
+ * `bytes_per_buffer` each (rounded down to 16 KiB; use >= 1 GiB, far beyond
+ * the 256 MB Infinity Cache), 16-byte accesses; kinds 2 and 3 are the same
+ * with non-temporal loads and stores.  Allocates and frees its buffers.
+ * *ms = mean per launch over `reps` launches timed with HIP events on
+ * `stream`; *bytes_moved = bytes read + written per launch. */
+int chysx_stream_bench(int kind, uint64_t bytes_per_buffer, int reps, uintptr_t stream, double* ms,
+                       double* bytes_moved);
 /* Algorithmic bytes of one plain SpMV: 40 per off-diagonal block + 64 per row + 4. */
 int chysx_cloth_spmv_bytes(const chysx_cloth* sim, double* bytes);
 

@@ -204,6 +204,11 @@ struct Cloth {
     std::vector<int> pin_idx;
     std::vector<T> pin_tgt;                // 3 per pin
     T pin_k = T(1.0e6);                    // pin_constraint.h:82
+    // Pin projection (extension, no reference counterpart; the reference's
+    // pins are penalties, pin_constraint.h:5-15): each pinned particle's
+    // displacement is fixed at d = target - x_n and eliminated from the
+    // system, the particle ends the step exactly at its target.
+    bool pin_projection = false;
     // material
     double gx = 0, gy = 0, gz = -9.81, damping = 0;  // cloth_material.h:20-43 defaults
     bool springs_active = false, force_active = false;
@@ -737,7 +742,7 @@ struct Cloth {
         }
         // gradient at x_n (:914-1110): pins, springs (extension), stretch.
         std::fill(rhs.begin(), rhs.end(), V3<T>{0, 0, 0});
-        for (size_t c = 0; c < pin_idx.size(); ++c) {  // pin_gradient_kernel (pin_constraint.cu:61-75)
+        for (size_t c = 0; c < pin_idx.size() && !pin_projection; ++c) {  // pin_gradient_kernel (pin_constraint.cu:61-75)
             const int i = pin_idx[c];
             const V3<T> t{pin_tgt[3 * c], pin_tgt[3 * c + 1], pin_tgt[3 * c + 2]};
             rhs[i] = rhs[i] + (xn[i] - t) * pin_k;
@@ -814,8 +819,9 @@ struct Cloth {
         ensure_topology();
         std::fill(diag.begin(), diag.end(), m3_zero<T>());
         std::fill(vals.begin(), vals.end(), m3_zero<T>());
-        for (size_t c = 0; c < pin_idx.size(); ++c)  // pin_hessian_scatter_kernel (pin_constraint.cu:79-92)
-            scatter(pin_slots[c], m3_scale(m3_ident<T>(), pin_k));
+        if (!pin_projection)
+            for (size_t c = 0; c < pin_idx.size(); ++c)  // pin_hessian_scatter_kernel (pin_constraint.cu:79-92)
+                scatter(pin_slots[c], m3_scale(m3_ident<T>(), pin_k));
         if (springs_active) {
             for (size_t e = 0; e < spr_a.size(); ++e) {
                 const M3<T> H = spring_hess(e, xn);
@@ -858,7 +864,39 @@ struct Cloth {
                 diag[i] = m3_add(diag[i], H);
             }
         }
+        // Pin projection: eliminate the pinned displacements d_p.  A free row
+        // j moves its coupling to a pinned column into the right-hand side
+        // (rhs_j -= A_jp d_p, A_jp = 0); a pinned row becomes I dx_p = d_p.
+        // The iterate starts at d_p, so the pinned residual is 0 throughout.
+        std::vector<int> pin_of;
+        std::vector<V3<T>> dp;
+        if (pin_projection) {
+            pin_of.assign(size_t(N), -1);
+            for (size_t c = 0; c < pin_idx.size(); ++c)
+                if (pin_of[size_t(pin_idx[c])] < 0) pin_of[size_t(pin_idx[c])] = int(c);
+            dp.assign(size_t(N), V3<T>{0, 0, 0});
+            for (int i = 0; i < N; ++i)
+                if (pin_of[i] >= 0) {
+                    const size_t c = size_t(pin_of[i]);
+                    dp[i] = V3<T>{pin_tgt[3 * c], pin_tgt[3 * c + 1], pin_tgt[3 * c + 2]} - xn[i];
+                }
+            for (int j = 0; j < N; ++j) {
+                const bool pj = pin_of[j] >= 0;
+                for (int k = pat.row_offsets[j]; k < pat.row_offsets[j + 1]; ++k) {
+                    const int c = pat.cols[k];
+                    if (!pj && pin_of[c] >= 0) rhs[j] = rhs[j] - m3_mul(vals[k], dp[c]);
+                    if (pj || pin_of[c] >= 0) vals[k] = m3_zero<T>();
+                }
+                if (pj) {
+                    diag[j] = m3_ident<T>();
+                    rhs[j] = dp[j];
+                }
+            }
+        }
         if (assemble_only) return;  // system only: no solve, no state change
+        if (pin_projection)
+            for (int i = 0; i < N; ++i)
+                if (pin_of[i] >= 0) dx[i] = V3<double>{double(dp[i].x), double(dp[i].y), double(dp[i].z)};
         // PCG with warm start dx (:1219-1249)
         pcg(rhs, dx);
         // finalize_step_kernel (:222-244)
@@ -867,8 +905,15 @@ struct Cloth {
         const T decay = damp ? T(std::exp(-T(damping) * dt)) : T(1);
         for (int i = 0; i < N; ++i) {
             // x_{n+1} = x_n + dx, v = dx / dt, in T like the reference kernel.
-            const V3<T> d{T(dx[i].x), T(dx[i].y), T(dx[i].z)};
-            pos[i] = xn[i] + d;
+            V3<T> d{T(dx[i].x), T(dx[i].y), T(dx[i].z)};
+            if (pin_projection && pin_of[i] >= 0) {  // exactly at the target
+                const size_t c = size_t(pin_of[i]);
+                const V3<T> t{pin_tgt[3 * c], pin_tgt[3 * c + 1], pin_tgt[3 * c + 2]};
+                d = t - xn[i];
+                pos[i] = t;
+            } else {
+                pos[i] = xn[i] + d;
+            }
             V3<T> v = d * inv_dt;
             if (damp) v = v * decay;
             vel[i] = v;
@@ -988,6 +1033,9 @@ int orc_set_pins(void* h, const int* idx, const double* targets, int npin, doubl
         c.pin_k = T(k);
         c.topo_dirty = true;
     });
+}
+int orc_set_pin_projection(void* h, int on) {
+    return dispatch(h, [&](auto& c) { c.pin_projection = on != 0; });
 }
 int orc_set_params(void* h, double gx, double gy, double gz, double damping, int springs_active,
                    int force_active, double tol, int max_iter, int threads) {

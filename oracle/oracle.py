@@ -37,6 +37,7 @@ def lib():
         L.orc_build_springs.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
         L.orc_build_stretch.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
         L.orc_set_pins.argtypes = [C.c_void_p, _ip, _dp, C.c_int, C.c_double]
+        L.orc_set_pin_projection.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_params.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
         L.orc_reset_warm_start.argtypes = [C.c_void_p]
@@ -181,6 +182,11 @@ class OracleCloth:
         i = np.ascontiguousarray(idx, np.int32)
         t = _f64(targets).reshape(-1, 3)
         _check(self._L.orc_set_pins(self._h, _i(i), _d(t), i.shape[0], float(k)))
+
+    def set_pin_projection(self, on=True):
+        """Pins as constraints (extension): pinned displacements fixed at
+        target - x_n and eliminated; the particle ends each step at its target."""
+        _check(self._L.orc_set_pin_projection(self._h, int(bool(on))))
 
     def set_params(self, gravity=(0.0, 0.0, -9.81), damping=0.0, springs_active=False,
                    force_active=False, tol=0.0, max_iter=50, threads=1):

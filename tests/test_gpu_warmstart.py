@@ -1,7 +1,9 @@
 """GPU checks of the learned warm start.
 
 The network runs with BF16 weights, inputs and activations on MFMA and FP32
-accumulation, so it is compared with the FP64 reference
+accumulation (at inference each layer is one fused kernel: neighbour mean
+inside the operand load; the simulator folds the decoder into the last
+layer), so it is compared with the FP64 reference
 (oracle/warmstart_oracle.py) evaluated on the same BF16-rounded weights and
 inputs; the remaining difference is the BF16 rounding of the activations:
   * forward:  |out_gpu - out_ref| / |out_ref| <= 3e-2 (first 3 columns);
@@ -65,6 +67,12 @@ def test_forward_matches_fp64_reference(cuda, seed):
                     ro.astype(np.int64), co.astype(np.int64))
     assert rel(out[:, :3], ref[:, :3]) <= 3e-2
     assert rel(out, ref) <= 3e-2
+    # the simulator's path: decoder folded into the last fused layer, the
+    # three direction columns only, from the same BF16-rounded last layer
+    d = chysx._chysx_native.wsnet_forward(theta, ro, co, F, E, direction_only=True)
+    assert d.shape == (n, 3)
+    assert rel(d, ref[:, :3]) <= 3e-2
+    assert rel(d, out[:, :3]) <= 1e-5
 
 
 @pytest.mark.parametrize("seed", [0, 1])
@@ -283,3 +291,51 @@ def test_trained_network_gains_over_previous_guess(cuda):
         its[mode] = float(np.mean(it[10:]))
     assert its["learned"] <= 0.95 * its["previous"], its
     assert its["projection"] <= 0.85 * its["previous"], its
+
+
+@pytest.mark.parametrize("mode", ["projection", "learned", "projection_learned"])
+def test_batch_warm_start_modes_match_oracle_and_single_cloths(cuda, mode):
+    """An uneven batch (config-4 recipe, W in 8..40) stepped in each warm-start
+    mode with the committed parameters: every cloth converges, its step from
+    the batch's state matches the FP64 oracle (the warm start only moves the
+    initial guess), the warm start ran, and the batch is bitwise equal to the
+    same cloths stepped one by one in the same mode (every stage is row-local
+    or summed per cloth in a fixed order)."""
+    from chysx import warmstart
+    from chysx.harness import make_batch
+    from oracle.oracle import oracle_from_scene
+    theta, _ = warmstart.load_params()
+    sc = scenes.config4(count=10, wmin=8, wmax=40, steps=8, seed=21)
+    b = make_batch(sc, tol=1e-6)
+    b.batch.ws_set_params(theta)
+    b.batch.set_warm_start_mode(mode)
+    singles = []
+    for s in sc:
+        g = make_sim(s, tol=1e-6, force_active=False)
+        g.sim.ws_set_params(theta)
+        g.sim.set_warm_start_mode(mode)
+        singles.append(g)
+    used = []
+    for k in range(6):
+        if k == 5:
+            pos0 = b.pos.cpu().numpy().astype(np.float64)
+            vel0 = b.vel.cpu().numpy().astype(np.float64)
+        b.batch.step(1.0 / 60.0)
+        for g in singles:
+            g.sim.step(1.0 / 60.0)
+        for c in range(len(sc)):
+            st = b.batch.stats(c)
+            assert st["converged"] and st["relres"] <= 1e-6, (mode, k, c, st)
+            assert st["iters"] == singles[c].sim.last_solve_stats()["iters"], (mode, k, c)
+            if k >= 2:
+                w = b.batch.ws_stats(c)
+                used.append(w["projection_used"] if mode == "projection" else w["used"])
+    assert np.mean(used) > 0.3, (mode, np.mean(used))
+    for c, s in enumerate(sc):
+        np.testing.assert_array_equal(b.positions(c), singles[c].positions())
+        a = b.offsets[c]
+        oc, _, _, im, _ = oracle_from_scene(s, precision=64, tol=1e-11, max_iter=100000, force_active=False)
+        p = np.ascontiguousarray(pos0[a:a + s.n])
+        v = np.ascontiguousarray(vel0[a:a + s.n])
+        oc.step(p, v, im, None, 1.0 / 60.0, 1)
+        assert rel(b.positions(c), p) <= 1e-5, (mode, c)

@@ -162,3 +162,64 @@ def test_mat3_inverse_matches_numpy(seed):
     a = rng.normal(size=(3, 3))
     a = a @ a.T + np.eye(3)
     np.testing.assert_allclose(mat3_inverse(a), np.linalg.inv(a), rtol=1e-12, atol=1e-12)
+
+
+# ---- the reference's guards on degenerate elements -------------------------
+
+def test_spring_coincident_ends_clamped_length():
+    """Coincident ends: the length is clamped to kMinLength = 1e-6
+    (spring_constraint.cu:35) and the direction d / l is 0.  Rest length 0:
+    l = 1e-6 >= L, a = k L / l = 0, so H = k I and the gradient is 0.  Rest
+    length 0.1: l < L, H = a d d^T = 0, gradient 0.  Without the clamp both
+    would divide by zero."""
+    x = np.array([0.3, -0.2, 0.7])
+    g, H = spring_element(x, x.copy(), 0.0, 250.0)
+    np.testing.assert_array_equal(g, 0.0)
+    np.testing.assert_allclose(H, 250.0 * np.eye(3), rtol=0, atol=0)
+    g, H = spring_element(x, x.copy(), 0.1, 250.0)
+    np.testing.assert_array_equal(g, 0.0)
+    np.testing.assert_array_equal(H, 0.0)
+
+
+def test_triangle_degenerate_rest_frames():
+    """Rest determinant lab * v_c <= 1e-6 gives the identity frame
+    (triangle_stretch_constraint.cu:95-99).  A collinear triangle has area 0,
+    so its forces and blocks vanish; a sliver 2.5e-6 off its base of 0.2 has
+    determinant 5e-7, the identity frame and area 0.2 * 2.5e-6 / 2.  Just
+    above the threshold (determinant 2e-6) the frame is the real inverse."""
+    col = np.array([[0.0, 0, 0], [0.2, 0, 0], [0.5, 0, 0]])
+    x = col + np.array([[0, 0.01, 0], [0.02, 0, 0], [0, 0, 0.03]])
+    g, H, dm, area = triangle_element(col, x, 1e3)
+    np.testing.assert_array_equal(dm, np.eye(2))
+    assert area == 0.0
+    np.testing.assert_array_equal(g, 0.0)
+    np.testing.assert_array_equal(H, 0.0)
+    sl = np.array([[0.0, 0, 0], [0.2, 0, 0], [0.1, 0, -2.5e-6]])
+    _, _, dm, area = triangle_element(sl, sl, 1e3)
+    np.testing.assert_array_equal(dm, np.eye(2))
+    assert area == pytest.approx(0.5 * 0.2 * 2.5e-6, rel=1e-9)
+    ok = np.array([[0.0, 0, 0], [0.2, 0, 0], [0.1, 0, -1e-5]])
+    _, _, dm, area = triangle_element(ok, ok, 1e3)
+    np.testing.assert_allclose(dm @ np.array([[0.2, 0.1], [0.0, 1e-5]]), np.eye(2), atol=1e-9)
+
+
+def test_triangle_collapsed_edge_clamped_norm():
+    """A deformed triangle whose first edge has collapsed (x1 = x0): w_u = 0 is
+    clamped to kMinNorm = 1e-6 (triangle_stretch_constraint.cu:39), so its
+    direction is 0 and only the v term acts: on the unit right triangle
+    (D_m^-1 = I, dw_v = (-1, 0, 1)) the gradient is k A c_v w_v_hat on
+    corner 2, its negative on corner 0 and 0 on corner 1."""
+    xr = np.array([[0.0, 0, 0], [1.0, 0, 0], [0.0, 1.0, 0]])
+    x = np.array([[0.0, 0, 0], [0.0, 0, 0], [0.0, 1.5, 0.2]])
+    k = 300.0
+    g, H, _, area = triangle_element(xr, x, k)
+    wv = x[2] - x[0]
+    lv = np.linalg.norm(wv)
+    expect = k * area * (lv - 1.0) * wv / lv
+    np.testing.assert_allclose(g[2], expect, rtol=1e-12)
+    np.testing.assert_allclose(g[0], -expect, rtol=1e-12)
+    np.testing.assert_allclose(g[1], 0.0, atol=1e-12)
+    assert np.all(np.isfinite(H))
+    # the u term is gone from the Hessian: corner 1 couples to nothing
+    np.testing.assert_allclose(H[1], 0.0, atol=1e-9)
+    np.testing.assert_allclose(H[:, 1], 0.0, atol=1e-9)
