@@ -268,6 +268,13 @@ int chysx_cloth_redistribute_mass_area_weighted(chysx_cloth* sim, float density,
         sim->sim.redistribute_mass_area_weighted(density, inv_mass, n, stream);
     });
 }
+int chysx_cloth_redistribute_mass_area_weighted_noalloc(chysx_cloth* sim, float density, uintptr_t inv_mass, int n,
+                                                        uintptr_t stream) {
+    return guard([&] {
+        need(sim, "sim");
+        sim->sim.redistribute_mass_area_weighted_noalloc(density, inv_mass, n, stream);
+    });
+}
 int chysx_cloth_set_pcg_iterations(chysx_cloth* sim, int max_iter) {
     return guard([&] {
         need(sim, "sim");

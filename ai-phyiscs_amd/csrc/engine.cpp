@@ -1,5 +1,6 @@
 // SPDX-License-Identifier: Apache-2.0
 #include "engine.hpp"
+#include "noalloc.hpp"
 
 #include <climits>
 
@@ -1221,6 +1222,27 @@ void ClothSim::redistribute_mass_area_weighted(float density, std::uintptr_t inv
     launch_area_mass(doff.data(), dids.data(), dtri.data(), reinterpret_cast<const float*>(pos_), density / 3.0f,
                      reinterpret_cast<float*>(inv_mass), n, s);
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(area mass)");
+}
+
+void ClothSim::redistribute_mass_area_weighted_noalloc(float density, std::uintptr_t inv_mass, int n,
+                                                       std::uintptr_t stream) {
+    if (n <= 0) return;
+    if (inv_mass == 0)
+        throw std::invalid_argument(
+            "ClothSimulator::redistribute_mass_area_weighted_noalloc: inv_mass_ptr must be non-null");
+    if (!(density > 0.0f))
+        throw std::invalid_argument(
+            "ClothSimulator::redistribute_mass_area_weighted_noalloc: surface_density must be positive");
+    const int T = int(inst_.tris.size() / 3);
+    if (T == 0)
+        throw std::runtime_error(
+            "ClothSimulator::redistribute_mass_area_weighted_noalloc: call set_mesh(...) before redistributing mass.");
+    if (pos_ == 0 || n_ < n)
+        throw std::runtime_error(
+            "ClothSimulator::redistribute_mass_area_weighted_noalloc: external positions must be set first via "
+            "set_external_buffers().");
+    area_mass_noalloc(inst_.tris.data(), T, n, reinterpret_cast<const float*>(pos_), density,
+                      reinterpret_cast<float*>(inv_mass), reinterpret_cast<hipStream_t>(stream));
 }
 
 void ClothSim::set_pcg_tolerance(double tol) {

@@ -395,6 +395,9 @@ public:
     void clear_pins();
     void update_pin_targets(const float* targets, int n, std::uintptr_t stream);
     void redistribute_mass_area_weighted(float density, std::uintptr_t inv_mass, int n, std::uintptr_t stream);
+    // The same without allocating device memory: the incidence is staged in
+    // the library's fixed scratch array (noalloc.hpp).
+    void redistribute_mass_area_weighted_noalloc(float density, std::uintptr_t inv_mass, int n, std::uintptr_t stream);
     void set_pcg_iterations(int it) { inst_.max_iter = it > 0 ? it : 1; params_dirty_ = true; }
     int pcg_iterations() const { return inst_.max_iter; }
     void set_pcg_tolerance(double tol);

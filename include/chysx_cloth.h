@@ -130,6 +130,18 @@ int chysx_cloth_update_pin_targets(chysx_cloth* sim, const float* targets, int n
 int chysx_cloth_redistribute_mass_area_weighted(chysx_cloth* sim, float surface_density, uintptr_t inv_mass,
                                                 int particle_count, uintptr_t stream);
 
+/* No reference counterpart: the entry point above without device allocation,
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The vertex-to-triangle incidence
+ * and the triangles are staged in the library's fixed 64 MiB device scratch
+ * array (one per GPU, held until the process ends) instead of fresh device
+ * buffers; (particle_count + 1 + 6 triangles) * 4 bytes must fit in it, or
+ * CHYSX_EINVAL.  Results are bitwise those of the allocating entry point.
+ * Calls that overlap on one GPU share the array and race.  Synchronises
+ * `stream` before returning. */
+int chysx_cloth_redistribute_mass_area_weighted_noalloc(chysx_cloth* sim, float surface_density, uintptr_t inv_mass,
+                                                        int particle_count, uintptr_t stream);
+
 /* set_pcg_iterations / pcg_iterations (cloth_simulator.h:597-600; bindings.cpp:544-549). */
 int chysx_cloth_set_pcg_iterations(chysx_cloth* sim, int max_iter);
 int chysx_cloth_pcg_iterations(const chysx_cloth* sim, int* out);
