@@ -324,6 +324,12 @@ PYBIND11_MODULE(_chysx_native, m) {
                  check(chysx_cloth_pcg_iterations(s.p(), &n));
                  return n;
              })
+        .def("num_off_diag_blocks",
+             [](const Sim& s) {
+                 int nnz = 0;
+                 check(chysx_cloth_num_off_diag_blocks(s.p(), &nnz));
+                 return nnz;
+             })
         .def(
             "debug_dump_last_solve",
             [](const Sim& s) {

@@ -367,8 +367,7 @@ def hbm_kernels(args, make_sim):
     g.sim.step(s.dt)
     st = g.sim.last_solve_stats()
     torch.cuda.synchronize()
-    _, ro, _, _, _, _ = g.sim.debug_dump_last_solve()
-    n, nnz = s.n, int(ro[-1])
+    n, nnz = s.n, g.sim.num_off_diag_blocks()
     kb = kernel_bytes(n, nnz)
     stream = torch.cuda.current_stream().cuda_stream
     mode = storage_of(g)
@@ -599,8 +598,7 @@ def main():
     steps_per_s = args.steps / elapsed
 
     n = s.n
-    _, ro, _, _, _, _ = g.sim.debug_dump_last_solve()
-    nnz = int(ro[-1])
+    nnz = g.sim.num_off_diag_blocks()
     kb = kernel_bytes(n, nnz)
     # Kernel timings with HIP events on the launch stream (null stream).
     stream = torch.cuda.current_stream().cuda_stream
