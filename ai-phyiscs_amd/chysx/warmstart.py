@@ -27,6 +27,17 @@ from .harness import make_batch
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "warmstart_cloth.npz")
+# parameters trained on top of the projection (mode projection_learned)
+PROJECTION_WEIGHTS = os.path.join(DATA_DIR, "warmstart_cloth_projection.npz")
+
+
+def weights_for(mode: str) -> str:
+    """The committed parameter file for a warm-start mode: the one trained on
+    top of the projection for projection_learned when it exists, else the one
+    trained from the previous step's dx."""
+    if mode == "projection_learned" and os.path.exists(PROJECTION_WEIGHTS):
+        return PROJECTION_WEIGHTS
+    return DEFAULT_WEIGHTS
 
 
 def num_params() -> int:

@@ -529,6 +529,12 @@ int chysx_batch_step(chysx_batch* b, float dt, uintptr_t stream) {
         b->batch.step(dt, stream);
     });
 }
+int chysx_batch_set_split_advance(chysx_batch* b, int mode) {
+    return guard([&] {
+        need(b, "batch");
+        b->batch.problem().set_split_advance(mode);
+    });
+}
 int chysx_batch_set_pin_projection(chysx_batch* b, int on) {
     return guard([&] {
         need(b, "batch");

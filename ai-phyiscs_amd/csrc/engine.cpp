@@ -862,8 +862,11 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
     launch_ws_finish(red, 3, state_.data(), ws_inst_.data(), ninst_, kWsPhaseBase, proj_ran_ ? 1 : 0, s);
     launch_ws_features(geom(), matrix(), v, ext, mesh(), rest_pos_.data(), h, xb_.data(), ws_inst_.data(), dt,
                        N.features(), N.edge_features(), s);
-    N.forward(ws_graph(), s, ws_train_, !ws_train_);
-    launch_ws_direction(geom(), matrix(), N.out(), ws_inst_.data(), rb_.data(), d_.data(), ad_.data(), red, s);
+    // inference: the last layer writes the direction d itself; training keeps
+    // the full output (and the activations for backward)
+    N.forward(ws_graph(), s, ws_train_, !ws_train_, ws_inst_.data(), d_.data());
+    launch_ws_direction(geom(), matrix(), ws_train_ ? N.out() : nullptr, ws_inst_.data(), rb_.data(), d_.data(),
+                        ad_.data(), red, s);
     launch_ws_finish(red, 4, state_.data(), ws_inst_.data(), ninst_, kWsPhaseLine, ws_accept_residual_ ? 1 : 0, s);
     if (learned) launch_ws_line_apply(geom(), ws_inst_.data(), d_.data(), x_.data(), s);
     ws_ran_ = true;

@@ -276,8 +276,16 @@ public:
     // its own is as fast at 1024 groups (1024^2) and faster beyond.
     static constexpr int kSplitAdvanceGroups = 512;
     bool split_advance() const {
+        // batches keep the in-pass advance by default: at 512 cloths (3130
+        // groups) it measured 1% faster than the split one, whose per-instance
+        // workgroups also run for finished cloths
         return cfg_.pcg_split_advance > 0 || (cfg_.pcg_split_advance < 0 && ninst_ == 1 &&
                                               int(slices_h_.size()) / kGroupSlices >= kSplitAdvanceGroups);
+    }
+    void set_split_advance(int mode) {
+        if (mode < -1 || mode > 1) throw std::invalid_argument("split advance: -1 (by size), 0 or 1");
+        cfg_.pcg_split_advance = mode;
+        destroy_graph();
     }
 
     const LaunchConfig& launch_config() const { return cfg_; }

@@ -666,6 +666,9 @@ PYBIND11_MODULE(_chysx_native, m) {
             "set_pin_projection",
             [](Batch& b, bool on) { check(chysx_batch_set_pin_projection(b.h, on ? 1 : 0)); }, py::arg("on"))
         .def(
+            "set_split_advance", [](Batch& b, int mode) { check(chysx_batch_set_split_advance(b.h, mode)); },
+            py::arg("mode"))
+        .def(
             "stats",
             [](const Batch& b, int k) {
                 int it = 0, rs = 0, st = 0;

@@ -57,6 +57,42 @@ const MatDesc kMats[NMAT] = {{OFF_WE, FIN, HID}, {OFF_W0, XL, HID}, {OFF_W0 + LA
 __device__ __forceinline__ float bf2f(__bf16 v) { return float(v); }
 __device__ __forceinline__ __bf16 f2bf(float v) { return __bf16(v); }
 
+// m[t] += valid ? v[t] : 0 for t = 0..7, one dot2 per column: v[t] * 1 + v[t^1]
+// * 0 + m[t] is exact before its one rounding, so the sum is that of m[t] +=
+// float(v[t]) (for finite v), without converting v first.  Invalid entries
+// leave m unchanged (m is never -0: it starts at +0 and an exact cancellation
+// rounds to +0).  The BF16 pair (1, 0) is kept in a register: as a constant
+// the compiler encodes it as the inline constant 1.0, whose meaning for a
+// packed BF16 operand is not that pair.
+__device__ __forceinline__ void add_bf16x8(float* m, const bf16x8& v, bool valid) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    unsigned one_lo = valid ? 0x3F80u : 0u;  // BF16 1.0 in the low half
+    asm volatile("" : "+v"(one_lo));
+    const bf16x2 lo = __builtin_bit_cast(bf16x2, one_lo), hi = __builtin_bit_cast(bf16x2, one_lo << 16);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const bf16x2 p = {v[2 * j], v[2 * j + 1]};
+        m[2 * j] = __builtin_amdgcn_fdot2_f32_bf16(p, lo, m[2 * j], false);
+        m[2 * j + 1] = __builtin_amdgcn_fdot2_f32_bf16(p, hi, m[2 * j + 1], false);
+    }
+}
+
+// Slice of this wavefront in a 4-wave workgroup, XCD-aware: workgroup b runs
+// on XCD b % 8, and each XCD walks a contiguous eighth of the slices, so a
+// row's +-1 and +-W neighbours (W up to a few thousand rows) were loaded into
+// the same XCD's L2 by the neighbouring wavefronts.  The grid is ws_grid.
+__device__ __forceinline__ int ws_slice(int nslices) {
+    const int nwg = (nslices + 3) >> 2;
+    const int per = (nwg + 7) >> 3;
+    const int b = int(blockIdx.x);
+    // wave-uniform: the slice's bounds and loops then live in scalar registers
+    return __builtin_amdgcn_readfirstlane(((b & 7) * per + (b >> 3)) * 4 + int(threadIdx.x >> 6));
+}
+inline unsigned ws_grid(int nslices) {
+    const int nwg = (nslices + 3) / 4;
+    return unsigned(8 * ((nwg + 7) / 8));
+}
+
 // ---------------------------------------------------------------------------
 // GEMM: Y = act(sum_s X_s W_s + bias) (+ resid), rows x NOUT, X row-major
 // BF16 segments of widths multiple of 32; weights in (NOUT, K) layout.
@@ -80,7 +116,7 @@ struct GemmArgs {
 template <int NT>
 __global__ __launch_bounds__(256) void ws_gemm_kernel(GemmArgs a) {
     constexpr int NOUT = 16 * NT;
-    const int wave = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int wave = ws_slice(a.rows / 64);
     const int row0 = wave * 64;
     if (row0 >= a.rows) return;
     const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
@@ -129,21 +165,33 @@ __global__ __launch_bounds__(256) void ws_gemm_kernel(GemmArgs a) {
 
 // ---------------------------------------------------------------------------
 // Inference layer, fused: h' = relu([h | mean_{j in N(i)} h_j | e] W + b) + h
-// for the 64 rows of one slice per wave, the neighbour mean formed while the
-// MFMA operand is built (it never goes to memory).  The MFMA operands are
-// swapped (weights as A, rows as B), so each lane ends with four consecutive
-// output columns of one row and stores them as one 8-byte vector.  With LAST
-// the decoder follows in the epilogue: only the output columns the warm start
-// uses (0..2) are formed, from the BF16-rounded h' (as the decoder GEMM would
-// read it), by FP32 FMAs and a sum over the four lanes of a row; h' itself is
-// not stored.  The neighbour mean is bitwise that of ws_agg_kernel (same
-// order, same 1/deg, same BF16 rounding).
+// for the 64 rows of one slice per wave, in two phases.
+//  1. The neighbour mean of the slice's rows goes to the wave's LDS tile
+//     (64 x HID BF16).  Eight lanes share a row and each reads 16 B of a
+//     neighbour's 128-B row, so one load instruction covers eight whole rows;
+//     the column indices and neighbour rows of two groups of eight rows are
+//     in flight together.  The mean is bitwise that of ws_agg_kernel (same
+//     order, same 1/deg, same BF16 rounding); it never goes to memory.
+//  2. The layer GEMM on MFMA 16x16x32 BF16 with the operands swapped
+//     (weights as A, rows as B), K = [h (2 chunks of 32) | mean (2, from LDS)
+//     | e (1)] as in the training GEMM.  The weight rows of output tile nt are
+//     taken in the order ch(nt, i) = 32 (nt >> 1) + 8 (i >> 2) + 4 (nt & 1) +
+//     (i & 3), so the lane (li, lg) ends with output columns 8 lg .. 8 lg + 7
+//     and 32 + 8 lg .. 32 + 8 lg + 7 of row 16 mt + li: the h columns it
+//     loaded for K chunks 0 and 1, which are the residual, and two 16-byte
+//     vector stores write them.  Permuting the output columns changes no
+//     result (each is the same dot product).
+// With LAST the decoder follows in the epilogue: only the output columns the
+// warm start uses (0..2) are formed, from the BF16-rounded h' (as the decoder
+// GEMM would read it), by FP32 FMAs and a sum over the four lanes of a row;
+// h' itself is not stored.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 constexpr int kDirCols = 3;  // output columns the warm start reads
 
 struct LayerArgs {
     int nslices;
+    const SliceDesc* slices;
     const int* slice_ptr;
     const int* col;
     const int* row_deg;
@@ -154,115 +202,195 @@ struct LayerArgs {
     __bf16* h_out;       // rows x HID (not LAST)
     const __bf16* wd;    // (NOUTD, HID): the decoder's weights (LAST)
     const float* bd;     // NOUTD (LAST)
-    float* out;          // rows x NOUTD, columns 0..kDirCols-1 written (LAST)
+    float* out;          // rows x NOUTD, columns 0..kDirCols-1 written (LAST, dir null)
+    // LAST with dir: the warm-start direction d = scale * out[:, 0:3] of the
+    // row's instance, 0 on padding rows, written instead of out
+    const WsInst* wi;
+    float* dir;
 };
 
-// Mean over the slice lane's neighbours of h[., c0 .. c0 + 7], as BF16.
-__device__ __forceinline__ bf16x8 neighbour_mean8(const LayerArgs& a, int s, int slane, int row, int c0) {
-    const int sb = a.slice_ptr[s], w = a.slice_ptr[s + 1] - sb;
-    const int* colp = a.col + (size_t)sb * 64 + slane;
-    float m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k0 = 0; k0 < w; k0 += 4) {
-        int c[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) c[u] = (k0 + u < w) ? colp[(k0 + u) * 64] : -1;
-        bf16x8 v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (c[u] >= 0) v[u] = *reinterpret_cast<const bf16x8*>(a.h + (size_t)c[u] * HID + c0);
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (c[u] >= 0) {
-#pragma unroll
-                for (int t = 0; t < 8; ++t) m[t] += bf2f(v[u][t]);
-            }
-    }
-    const int deg = a.row_deg[row];
-    const float inv = deg > 0 ? 1.0f / float(deg) : 0.0f;
-    bf16x8 r;
-#pragma unroll
-    for (int t = 0; t < 8; ++t) r[t] = f2bf(m[t] * inv);
-    return r;
-}
+constexpr int kStageLd = HID + 8;  // LDS row stride (BF16) of the mean tile: 144 B, against bank conflicts
+static_assert(HID == 64 && XL == 2 * HID + 32, "the layer kernel tiles h as 2 K chunks and e as 1");
+static_assert(OFF_W0 % 8 == 0 && LAYER_STRIDE % 8 == 0 && OFF_WD % 8 == 0, "16-byte weight fragments");
+
+// Output column held in register r of lane group lg for output tile nt (and
+// the weight row an A-operand lane li = 4 lg + r supplies).
+__device__ __forceinline__ int ws_out_col(int nt, int i) { return 32 * (nt >> 1) + 8 * (i >> 2) + 4 * (nt & 1) + (i & 3); }
 
 template <bool LAST>
-__global__ __launch_bounds__(256) void ws_layer_kernel(LayerArgs a) {
-    constexpr int NT = HID / 16;
-    constexpr int MH = 2;  // row tiles of 16 per pass: the wave's 64 rows in two halves (fewer registers)
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
-    if (s >= a.nslices) return;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void ws_layer_kernel(LayerArgs a) {
+    constexpr int NT = HID / 16, KC = XL / 32;
+    constexpr int KB = 8;  // neighbours per gather batch
+    // the weights as MFMA A fragments: wfrag[(kc NT + nt) 64 + lane] = W row
+    // ws_out_col(nt, li), K columns 32 kc + 8 lg .. 32 kc + 8 lg + 7
+    __shared__ bf16x8 wfrag[KC * NT * 64];
+    // per wave: the neighbour mean of 32 rows
+    __shared__ __attribute__((aligned(16))) __bf16 stage[4][32 * kStageLd];
+    for (int f = int(threadIdx.x); f < KC * NT * 64; f += 256) {
+        const int lane = f & 63, nt = (f >> 6) % NT, kc = (f >> 6) / NT;
+        wfrag[f] = *reinterpret_cast<const bf16x8*>(a.w + (size_t)ws_out_col(nt, lane & 15) * XL + 32 * kc +
+                                                    8 * (lane >> 4));
+    }
+    const int s = ws_slice(a.nslices);
+    // a wave past the last slice still reaches the barriers
+    const bool live = s < a.nslices;
+    __bf16* tile = stage[threadIdx.x >> 6];
     const int row0 = s * 64;
     const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
-    for (int half = 0; half < 4 / MH; ++half) {
-        f32x4 acc[MH][NT];
+    int sb = 0, w = 0;
+    if (live) {
+        sb = a.slice_ptr[s];
+        w = a.slice_ptr[s + 1] - sb;
+    }
+    for (int half = 0; half < 2; ++half) {
+        if (live) {
+            const int r8 = l >> 3, q = l & 7;  // row r8 of a group of eight, columns 8 q .. 8 q + 7
+            for (int g = 0; g < 4; ++g) {
+                const int sl = 32 * half + 8 * g + r8;
+                float m[8];
 #pragma unroll
-        for (int mt = 0; mt < MH; ++mt)
+                for (int t = 0; t < 8; ++t) m[t] = 0.0f;
+                for (int k0 = 0; k0 < w; k0 += KB) {
+                    int c[KB];
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        // K = [h (2 chunks of 32) | mean (2 chunks) | e (1 chunk)], in that order
-        for (int kc = 0; kc < XL / 32; ++kc) {
-            bf16x8 xf[MH];
+                    for (int u = 0; u < KB; ++u) c[u] = (k0 + u < w) ? a.col[(size_t)(sb + k0 + u) * 64 + sl] : -1;
+                    // padding entries (c < 0) read the row itself and add nothing
+                    bf16x8 v[KB];
 #pragma unroll
-            for (int mt = 0; mt < MH; ++mt) {
-                const int slane = (half * MH + mt) * 16 + li, row = row0 + slane;
-                if (kc < 2)
-                    xf[mt] = *reinterpret_cast<const bf16x8*>(a.h + (size_t)row * HID + 32 * kc + 8 * lg);
-                else if (kc < 4)
-                    xf[mt] = neighbour_mean8(a, s, slane, row, 32 * (kc - 2) + 8 * lg);
-                else
-                    xf[mt] = *reinterpret_cast<const bf16x8*>(a.e + (size_t)row * 32 + 8 * lg);
-            }
+                    for (int u = 0; u < KB; ++u)
+                        if (k0 + u < w)
+                            v[u] = *reinterpret_cast<const bf16x8*>(a.h + (size_t)(c[u] >= 0 ? c[u] : row0 + sl) * HID +
+                                                                    8 * q);
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const bf16x8 wf =
-                    *reinterpret_cast<const bf16x8*>(a.w + (size_t)(nt * 16 + li) * XL + 32 * kc + 8 * lg);
+                    for (int u = 0; u < KB; ++u)
+                        if (k0 + u < w) add_bf16x8(m, v[u], c[u] >= 0);
+                }
+                const int deg = a.row_deg[row0 + sl];
+                const float inv = deg > 0 ? 1.0f / float(deg) : 0.0f;
+                bf16x8 o;
 #pragma unroll
-                for (int mt = 0; mt < MH; ++mt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xf[mt], acc[mt][nt], 0, 0, 0);
+                for (int t = 0; t < 8; ++t) o[t] = f2bf(m[t] * inv);
+                *reinterpret_cast<bf16x8*>(tile + (8 * g + r8) * kStageLd + 8 * q) = o;
             }
         }
-        // lane (li, lg): acc[mt][nt][r] = output (row0 + 16 (half MH + mt) + li, column 16 nt + 4 lg + r)
+        __syncthreads();  // the mean tile (and, the first time, the weight fragments)
+        if (live) {
+            f32x4 acc[2][NT];
+            bf16x8 hres[2][2];
 #pragma unroll
-        for (int mt = 0; mt < MH; ++mt) {
-            const int row = row0 + (half * MH + mt) * 16 + li;
-            float dsum[kDirCols] = {0.0f, 0.0f, 0.0f};
+            for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const int n0 = nt * 16 + 4 * lg;
-                const bf16x4 res = *reinterpret_cast<const bf16x4*>(a.h + (size_t)row * HID + n0);
-                const float4 bb = *reinterpret_cast<const float4*>(a.bias + n0);
-                const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
-                bf16x4 o;
+                for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    float v = fmaxf(acc[mt][nt][r] + bv[r], 0.0f);
-                    v += bf2f(res[r]);
-                    o[r] = f2bf(v);
+            for (int kc = 0; kc < KC; ++kc) {
+                bf16x8 xf[2];
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt) {
+                    const int tl = 16 * mt + li, row = row0 + 32 * half + tl;
+                    if (kc < 2) {
+                        xf[mt] = *reinterpret_cast<const bf16x8*>(a.h + (size_t)row * HID + 32 * kc + 8 * lg);
+                        hres[mt][kc] = xf[mt];
+                    } else if (kc < 4) {
+                        xf[mt] = *reinterpret_cast<const bf16x8*>(tile + tl * kStageLd + 32 * (kc - 2) + 8 * lg);
+                    } else {
+                        xf[mt] = *reinterpret_cast<const bf16x8*>(a.e + (size_t)row * 32 + 8 * lg);
+                    }
+                }
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const bf16x8 wf = wfrag[(kc * NT + nt) * 64 + l];
+#pragma unroll
+                    for (int mt = 0; mt < 2; ++mt)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xf[mt], acc[mt][nt], 0, 0, 0);
+                }
+            }
+            // lane (li, lg): acc[mt][nt][r] = output (row0 + 32 half + 16 mt + li,
+            // column ws_out_col(nt, 4 lg + r))
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const int slane = 32 * half + 16 * mt + li, row = row0 + slane;
+                bf16x8 o[2];
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int n0 = ws_out_col(nt, 4 * lg);
+                    const float4 bb = *reinterpret_cast<const float4*>(a.bias + n0);
+                    const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        float v = fmaxf(acc[mt][nt][r] + bv[r], 0.0f);
+                        v += bf2f(hres[mt][nt >> 1][4 * (nt & 1) + r]);
+                        o[nt >> 1][4 * (nt & 1) + r] = f2bf(v);
+                    }
                 }
                 if constexpr (LAST) {
+                    float dsum[kDirCols] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int q = 0; q < kDirCols; ++q)
+#pragma unroll
+                        for (int hh = 0; hh < 2; ++hh) {
+                            const bf16x8 wq =
+                                *reinterpret_cast<const bf16x8*>(a.wd + (size_t)q * HID + 32 * hh + 8 * lg);
+#pragma unroll
+                            for (int t = 0; t < 8; ++t) dsum[q] = fmaf(bf2f(o[hh][t]), bf2f(wq[t]), dsum[q]);
+                        }
 #pragma unroll
                     for (int q = 0; q < kDirCols; ++q) {
-                        const bf16x4 wq = *reinterpret_cast<const bf16x4*>(a.wd + (size_t)q * HID + n0);
+                        float t = dsum[q];
+                        t += __shfl_xor(t, 16, 64);
+                        t += __shfl_xor(t, 32, 64);
+                        dsum[q] = t + a.bd[q];
+                    }
+                    if (lg == 0) {
+                        if (a.dir != nullptr) {
+                            const SliceDesc sd = a.slices[s];
+                            const bool valid = slane < sd.nvalid;
+                            const float sc = float(a.wi[sd.inst].scale);
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) dsum[q] = fmaf(bf2f(o[r]), bf2f(wq[r]), dsum[q]);
+                            for (int q = 0; q < kDirCols; ++q) a.dir[3 * (size_t)row + q] = valid ? sc * dsum[q] : 0.0f;
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < kDirCols; ++q) a.out[(size_t)row * NOUTD + q] = dsum[q];
+                        }
                     }
                 } else {
-                    *reinterpret_cast<bf16x4*>(a.h_out + (size_t)row * HID + n0) = o;
+                    bf16x8* dst = reinterpret_cast<bf16x8*>(a.h_out + (size_t)row * HID + 8 * lg);
+                    dst[0] = o[0];
+                    dst[4] = o[1];
                 }
             }
-            if constexpr (LAST) {
+        }
+        __syncthreads();  // the next half rewrites the mean tile
+    }
+}
+
+// Inference encoder: h0 = relu(F We + be) for the 64 rows of a slice per
+// wave, MFMA operands swapped like the layers (8-byte vector stores).
+__global__ __launch_bounds__(256) void ws_encode_kernel(int nslices, const __bf16* __restrict__ F,
+                                                        const __bf16* __restrict__ w, const float* __restrict__ bias,
+                                                        __bf16* __restrict__ h) {
+    constexpr int NT = HID / 16;
+    const int s = ws_slice(nslices);
+    if (s >= nslices) return;
+    const int row0 = s * 64;
+    const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
+    bf16x8 wf[NT];
 #pragma unroll
-                for (int q = 0; q < kDirCols; ++q) {
-                    float t = dsum[q];
-                    t += __shfl_xor(t, 16, 64);
-                    t += __shfl_xor(t, 32, 64);
-                    dsum[q] = t + a.bd[q];
-                }
-                if (lg == 0) {
+    for (int nt = 0; nt < NT; ++nt) wf[nt] = *reinterpret_cast<const bf16x8*>(w + (size_t)(nt * 16 + li) * FIN + 8 * lg);
 #pragma unroll
-                    for (int q = 0; q < kDirCols; ++q) a.out[(size_t)row * NOUTD + q] = dsum[q];
-                }
-            }
+    for (int mt = 0; mt < 4; ++mt) {
+        const int row = row0 + mt * 16 + li;
+        const bf16x8 xf = *reinterpret_cast<const bf16x8*>(F + (size_t)row * FIN + 8 * lg);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[nt], xf, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+            const int n0 = nt * 16 + 4 * lg;
+            const float4 bb = *reinterpret_cast<const float4*>(bias + n0);
+            bf16x4 o;
+            o[0] = f2bf(fmaxf(acc[0] + bb.x, 0.0f));
+            o[1] = f2bf(fmaxf(acc[1] + bb.y, 0.0f));
+            o[2] = f2bf(fmaxf(acc[2] + bb.z, 0.0f));
+            o[3] = f2bf(fmaxf(acc[3] + bb.w, 0.0f));
+            *reinterpret_cast<bf16x4*>(h + (size_t)row * HID + n0) = o;
         }
     }
 }
@@ -377,7 +505,7 @@ __global__ void ws_reduce_parts_kernel(const float* __restrict__ part, int nwg, 
 // ---------------------------------------------------------------------------
 // out_i = mean_{j in N(i)} h_j  (64 columns, BF16 in/out, FP32 sums)
 __global__ __launch_bounds__(256) void ws_agg_kernel(WsGraph g, const __bf16* __restrict__ h, __bf16* __restrict__ out) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
     float acc[HID];
@@ -390,11 +518,7 @@ __global__ __launch_bounds__(256) void ws_agg_kernel(WsGraph g, const __bf16* __
         if (c < 0) continue;
         const bf16x8* src = reinterpret_cast<const bf16x8*>(h + (size_t)c * HID);
 #pragma unroll
-        for (int u = 0; u < HID / 8; ++u) {
-            const bf16x8 v = src[u];
-#pragma unroll
-            for (int t = 0; t < 8; ++t) acc[8 * u + t] += bf2f(v[t]);
-        }
+        for (int u = 0; u < HID / 8; ++u) add_bf16x8(acc + 8 * u, src[u], true);
     }
     const int deg = g.row_deg[row];
     const float inv = deg > 0 ? 1.0f / float(deg) : 0.0f;
@@ -412,7 +536,7 @@ __global__ __launch_bounds__(256) void ws_agg_kernel(WsGraph g, const __bf16* __
 //   + sum_{j in N(i)} dx_j[64:128] / deg_j   (adjoint of the mean aggregation)
 __global__ __launch_bounds__(256) void ws_dh_kernel(WsGraph g, const float* __restrict__ dh_in,
                                                     const float* __restrict__ dx, float* __restrict__ dh_out) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
     float acc[HID];
@@ -461,7 +585,7 @@ __global__ void ws_relu_grad_kernel(const float* __restrict__ dh, const __bf16* 
 __global__ __launch_bounds__(256) void ws_loss_kernel(WsGraph g, const float* __restrict__ out,
                                                       const float* __restrict__ target, float inv_nv,
                                                       __bf16* __restrict__ dout, double* __restrict__ lpart) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
     const bool valid = lane < g.slices[s].nvalid;
@@ -503,7 +627,7 @@ __global__ void ws_bf16_kernel(const float* __restrict__ th, __bf16* __restrict_
     wnk[off + n * K + k] = b;
 }
 
-int grid_rows(int rows) { return (rows / 64 + 3) / 4; }
+unsigned grid_rows(int rows) { return ws_grid(rows / 64); }
 
 __global__ void ws_to_bf16_kernel(const float* __restrict__ in, __bf16* __restrict__ out, size_t n_in, size_t n_out) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -680,24 +804,31 @@ void WarmStartNet::copy_grad(float* host, hipStream_t s) const {
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
-void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep, bool direction_only) {
+void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep, bool direction_only, const WsInst* wi,
+                           float* dir) {
     const int rows = g.nslices * 64;
     if (rows > rows_) throw std::invalid_argument("warm start: reserve() rows first");
     if (keep && !training_) throw std::invalid_argument("warm start: reserve(training) before a kept forward");
     // Pre-activations are only needed by backward().
-    // encoder
-    GemmArgs a{};
-    a.rows = rows;
-    a.seg[0] = F_;
-    a.segw[0] = FIN;
-    a.nseg = 1;
-    a.K = FIN;
-    a.wnk = wnk_ + OFF_WE;
-    a.bias = theta_ + OFF_BE;
-    a.relu = 1;
-    a.out_b = H_[0];
-    a.pre_b = keep ? P_[0] : nullptr;
-    run_gemm<HID / 16>(a, s);
+    if (!keep) {
+        // inference encoder, vector stores
+        hipLaunchKernelGGL(ws_encode_kernel, dim3(grid_rows(rows)), dim3(256), 0, s, g.nslices, F_, wnk_ + OFF_WE,
+                           theta_ + OFF_BE, H_[0]);
+        check_hip(hipGetLastError(), "ws_encode_kernel");
+    } else {
+        GemmArgs a{};
+        a.rows = rows;
+        a.seg[0] = F_;
+        a.segw[0] = FIN;
+        a.nseg = 1;
+        a.K = FIN;
+        a.wnk = wnk_ + OFF_WE;
+        a.bias = theta_ + OFF_BE;
+        a.relu = 1;
+        a.out_b = H_[0];
+        a.pre_b = P_[0];
+        run_gemm<HID / 16>(a, s);
+    }
     if (!keep) {
         // inference: each layer in one fused kernel; with direction_only the
         // decoder runs in the last one's epilogue (output columns 0..2, the
@@ -705,6 +836,7 @@ void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep, bool dire
         for (int l = 0; l < NL; ++l) {
             LayerArgs la{};
             la.nslices = g.nslices;
+            la.slices = g.slices;
             la.slice_ptr = g.slice_ptr;
             la.col = g.col;
             la.row_deg = g.row_deg;
@@ -716,6 +848,8 @@ void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep, bool dire
             la.wd = wnk_ + OFF_WD;
             la.bd = theta_ + OFF_BD;
             la.out = out_;
+            la.wi = wi;
+            la.dir = (direction_only && wi != nullptr) ? dir : nullptr;
             if (l + 1 < NL || !direction_only)
                 hipLaunchKernelGGL(ws_layer_kernel<false>, dim3(grid_rows(rows)), dim3(256), 0, s, la);
             else
@@ -725,7 +859,7 @@ void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep, bool dire
         if (direction_only) return;
     } else {
         for (int l = 0; l < NL; ++l) {
-            hipLaunchKernelGGL(ws_agg_kernel, dim3((g.nslices + 3) / 4), dim3(256), 0, s, g, H_[l], A_[l]);
+            hipLaunchKernelGGL(ws_agg_kernel, dim3(ws_grid(g.nslices)), dim3(256), 0, s, g, H_[l], A_[l]);
             check_hip(hipGetLastError(), "ws_agg_kernel");
             GemmArgs b{};
             b.rows = rows;
@@ -767,7 +901,7 @@ void WarmStartNet::loss(const WsGraph& g, const float* target, hipStream_t s) {
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
     long long nv = 0;
     for (const auto& q : sd) nv += q.nvalid;
-    hipLaunchKernelGGL(ws_loss_kernel, dim3((g.nslices + 3) / 4), dim3(256), 0, s, g, out_, target,
+    hipLaunchKernelGGL(ws_loss_kernel, dim3(ws_grid(g.nslices)), dim3(256), 0, s, g, out_, target,
                        1.0f / float(std::max<long long>(nv, 1)), dout_, lpart_);
     check_hip(hipGetLastError(), "ws_loss_kernel");
     lpart_n_ = g.nslices;
@@ -835,7 +969,7 @@ void WarmStartNet::backward(const WsGraph& g, hipStream_t s) {
         a.wnk = wkn_ + off_w(l);  // (XL, HID)
         a.out_f = dx_;
         run_gemm<XL / 16>(a, s);
-        hipLaunchKernelGGL(ws_dh_kernel, dim3((g.nslices + 3) / 4), dim3(256), 0, s, g, dh, dx_, dh_next);
+        hipLaunchKernelGGL(ws_dh_kernel, dim3(ws_grid(g.nslices)), dim3(256), 0, s, g, dh, dx_, dh_next);
         check_hip(hipGetLastError(), "ws_dh_kernel");
         std::swap(dh, dh_next);
     }
@@ -956,7 +1090,7 @@ __device__ __forceinline__ void put_partial(const WsReduce& red, int o, int s, i
 template <int K>
 __global__ __launch_bounds__(256) void ws_gram_kernel(LaunchGeom g, DevMatrix A, const float* __restrict__ rhs,
                                                       WsHist h, WsReduce red) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
@@ -1130,7 +1264,7 @@ __global__ __launch_bounds__(128) void ws_finish_kernel(WsReduce red, int nv, co
 
 __global__ __launch_bounds__(256) void ws_proj_apply_kernel(LaunchGeom g, WsHist h, const WsInst* __restrict__ wi,
                                                             double* __restrict__ x) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
@@ -1153,7 +1287,7 @@ __global__ __launch_bounds__(256) void ws_proj_apply_kernel(LaunchGeom g, WsHist
 __global__ __launch_bounds__(256) void ws_base_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
                                                       const double* __restrict__ x, double* __restrict__ xb,
                                                       float* __restrict__ rb, WsReduce red) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
@@ -1219,7 +1353,7 @@ __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatri
                                                           const double* __restrict__ xb,
                                                           const WsInst* __restrict__ wi, float dt,
                                                           __bf16* __restrict__ F, __bf16* __restrict__ E) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
@@ -1306,7 +1440,7 @@ __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatri
 // d = scale * out[:, 0:3] (FP32; padding rows 0).
 __global__ __launch_bounds__(256) void ws_dir_write_kernel(LaunchGeom g, const float* __restrict__ out,
                                                            const WsInst* __restrict__ wi, float* __restrict__ d) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
@@ -1320,7 +1454,7 @@ __global__ __launch_bounds__(256) void ws_dir_write_kernel(LaunchGeom g, const f
 __global__ __launch_bounds__(256) void ws_dir_spmv_kernel(LaunchGeom g, DevMatrix A, const float* __restrict__ d,
                                                           const float* __restrict__ rb, float* __restrict__ ad,
                                                           WsReduce red) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
@@ -1365,7 +1499,7 @@ __global__ __launch_bounds__(256) void ws_dir_spmv_kernel(LaunchGeom g, DevMatri
 
 __global__ __launch_bounds__(256) void ws_line_apply_kernel(LaunchGeom g, const WsInst* __restrict__ wi,
                                                             const float* __restrict__ d, double* __restrict__ x) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
@@ -1379,7 +1513,7 @@ __global__ __launch_bounds__(256) void ws_line_apply_kernel(LaunchGeom g, const 
 __global__ __launch_bounds__(256) void ws_energy_kernel(LaunchGeom g, const double* __restrict__ x,
                                                         const double* __restrict__ xb, const float* __restrict__ rb,
                                                         WsReduce red) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
@@ -1392,7 +1526,7 @@ __global__ __launch_bounds__(256) void ws_energy_kernel(LaunchGeom g, const doub
 __global__ __launch_bounds__(256) void ws_dout_kernel(LaunchGeom g, const WsInst* __restrict__ wi,
                                                       const float* __restrict__ rb, const float* __restrict__ ad,
                                                       __bf16* __restrict__ dout) {
-    const int s = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
@@ -1417,7 +1551,7 @@ __global__ void ws_history_kernel(const double* __restrict__ x, float* __restric
     if (i < n) hist[i] = float(x[i]);
 }
 
-inline unsigned grid_slices(int nslices) { return unsigned((nslices + 3) / 4); }
+inline unsigned grid_slices(int nslices) { return ws_grid(nslices); }
 
 template <int K>
 void gram_k(const LaunchGeom& g, const DevMatrix& A, const float* rhs, const WsHist& h, const WsReduce& red,
@@ -1479,8 +1613,10 @@ void launch_ws_features(const LaunchGeom& g, const DevMatrix& A, const DevVector
 void launch_ws_direction(const LaunchGeom& g, const DevMatrix& A, const float* out, const WsInst* wi,
                          const float* rb, float* d, float* ad, const WsReduce& red, hipStream_t s) {
     if (g.nslices <= 0) return;
-    hipLaunchKernelGGL(ws_dir_write_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, out, wi, d);
-    check_hip(hipGetLastError(), "ws_dir_write_kernel");
+    if (out != nullptr) {  // else the forward wrote d itself
+        hipLaunchKernelGGL(ws_dir_write_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, out, wi, d);
+        check_hip(hipGetLastError(), "ws_dir_write_kernel");
+    }
     hipLaunchKernelGGL(ws_dir_spmv_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, d, rb, ad, red);
     check_hip(hipGetLastError(), "ws_dir_spmv_kernel");
 }

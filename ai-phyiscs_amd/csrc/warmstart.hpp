@@ -32,6 +32,8 @@ struct WsGraph {
     const int* row_deg;
 };
 
+struct WsInst;
+
 class WarmStartNet {
 public:
     static constexpr int kFin = 32;
@@ -60,10 +62,14 @@ public:
     __bf16* edge_features() { return E_; }
     // Forward: out (rows x 32, FP32).  keep = store activations for backward
     // (one kernel per aggregation and per GEMM).  Without keep each layer is
-    // one fused kernel (aggregation inside the GEMM's operand load); with
+    // one fused kernel (the neighbour mean staged in LDS, never in memory); with
     // direction_only the decoder is folded into the last layer and only the
     // output columns 0..2 (the warm-start direction) are written.
-    void forward(const WsGraph& g, hipStream_t s, bool keep, bool direction_only = false);
+    // With direction_only, wi and dir given, the last layer writes the
+    // warm-start direction d = scale * out[:, 0:3] (0 on padding rows) into
+    // dir itself (rows x 3) instead of out.
+    void forward(const WsGraph& g, hipStream_t s, bool keep, bool direction_only = false, const WsInst* wi = nullptr,
+                 float* dir = nullptr);
     const float* out() const { return out_; }
     // Loss against target (rows x 3, FP32; rows with valid[row] == 0 or
     // beyond a slice's valid count are ignored): mean over valid rows of the

@@ -97,6 +97,8 @@ def bench_batch(args):
     sc = scenes.config4(steps=args.warmup + args.steps)
     t0 = time.perf_counter()
     b = make_batch(sc, tol=args.tol, max_iter=20000)
+    if args.split_advance is not None:
+        b.batch.set_split_advance(args.split_advance)
     b.batch.step(sc[0].dt)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
@@ -187,18 +189,12 @@ def warm_start_comparison(args, scene, make_sim):
     import numpy as np
     import torch
     from chysx import warmstart as ws
-    path = args.ws_weights or ws.DEFAULT_WEIGHTS
-    theta, meta = (None, None)
-    if os.path.exists(path):
-        theta, meta = ws.load_params(path)
-    out = {"warmup": args.warmup, "steps": args.ws_steps, "weights": os.path.relpath(path, ROOT) if theta is not None
-           else None}
-    if meta is not None:
-        out["trained_pairs"] = meta.get("pairs")
-        out["train_seconds"] = meta.get("train_seconds")
+    out = {"warmup": args.warmup, "steps": args.ws_steps}
     pos = {}
     modes = ("previous", "learned", "projection", "projection_learned")
     for mode in modes:
+        path = args.ws_weights or ws.weights_for(mode)
+        theta = ws.load_params(path)[0] if os.path.exists(path) else None
         if mode.endswith("learned") and theta is None:
             out[mode] = None
             continue
@@ -225,7 +221,8 @@ def warm_start_comparison(args, scene, make_sim):
         pos[mode] = g.positions()
         out[mode] = {"steps_per_s": round(args.ws_steps / el, 3), "pcg_iters_per_step": round(float(np.mean(its)), 2),
                      "projection_used_frac": round(float(np.mean(proj)), 3),
-                     "learned_used_frac": round(float(np.mean(used)), 3)}
+                     "learned_used_frac": round(float(np.mean(used)), 3),
+                     "weights": os.path.relpath(path, ROOT) if mode.endswith("learned") else None}
         del g
     for mode in modes[1:]:
         if out.get(mode):
@@ -241,12 +238,11 @@ def batch_warm_start_comparison(args, sc, make_batch):
     import numpy as np
     import torch
     from chysx import warmstart as ws
-    path = args.ws_weights or ws.DEFAULT_WEIGHTS
-    theta = ws.load_params(path)[0] if os.path.exists(path) else None
-    out = {"warmup": args.warmup, "steps": args.ws_steps,
-           "weights": os.path.relpath(path, ROOT) if theta is not None else None}
+    out = {"warmup": args.warmup, "steps": args.ws_steps}
     modes = ("previous", "learned", "projection", "projection_learned")
     for mode in modes:
+        path = args.ws_weights or ws.weights_for(mode)
+        theta = ws.load_params(path)[0] if os.path.exists(path) else None
         if mode.endswith("learned") and theta is None:
             out[mode] = None
             continue
@@ -569,7 +565,7 @@ def main():
     if args.warm_start != "previous":
         from chysx import warmstart as ws
         if args.warm_start.endswith("learned"):
-            g.sim.ws_set_params(ws.load_params(args.ws_weights or ws.DEFAULT_WEIGHTS)[0])
+            g.sim.ws_set_params(ws.load_params(args.ws_weights or ws.weights_for(args.warm_start))[0])
         g.sim.set_warm_start_mode(args.warm_start)
     g.sim.step(s.dt)  # builds topology + first solve
     torch.cuda.synchronize()

@@ -247,6 +247,10 @@ int chysx_batch_set_options(chysx_batch* b, double tol, int max_iter, int spring
  * when every cloth runs a fixed iteration count. */
 int chysx_batch_step(chysx_batch* b, float dt, uintptr_t stream);
 int chysx_batch_set_blocking_step(chysx_batch* b, int on);
+/* Where the PCG state is advanced (launch config's split advance): -1 by
+ * size (default: inside the SpMV workgroups for a batch), 0 inside every SpMV
+ * workgroup, 1 in a kernel of its own.  Speed only; results are unchanged. */
+int chysx_batch_set_split_advance(chysx_batch* b, int mode);
 /* Pin projection (as chysx_cloth_set_pin_projection) for every cloth of the batch. */
 int chysx_batch_set_pin_projection(chysx_batch* b, int on);
 int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts, int* status, double* relres);
