@@ -221,15 +221,24 @@ template <bool LAST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void ws_layer_kernel(LayerArgs a) {
     constexpr int NT = HID / 16, KC = XL / 32;
     constexpr int KB = 8;  // neighbours per gather batch
-    // the weights as MFMA A fragments: wfrag[(kc NT + nt) 64 + lane] = W row
-    // ws_out_col(nt, li), K columns 32 kc + 8 lg .. 32 kc + 8 lg + 7
+    // The weights as MFMA A fragments: wfrag[(kc NT + nt) 64 + lane] = W row
+    // ws_out_col(nt, li), K columns 32 kc + 8 lg .. 32 kc + 8 lg + 7; the
+    // bias, and with LAST the decoder rows 0..2 and biases, beside them.
     __shared__ bf16x8 wfrag[KC * NT * 64];
+    __shared__ float4 bias_s[HID / 4];
+    __shared__ bf16x8 wd_s[kDirCols * HID / 8];
+    __shared__ float bd_s[kDirCols];
     // per wave: the neighbour mean of 32 rows
     __shared__ __attribute__((aligned(16))) __bf16 stage[4][32 * kStageLd];
     for (int f = int(threadIdx.x); f < KC * NT * 64; f += 256) {
         const int lane = f & 63, nt = (f >> 6) % NT, kc = (f >> 6) / NT;
         wfrag[f] = *reinterpret_cast<const bf16x8*>(a.w + (size_t)ws_out_col(nt, lane & 15) * XL + 32 * kc +
                                                     8 * (lane >> 4));
+    }
+    if (threadIdx.x < HID / 4) bias_s[threadIdx.x] = reinterpret_cast<const float4*>(a.bias)[threadIdx.x];
+    if constexpr (LAST) {
+        if (threadIdx.x < kDirCols * HID / 8) wd_s[threadIdx.x] = reinterpret_cast<const bf16x8*>(a.wd)[threadIdx.x];
+        if (threadIdx.x < kDirCols) bd_s[threadIdx.x] = a.bd[threadIdx.x];
     }
     const int s = ws_slice(a.nslices);
     // a wave past the last slice still reaches the barriers
@@ -244,28 +253,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void w
     }
     for (int half = 0; half < 2; ++half) {
         if (live) {
+            // the first KB column indices and the degree of row l of the slice
+            // (one coalesced load each); the gather lanes take theirs by a lane
+            // shuffle
+            int cl[KB];
+#pragma unroll
+            for (int u = 0; u < KB; ++u) cl[u] = (u < w) ? a.col[(size_t)(sb + u) * 64 + l] : -1;
+            const int degl = a.row_deg[row0 + l];
             const int r8 = l >> 3, q = l & 7;  // row r8 of a group of eight, columns 8 q .. 8 q + 7
+#pragma unroll 1
             for (int g = 0; g < 4; ++g) {
                 const int sl = 32 * half + 8 * g + r8;
                 float m[8];
 #pragma unroll
                 for (int t = 0; t < 8; ++t) m[t] = 0.0f;
                 for (int k0 = 0; k0 < w; k0 += KB) {
-                    int c[KB];
 #pragma unroll
-                    for (int u = 0; u < KB; ++u) c[u] = (k0 + u < w) ? a.col[(size_t)(sb + k0 + u) * 64 + sl] : -1;
-                    // padding entries (c < 0) read the row itself and add nothing
-                    bf16x8 v[KB];
+                    for (int u0 = 0; u0 < KB; u0 += KB / 2) {
+                        // padding entries (c < 0) read the row itself and add nothing
+                        int c[KB / 2];
+                        bf16x8 v[KB / 2];
 #pragma unroll
-                    for (int u = 0; u < KB; ++u)
-                        if (k0 + u < w)
-                            v[u] = *reinterpret_cast<const bf16x8*>(a.h + (size_t)(c[u] >= 0 ? c[u] : row0 + sl) * HID +
-                                                                    8 * q);
+                        for (int u = 0; u < KB / 2; ++u)
+                            if (k0 + u0 + u < w) {
+                                const int cu = (k0 == 0) ? cl[u0 + u] : a.col[(size_t)(sb + k0 + u0 + u) * 64 + l];
+                                c[u] = __shfl(cu, sl, 64);
+                                v[u] = *reinterpret_cast<const bf16x8*>(
+                                    a.h + (size_t)(c[u] >= 0 ? c[u] : row0 + sl) * HID + 8 * q);
+                            }
 #pragma unroll
-                    for (int u = 0; u < KB; ++u)
-                        if (k0 + u < w) add_bf16x8(m, v[u], c[u] >= 0);
+                        for (int u = 0; u < KB / 2; ++u)
+                            if (k0 + u0 + u < w) add_bf16x8(m, v[u], c[u] >= 0);
+                    }
                 }
-                const int deg = a.row_deg[row0 + sl];
+                const int deg = __shfl(degl, sl, 64);
                 const float inv = deg > 0 ? 1.0f / float(deg) : 0.0f;
                 bf16x8 o;
 #pragma unroll
@@ -273,7 +294,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void w
                 *reinterpret_cast<bf16x8*>(tile + (8 * g + r8) * kStageLd + 8 * q) = o;
             }
         }
-        __syncthreads();  // the mean tile (and, the first time, the weight fragments)
+        __syncthreads();  // the mean tile (and, the first time, the staged parameters)
         if (live) {
             f32x4 acc[2][NT];
             bf16x8 hres[2][2];
@@ -312,8 +333,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void w
                 bf16x8 o[2];
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
-                    const int n0 = ws_out_col(nt, 4 * lg);
-                    const float4 bb = *reinterpret_cast<const float4*>(a.bias + n0);
+                    const float4 bb = bias_s[ws_out_col(nt, 4 * lg) / 4];
                     const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
@@ -328,8 +348,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void w
                     for (int q = 0; q < kDirCols; ++q)
 #pragma unroll
                         for (int hh = 0; hh < 2; ++hh) {
-                            const bf16x8 wq =
-                                *reinterpret_cast<const bf16x8*>(a.wd + (size_t)q * HID + 32 * hh + 8 * lg);
+                            const bf16x8 wq = wd_s[(q * HID + 32 * hh + 8 * lg) / 8];
 #pragma unroll
                             for (int t = 0; t < 8; ++t) dsum[q] = fmaf(bf2f(o[hh][t]), bf2f(wq[t]), dsum[q]);
                         }
@@ -338,7 +357,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void w
                         float t = dsum[q];
                         t += __shfl_xor(t, 16, 64);
                         t += __shfl_xor(t, 32, 64);
-                        dsum[q] = t + a.bd[q];
+                        dsum[q] = t + bd_s[q];
                     }
                     if (lg == 0) {
                         if (a.dir != nullptr) {
