@@ -173,6 +173,7 @@ def bench_batch(args):
                    "particles": n_total, "cloths": len(sc), "parallelism": "1 GPU"},
         "pcg_iters_per_step_mean": round(float(np.mean(it_mean)), 2),
         "pcg_iters_per_step_max": round(float(np.mean(it_max)), 2),
+        "cloth_iterations": batch_tail(sc, it_sum / args.steps),
         "warm_start_comparison": ws_cmp,
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
@@ -230,6 +231,29 @@ def warm_start_comparison(args, scene, make_sim):
                                                          out[mode]["pcg_iters_per_step"], 3)
             out[mode]["speedup_vs_previous"] = round(out[mode]["steps_per_s"] / out["previous"]["steps_per_s"], 3)
             out[mode]["max_pos_diff_vs_previous_m"] = float(np.abs(pos[mode] - pos["previous"]).max())
+    return out
+
+
+def batch_tail(sc, its):
+    """Where the batch's rounds go: percentiles of the cloths' mean PCG
+    iterations per step, and the cloths, 1024-row groups and rows still
+    iterating after r rounds (the passes launch every group and a finished
+    group's workgroups exit at once), with the share of small cloths (at most
+    4096 rows) among them."""
+    import numpy as np
+    rows = np.array([s.n for s in sc])
+    groups = np.ceil(rows / 1024.0)
+    out = {"percentiles_0_25_50_75_90_100": [round(float(x), 1) for x in np.percentile(its, [0, 25, 50, 75, 90, 100])],
+           "iterating_after_round": {}}
+    w = np.array([s.grid_w for s in sc])
+    out["mean_iterations_by_width"] = {f"{lo}-{hi}": round(float(its[(w >= lo) & (w <= hi)].mean()), 1)
+                                       for lo, hi in ((16, 40), (41, 64), (65, 96), (97, 128))
+                                       if ((w >= lo) & (w <= hi)).any()}
+    for r in (0, 50, 100, 150, 200, 250, 300, 350):
+        live = its > r
+        out["iterating_after_round"][str(r)] = {"cloths": int(live.sum()), "groups": int(groups[live].sum()),
+                                                "rows": int(rows[live].sum()),
+                                                "small_cloths": int((live & (rows <= 4096)).sum())}
     return out
 
 
