@@ -680,10 +680,15 @@ void Problem::prime_states(InstState* st, hipStream_t s) const {
                   "hipMemsetD32Async(active groups)");
 }
 
-// Autotuner: times every SpMV-pass variant (unroll x branch-free padding) on
-// this problem's own matrix layout, on scratch vectors and a scratch solver
-// state, and keeps the fastest.  Launch shapes never change results.
-void Problem::autotune(hipStream_t s) {
+// Autotuner: on this problem's own matrix layout, times every SpMV-pass
+// variant (unroll x branch-free padding) on scratch vectors and a scratch
+// solver state; the assembly at each width of workgroup (it writes this
+// step's system, as the step's own assembly will again); and, when the
+// launch config leaves it open, whole PCG iterations with the state advanced
+// in the SpMV pass and in a kernel of its own, on a scratch state that keeps
+// iterating.  It keeps the fastest of each.  Launch shapes never change
+// results.
+void Problem::autotune(hipStream_t s, float dt, const DevExt& ext) {
     if (!built_ || n_pad_ < kAutotuneMinRows) return;
     const size_t v3 = size_t(n_pad_) * 3;
     DeviceArray<float> f[6];
@@ -695,8 +700,9 @@ void Problem::autotune(hipStream_t s) {
     xs.resize(v3);
     xs.zero(s);
     ps.resize(partials_.size());
-    DeviceArray<InstState> sst;
+    DeviceArray<InstState> sst, sst_b;
     sst.resize(size_t(ninst_));
+    sst_b.resize(size_t(ninst_));
     check_hip(hipMemcpyAsync(sst.data(), state_.data(), sizeof(InstState) * ninst_, hipMemcpyDeviceToDevice, s),
               "hipMemcpyAsync(scratch state)");
     prime_states(sst.data(), s);
@@ -715,9 +721,10 @@ void Problem::autotune(hipStream_t s) {
         check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
         return double(ms) / kAutotuneReps;
     };
+    tune_log_.clear();
+    // 1. the SpMV pass
     const int unroll_list[] = {1, 2, 4};
     double best = 1e30;
-    tune_log_.clear();
     for (int bf = 0; bf < 2; ++bf)
         for (int u : unroll_list) {
             const double t = time_it(
@@ -729,6 +736,63 @@ void Problem::autotune(hipStream_t s) {
                 cfg_.spmv_branch_free = bf;
             }
         }
+    // 2. the assembly's waves per workgroup (the LDS slab caps the width)
+    if (ext.pos != nullptr && ext.vel != nullptr) {
+        best = 1e30;
+        for (int w : {1, 2, 4, 8}) {
+            const double t = time_it([&] {
+                chysx_amd::launch_assemble(geom(), mesh(), matrix(), vectors(), ext, params_.data(), dt, w,
+                                           max_width_, s);
+            });
+            tune_log_.push_back({3, w, 0, t});
+            if (t < best) {
+                best = t;
+                cfg_.assemble_waves = w;
+            }
+        }
+    }
+    // 3. where the state is advanced, over whole iterations: scratch
+    // parameters that never stop the solve and partials of 1, so every
+    // advance does its full work
+    split_tuned_ = -1;
+    if (cfg_.pcg_split_advance < 0) {
+        std::vector<InstParams> ph = params_h_;
+        for (InstParams& q : ph) {
+            q.tol2 = 0.0;
+            q.max_iter = 1 << 30;
+        }
+        DeviceArray<InstParams> sp;
+        sp.assign(ph.data(), ph.size(), s);
+        const std::vector<double> ones(partials_.size(), 1.0);
+        ps.assign(ones.data(), ones.size(), s);
+        const LaunchGeom g = geom();
+        int parity = 0;
+        auto iteration = [&](bool split) {
+            InstState* in = parity ? sst_b.data() : sst.data();
+            InstState* out = parity ? sst.data() : sst_b.data();
+            parity ^= 1;
+            if (split) {
+                launch_pcg_advance_copy(in, out, sp.data(), ps.data(), ninst_, g.ngroups, s, g.gactive);
+                launch_pcg_spmv(g, matrix(), v, out, nullptr, nullptr, cfg_.spmv_unroll, s, cfg_.spmv_branch_free,
+                                cfg_.spmv_symmetric);
+            } else {
+                launch_pcg_spmv(g, matrix(), v, in, out, sp.data(), cfg_.spmv_unroll, s, cfg_.spmv_branch_free,
+                                cfg_.spmv_symmetric);
+            }
+            launch_pcg_update(g, matrix(), v, out, s);
+        };
+        double t2[2];
+        for (int split = 0; split < 2; ++split) {
+            prime_states(sst.data(), s);
+            parity = 0;
+            t2[split] = time_it([&] { iteration(split != 0); });
+            tune_log_.push_back({4 + split, kGroupSlices, cfg_.spmv_unroll, t2[split]});
+        }
+        split_tuned_ = t2[1] < t2[0] ? 1 : 0;
+        if (ninst_ > 1 && gactive_.size() > 0)  // the timings left every group flagged; the start resets them
+            check_hip(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(gactive_.data()), 1, gactive_.size(), s),
+                      "hipMemsetD32Async(active groups)");
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize(autotune)");
@@ -1265,7 +1329,7 @@ void ClothSim::ensure_topology() {
     inst_.n = n_;
     inst_.part0 = 0;
     prob_.build({&inst_});
-    if (autotune_) prob_.autotune(nullptr);
+    tune_pending_ = autotune_;
     topo_dirty_ = false;
     params_dirty_ = true;
 }
@@ -1283,6 +1347,10 @@ void ClothSim::step(float dt, std::uintptr_t stream) {
     params_dirty_ = false;
     DevExt e{reinterpret_cast<float*>(pos_), reinterpret_cast<float*>(vel_),
              reinterpret_cast<const float*>(inv_mass_), reinterpret_cast<const float*>(force_)};
+    if (tune_pending_) {
+        prob_.autotune(s, dt, e);
+        tune_pending_ = false;
+    }
     prob_.step(dt, e, s);
     dumped_n_ = n_;
 }
@@ -1410,13 +1478,17 @@ void ClothBatch::step(float dt, std::uintptr_t stream) {
     for (auto& I : insts_) v.push_back(I.get());
     if (dirty_ || !prob_.built()) {
         prob_.build(v);
-        if (autotune_) prob_.autotune(s);
+        tune_pending_ = autotune_;
         dirty_ = false;
     }
     prob_.upload_params(v, s);
     params_dirty_ = false;
     DevExt e{reinterpret_cast<float*>(pos_), reinterpret_cast<float*>(vel_),
              reinterpret_cast<const float*>(inv_mass_), reinterpret_cast<const float*>(force_)};
+    if (tune_pending_) {
+        prob_.autotune(s, dt, e);
+        tune_pending_ = false;
+    }
     prob_.step(dt, e, s);
 }
 

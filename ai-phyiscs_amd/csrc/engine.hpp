@@ -126,7 +126,10 @@ struct LaunchConfig {
 };
 
 struct TuneSample {
-    int kernel;  // 0: SpMV pass, 1: update pass, 2: branch-free SpMV pass
+    // 0: SpMV pass, 1: update pass, 2: branch-free SpMV pass, 3: assembly
+    // (waves per workgroup), 4 / 5: a whole PCG iteration with the state
+    // advanced in the SpMV pass / in a kernel of its own
+    int kernel;
     int waves;
     int unroll;
     double ms;
@@ -279,20 +282,28 @@ public:
         // batches keep the in-pass advance by default: at 512 cloths (3130
         // groups) it measured 1% faster than the split one, whose per-instance
         // workgroups also run for finished cloths
-        return cfg_.pcg_split_advance > 0 || (cfg_.pcg_split_advance < 0 && ninst_ == 1 &&
-                                              int(slices_h_.size()) / kGroupSlices >= kSplitAdvanceGroups);
+        if (cfg_.pcg_split_advance >= 0) return cfg_.pcg_split_advance > 0;
+        if (split_tuned_ >= 0) return split_tuned_ > 0;
+        return ninst_ == 1 && int(slices_h_.size()) / kGroupSlices >= kSplitAdvanceGroups;
     }
     void set_split_advance(int mode) {
         if (mode < -1 || mode > 1) throw std::invalid_argument("split advance: -1 (by size), 0 or 1");
         cfg_.pcg_split_advance = mode;
+        split_tuned_ = -1;
         destroy_graph();
     }
 
     const LaunchConfig& launch_config() const { return cfg_; }
     void set_launch_config(const LaunchConfig& c);
-    // Pick the fastest SpMV / update shapes for this problem (no-op for small
-    // problems).  Uses scratch buffers; solver state and vectors are untouched.
-    void autotune(hipStream_t s);
+    // Pick the fastest launch shapes for this problem (no-op for small
+    // problems): the SpMV pass variant (entries per batch, branch-free
+    // padding), the assembly's waves per workgroup, and, when the launch
+    // config leaves it to the size rule (-1), where the PCG state is advanced
+    // (timed over whole iterations).  None of these changes results.  The
+    // PCG timings use scratch vectors and a scratch solver state; the assembly
+    // timings assemble this step's system from `ext` (idempotent: the step's
+    // own assembly writes the same values), so call it after upload_params().
+    void autotune(hipStream_t s, float dt, const DevExt& ext);
     const std::vector<TuneSample>& tune_log() const { return tune_log_; }
     // Mark every instance of `st` as running (timing hooks only).
     void prime_states(InstState* st, hipStream_t s) const;
@@ -362,6 +373,7 @@ private:
 
     LaunchConfig cfg_;
     std::vector<TuneSample> tune_log_;
+    int split_tuned_ = -1;  // the autotuner's split-advance choice (-1: none)
     // warm start
     int ws_mode_ = kWarmPrevious;
     bool ws_train_ = false;
@@ -458,6 +470,7 @@ private:
     bool topo_dirty_ = true;
     bool params_dirty_ = true;
     bool autotune_ = true;
+    bool tune_pending_ = false;  // topology rebuilt: tune at the next step
     int dumped_n_ = 0;
 };
 
@@ -495,6 +508,7 @@ private:
     int ext_n_ = 0;
     bool dirty_ = true;
     bool autotune_ = true;
+    bool tune_pending_ = false;  // rebuilt: tune at this step, after the parameters are uploaded
     bool params_dirty_ = true;
 };
 

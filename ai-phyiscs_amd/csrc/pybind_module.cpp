@@ -504,11 +504,13 @@ PYBIND11_MODULE(_chysx_native, m) {
                  std::vector<double> ms(static_cast<size_t>(n));
                  check(chysx_cloth_tune_log(s.p(), &n, kwu.data(), ms.data()));
                  py::list out;
-                 for (int i = 0; i < n; ++i)
-                     out.append(py::make_tuple(kwu[3 * i] == 0   ? "spmv"
-                                               : kwu[3 * i] == 2 ? "spmv_branch_free"
-                                                                 : "update",
-                                               kwu[3 * i + 1], kwu[3 * i + 2], ms[i]));
+                 static const char* const kinds[] = {"spmv", "update", "spmv_branch_free", "assemble",
+                                                     "iteration_advance_in_pass", "iteration_advance_kernel"};
+                 for (int i = 0; i < n; ++i) {
+                     const int k = kwu[3 * i];
+                     out.append(py::make_tuple(k >= 0 && k < 6 ? kinds[k] : "unknown", kwu[3 * i + 1],
+                                               kwu[3 * i + 2], ms[i]));
+                 }
                  return out;
              })
         .def(

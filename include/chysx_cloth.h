@@ -294,8 +294,11 @@ int chysx_cloth_set_launch_config(chysx_cloth* sim, const int* cfg);
 int chysx_cloth_block_floats(const chysx_cloth* sim, int* floats);
 /* Bits of the column / mirror offsets the SpMV pass reads: 16 or 32. */
 int chysx_cloth_index_bits(const chysx_cloth* sim, int* bits);
-/* Autotuner samples: count in/out; per sample {kernel (0 SpMV, 2 branch-free
- * SpMV), waves, unroll} and mean ms. */
+/* Autotuner samples: count in/out; per sample {kernel (0 SpMV pass, 2
+ * branch-free SpMV pass, 3 assembly, 4 / 5 a whole PCG iteration with the
+ * state advanced in the SpMV pass / in a kernel of its own), waves, unroll}
+ * and mean ms.  The tuner runs at the first step after the topology is built
+ * (problems of 32768 padded rows or more) and keeps the fastest of each. */
 int chysx_cloth_tune_log(const chysx_cloth* sim, int* count, int* kernel_waves_unroll, double* ms);
 int chysx_batch_set_autotune(chysx_batch* b, int on);
 /* ---- PCG warm start (extension) ------------------------------------------

@@ -257,6 +257,31 @@ def test_launch_shapes_do_not_change_results(cuda):
         np.testing.assert_array_equal(x, runs[0][1])
 
 
+def test_autotuner_picks_shapes_without_changing_results(cuda):
+    """The autotuner (problems of 32768 padded rows or more) times the SpMV
+    pass variants, the assembly's workgroup widths and the two places the PCG
+    state can be advanced, and keeps the fastest of each; stepping with it
+    gives bitwise the positions and iterate of stepping without it."""
+    s = scenes.config2(w=192, steps=4)
+    runs = []
+    for tune in (True, False):
+        g = make_sim(s, tol=1e-6)
+        g.sim.set_autotune(tune)
+        for _ in range(3):
+            g.sim.step(s.dt)
+            assert g.sim.last_solve_stats()["converged"]
+        runs.append((g.positions(), g.sim.get_dx64()))
+        kinds = {t[0] for t in g.sim.tune_log()}
+        if tune:
+            assert {"spmv", "spmv_branch_free", "assemble", "iteration_advance_in_pass",
+                    "iteration_advance_kernel"} <= kinds, kinds
+            assert all(t[3] > 0.0 for t in g.sim.tune_log())
+        else:
+            assert not kinds
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+
+
 def test_batch_uneven_cloths_each_match_oracle(cuda):
     """Uneven batch (config-4 recipe, smaller sizes): every cloth converges on
     its own and each cloth's step matches the oracle from the same state."""
