@@ -176,6 +176,9 @@ void Problem::destroy_graph() {
         if (g) (void)hipGraphExecDestroy(g);
         g = nullptr;
     }
+    if (step_exec_) (void)hipGraphExecDestroy(step_exec_);
+    step_exec_ = nullptr;
+    step_key_.clear();
 }
 
 void Problem::build(const std::vector<const Instance*>& insts) {
@@ -823,6 +826,71 @@ void Problem::launch_pcg_chunks(hipStream_t s, int count) {
         }
     }
 }
+std::vector<unsigned char> Problem::fixed_step_key(float dt, const DevExt& ext, int chunks) const {
+    // every field is a pointer or a 4-byte integer, so the structs carry no
+    // padding and compare bytewise
+    std::vector<unsigned char> k;
+    auto put = [&k](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        k.insert(k.end(), b, b + n);
+    };
+    const LaunchGeom g = geom();
+    const DevMesh m = mesh();
+    const DevMatrix A = matrix();
+    const DevVectors v = vectors();
+    const void* ptrs[5] = {params_.data(), state_.data(), state_b_.data(), partials_.data(), host_states_};
+    const int ints[4] = {chunks, max_width_, split_advance() ? 1 : 0, ninst_};
+    put(&dt, sizeof dt);
+    put(&ext, sizeof ext);
+    put(&g, sizeof g);
+    put(&m, sizeof m);
+    put(&A, sizeof A);
+    put(&v, sizeof v);
+    put(ptrs, sizeof ptrs);
+    put(ints, sizeof ints);
+    put(&cfg_, sizeof cfg_);
+    return k;
+}
+
+// The fixed-count step (every instance at tol = 0) as one graph: assembly,
+// the PCG start, `chunks` chunks of iterations, the true residual of the
+// stopped solves, the final reduction, finalize and the copy of the solver
+// states for the statistics.  Captured once and replayed while nothing it
+// was captured with changes; no memset is recorded.
+void Problem::launch_fixed_step_graph(float dt, const DevExt& ext, int chunks, hipStream_t s) {
+    std::vector<unsigned char> key = fixed_step_key(dt, ext, chunks);
+    if (!step_exec_ || key != step_key_) {
+        if (step_exec_) (void)hipGraphExecDestroy(step_exec_);
+        step_exec_ = nullptr;
+        step_key_.clear();
+        if (!cap_stream_) check_hip(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking), "hipStreamCreate");
+        hipGraph_t graph = nullptr;
+        check_hip(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            launch_assemble(dt, ext, cap_stream_);
+            launch_pcg_start(cap_stream_);
+            for (int c = 0; c < chunks; ++c) launch_pcg_chunk_body(cap_stream_);
+            launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 1, cap_stream_);
+            launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, geom().ngroups, kPhaseFinal,
+                              cap_stream_);
+            launch_finalize(dt, ext, cap_stream_);
+            check_hip(hipMemcpyAsync(host_states_, state_.data(), sizeof(InstState) * ninst_, hipMemcpyDeviceToHost,
+                                     cap_stream_),
+                      "hipMemcpyAsync(states)");
+        } catch (...) {
+            (void)hipStreamEndCapture(cap_stream_, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        check_hip(hipStreamEndCapture(cap_stream_, &graph), "hipStreamEndCapture");
+        hipError_t e = hipGraphInstantiate(&step_exec_, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        check_hip(e, "hipGraphInstantiate(step)");
+        step_key_ = std::move(key);
+    }
+    check_hip(hipGraphLaunch(step_exec_, s), "hipGraphLaunch(step)");
+}
+
 void Problem::launch_finalize(float dt, const DevExt& ext, hipStream_t s) {
     chysx_amd::launch_finalize(geom(), vectors(), ext, params_.data(), dt, mesh(), cfg_.finalize_waves, s);
 }
@@ -940,7 +1008,6 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
     if (!built_ || ninst_ == 0) return;
     wait_pending(s);
     stats_pending_ = false;
-    launch_assemble(dt, ext, s);
     ws_ran_ = false;
     proj_ran_ = false;
     const bool ws_on = ws_mode_ != kWarmPrevious || ws_train_;
@@ -953,22 +1020,33 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
         fixed_iters = std::max(fixed_iters, q.max_iter);
     }
     if (fixed && !blocking_) {
-        launch_pcg_start(s);
         // max_iter iterations plus a flush round and the advance that stops
-        // the solve; rounds after the stop exit at once
-        launch_pcg_chunks(s, std::max(1, (fixed_iters + 2 + kChunkIters - 1) / kChunkIters));
-        // the true residual of every stopped solve, for the statistics
-        launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 1, s);
-        launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, geom().ngroups, kPhaseFinal, s);
-        launch_finalize(dt, ext, s);
-        check_hip(hipMemcpyAsync(host_states_, state_.data(), sizeof(InstState) * ninst_, hipMemcpyDeviceToHost, s),
-                  "hipMemcpyAsync(states)");
+        // the solve (rounds after the stop exit at once): the whole step,
+        // assembly to finalize and the statistics' copy, is one graph; very
+        // long fixed solves replay the chunk graphs instead, so a graph never
+        // holds more than kStepGraphMaxChunks chunks
+        const int chunks = std::max(1, (fixed_iters + 2 + kChunkIters - 1) / kChunkIters);
+        if (chunks <= kStepGraphMaxChunks) {
+            launch_fixed_step_graph(dt, ext, chunks, s);
+        } else {
+            launch_assemble(dt, ext, s);
+            launch_pcg_start(s);
+            launch_pcg_chunks(s, chunks);
+            launch_pcg_residual(geom(), matrix(), vectors(), state_.data(), 1, s);
+            launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, geom().ngroups, kPhaseFinal,
+                              s);
+            launch_finalize(dt, ext, s);
+            check_hip(
+                hipMemcpyAsync(host_states_, state_.data(), sizeof(InstState) * ninst_, hipMemcpyDeviceToHost, s),
+                "hipMemcpyAsync(states)");
+        }
         if (!done_event_) check_hip(hipEventCreateWithFlags(&done_event_, hipEventDisableTiming), "hipEventCreate");
         check_hip(hipEventRecord(done_event_, s), "hipEventRecord(step)");
         stats_pending_ = true;
         last_iters_ = fixed_iters;
         return;
     }
+    launch_assemble(dt, ext, s);
     if (ws_on && hist_n_ >= 2) warm_start_stage(dt, ext, s);
     launch_pcg_start(s);
     // Launch as many rounds as the previous step used (its iterations plus a

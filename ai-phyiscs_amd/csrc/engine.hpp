@@ -271,6 +271,9 @@ public:
     double last_ws_loss() const { return ws_loss_; }
 
     static constexpr int kChunkIters = 8;
+    // Longest fixed-count solve captured with its whole step in one graph
+    // (64 chunks: 512 iterations, about 1,600 kernel nodes).
+    static constexpr int kStepGraphMaxChunks = 64;
     static constexpr int kAutotuneMinRows = 1 << 15;
     static constexpr int kAutotuneReps = 8;
     // Groups from which the solver state is advanced in a kernel of its own
@@ -395,6 +398,13 @@ private:
     hipStream_t cap_stream_ = nullptr;
     static constexpr int kChunkGraphLevels = 4;  // graphs of 1, 2, 4 and 8 chunks
     hipGraphExec_t chunk_exec_[kChunkGraphLevels] = {};
+    // The whole fixed-count step as one graph, and what it was captured with
+    // (dt, caller buffers, chunk count, every device pointer and launch shape
+    // its kernels take): a step whose key differs captures it again.
+    hipGraphExec_t step_exec_ = nullptr;
+    std::vector<unsigned char> step_key_;
+    std::vector<unsigned char> fixed_step_key(float dt, const DevExt& ext, int chunks) const;
+    void launch_fixed_step_graph(float dt, const DevExt& ext, int chunks, hipStream_t s);
 };
 
 // The reference's ClothSimulator behaviour on one instance.

@@ -454,6 +454,34 @@ def test_reference_mode_step_returns_before_its_stream_finishes(cuda):
     assert np.array_equal(p0, p1) and np.array_equal(v0, v1) and np.array_equal(x0, x1) and r0 == r1
 
 
+def test_reference_mode_step_graph_replays_and_recaptures(cuda):
+    """In reference mode (tol = 0) the whole step, assembly to finalize, is one
+    graph, replayed while nothing it was captured with changes: a new dt
+    captures it again and new pin targets reach it through its parameters.
+    Every step is bitwise the polled step's.  A fixed solve longer than the
+    graph's bound (600 iterations) replays the chunk graphs instead."""
+    s = scenes.config3(w=64, steps=8)
+    out = []
+    for blocking in (False, True):
+        g = make_sim(s, tol=0.0, max_iter=50)
+        g.sim.set_blocking_step(blocking)
+        tgt = s.pin_targets.copy()
+        for k in range(8):
+            if k == 6:
+                tgt = (tgt + np.array([0.0, 0.01, 0.0], np.float32)).astype(np.float32)
+                g.sim.update_pin_targets(tgt)
+            g.sim.step(s.dt if k < 4 else 0.5 * s.dt)
+            st = g.sim.last_solve_stats()
+            assert st["iters"] == 50 and st["status"] == STOPPED, (k, st)
+        out.append((g.positions(), g.velocities(), g.sim.get_dx64()))
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+    g = make_sim(s, tol=0.0, max_iter=600)
+    for _ in range(2):
+        g.sim.step(s.dt)
+        assert g.sim.last_solve_stats()["iters"] == 600
+
+
 def test_reference_mode_batch_enqueues_and_matches_blocking(cuda):
     """A batch whose cloths all run fixed iteration counts is enqueued the same
     way; its results are bitwise those of the blocking step."""
