@@ -1079,6 +1079,88 @@ __host__ __device__ inline int ws_small_solve(int k, const double* G, const doub
     return kept;
 }
 
+// ws_small_solve with k fixed at compile time (every loop unrolled, the
+// matrices in registers): the same operations in the same order.
+// Index of entry (i, j), i <= j, of a K x K symmetric matrix packed as its
+// upper triangle row by row.
+template <int K>
+__device__ __forceinline__ constexpr int ws_tri(int i, int j) {
+    return i <= j ? i * K - i * (i - 1) / 2 + (j - i) : j * K - j * (j - 1) / 2 + (i - j);
+}
+// G is the packed upper triangle (ws_tri); a[i][j] = (G_ij + G_ji) / 2 as
+// ws_small_solve forms it from the full matrix.
+template <int K>
+__device__ __forceinline__ int ws_small_solve_k(const double* G, const double* g, double* c) {
+    double a[K][K], v[K][K];
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const double gij = G[ws_tri<K>(i, j)];
+            a[i][j] = 0.5 * (gij + gij);
+            v[i][j] = (i == j) ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0.0, dia = 0.0;
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+            dia += a[p][p] * a[p][p];
+#pragma unroll
+            for (int q = p + 1; q < K; ++q) off += a[p][q] * a[p][q];
+        }
+        if (!(off > 1e-30 * dia)) break;
+#pragma unroll
+        for (int p = 0; p < K; ++p)
+#pragma unroll
+            for (int q = p + 1; q < K; ++q) {
+                if (a[p][q] == 0.0) continue;
+                const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                const double t = fabs(theta) > 1e150 ? 0.5 / theta
+                                                      : (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+#pragma unroll
+                for (int r = 0; r < K; ++r) {
+                    const double arp = a[r][p], arq = a[r][q];
+                    a[r][p] = cs * arp - sn * arq;
+                    a[r][q] = sn * arp + cs * arq;
+                }
+#pragma unroll
+                for (int r = 0; r < K; ++r) {
+                    const double apr = a[p][r], aqr = a[q][r];
+                    a[p][r] = cs * apr - sn * aqr;
+                    a[q][r] = sn * apr + cs * aqr;
+                }
+#pragma unroll
+                for (int r = 0; r < K; ++r) {
+                    const double vrp = v[r][p], vrq = v[r][q];
+                    v[r][p] = cs * vrp - sn * vrq;
+                    v[r][q] = sn * vrp + cs * vrq;
+                }
+            }
+    }
+    double lmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        c[i] = 0.0;
+        lmax = fmax(lmax, a[i][i]);
+    }
+    int kept = 0;
+    if (!(lmax > 0.0)) return 0;
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        const double lam = a[m][m];
+        if (!(lam > 1e-12 * lmax)) continue;
+        double pr = 0.0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) pr += v[i][m] * g[i];
+        const double cf = pr / lam;
+#pragma unroll
+        for (int i = 0; i < K; ++i) c[i] += cf * v[i][m];
+        ++kept;
+    }
+    return kept;
+}
+
 int ws_small_solve_host(int k, const double* G, const double* g, double* c) {
     if (k < 1 || k > kWsHistMax) throw std::invalid_argument("ws_small_solve: 1 <= k <= 8");
     return ws_small_solve(k, G, g, c);
@@ -1183,21 +1265,60 @@ __global__ __launch_bounds__(256) void ws_gram_kernel(LaunchGeom g, DevMatrix A,
 
 // Sums of `nv` per-slice partials over chunk blockIdx.y of instance
 // blockIdx.x; fixed order, so the result does not depend on timing.
+// One workgroup per (instance, chunk of kWsChunkSlices slices, value): thread
+// t adds slices t, t + 256, ... of the chunk in order, then the wave sums and
+// the four waves in order, so every value's sum is fixed whatever the launch.
 __global__ __launch_bounds__(256) void ws_chunk_kernel(WsReduce red, int nv, const InstState* __restrict__ st) {
-    const int inst = blockIdx.x, c = blockIdx.y;
+    const int inst = blockIdx.x, c = blockIdx.y, v = blockIdx.z;
     const int s0 = st[inst].slice0, ns = st[inst].nslices;
     const int a = c * kWsChunkSlices, e = min(ns, a + kWsChunkSlices);
     const int wv = int(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    __shared__ double sh[4][kWsMaxValues];
-    for (int v = 0; v < nv; ++v) {
-        double acc = 0.0;
-        for (int q = a + int(threadIdx.x); q < e; q += 256) acc += red.part[(size_t)v * red.ldp + s0 + q];
-        acc = wave_sum(acc);
-        if (lane == 0) sh[wv][v] = acc;
-    }
+    __shared__ double sh[4];
+    double acc = 0.0;
+    const double* __restrict__ pv = red.part + (size_t)v * red.ldp + s0;
+    for (int q = a + int(threadIdx.x); q < e; q += 256) acc += pv[q];
+    acc = wave_sum(acc);
+    if (lane == 0) sh[wv] = acc;
     __syncthreads();
-    const int t = threadIdx.x;
-    if (t < nv) red.part2[((size_t)inst * red.chunks + c) * nv + t] = ((sh[0][t] + sh[1][t]) + sh[2][t]) + sh[3][t];
+    if (threadIdx.x == 0) red.part2[((size_t)inst * red.chunks + c) * nv + v] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// The projection's decision for K history vectors, with K fixed at compile
+// time so the eigen-solve stays in registers; the Gram matrices are read
+// from the packed sums in LDS where they are needed.
+template <int K>
+__device__ __forceinline__ void ws_finish_projection(const double* __restrict__ tot, WsInst& W) {
+    constexpr int NT = K * (K + 1) / 2;
+    const double* G = tot;       // V^T A V, packed
+    const double* Q = tot + NT;  // (AV)^T (AV), packed
+    double gv[K], bav[K], c[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        gv[i] = tot[2 * NT + i];
+        bav[i] = tot[2 * NT + K + i];
+    }
+    const double bb = tot[2 * NT + 2 * K];
+    const int kept = ws_small_solve_k<K>(G, gv, c);
+    double phi = 0.0, rr = bb;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        double gc = 0.0, qc = 0.0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            gc += G[ws_tri<K>(i, j)] * c[j];
+            qc += Q[ws_tri<K>(i, j)] * c[j];
+        }
+        phi += c[i] * (0.5 * gc - gv[i]);
+        rr += c[i] * (qc - 2.0 * bav[i]);
+    }
+    const double phi_prev = 0.5 * G[0] - gv[0];
+    W.bb = bb;
+    W.rr_prev = fmax(bb - 2.0 * bav[0] + Q[0], 0.0);
+    W.rr_proj = fmax(rr, 0.0);
+    W.kdim = kept;
+    W.use_proj = (kept > 0 && phi < phi_prev) ? 1 : 0;
+#pragma unroll
+    for (int i = 0; i < kWsHistMax; ++i) W.coef[i] = (i < K) ? c[i] : 0.0;
 }
 
 __global__ __launch_bounds__(128) void ws_finish_kernel(WsReduce red, int nv, const InstState* __restrict__ st,
@@ -1214,35 +1335,7 @@ __global__ __launch_bounds__(128) void ws_finish_kernel(WsReduce red, int nv, co
     if (t != 0) return;
     WsInst& W = wi[inst];
     if (phase == kWsPhaseProj) {
-        const int K = hk;
-        double G[kWsHistMax * kWsHistMax], Q[kWsHistMax * kWsHistMax], gv[kWsHistMax], bav[kWsHistMax],
-            c[kWsHistMax];
-        int o = 0;
-        for (int i = 0; i < K; ++i)
-            for (int j = i; j < K; ++j) G[i * K + j] = G[j * K + i] = tot[o++];
-        for (int i = 0; i < K; ++i)
-            for (int j = i; j < K; ++j) Q[i * K + j] = Q[j * K + i] = tot[o++];
-        for (int i = 0; i < K; ++i) gv[i] = tot[o++];
-        for (int i = 0; i < K; ++i) bav[i] = tot[o++];
-        const double bb = tot[o++];
-        const int kept = ws_small_solve(K, G, gv, c);
-        double phi = 0.0, rr = bb;
-        for (int i = 0; i < K; ++i) {
-            double gc = 0.0, qc = 0.0;
-            for (int j = 0; j < K; ++j) {
-                gc += G[i * K + j] * c[j];
-                qc += Q[i * K + j] * c[j];
-            }
-            phi += c[i] * (0.5 * gc - gv[i]);
-            rr += c[i] * (qc - 2.0 * bav[i]);
-        }
-        const double phi_prev = 0.5 * G[0] - gv[0];
-        W.bb = bb;
-        W.rr_prev = fmax(bb - 2.0 * bav[0] + Q[0], 0.0);
-        W.rr_proj = fmax(rr, 0.0);
-        W.kdim = kept;
-        W.use_proj = (kept > 0 && phi < phi_prev) ? 1 : 0;
-        for (int i = 0; i < kWsHistMax; ++i) W.coef[i] = (i < K) ? c[i] : 0.0;
+        return;  // ws_finish_proj_kernel
     } else if (phase == kWsPhaseBase) {
         W.rr_base = tot[0];
         W.scale = fmax(sqrt(tot[1] / (3.0 * 64.0 * double(max(1, st[inst].nslices)))), 1e-12);
@@ -1279,6 +1372,23 @@ __global__ __launch_bounds__(128) void ws_finish_kernel(WsReduce red, int nv, co
             W.gcoef = 0.0;
         }
     }
+}
+
+// The projection phase of ws_finish_kernel, one kernel per history length K
+// (its eigen-solve held in registers without spilling the others).
+template <int K>
+__global__ __launch_bounds__(128) void ws_finish_proj_kernel(WsReduce red, int nv, WsInst* __restrict__ wi) {
+    const int inst = blockIdx.x;
+    __shared__ double tot[kWsMaxValues];
+    const int t = threadIdx.x;
+    if (t < nv) {
+        double a = 0.0;
+        for (int c = 0; c < red.chunks; ++c) a += red.part2[((size_t)inst * red.chunks + c) * nv + t];
+        tot[t] = a;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    ws_finish_projection<K>(tot, wi[inst]);
 }
 
 __global__ __launch_bounds__(256) void ws_proj_apply_kernel(LaunchGeom g, WsHist h, const WsInst* __restrict__ wi,
@@ -1600,8 +1710,26 @@ void launch_ws_finish(const WsReduce& red, int nvalues, const InstState* st, WsI
                       int hist_k, hipStream_t s) {
     if (ninst <= 0) return;
     if (nvalues < 1 || nvalues > kWsMaxValues) throw std::invalid_argument("ws_finish: bad value count");
-    hipLaunchKernelGGL(ws_chunk_kernel, dim3(unsigned(ninst), unsigned(red.chunks)), dim3(256), 0, s, red, nvalues, st);
+    if (phase == kWsPhaseProj && (hist_k < 1 || hist_k > kWsHistMax))
+        throw std::invalid_argument("ws_finish: the projection needs 1 to 8 history vectors");
+    hipLaunchKernelGGL(ws_chunk_kernel, dim3(unsigned(ninst), unsigned(red.chunks), unsigned(nvalues)), dim3(256), 0, s,
+                       red, nvalues, st);
     check_hip(hipGetLastError(), "ws_chunk_kernel");
+    if (phase == kWsPhaseProj) {
+        const dim3 grid{unsigned(ninst)}, block{128u};
+        switch (hist_k) {
+            case 1: hipLaunchKernelGGL(ws_finish_proj_kernel<1>, grid, block, 0, s, red, nvalues, wi); break;
+            case 2: hipLaunchKernelGGL(ws_finish_proj_kernel<2>, grid, block, 0, s, red, nvalues, wi); break;
+            case 3: hipLaunchKernelGGL(ws_finish_proj_kernel<3>, grid, block, 0, s, red, nvalues, wi); break;
+            case 4: hipLaunchKernelGGL(ws_finish_proj_kernel<4>, grid, block, 0, s, red, nvalues, wi); break;
+            case 5: hipLaunchKernelGGL(ws_finish_proj_kernel<5>, grid, block, 0, s, red, nvalues, wi); break;
+            case 6: hipLaunchKernelGGL(ws_finish_proj_kernel<6>, grid, block, 0, s, red, nvalues, wi); break;
+            case 7: hipLaunchKernelGGL(ws_finish_proj_kernel<7>, grid, block, 0, s, red, nvalues, wi); break;
+            default: hipLaunchKernelGGL(ws_finish_proj_kernel<8>, grid, block, 0, s, red, nvalues, wi); break;
+        }
+        check_hip(hipGetLastError(), "ws_finish_proj_kernel");
+        return;
+    }
     hipLaunchKernelGGL(ws_finish_kernel, dim3(unsigned(ninst)), dim3(128), 0, s, red, nvalues, st, wi, phase, hist_k,
                        1.0 / double(ninst));
     check_hip(hipGetLastError(), "ws_finish_kernel");
