@@ -488,6 +488,27 @@ def cpu_baseline(args, s, g, cpu_state):
     return cpu, fp_err
 
 
+def reference_mode(args, scene, make_sim):
+    """The reference's own stopping rule on the same cloth: a fixed 50 PCG
+    iterations per step, no convergence test (pcg_solver.cu:542-592).  Each
+    step is one captured graph launched without a host wait; the window is
+    timed from the first launch to the stream's end."""
+    import torch
+    g = make_sim(scene, tol=0.0, max_iter=50)
+    for _ in range(5):
+        g.sim.step(scene.dt)
+    torch.cuda.synchronize()
+    steps = 20
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.sim.step(scene.dt)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    st = g.sim.last_solve_stats()
+    return {"pcg_iters_per_step": st["iters"], "steps": steps, "steps_per_s": round(steps / el, 3),
+            "ms_per_step": round(1e3 * el / steps, 3), "relres_after_50": float(st["relres"])}
+
+
 def full_run(args, make_sim):
     """The whole 300-step scenario from rest: PCG iterations of every step,
     so the timed window's iteration count can be set against the run's."""
@@ -650,6 +671,7 @@ def main():
     del g
 
     full = full_run(args, make_sim) if args.full_run else None
+    ref_mode = reference_mode(args, s, make_sim) if args.full_run else None
 
     ws_cmp = None
     if args.ws_steps > 0:
@@ -699,6 +721,7 @@ def main():
         "kernels": extra_kernels,
         "kernels_hbm": hbm,
         "full_run": full,
+        "reference_mode": ref_mode,
         "cpu_baseline": cpu,
         "fp32_vs_fp64_oracle": fp_err,
         "launch_config": launch_cfg,
