@@ -1057,7 +1057,7 @@ __device__ __forceinline__ void advance_in_pass(const LaunchGeom& g, const DevVe
 // (the SpMV pass 33.4 instead of 27.4 us at 1024^2), although the compiler's
 // occupancy estimate still reports 8 waves per SIMD.  At 80 the compiler
 // spills a few scalars into vector-register lanes.
-template <int UNROLL, bool BF, bool SYM, bool PK, bool I16>
+template <int UNROLL, bool BF, bool SYM, bool PK, bool I16, bool ADV>
 __global__ __launch_bounds__(64 * kGroupSlices) __attribute__((amdgpu_num_sgpr(80))) void pcg_spmv_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
                                                                     const InstState* __restrict__ st,
                                                                     InstState* __restrict__ st_out,
@@ -1067,14 +1067,62 @@ __global__ __launch_bounds__(64 * kGroupSlices) __attribute__((amdgpu_num_sgpr(8
     const int grp = s / kGroupSlices;
     const SliceDesc sd = g.slices[s];
     // a finished cloth (its first workgroup still copies the state)
-    if (g.gactive != nullptr && g.gactive[grp] == 0 && (st_out == nullptr || grp != st[sd.inst].group0)) return;
-    InstState S = st[sd.inst];
-    if (st_out != nullptr) advance_in_pass(g, v, S, params, st_out, sd.inst, grp);
-    if (S.status != kRunning) return;
-    const int parity = S.parity;
+    if (g.gactive != nullptr && g.gactive[grp] == 0 && (!ADV || grp != st[sd.inst].group0)) return;
     const int lane = threadIdx.x & 63;
     const bool valid = lane < sd.nvalid;
     const int row = s * 64 + lane;
+    constexpr int BS = PK ? 6 : 9;
+    const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
+    const int nlow = SYM ? sd.nlower : 0;  // entries >= nlow hold no lower block in any lane
+    const int* colp = A.col + (size_t)sb * 64 + lane;
+    const int* mirp = A.mir + (size_t)sb * 64 + lane;
+    const int16_t* colp16 = A.col16 + (size_t)sb * 64 + lane;
+    const int16_t* mirp16 = A.mir16 + (size_t)sb * 64 + lane;
+    const int e0 = sb * 64 + lane;  // entry index of the row's entry 0
+    const float* vp = A.val + (size_t)sb * BS * 64 + lane;
+    // A batch of UNROLL entries runs in three phases: (1) the column and
+    // mirror offsets of every entry, (2) every gather and block load,
+    // (3) the FMAs.  No load of phase 2 depends on another load of the
+    // batch, so a batch costs two memory round trips.  Whether an entry
+    // needs its mirror offset (kk < nlow) is wave-uniform and known
+    // before any vector load returns.
+    int kk[UNROLL], cix[UNROLL], mix[UNROLL];
+    F3 zj[UNROLL], qj[UNROLL];
+    float blk[UNROLL][BS];
+    auto load_offsets = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const int k = k0 + u;
+            kk[u] = SYM ? sym_order(k, w) : k;
+            cix[u] = I16 ? int(kNoCol16) : -1;
+            mix[u] = 0;
+            if (k < w) {  // wave-uniform
+                cix[u] = I16 ? int(colp16[kk[u] * 64]) : colp[kk[u] * 64];
+                if (SYM && kk[u] < nlow) mix[u] = I16 ? int(mirp16[kk[u] * 64]) : mirp[kk[u] * 64];
+            }
+        }
+    };
+    // The first batch's offsets, the diagonal block and the row's z depend
+    // on no solver state.  Without the in-pass advance they go out before
+    // the state is read, so the two latencies overlap; with it they follow
+    // the advance (held across its sums they would cost the pass occupancy).
+    F3 zi = f3(0.0f, 0.0f, 0.0f);
+    float dg[BS];
+    auto load_first = [&] {
+        if (valid) {
+            load_offsets(0);
+            zi = ld3(v.z, row);
+            const float* dp = A.diag + (size_t)s * BS * 64 + lane;
+#pragma unroll
+            for (int q = 0; q < BS; ++q) dg[q] = dp[q * 64];
+        }
+    };
+    if constexpr (!ADV) load_first();
+    InstState S = st[sd.inst];
+    if constexpr (ADV) advance_in_pass(g, v, S, params, st_out, sd.inst, grp);
+    if (S.status != kRunning) return;
+    if constexpr (ADV) load_first();
+    const int parity = S.parity;
     if (S.mode == kModeCheck) {
         // true |r|^2 and r . z, into slots no workgroup of this pass reads
         // (the advance above read the update pass's slots 0-1)
@@ -1108,37 +1156,6 @@ __global__ __launch_bounds__(64 * kGroupSlices) __attribute__((amdgpu_num_sgpr(8
     const float* __restrict__ z = v.z;
     double part[1] = {0.0};
     if (valid) {
-        constexpr int BS = PK ? 6 : 9;
-        const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
-        const int nlow = SYM ? sd.nlower : 0;  // entries >= nlow hold no lower block in any lane
-        const int* colp = A.col + (size_t)sb * 64 + lane;
-        const int* mirp = A.mir + (size_t)sb * 64 + lane;
-        const int16_t* colp16 = A.col16 + (size_t)sb * 64 + lane;
-        const int16_t* mirp16 = A.mir16 + (size_t)sb * 64 + lane;
-        const int e0 = sb * 64 + lane;  // entry index of the row's entry 0
-        const float* vp = A.val + (size_t)sb * BS * 64 + lane;
-        // A batch of UNROLL entries runs in three phases: (1) the column and
-        // mirror offsets of every entry, (2) every gather and block load,
-        // (3) the FMAs.  No load of phase 2 depends on another load of the
-        // batch, so a batch costs two memory round trips.  Whether an entry
-        // needs its mirror offset (kk < nlow) is wave-uniform and known
-        // before any vector load returns.
-        int kk[UNROLL], cix[UNROLL], mix[UNROLL];
-        F3 zj[UNROLL], qj[UNROLL];
-        float blk[UNROLL][BS];
-        auto load_offsets = [&](int k0) {
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                const int k = k0 + u;
-                kk[u] = SYM ? sym_order(k, w) : k;
-                cix[u] = I16 ? int(kNoCol16) : -1;
-                mix[u] = 0;
-                if (k < w) {  // wave-uniform
-                    cix[u] = I16 ? int(colp16[kk[u] * 64]) : colp[kk[u] * 64];
-                    if (SYM && kk[u] < nlow) mix[u] = I16 ? int(mirp16[kk[u] * 64]) : mirp[kk[u] * 64];
-                }
-            }
-        };
         auto column = [&](int u) { return I16 ? (cix[u] == kNoCol16 ? -1 : row + cix[u]) : cix[u]; };
         // BF: padding entries hold zero blocks and gather the row's own
         // vectors instead of branching (k0 + u < w is wave-uniform).
@@ -1185,15 +1202,8 @@ __global__ __launch_bounds__(64 * kGroupSlices) __attribute__((amdgpu_num_sgpr(8
                 }
             }
         };
-        // The first batch is issued together with the row's own loads.
-        load_offsets(0);
-        const F3 zi = ld3(z, row), qi = ld3(pold, row);
-        float dg[BS];
-        {
-            const float* dp = A.diag + (size_t)s * BS * 64 + lane;
-#pragma unroll
-            for (int q = 0; q < BS; ++q) dg[q] = dp[q * 64];
-        }
+        // The first batch's operands go out with the row's p_prev.
+        const F3 qi = ld3(pold, row);
         load_operands(0);
         const F3 pi = dir3(zi, qi, beta);
         if (PK) {
@@ -1580,9 +1590,19 @@ template <bool BF, bool SYM, bool PK, bool I16>
 void pcg_spmv_launch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
                      InstState* st_out, const InstParams* params, int unroll, hipStream_t s) {
     const int grid = grid_for(g.nslices, kGroupSlices), block = 64 * kGroupSlices;
-    if (unroll == 1) pcg_spmv_kernel<1, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st, st_out, params);
-    else if (unroll == 2) pcg_spmv_kernel<2, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st, st_out, params);
-    else pcg_spmv_kernel<4, BF, SYM, PK, I16><<<grid, block, 0, s>>>(g, A, v, st, st_out, params);
+    // the in-pass advance is its own instantiation, so the pass without it
+    // carries none of its registers
+#define L(U, ADV) pcg_spmv_kernel<U, BF, SYM, PK, I16, ADV><<<grid, block, 0, s>>>(g, A, v, st, st_out, params)
+    if (st_out != nullptr) {
+        if (unroll == 1) L(1, true);
+        else if (unroll == 2) L(2, true);
+        else L(4, true);
+    } else {
+        if (unroll == 1) L(1, false);
+        else if (unroll == 2) L(2, false);
+        else L(4, false);
+    }
+#undef L
 }
 template <bool BF, bool SYM, bool PK>
 void pcg_spmv_dispatch(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
