@@ -221,4 +221,5 @@ def test_committed_warm_start_parameters_load():
     theta, meta = warmstart.load_params()
     assert theta.dtype == np.float32 and theta.size == chysx._chysx_native.ws_num_params()
     assert np.all(np.isfinite(theta))
-    assert meta["mode"] == "learned" and meta["pairs"] >= 50000 and meta["loss_last"] < meta["loss_first"]
+    assert meta["mode"] == "learned" and meta["pairs"] * meta["width"] ** 2 >= 50000 * 256 ** 2
+    assert meta["loss_last"] < meta["loss_first"]
