@@ -992,8 +992,10 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
     const DevVectors v = vectors();
     launch_ws_base(geom(), matrix(), v, x_.data(), xb_.data(), rb_.data(), red, s);
     launch_ws_finish(red, 3, state_.data(), ws_inst_.data(), ninst_, kWsPhaseBase, proj_ran_ ? 1 : 0, s);
+    // inference reads the compact edge features; training's unfused GEMMs the 32 columns
+    N.set_compact_edges(!ws_train_);
     launch_ws_features(geom(), matrix(), v, ext, mesh(), rest_pos_.data(), h, xb_.data(), ws_inst_.data(), dt,
-                       N.features(), N.edge_features(), s);
+                       N.features(), N.edge_features(), ws_train_ ? nullptr : N.edge_features_compact(), s);
     // inference: the last layer writes the direction d itself; training keeps
     // the full output (and the activations for backward)
     N.forward(ws_graph(), s, ws_train_, !ws_train_, ws_inst_.data(), d_.data());

@@ -197,6 +197,7 @@ struct LayerArgs {
     const int* row_deg;
     const __bf16* h;     // rows x HID
     const __bf16* e;     // rows x 32
+    const __bf16* e4;    // rows x 4 (columns 0..3 of e, the rest zero), read instead of e when given
     const __bf16* w;     // (HID, XL): the layer's weights, (NOUT, K) layout
     const float* bias;   // HID
     __bf16* h_out;       // rows x HID (not LAST)
@@ -313,6 +314,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void w
                         hres[mt][kc] = xf[mt];
                     } else if (kc < 4) {
                         xf[mt] = *reinterpret_cast<const bf16x8*>(tile + tl * kStageLd + 32 * (kc - 2) + 8 * lg);
+                    } else if (a.e4 != nullptr) {
+                        bf16x8 t;
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) t[u] = f2bf(0.0f);
+                        if (lg == 0) {
+                            const bf16x4 e4 = *reinterpret_cast<const bf16x4*>(a.e4 + (size_t)row * 4);
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) t[u] = e4[u];
+                        }
+                        xf[mt] = t;
                     } else {
                         xf[mt] = *reinterpret_cast<const bf16x8*>(a.e + (size_t)row * 32 + 8 * lg);
                     }
@@ -383,7 +394,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void w
 }
 
 // Inference encoder: h0 = relu(F We + be) for the 64 rows of a slice per
-// wave, MFMA operands swapped like the layers (8-byte vector stores).
+// wave, MFMA operands swapped like the layers and the weight rows permuted the
+// same way (ws_out_col), so that lane (li, lg) holds columns 8 lg .. 8 lg + 7
+// and 32 + 8 lg .. 32 + 8 lg + 7 of its row: two 16-byte stores.
 __global__ __launch_bounds__(256) void ws_encode_kernel(int nslices, const __bf16* __restrict__ F,
                                                         const __bf16* __restrict__ w, const float* __restrict__ bias,
                                                         __bf16* __restrict__ h) {
@@ -393,24 +406,30 @@ __global__ __launch_bounds__(256) void ws_encode_kernel(int nslices, const __bf1
     const int row0 = s * 64;
     const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4;
     bf16x8 wf[NT];
+    float bv[NT][4];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) wf[nt] = *reinterpret_cast<const bf16x8*>(w + (size_t)(nt * 16 + li) * FIN + 8 * lg);
+    for (int nt = 0; nt < NT; ++nt) {
+        wf[nt] = *reinterpret_cast<const bf16x8*>(w + (size_t)ws_out_col(nt, li) * FIN + 8 * lg);
+        const float4 bb = *reinterpret_cast<const float4*>(bias + ws_out_col(nt, 4 * lg));
+        bv[nt][0] = bb.x;
+        bv[nt][1] = bb.y;
+        bv[nt][2] = bb.z;
+        bv[nt][3] = bb.w;
+    }
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
         const int row = row0 + mt * 16 + li;
         const bf16x8 xf = *reinterpret_cast<const bf16x8*>(F + (size_t)row * FIN + 8 * lg);
+        bf16x8 o[2];
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[nt], xf, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-            const int n0 = nt * 16 + 4 * lg;
-            const float4 bb = *reinterpret_cast<const float4*>(bias + n0);
-            bf16x4 o;
-            o[0] = f2bf(fmaxf(acc[0] + bb.x, 0.0f));
-            o[1] = f2bf(fmaxf(acc[1] + bb.y, 0.0f));
-            o[2] = f2bf(fmaxf(acc[2] + bb.z, 0.0f));
-            o[3] = f2bf(fmaxf(acc[3] + bb.w, 0.0f));
-            *reinterpret_cast<bf16x4*>(h + (size_t)row * HID + n0) = o;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) o[nt >> 1][4 * (nt & 1) + r] = f2bf(fmaxf(acc[r] + bv[nt][r], 0.0f));
         }
+        bf16x8* dst = reinterpret_cast<bf16x8*>(h + (size_t)row * HID + 8 * lg);
+        dst[0] = o[0];
+        dst[4] = o[1];
     }
 }
 
@@ -721,6 +740,7 @@ WarmStartNet::~WarmStartNet() {
 void WarmStartNet::free_all() {
     dfree(F_);
     dfree(E_);
+    dfree(E4_);
     for (auto& p : H_) dfree(p);
     for (auto& p : A_) dfree(p);
     for (auto& p : P_) dfree(p);
@@ -743,12 +763,14 @@ void WarmStartNet::reserve(int rows, bool training) {
     const size_t R = size_t(rows);
     dalloc(F_, R * FIN);
     dalloc(E_, R * 32);
+    dalloc(E4_, R * 4);
     for (auto& p : H_) dalloc(p, R * HID);
     for (auto& p : A_) dalloc(p, R * HID);
     for (auto& p : P_) dalloc(p, R * HID);
     dalloc(out_, R * NOUTD);
     check_hip(hipMemset(F_, 0, R * FIN * sizeof(__bf16)), "hipMemset");
     check_hip(hipMemset(E_, 0, R * 32 * sizeof(__bf16)), "hipMemset");
+    check_hip(hipMemset(E4_, 0, R * 4 * sizeof(__bf16)), "hipMemset");
     if (training) {
         dalloc(dout_, R * NOUTD);
         dalloc(dh_, R * HID);
@@ -799,6 +821,7 @@ void WarmStartNet::init_params(uint64_t seed, hipStream_t s) {
 
 void WarmStartNet::set_inputs(const float* F, const float* E, int n, hipStream_t s) {
     if (n > rows_) throw std::invalid_argument("warm start: reserve() rows first");
+    compact_edges_ = false;
     float* tmp = nullptr;
     dalloc(tmp, size_t(n) * 32);
     const size_t nin = size_t(n) * 32, nout = size_t(rows_) * 32;
@@ -861,6 +884,7 @@ void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep, bool dire
             la.row_deg = g.row_deg;
             la.h = H_[l];
             la.e = E_;
+            la.e4 = compact_edges_ ? E4_ : nullptr;
             la.w = wnk_ + off_w(l);
             la.bias = theta_ + off_b(l);
             la.h_out = H_[l + 1];
@@ -1481,7 +1505,8 @@ __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatri
                                                           DevMesh m, const float* __restrict__ rest, WsHist h,
                                                           const double* __restrict__ xb,
                                                           const WsInst* __restrict__ wi, float dt,
-                                                          __bf16* __restrict__ F, __bf16* __restrict__ E) {
+                                                          __bf16* __restrict__ F, __bf16* __restrict__ E,
+                                                          __bf16* __restrict__ E4) {
     const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
@@ -1555,6 +1580,13 @@ __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatri
 #pragma unroll
         for (int q = 0; q < 8; ++q) t[q] = f2bf(f[8 * u + q]);
         fo[u] = t;
+    }
+    if (E4 != nullptr) {
+        bf16x4 t;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[q] = f2bf(ef[q]);
+        *reinterpret_cast<bf16x4*>(E4 + (size_t)row * 4) = t;
+        return;
     }
     bf16x8* eo = reinterpret_cast<bf16x8*>(E + (size_t)row * 32);
 #pragma unroll
@@ -1750,10 +1782,10 @@ void launch_ws_base(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v
 
 void launch_ws_features(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const DevExt& e,
                         const DevMesh& m, const float* rest_pos, const WsHist& h, const double* xb,
-                        const WsInst* wi, float dt, __bf16* F, __bf16* E, hipStream_t s) {
+                        const WsInst* wi, float dt, __bf16* F, __bf16* E, __bf16* E4, hipStream_t s) {
     if (g.nslices <= 0) return;
     hipLaunchKernelGGL(ws_features_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, v, e, m, rest_pos, h,
-                       xb, wi, dt, F, E);
+                       xb, wi, dt, F, E, E4);
     check_hip(hipGetLastError(), "ws_features_kernel");
 }
 

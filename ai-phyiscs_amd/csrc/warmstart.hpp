@@ -60,6 +60,13 @@ public:
     // Inputs written by the caller before forward().
     __bf16* features() { return F_; }
     __bf16* edge_features() { return E_; }
+    // Compact edge features (rows x 4, BF16): the four columns the simulator's
+    // features fill, the other 28 being zero.  With set_compact_edges(true) the
+    // fused inference layers read these instead of edge_features() (8 bytes a
+    // row instead of 64; the same products, since the rest are zeros).
+    // set_inputs() switches back to the full 32 columns.
+    __bf16* edge_features_compact() { return E4_; }
+    void set_compact_edges(bool on) { compact_edges_ = on; }
     // Forward: out (rows x 32, FP32).  keep = store activations for backward
     // (one kernel per aggregation and per GEMM).  Without keep each layer is
     // one fused kernel (the neighbour mean staged in LDS, never in memory); with
@@ -107,6 +114,8 @@ private:
     // activations (rows x width, BF16 unless noted)
     __bf16* F_ = nullptr;      // 32
     __bf16* E_ = nullptr;      // 32 (mean edge features)
+    __bf16* E4_ = nullptr;     // 4 (the same, compact)
+    bool compact_edges_ = false;
     __bf16* H_[4] = {};        // 64 each (H_[0] = encoder output)
     __bf16* A_[3] = {};        // 64 each (mean neighbour h per layer)
     __bf16* P_[4] = {};        // pre-activations: P_[0] encoder, P_[1..3] layers
@@ -195,10 +204,12 @@ void launch_ws_proj_apply(const LaunchGeom& g, const WsHist& h, const WsInst* wi
 // z = M^-1 r_b into v.z, xb <- x; partials [|r_b|^2, |x|^2].
 void launch_ws_base(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const double* x, double* xb,
                     float* rb, const WsReduce& red, hipStream_t s);
-// Network inputs for every row: features F and mean edge features E (BF16).
+// Network inputs for every row: features F and mean edge features (BF16),
+// into E4 (rows x 4) when it is given, else into E (rows x 32, columns 4..31
+// zero).
 void launch_ws_features(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const DevExt& e,
                         const DevMesh& m, const float* rest_pos, const WsHist& h, const double* xb,
-                        const WsInst* wi, float dt, __bf16* F, __bf16* E, hipStream_t s);
+                        const WsInst* wi, float dt, __bf16* F, __bf16* E, __bf16* E4, hipStream_t s);
 // d = scale * out[:, 0:3] and A d; partials [d.r_b, d.Ad, Ad.r_b, Ad.Ad].
 void launch_ws_direction(const LaunchGeom& g, const DevMatrix& A, const float* out, const WsInst* wi,
                          const float* rb, float* d, float* ad, const WsReduce& red, hipStream_t s);
