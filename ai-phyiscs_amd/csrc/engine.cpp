@@ -992,13 +992,19 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
     const DevVectors v = vectors();
     launch_ws_base(geom(), matrix(), v, x_.data(), xb_.data(), rb_.data(), red, s);
     launch_ws_finish(red, 3, state_.data(), ws_inst_.data(), ninst_, kWsPhaseBase, proj_ran_ ? 1 : 0, s);
-    // inference reads the compact edge features; training's unfused GEMMs the 32 columns
-    N.set_compact_edges(!ws_train_);
-    launch_ws_features(geom(), matrix(), v, ext, mesh(), rest_pos_.data(), h, xb_.data(), ws_inst_.data(), dt,
-                       N.features(), N.edge_features(), ws_train_ ? nullptr : N.edge_features_compact(), s);
-    // inference: the last layer writes the direction d itself; training keeps
-    // the full output (and the activations for backward)
-    N.forward(ws_graph(), s, ws_train_, !ws_train_, ws_inst_.data(), d_.data());
+    // inference: the features go straight into the encoder, the layers read
+    // the compact edge features and the last layer writes the direction d
+    // itself; training keeps F, the 32 edge-feature columns, the full output
+    // and the activations for backward
+    if (ws_train_) {
+        N.set_compact_edges(false);
+        launch_ws_features(geom(), matrix(), v, ext, mesh(), rest_pos_.data(), h, xb_.data(), ws_inst_.data(), dt,
+                           N.features(), N.edge_features(), nullptr, s);
+        N.forward(ws_graph(), s, true, false, ws_inst_.data(), d_.data());
+    } else {
+        N.features_encode(geom(), matrix(), v, ext, mesh(), rest_pos_.data(), h, xb_.data(), ws_inst_.data(), dt, s);
+        N.forward(ws_graph(), s, false, true, ws_inst_.data(), d_.data(), true);
+    }
     launch_ws_direction(geom(), matrix(), ws_train_ ? N.out() : nullptr, ws_inst_.data(), rb_.data(), d_.data(),
                         ad_.data(), red, s);
     launch_ws_finish(red, 4, state_.data(), ws_inst_.data(), ninst_, kWsPhaseLine, ws_accept_residual_ ? 1 : 0, s);

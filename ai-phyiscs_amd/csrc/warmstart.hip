@@ -178,9 +178,12 @@ __global__ __launch_bounds__(256) void ws_gemm_kernel(GemmArgs a) {
 //     taken in the order ch(nt, i) = 32 (nt >> 1) + 8 (i >> 2) + 4 (nt & 1) +
 //     (i & 3), so the lane (li, lg) ends with output columns 8 lg .. 8 lg + 7
 //     and 32 + 8 lg .. 32 + 8 lg + 7 of row 16 mt + li: the h columns it
-//     loaded for K chunks 0 and 1, which are the residual, and two 16-byte
-//     vector stores write them.  Permuting the output columns changes no
-//     result (each is the same dot product).
+//     loaded for K chunks 0 and 1 (before the gathers), which are the
+//     residual.  Permuting the output columns changes no result (each is the
+//     same dot product).  The rows go back through the wave's tile, so each
+//     store instruction writes eight whole 128-B rows: written from the MFMA
+//     layout directly, each store covered half of 16 rows, and the partial
+//     lines made the layer 30% slower.
 // With LAST the decoder follows in the epilogue: only the output columns the
 // warm start uses (0..2) are formed, from the BF16-rounded h' (as the decoder
 // GEMM would read it), by FP32 FMAs and a sum over the four lanes of a row;
@@ -218,6 +221,14 @@ static_assert(OFF_W0 % 8 == 0 && LAYER_STRIDE % 8 == 0 && OFF_WD % 8 == 0, "16-b
 // the weight row an A-operand lane li = 4 lg + r supplies).
 __device__ __forceinline__ int ws_out_col(int nt, int i) { return 32 * (nt >> 1) + 8 * (i >> 2) + 4 * (nt & 1) + (i & 3); }
 
+// Orders one wave's LDS accesses around a point: LDS operations of a wave
+// complete in issue order, so only the compiler's reordering is prevented.
+__device__ __forceinline__ void ws_wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool LAST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void ws_layer_kernel(LayerArgs a) {
     constexpr int NT = HID / 16, KC = XL / 32;
@@ -253,7 +264,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void w
         w = a.slice_ptr[s + 1] - sb;
     }
     for (int half = 0; half < 2; ++half) {
+        bf16x8 hres[2][2], xe[2];
         if (live) {
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const int row = row0 + 32 * half + 16 * mt + li;
+                hres[mt][0] = *reinterpret_cast<const bf16x8*>(a.h + (size_t)row * HID + 8 * lg);
+                hres[mt][1] = *reinterpret_cast<const bf16x8*>(a.h + (size_t)row * HID + 32 + 8 * lg);
+                if (a.e4 != nullptr) {
+                    bf16x8 t;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) t[u] = f2bf(0.0f);
+                    if (lg == 0) {
+                        const bf16x4 e4 = *reinterpret_cast<const bf16x4*>(a.e4 + (size_t)row * 4);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) t[u] = e4[u];
+                    }
+                    xe[mt] = t;
+                } else {
+                    xe[mt] = *reinterpret_cast<const bf16x8*>(a.e + (size_t)row * 32 + 8 * lg);
+                }
+            }
             // the first KB column indices and the degree of row l of the slice
             // (one coalesced load each); the gather lanes take theirs by a lane
             // shuffle
@@ -298,7 +329,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void w
         __syncthreads();  // the mean tile (and, the first time, the staged parameters)
         if (live) {
             f32x4 acc[2][NT];
-            bf16x8 hres[2][2];
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
@@ -308,24 +338,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void w
                 bf16x8 xf[2];
 #pragma unroll
                 for (int mt = 0; mt < 2; ++mt) {
-                    const int tl = 16 * mt + li, row = row0 + 32 * half + tl;
+                    const int tl = 16 * mt + li;
                     if (kc < 2) {
-                        xf[mt] = *reinterpret_cast<const bf16x8*>(a.h + (size_t)row * HID + 32 * kc + 8 * lg);
-                        hres[mt][kc] = xf[mt];
+                        xf[mt] = hres[mt][kc];
                     } else if (kc < 4) {
                         xf[mt] = *reinterpret_cast<const bf16x8*>(tile + tl * kStageLd + 32 * (kc - 2) + 8 * lg);
-                    } else if (a.e4 != nullptr) {
-                        bf16x8 t;
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) t[u] = f2bf(0.0f);
-                        if (lg == 0) {
-                            const bf16x4 e4 = *reinterpret_cast<const bf16x4*>(a.e4 + (size_t)row * 4);
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) t[u] = e4[u];
-                        }
-                        xf[mt] = t;
                     } else {
-                        xf[mt] = *reinterpret_cast<const bf16x8*>(a.e + (size_t)row * 32 + 8 * lg);
+                        xf[mt] = xe[mt];
                     }
                 }
 #pragma unroll
@@ -383,9 +402,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void w
                         }
                     }
                 } else {
-                    bf16x8* dst = reinterpret_cast<bf16x8*>(a.h_out + (size_t)row * HID + 8 * lg);
-                    dst[0] = o[0];
-                    dst[4] = o[1];
+                    // staged in the wave's tile (its mean rows are read) for row-contiguous stores
+                    if (mt == 0) ws_wave_lds_sync();
+                    bf16x8* t = reinterpret_cast<bf16x8*>(tile + (16 * mt + li) * kStageLd + 8 * lg);
+                    t[0] = o[0];
+                    t[4] = o[1];
+                }
+            }
+            if constexpr (!LAST) {
+                ws_wave_lds_sync();
+                const int r8 = l >> 3, q = l & 7;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int tl = 8 * g + r8;
+                    *reinterpret_cast<bf16x8*>(a.h_out + (size_t)(row0 + 32 * half + tl) * HID + 8 * q) =
+                        *reinterpret_cast<const bf16x8*>(tile + tl * kStageLd + 8 * q);
                 }
             }
         }
@@ -847,12 +878,15 @@ void WarmStartNet::copy_grad(float* host, hipStream_t s) const {
 }
 
 void WarmStartNet::forward(const WsGraph& g, hipStream_t s, bool keep, bool direction_only, const WsInst* wi,
-                           float* dir) {
+                           float* dir, bool encoded) {
     const int rows = g.nslices * 64;
     if (rows > rows_) throw std::invalid_argument("warm start: reserve() rows first");
     if (keep && !training_) throw std::invalid_argument("warm start: reserve(training) before a kept forward");
+    if (keep && encoded) throw std::invalid_argument("warm start: a kept forward runs its own encoder");
     // Pre-activations are only needed by backward().
-    if (!keep) {
+    if (encoded) {
+        // h0 written by features_encode
+    } else if (!keep) {
         // inference encoder, vector stores
         hipLaunchKernelGGL(ws_encode_kernel, dim3(grid_rows(rows)), dim3(256), 0, s, g.nslices, F_, wnk_ + OFF_WE,
                            theta_ + OFF_BE, H_[0]);
@@ -1501,6 +1535,78 @@ __global__ __launch_bounds__(256) void ws_base_kernel(LaunchGeom g, DevMatrix A,
 //   0-2 dx_prev, 3-5 dx_prev - dx_prev2, 6-8 z = M^-1 r_b, 9-11 x_b - dx_prev,
 //   12 pinned, 13 inertia share of the diagonal, 14 constant 1.
 // Edge features (mean over mesh neighbours): (x_j - x_i)/L0 (3), |x_j - x_i|/L0 - 1.
+// Zeros for the rows of a slice beyond its valid count.
+__device__ __forceinline__ void ws_row_features(const DevMatrix& A, const DevVectors& v, const DevExt& e,
+                                                const DevMesh& m, const float* __restrict__ rest, const WsHist& h,
+                                                const double* __restrict__ xb, const WsInst* __restrict__ wi,
+                                                float dt, int s, const SliceDesc& sd, int lane, float (&f)[FIN],
+                                                float (&ef)[4]) {
+    const int row = s * 64 + lane;
+#pragma unroll
+    for (int q = 0; q < FIN; ++q) f[q] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ef[q] = 0.0f;
+    if (lane >= sd.nvalid) return;
+    const float is = float(1.0 / fmax(wi[sd.inst].scale, 1e-30));
+    float h0, h1, h2, p0, p1, p2;
+    ld3f(h.v[0], row, h0, h1, h2);
+    ld3f(h.v[1], row, p0, p1, p2);
+    f[0] = h0 * is;
+    f[1] = h1 * is;
+    f[2] = h2 * is;
+    f[3] = (h0 - p0) * is;
+    f[4] = (h1 - p1) * is;
+    f[5] = (h2 - p2) * is;
+    f[6] = v.z[3 * row] * is;
+    f[7] = v.z[3 * row + 1] * is;
+    f[8] = v.z[3 * row + 2] * is;
+    f[9] = float(xb[3 * row] - double(h0)) * is;
+    f[10] = float(xb[3 * row + 1] - double(h1)) * is;
+    f[11] = float(xb[3 * row + 2] - double(h2)) * is;
+    f[12] = (m.pin_inc_off[row + 1] > m.pin_inc_off[row]) ? 1.0f : 0.0f;
+    const int p = sd.part0 + lane;
+    float mass = 1.0f;
+    if (e.inv_mass != nullptr) {
+        const float w = e.inv_mass[p];
+        mass = (w > 1.0e-12f) ? (1.0f / w) : 1.0e8f;
+    }
+    float dm[9];
+    load_diag(A, s, lane, dm);
+    const float tr = (dm[0] + dm[4] + dm[8]) * (1.0f / 3.0f);
+    f[13] = (tr > 0.0f) ? fminf(mass / (dt * dt) / tr, 1.0f) : 0.0f;
+    f[14] = 1.0f;
+    float px, py, pz, rx, ry, rz;
+    ld3f(e.pos, p, px, py, pz);
+    ld3f(rest, p, rx, ry, rz);
+    const int shift = sd.part0 - 64 * s;  // particle = row + shift within an instance
+    const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
+    const int* colp = A.col + (size_t)sb * 64 + lane;
+    int cnt = 0;
+    for (int k = 0; k < w; ++k) {
+        const int c = colp[k * 64];
+        if (c < 0) continue;
+        const int pj = c + shift;
+        float qx, qy, qz, sx, sy, sz;
+        ld3f(e.pos, pj, qx, qy, qz);
+        ld3f(rest, pj, sx, sy, sz);
+        const float dx = qx - px, dy = qy - py, dz = qz - pz;
+        const float l0 = sqrtf((sx - rx) * (sx - rx) + (sy - ry) * (sy - ry) + (sz - rz) * (sz - rz));
+        const float il = l0 > 1e-12f ? 1.0f / l0 : 0.0f;
+        ef[0] += dx * il;
+        ef[1] += dy * il;
+        ef[2] += dz * il;
+        ef[3] += sqrtf(dx * dx + dy * dy + dz * dz) * il - 1.0f;
+        ++cnt;
+    }
+    if (cnt) {
+        const float ic = 1.0f / float(cnt);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ef[q] *= ic;
+    }
+}
+
+// The features of every row into F (rows x 32) and the edge features into E4
+// (rows x 4) when it is given, else into E (rows x 32, columns 4..31 zero).
 __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatrix A, DevVectors v, DevExt e,
                                                           DevMesh m, const float* __restrict__ rest, WsHist h,
                                                           const double* __restrict__ xb,
@@ -1511,68 +1617,8 @@ __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatri
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63, row = s * 64 + lane;
-    float f[FIN];
-#pragma unroll
-    for (int q = 0; q < FIN; ++q) f[q] = 0.0f;
-    float ef[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (lane < sd.nvalid) {
-        const float is = float(1.0 / fmax(wi[sd.inst].scale, 1e-30));
-        float h0, h1, h2, p0, p1, p2;
-        ld3f(h.v[0], row, h0, h1, h2);
-        ld3f(h.v[1], row, p0, p1, p2);
-        f[0] = h0 * is;
-        f[1] = h1 * is;
-        f[2] = h2 * is;
-        f[3] = (h0 - p0) * is;
-        f[4] = (h1 - p1) * is;
-        f[5] = (h2 - p2) * is;
-        f[6] = v.z[3 * row] * is;
-        f[7] = v.z[3 * row + 1] * is;
-        f[8] = v.z[3 * row + 2] * is;
-        f[9] = float(xb[3 * row] - double(h0)) * is;
-        f[10] = float(xb[3 * row + 1] - double(h1)) * is;
-        f[11] = float(xb[3 * row + 2] - double(h2)) * is;
-        f[12] = (m.pin_inc_off[row + 1] > m.pin_inc_off[row]) ? 1.0f : 0.0f;
-        const int p = sd.part0 + lane;
-        float mass = 1.0f;
-        if (e.inv_mass != nullptr) {
-            const float w = e.inv_mass[p];
-            mass = (w > 1.0e-12f) ? (1.0f / w) : 1.0e8f;
-        }
-        float dm[9];
-        load_diag(A, s, lane, dm);
-        const float tr = (dm[0] + dm[4] + dm[8]) * (1.0f / 3.0f);
-        f[13] = (tr > 0.0f) ? fminf(mass / (dt * dt) / tr, 1.0f) : 0.0f;
-        f[14] = 1.0f;
-        float px, py, pz, rx, ry, rz;
-        ld3f(e.pos, p, px, py, pz);
-        ld3f(rest, p, rx, ry, rz);
-        const int shift = sd.part0 - 64 * s;  // particle = row + shift within an instance
-        const int sb = A.slice_ptr[s], w = A.slice_ptr[s + 1] - sb;
-        const int* colp = A.col + (size_t)sb * 64 + lane;
-        int cnt = 0;
-        for (int k = 0; k < w; ++k) {
-            const int c = colp[k * 64];
-            if (c < 0) continue;
-            const int pj = c + shift;
-            float qx, qy, qz, sx, sy, sz;
-            ld3f(e.pos, pj, qx, qy, qz);
-            ld3f(rest, pj, sx, sy, sz);
-            const float dx = qx - px, dy = qy - py, dz = qz - pz;
-            const float l0 = sqrtf((sx - rx) * (sx - rx) + (sy - ry) * (sy - ry) + (sz - rz) * (sz - rz));
-            const float il = l0 > 1e-12f ? 1.0f / l0 : 0.0f;
-            ef[0] += dx * il;
-            ef[1] += dy * il;
-            ef[2] += dz * il;
-            ef[3] += sqrtf(dx * dx + dy * dy + dz * dz) * il - 1.0f;
-            ++cnt;
-        }
-        if (cnt) {
-            const float ic = 1.0f / float(cnt);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) ef[q] *= ic;
-        }
-    }
+    float f[FIN], ef[4];
+    ws_row_features(A, v, e, m, rest, h, xb, wi, dt, s, sd, lane, f, ef);
     bf16x8* fo = reinterpret_cast<bf16x8*>(F + (size_t)row * FIN);
 #pragma unroll
     for (int u = 0; u < FIN / 8; ++u) {
@@ -1595,6 +1641,80 @@ __global__ __launch_bounds__(256) void ws_features_kernel(LaunchGeom g, DevMatri
 #pragma unroll
         for (int q = 0; q < 8; ++q) t[q] = f2bf((u == 0 && q < 4) ? ef[q] : 0.0f);
         eo[u] = t;
+    }
+}
+
+// Inference: the features of the wave's 64 rows go to an LDS tile instead of
+// memory, and the encoder h0 = relu(F We + be) runs on them as in
+// ws_encode_kernel (same operands, so bitwise its result); the compact edge
+// features go to E4.  Each 16-row tile of h0 goes back through LDS, so a store
+// instruction writes eight whole 128-B rows.
+constexpr int kFeatLd = FIN + 8;  // LDS row stride (BF16) of the feature tile
+__global__ __launch_bounds__(256) void ws_features_encode_kernel(LaunchGeom g, DevMatrix A, DevVectors v, DevExt e,
+                                                                 DevMesh m, const float* __restrict__ rest,
+                                                                 WsHist h, const double* __restrict__ xb,
+                                                                 const WsInst* __restrict__ wi, float dt,
+                                                                 const __bf16* __restrict__ w,
+                                                                 const float* __restrict__ bias,
+                                                                 __bf16* __restrict__ E4, __bf16* __restrict__ H0) {
+    constexpr int NT = HID / 16;
+    __shared__ __attribute__((aligned(16))) __bf16 ftile[4][64 * kFeatLd];
+    __shared__ __attribute__((aligned(16))) __bf16 otile[4][16 * kStageLd];
+    const int s = ws_slice(g.nslices);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const int l = threadIdx.x & 63, li = l & 15, lg = l >> 4, row0 = s * 64;
+    __bf16* ft = ftile[threadIdx.x >> 6];
+    __bf16* ot = otile[threadIdx.x >> 6];
+    {
+        float f[FIN], ef[4];
+        ws_row_features(A, v, e, m, rest, h, xb, wi, dt, s, sd, l, f, ef);
+#pragma unroll
+        for (int u = 0; u < FIN / 8; ++u) {
+            bf16x8 t;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) t[q] = f2bf(f[8 * u + q]);
+            *reinterpret_cast<bf16x8*>(ft + l * kFeatLd + 8 * u) = t;
+        }
+        bf16x4 t4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t4[q] = f2bf(ef[q]);
+        *reinterpret_cast<bf16x4*>(E4 + (size_t)(row0 + l) * 4) = t4;
+    }
+    bf16x8 wf[NT];
+    float bv[NT][4];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        wf[nt] = *reinterpret_cast<const bf16x8*>(w + (size_t)ws_out_col(nt, li) * FIN + 8 * lg);
+        const float4 bb = *reinterpret_cast<const float4*>(bias + ws_out_col(nt, 4 * lg));
+        bv[nt][0] = bb.x;
+        bv[nt][1] = bb.y;
+        bv[nt][2] = bb.z;
+        bv[nt][3] = bb.w;
+    }
+    ws_wave_lds_sync();
+    const int r8 = l >> 3, q8 = l & 7;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const bf16x8 xf = *reinterpret_cast<const bf16x8*>(ft + (16 * mt + li) * kFeatLd + 8 * lg);
+        bf16x8 o[2];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[nt], xf, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) o[nt >> 1][4 * (nt & 1) + r] = f2bf(fmaxf(acc[r] + bv[nt][r], 0.0f));
+        }
+        ws_wave_lds_sync();  // the previous tile's rows are read
+        bf16x8* t = reinterpret_cast<bf16x8*>(ot + li * kStageLd + 8 * lg);
+        t[0] = o[0];
+        t[4] = o[1];
+        ws_wave_lds_sync();
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq) {
+            const int tl = 8 * gq + r8;
+            *reinterpret_cast<bf16x8*>(H0 + (size_t)(row0 + 16 * mt + tl) * HID + 8 * q8) =
+                *reinterpret_cast<const bf16x8*>(ot + tl * kStageLd + 8 * q8);
+        }
     }
 }
 
@@ -1787,6 +1907,17 @@ void launch_ws_features(const LaunchGeom& g, const DevMatrix& A, const DevVector
     hipLaunchKernelGGL(ws_features_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, v, e, m, rest_pos, h,
                        xb, wi, dt, F, E, E4);
     check_hip(hipGetLastError(), "ws_features_kernel");
+}
+
+void WarmStartNet::features_encode(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const DevExt& e,
+                                   const DevMesh& m, const float* rest_pos, const WsHist& h, const double* xb,
+                                   const WsInst* wi, float dt, hipStream_t s) {
+    if (g.nslices * 64 > rows_) throw std::invalid_argument("warm start: reserve() rows first");
+    compact_edges_ = true;
+    if (g.nslices <= 0) return;
+    hipLaunchKernelGGL(ws_features_encode_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, A, v, e, m,
+                       rest_pos, h, xb, wi, dt, wnk_ + OFF_WE, theta_ + OFF_BE, E4_, H_[0]);
+    check_hip(hipGetLastError(), "ws_features_encode_kernel");
 }
 
 void launch_ws_direction(const LaunchGeom& g, const DevMatrix& A, const float* out, const WsInst* wi,

@@ -33,6 +33,7 @@ struct WsGraph {
 };
 
 struct WsInst;
+struct WsHist;
 
 class WarmStartNet {
 public:
@@ -75,8 +76,17 @@ public:
     // With direction_only, wi and dir given, the last layer writes the
     // warm-start direction d = scale * out[:, 0:3] (0 on padding rows) into
     // dir itself (rows x 3) instead of out.
+    // With encoded (inference only) h0 is already in place (features_encode)
+    // and the encoder is skipped.
     void forward(const WsGraph& g, hipStream_t s, bool keep, bool direction_only = false, const WsInst* wi = nullptr,
-                 float* dir = nullptr);
+                 float* dir = nullptr, bool encoded = false);
+    // Inference from a simulator's state: the features of every row (as
+    // launch_ws_features forms them) go straight into the encoder in one
+    // kernel, which writes h0 and the compact edge features (F is not
+    // written) and switches the layers to the compact edge features.
+    void features_encode(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const DevExt& e,
+                         const DevMesh& m, const float* rest_pos, const WsHist& h, const double* xb,
+                         const WsInst* wi, float dt, hipStream_t s);
     const float* out() const { return out_; }
     // Loss against target (rows x 3, FP32; rows with valid[row] == 0 or
     // beyond a slice's valid count are ignored): mean over valid rows of the

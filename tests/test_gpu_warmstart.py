@@ -220,6 +220,38 @@ def test_learned_direction_applied_and_steps_stay_exact(cuda):
     assert np.mean(used) > 0.5
 
 
+def test_fused_inference_matches_training_path(cuda):
+    """Inference runs the features and the encoder in one kernel, the layers
+    on the compact edge features and the decoder in the last layer's epilogue;
+    training (here with a zero learning rate, so the parameters stay put) runs
+    the features kernel, separate GEMMs and the 32 edge-feature columns.  From
+    the same state both give the same direction: the predicted residual after
+    the line search agrees to FP32 rounding of the decoder, and so do the
+    steps.  A fused kernel that read the wrong rows or columns of its tiles
+    would give O(1) differences."""
+    from chysx import warmstart
+    theta, _ = warmstart.load_params()
+    s = scenes.config3(w=72, steps=6)
+    sims = []
+    for train in (False, True):
+        g = make_sim(s, tol=1e-6)
+        g.sim.ws_set_params(theta)
+        g.sim.set_warm_start_mode("learned")
+        if train:
+            g.sim.ws_set_training(True, 0.0)
+        sims.append(g)
+    for k in range(3):
+        for g in sims:
+            g.sim.step(s.dt)
+            assert g.sim.last_solve_stats()["converged"]
+    a, b = sims[0].sim.ws_stats(), sims[1].sim.ws_stats()
+    assert a["used"] == 1 and b["used"] == 1, (a, b)
+    assert a["r0_learned"] > 0
+    assert abs(a["r0_learned"] - b["r0_learned"]) <= 1e-4 * a["r0_learned"], (a, b)
+    assert rel(sims[0].positions(), sims[1].positions()) <= 1e-6
+    np.testing.assert_array_equal(sims[0].sim.ws_get_params(), sims[1].sim.ws_get_params())
+
+
 def test_residual_acceptance_falls_back_when_residual_rises(cuda):
     """With the residual rule the network's direction is kept only when the
     predicted residual drops below the previous step's guess; otherwise the
