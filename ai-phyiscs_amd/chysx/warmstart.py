@@ -2,9 +2,10 @@
 
 The network itself lives in the library (BF16 MFMA forward and backward
 kernels, Adam on the GPU).  Training pairs (state, converged dx) come from
-stepping random 256x256 cloths (random pins, wind, stiffness and mass) with
-training on: each step of a batch of cloths gives one pair per cloth and one
-Adam step over the whole batch on the line-search loss (the share of the
+stepping random W x W cloths (random pins, wind, stiffness and mass; W = 256
+by default, 1024 for the committed parameters) with training on: each step
+of a batch of cloths gives one pair per cloth and one Adam step over the
+whole batch on the line-search loss (the share of the
 base guess's A-norm error the network's direction leaves).  The base guess
 is the previous step's dx for the 'learned' mode and the Galerkin
 projection for 'projection_learned'; a parameter file is trained for one of
