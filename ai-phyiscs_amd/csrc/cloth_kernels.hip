@@ -1241,23 +1241,38 @@ __global__ __launch_bounds__(64 * kGroupSlices) __attribute__((amdgpu_num_sgpr(8
 // issued before the p . Ap sum and waited for once.
 // sig_t: the first round of sigma's partial loads (block_sum_issue), issued
 // by update_group before the solver state was read.
+// The FP64 iterate of a wave's 64 rows is 192 contiguous doubles; it is read
+// and written 64 consecutive doubles per instruction and moved between that
+// order and the rows' (x0, x1, x2) through the wave's LDS slab (three plain
+// 8-byte accesses per row, 24 bytes apart, covered each line a third at a
+// time: the 1024^2 update pass that adds one term took 23.3 us that way,
+// 22.3 us staged; 108.0 -> 103.9 us adding two terms at 2048^2).
 constexpr int kSigmaKU = 2;
+// Orders one wave's LDS accesses around a point (LDS operations of a wave
+// complete in issue order; only the compiler's reordering is prevented).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 template <int X>
 __device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVectors& v, InstState& S, int s,
                                            const SliceDesc& sd, F3 r0, F3 api, const float (&m)[6],
-                                           const double (&sig_t)[1][kSigmaKU], double& sig_cache, int& sig_inst) {
+                                           const double (&sig_t)[1][kSigmaKU], double& sig_cache, int& sig_inst,
+                                           double* __restrict__ slab) {
     const float* __restrict__ p = S.parity ? v.p0 : v.p1;      // written by the SpMV pass
     const float* __restrict__ pprev = S.parity ? v.p1 : v.p0;  // the previous direction
     const int lane = threadIdx.x & 63;
     const int row = s * 64 + lane;
     F3 pi = f3(0.0f, 0.0f, 0.0f), pp = pi;
     double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-    double* x = v.x + 3 * row;
+    double* xw = v.x + (size_t)s * 192;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
     if (X != 0) {
         pi = ld3(p, row);
-        x0 = x[0];
-        x1 = x[1];
-        x2 = x[2];
+        t0 = xw[lane];
+        t1 = xw[lane + 64];
+        t2 = xw[lane + 128];
     }
     if (X == 1) pp = ld3(pprev, row);
     // sigma = p . Ap of this instance, in the fixed order of every per-iteration
@@ -1277,16 +1292,27 @@ __device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVector
         S.alpha = alpha;
     }
     double part[2] = {0.0, 0.0};
+    if (X != 0) {
+        slab[lane] = t0;
+        slab[lane + 64] = t1;
+        slab[lane + 128] = t2;
+        wave_lds_sync();
+        x0 = slab[3 * lane];
+        x1 = slab[3 * lane + 1];
+        x2 = slab[3 * lane + 2];
+        wave_lds_sync();
+    }
+    double n0 = x0, n1 = x1, n2 = x2;
     if (lane < sd.nvalid) {
         if (X == 1) {  // the owed term and this one
             const double ap = S.alpha_pend;
-            x[0] = x0 + (ap * double(pp.x) + alpha * double(pi.x));
-            x[1] = x1 + (ap * double(pp.y) + alpha * double(pi.y));
-            x[2] = x2 + (ap * double(pp.z) + alpha * double(pi.z));
+            n0 = x0 + (ap * double(pp.x) + alpha * double(pi.x));
+            n1 = x1 + (ap * double(pp.y) + alpha * double(pi.y));
+            n2 = x2 + (ap * double(pp.z) + alpha * double(pi.z));
         } else if (X == 2) {
-            x[0] = x0 + alpha * double(pi.x);
-            x[1] = x1 + alpha * double(pi.y);
-            x[2] = x2 + alpha * double(pi.z);
+            n0 = x0 + alpha * double(pi.x);
+            n1 = x1 + alpha * double(pi.y);
+            n2 = x2 + alpha * double(pi.z);
         }
         const F3 ri = r0 - api * af;
         const F3 zi = f3(m[0] * ri.x + m[1] * ri.y + m[2] * ri.z, m[1] * ri.x + m[3] * ri.y + m[4] * ri.z,
@@ -1295,6 +1321,18 @@ __device__ __forceinline__ void update_rows(const LaunchGeom& g, const DevVector
         st3(v.z, row, zi);
         part[0] = double(ri.x) * zi.x + double(ri.y) * zi.y + double(ri.z) * zi.z;
         part[1] = double(ri.x) * ri.x + double(ri.y) * ri.y + double(ri.z) * ri.z;
+    }
+    if (X != 0) {
+        // the rows' x go out through the wave's slab, each store covering 512 contiguous bytes
+        slab[3 * lane] = n0;
+        slab[3 * lane + 1] = n1;
+        slab[3 * lane + 2] = n2;
+        wave_lds_sync();
+        const int nv3 = 3 * sd.nvalid;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (lane + 64 * c < nv3) xw[lane + 64 * c] = slab[lane + 64 * c];
+        wave_lds_sync();
     }
     const int sl[2] = {0, 1};
     group_partials<2>(part, sl, v.partials, s / kGroupSlices, g.ngroups);
@@ -1328,9 +1366,11 @@ __device__ __forceinline__ void update_group(const LaunchGeom& g, const DevMatri
     }
     InstState& S = st[sd.inst];
     if (S.status != kRunning || S.mode != kModeIter) return;  // check and flush rounds update nothing
-    if (S.pend) update_rows<1>(g, v, S, s, sd, r0, api, m, sig_t, sig_cache, sig_inst);
-    else if (S.defer) update_rows<0>(g, v, S, s, sd, r0, api, m, sig_t, sig_cache, sig_inst);
-    else update_rows<2>(g, v, S, s, sd, r0, api, m, sig_t, sig_cache, sig_inst);
+    __shared__ double xslab[kGroupSlices][192];
+    double* slab = xslab[threadIdx.x >> 6];
+    if (S.pend) update_rows<1>(g, v, S, s, sd, r0, api, m, sig_t, sig_cache, sig_inst, slab);
+    else if (S.defer) update_rows<0>(g, v, S, s, sd, r0, api, m, sig_t, sig_cache, sig_inst, slab);
+    else update_rows<2>(g, v, S, s, sd, r0, api, m, sig_t, sig_cache, sig_inst, slab);
 }
 
 // With `pairs` (one instance), each workgroup updates two groups, i and
@@ -1340,8 +1380,9 @@ __device__ __forceinline__ void update_group(const LaunchGeom& g, const DevMatri
 // different cloths, and a finished cloth's group would hold up a live one
 // (the 512-cloth batch ran 3.8% slower paired).  Workgroup ids are remapped
 // so each XCD walks contiguous ranges of groups.
-__global__ __launch_bounds__(64 * kGroupSlices) void pcg_update_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
-                                                                      InstState* __restrict__ st, int pairs) {
+// At most 64 VGPRs (two 16-wave workgroups per CU).
+__global__ __launch_bounds__(64 * kGroupSlices) __attribute__((amdgpu_waves_per_eu(8))) void pcg_update_kernel(
+    LaunchGeom g, DevMatrix A, DevVectors v, InstState* __restrict__ st, int pairs) {
     const int nfirst = pairs ? (g.ngroups + 1) / 2 : g.ngroups;
     const int per = (nfirst + 7) >> 3;
     const int b = blockIdx.x;
