@@ -75,6 +75,44 @@ def test_forward_matches_fp64_reference(cuda, seed):
     assert rel(d, out[:, :3]) <= 1e-5
 
 
+def test_forward_matches_fp64_reference_wide_and_empty_rows(cuda):
+    """Rows with up to 12 neighbours (the grid's second ring: more neighbour
+    slots than one gather batch of the fused layer holds) and rows with none
+    (their neighbour mean is zero): the fused layers and the folded decoder
+    still match the FP64 reference."""
+    w = 17
+    n = w * w
+    rows = []
+    for i in range(w):
+        for j in range(w):
+            nb = []
+            if i < w - 1:  # the last grid row is isolated
+                for di in range(-2, 3):
+                    for dj in range(-2, 3):
+                        if 0 < abs(di) + abs(dj) <= 2 and 0 <= i + di < w - 1 and 0 <= j + dj < w:
+                            nb.append((i + di) * w + j + dj)
+            rows.append(sorted(nb))
+    ro = np.zeros(n + 1, np.int32)
+    ro[1:] = np.cumsum([len(r) for r in rows])
+    co = np.array([c for r in rows for c in r], np.int32)
+    assert np.diff(ro).max() == 12 and np.diff(ro).min() == 0
+    rng = np.random.default_rng(7)
+    F = np.zeros((n, 32), np.float32)
+    F[:, :15] = rng.normal(size=(n, 15)) * 0.7
+    E = np.zeros((n, 32), np.float32)
+    E[:, :4] = rng.normal(size=(n, 4)) * 0.3
+    F, E = bf16(F), bf16(E)
+    theta = W.init_params(3).astype(np.float32)
+    theta += np.random.default_rng(3).normal(size=theta.size).astype(np.float32) * 0.02
+    out = chysx._chysx_native.wsnet_forward(theta, ro, co, F, E)
+    ref = W.forward(quantize_params(theta.astype(np.float64)), F.astype(np.float64), E.astype(np.float64),
+                    ro.astype(np.int64), co.astype(np.int64))
+    assert rel(out, ref) <= 3e-2
+    d = chysx._chysx_native.wsnet_forward(theta, ro, co, F, E, direction_only=True)
+    assert rel(d, ref[:, :3]) <= 3e-2
+    assert rel(d, out[:, :3]) <= 1e-5
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_gradient_matches_fp64_reference(cuda, seed):
     n, ro, co, F, E = graph(19, seed)
