@@ -990,7 +990,10 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
         if (a->size() != v3) a->resize(v3);
     if (xb_.size() != v3) xb_.resize(v3);
     const DevVectors v = vectors();
-    launch_ws_base(geom(), matrix(), v, x_.data(), xb_.data(), rb_.data(), red, s);
+    // inference reads the base guess from x itself (unchanged until the line
+    // apply); training keeps the copy for the energy after the solve
+    double* xb = ws_train_ ? xb_.data() : x_.data();
+    launch_ws_base(geom(), matrix(), v, x_.data(), ws_train_ ? xb : nullptr, rb_.data(), red, s);
     launch_ws_finish(red, 3, state_.data(), ws_inst_.data(), ninst_, kWsPhaseBase, proj_ran_ ? 1 : 0, s);
     // inference: the features go straight into the encoder, the layers read
     // the compact edge features and the last layer writes the direction d
@@ -1002,7 +1005,7 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
                            N.features(), N.edge_features(), nullptr, s);
         N.forward(ws_graph(), s, true, false, ws_inst_.data(), d_.data());
     } else {
-        N.features_encode(geom(), matrix(), v, ext, mesh(), rest_pos_.data(), h, xb_.data(), ws_inst_.data(), dt, s);
+        N.features_encode(geom(), matrix(), v, ext, mesh(), rest_pos_.data(), h, xb, ws_inst_.data(), dt, s);
         N.forward(ws_graph(), s, false, true, ws_inst_.data(), d_.data(), true);
     }
     launch_ws_direction(geom(), matrix(), ws_train_ ? N.out() : nullptr, ws_inst_.data(), rb_.data(), d_.data(),

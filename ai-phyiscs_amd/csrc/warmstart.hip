@@ -1226,6 +1226,10 @@ int ws_small_solve_host(int k, const double* G, const double* g, double* c) {
 
 namespace {
 
+struct Flt3 {
+    float a, b, c;
+};
+
 __device__ __forceinline__ void ld3f(const float* p, int i, float& a, float& b, float& c) {
     a = p[3 * i];
     b = p[3 * i + 1];
@@ -1470,7 +1474,7 @@ __global__ __launch_bounds__(256) void ws_proj_apply_kernel(LaunchGeom g, WsHist
 }
 
 // r_b = b - A x with FP64 products and sums (stored FP32), z = M^-1 r_b,
-// xb <- x; partials [|r_b|^2, |x|^2, |b|^2].
+// xb <- x (unless xb is null); partials [|r_b|^2, |x|^2, |b|^2].
 __global__ __launch_bounds__(256) void ws_base_kernel(LaunchGeom g, DevMatrix A, DevVectors v,
                                                       const double* __restrict__ x, double* __restrict__ xb,
                                                       float* __restrict__ rb, WsReduce red) {
@@ -1517,15 +1521,16 @@ __global__ __launch_bounds__(256) void ws_base_kernel(LaunchGeom g, DevMatrix A,
     const float* mv = A.minv + (size_t)s * 6 * 64 + lane;
     const float m0 = mv[0], m1 = mv[64], m2 = mv[128], m3 = mv[192], m4 = mv[256], m5 = mv[320];
     const bool live = lane < sd.nvalid;
-    rb[3 * row] = f0;
-    rb[3 * row + 1] = f1;
-    rb[3 * row + 2] = f2;
-    v.z[3 * row] = live ? m0 * f0 + m1 * f1 + m2 * f2 : 0.0f;
-    v.z[3 * row + 1] = live ? m1 * f0 + m3 * f1 + m4 * f2 : 0.0f;
-    v.z[3 * row + 2] = live ? m2 * f0 + m4 * f1 + m5 * f2 : 0.0f;
-    xb[3 * row] = x0;
-    xb[3 * row + 1] = x1;
-    xb[3 * row + 2] = x2;
+    // one 12-byte store per row and vector, so a wave's stores cover whole lines
+    *reinterpret_cast<Flt3*>(rb + 3 * row) = Flt3{f0, f1, f2};
+    *reinterpret_cast<Flt3*>(v.z + 3 * row) =
+        live ? Flt3{m0 * f0 + m1 * f1 + m2 * f2, m1 * f0 + m3 * f1 + m4 * f2, m2 * f0 + m4 * f1 + m5 * f2}
+             : Flt3{0.0f, 0.0f, 0.0f};
+    if (xb != nullptr) {
+        xb[3 * row] = x0;
+        xb[3 * row + 1] = x1;
+        xb[3 * row + 2] = x2;
+    }
     put_partial(red, 0, s, lane, rr);
     put_partial(red, 1, s, lane, xx);
     put_partial(red, 2, s, lane, bb);

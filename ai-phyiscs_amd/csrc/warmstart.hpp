@@ -211,7 +211,9 @@ void launch_ws_finish(const WsReduce& red, int nvalues, const InstState* st, WsI
 // x <- V c for instances whose projection was accepted.
 void launch_ws_proj_apply(const LaunchGeom& g, const WsHist& h, const WsInst* wi, double* x, hipStream_t s);
 // Base guess of the learned stage: r_b = b - A x (FP64 sums, stored FP32),
-// z = M^-1 r_b into v.z, xb <- x; partials [|r_b|^2, |x|^2].
+// z = M^-1 r_b into v.z, xb <- x (skipped when xb is null: inference reads x
+// itself, which stays the base guess until the line search applies);
+// partials [|r_b|^2, |x|^2, |b|^2].
 void launch_ws_base(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const double* x, double* xb,
                     float* rb, const WsReduce& red, hipStream_t s);
 // Network inputs for every row: features F and mean edge features (BF16),
