@@ -1458,19 +1458,21 @@ __global__ __launch_bounds__(256) void ws_proj_apply_kernel(LaunchGeom g, WsHist
     const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
-    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    const int lane = threadIdx.x & 63;
     const WsInst& W = wi[sd.inst];
-    if (lane >= sd.nvalid || !W.use_proj) return;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int j = 0; j < h.k; ++j) {
-        const double cj = W.coef[j];
-        a0 += cj * double(h.v[j][3 * row]);
-        a1 += cj * double(h.v[j][3 * row + 1]);
-        a2 += cj * double(h.v[j][3 * row + 2]);
+    if (!W.use_proj) return;
+    // element-wise over the slice's 3 x 64 components, 64 consecutive ones per
+    // access (x[3 row + c] is component 3 (row - 64 s) + c of the slice)
+    const size_t base = (size_t)s * 192;
+    const int nv3 = 3 * sd.nvalid;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int e = lane + 64 * q;
+        if (e >= nv3) break;
+        double a = 0.0;
+        for (int j = 0; j < h.k; ++j) a += W.coef[j] * double(h.v[j][base + e]);
+        x[base + e] = a;
     }
-    x[3 * row] = a0;
-    x[3 * row + 1] = a1;
-    x[3 * row + 2] = a2;
 }
 
 // r_b = b - A x with FP64 products and sums (stored FP32), z = M^-1 r_b,
@@ -1788,12 +1790,18 @@ __global__ __launch_bounds__(256) void ws_line_apply_kernel(LaunchGeom g, const 
     const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
-    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    const int lane = threadIdx.x & 63;
     const WsInst& W = wi[sd.inst];
-    if (lane >= sd.nvalid || !W.use_cand) return;
+    if (!W.use_cand) return;
     const double al = W.alpha;
+    // element-wise over the slice's 3 x 64 components, 64 consecutive ones per access
+    const size_t base = (size_t)s * 192;
+    const int nv3 = 3 * sd.nvalid;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) x[3 * row + c] += al * double(d[3 * row + c]);
+    for (int q = 0; q < 3; ++q) {
+        const int e = lane + 64 * q;
+        if (e < nv3) x[base + e] += al * double(d[base + e]);
+    }
 }
 
 __global__ __launch_bounds__(256) void ws_energy_kernel(LaunchGeom g, const double* __restrict__ x,
