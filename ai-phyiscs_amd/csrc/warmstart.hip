@@ -165,16 +165,18 @@ __global__ __launch_bounds__(256) void ws_gemm_kernel(GemmArgs a) {
 
 // ---------------------------------------------------------------------------
 // Inference layer, fused: h' = relu([h | mean_{j in N(i)} h_j | e] W + b) + h
-// for the 64 rows of one slice per wave, in two phases.
-//  1. The neighbour mean of the slice's rows goes to the wave's LDS tile
-//     (64 x HID BF16).  Eight lanes share a row and each reads 16 B of a
+// for the 64 rows of one slice per wave, 32 rows at a time, in two phases.
+//  1. The neighbour mean of the 32 rows goes to the wave's LDS tile
+//     (32 x HID BF16).  Eight lanes share a row and each reads 16 B of a
 //     neighbour's 128-B row, so one load instruction covers eight whole rows;
-//     the column indices and neighbour rows of two groups of eight rows are
-//     in flight together.  The mean is bitwise that of ws_agg_kernel (same
-//     order, same 1/deg, same BF16 rounding); it never goes to memory.
+//     the rows' column indices come from one coalesced load per neighbour
+//     slot and a lane shuffle, and four neighbour rows of a group of eight
+//     rows are in flight together.  The mean is bitwise that of ws_agg_kernel
+//     (same order, same 1/deg, same BF16 rounding); it never goes to memory.
 //  2. The layer GEMM on MFMA 16x16x32 BF16 with the operands swapped
 //     (weights as A, rows as B), K = [h (2 chunks of 32) | mean (2, from LDS)
-//     | e (1)] as in the training GEMM.  The weight rows of output tile nt are
+//     | e (1: the 32 edge-feature columns, or the compact 4 with zeros)] as in
+//     the training GEMM.  The weight rows of output tile nt are
 //     taken in the order ch(nt, i) = 32 (nt >> 1) + 8 (i >> 2) + 4 (nt & 1) +
 //     (i & 3), so the lane (li, lg) ends with output columns 8 lg .. 8 lg + 7
 //     and 32 + 8 lg .. 32 + 8 lg + 7 of row 16 mt + li: the h columns it
