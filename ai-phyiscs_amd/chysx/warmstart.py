@@ -169,6 +169,7 @@ def _main(argv=None):
     t.add_argument("--time-budget", type=float, default=None)
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
     t.add_argument("--mode", choices=["learned", "projection_learned"], default="learned")
+    t.add_argument("--init", default=None, help="continue from this parameter file (same mode)")
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
     e.add_argument("--width", type=int, default=1024)
@@ -178,8 +179,17 @@ def _main(argv=None):
     e.add_argument("--accept", choices=["energy", "residual"], default="energy")
     a = ap.parse_args(argv)
     if a.cmd == "train":
-        theta, meta = train(a.pairs, a.cloths, a.width, a.episode, a.lr, a.lr_final, a.seed,
+        theta0, meta0 = load_params(a.init) if a.init else (None, None)
+        if meta0 is not None and meta0.get("mode") != a.mode:
+            raise SystemExit(f"--init {a.init} was trained for mode {meta0.get('mode')}, not {a.mode}")
+        theta, meta = train(a.pairs, a.cloths, a.width, a.episode, a.lr, a.lr_final, a.seed, theta0=theta0,
                             time_budget_s=a.time_budget, mode=a.mode)
+        if meta0 is not None:
+            # the pairs this file has seen in all: the starting file's and this run's
+            meta["init"] = {k: meta0[k] for k in ("pairs", "cloths_per_batch", "width", "lr", "lr_final", "seed",
+                                                  "train_seconds") if k in meta0}
+            meta["pairs_this_run"] = meta["pairs"]
+            meta["pairs"] = int(meta0["pairs"]) + int(meta["pairs"])
         save_params(a.out, theta, meta)
         print(json.dumps({k: v for k, v in meta.items() if k != "loss_curve"}), flush=True)
     else:
