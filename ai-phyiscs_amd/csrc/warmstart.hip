@@ -1228,6 +1228,15 @@ int ws_small_solve_host(int k, const double* G, const double* g, double* c) {
 
 namespace {
 
+// Entry order of the symmetric storage's products: 0, w-1, 1, w-2, ...  A
+// lower entry's block is its owner row's upper entry; the owner (on a grid,
+// a row W earlier, in a wavefront that runs a little earlier on the same XCD)
+// streams that entry at the same step from its end, so the mirror read hits
+// L2 (in ascending order the owner reads it last, long after).
+__device__ __forceinline__ int ws_entry(const DevMatrix& A, int k, int w) {
+    return A.sym ? ((k & 1) ? w - 1 - (k >> 1) : (k >> 1)) : k;
+}
+
 struct Flt3 {
     float a, b, c;
 };
@@ -1287,7 +1296,8 @@ __global__ __launch_bounds__(256) void ws_gram_kernel(LaunchGeom g, DevMatrix A,
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;
         const float* vp = A.val + (size_t)sb * A.bs * 64 + lane;
-        for (int k = 0; k < w; ++k) {
+        for (int t = 0; t < w; ++t) {
+            const int k = ws_entry(A, t, w);
             const int c = colp[k * 64];
             if (c < 0) continue;
             float mf[9];
@@ -1502,7 +1512,8 @@ __global__ __launch_bounds__(256) void ws_base_kernel(LaunchGeom g, DevMatrix A,
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;
         const float* vp = A.val + (size_t)sb * A.bs * 64 + lane;
-        for (int k = 0; k < w; ++k) {
+        for (int t = 0; t < w; ++t) {
+            const int k = ws_entry(A, t, w);
             const int c = colp[k * 64];
             if (c >= 0) {
                 float m[9];
@@ -1762,7 +1773,8 @@ __global__ __launch_bounds__(256) void ws_dir_spmv_kernel(LaunchGeom g, DevMatri
         const int* colp = A.col + (size_t)sb * 64 + lane;
         const int* mirp = A.mir + (size_t)sb * 64 + lane;
         const float* vp = A.val + (size_t)sb * A.bs * 64 + lane;
-        for (int k = 0; k < w; ++k) {
+        for (int t = 0; t < w; ++t) {
+            const int k = ws_entry(A, t, w);
             const int c = colp[k * 64];
             if (c >= 0) {
                 float m[9];
