@@ -290,6 +290,40 @@ def test_fused_inference_matches_training_path(cuda):
     np.testing.assert_array_equal(sims[0].sim.ws_get_params(), sims[1].sim.ws_get_params())
 
 
+@pytest.mark.parametrize("mode", ["learned", "projection_learned"])
+def test_warm_start_on_bending_cloth_matches_oracle(cuda, mode):
+    """The reference demo's crumpled drape (stretch, shear and dihedral
+    bending: 9-float blocks and rows of up to 12 neighbours, more slots than
+    one gather batch of the fused layers), stepped with the committed
+    parameters: every solve converges, the direction runs, and a step from
+    the simulator's state matches the FP64 oracle's step from that state."""
+    from chysx import warmstart
+    from oracle.oracle import oracle_from_scene
+    theta, _ = warmstart.load_params(warmstart.weights_for(mode))
+    s = scenes.drape(w=24, seed=5)
+    g = make_sim(s, tol=1e-6, springs_active=False, force_active=False)
+    g.sim.ws_set_params(theta)
+    g.sim.set_warm_start_mode(mode)
+    oc, _, _, im, f = oracle_from_scene(s, precision=64, springs_active=False, force_active=False, tol=1e-11,
+                                        max_iter=100000)
+    used = []
+    for k in range(10):
+        check = k == 7
+        if check:
+            p0 = g.positions().astype(np.float64)
+            v0 = g.velocities().astype(np.float64)
+            dx0 = g.sim.get_dx64()
+        g.sim.step(s.dt)
+        assert g.sim.last_solve_stats()["converged"], (mode, k)
+        if k >= 2:
+            used.append(g.sim.ws_stats()["used"])
+        if check:
+            oc.set_dx(dx0)
+            oc.step(p0, v0, im, f, s.dt, 1)
+            assert rel(g.positions(), p0) <= 1e-5, (mode, k)
+    assert np.mean(used) > 0.3, (mode, used)
+
+
 def test_residual_acceptance_falls_back_when_residual_rises(cuda):
     """With the residual rule the network's direction is kept only when the
     predicted residual drops below the previous step's guess; otherwise the
