@@ -190,7 +190,7 @@ def warm_start_comparison(args, scene, make_sim):
     import numpy as np
     import torch
     from chysx import warmstart as ws
-    out = {"warmup": args.warmup, "steps": args.ws_steps}
+    out = {"warmup": args.warmup, "steps": args.ws_steps, "accept": args.ws_accept}
     pos = {}
     modes = ("previous", "learned", "projection", "projection_learned")
     for mode in modes:
@@ -203,6 +203,7 @@ def warm_start_comparison(args, scene, make_sim):
         if theta is not None:
             g.sim.ws_set_params(theta)
         g.sim.set_warm_start_mode(mode)
+        g.sim.ws_set_acceptance(args.ws_accept)
         for _ in range(args.warmup):
             g.sim.step(scene.dt)
         torch.cuda.synchronize()
@@ -262,7 +263,7 @@ def batch_warm_start_comparison(args, sc, make_batch):
     import numpy as np
     import torch
     from chysx import warmstart as ws
-    out = {"warmup": args.warmup, "steps": args.ws_steps}
+    out = {"warmup": args.warmup, "steps": args.ws_steps, "accept": args.ws_accept}
     modes = ("previous", "learned", "projection", "projection_learned")
     for mode in modes:
         path = args.ws_weights or ws.weights_for(mode)
@@ -274,6 +275,7 @@ def batch_warm_start_comparison(args, sc, make_batch):
         if theta is not None:
             b.batch.ws_set_params(theta)
         b.batch.set_warm_start_mode(mode)
+        b.batch.ws_set_acceptance(args.ws_accept)
         for _ in range(args.warmup):
             b.batch.step(sc[0].dt)
         torch.cuda.synchronize()
@@ -567,6 +569,8 @@ def main():
     ap.add_argument("--ws-steps", type=int, default=40,
                     help="timed steps per warm-start mode in the projection / learned comparison (0: skip)")
     ap.add_argument("--ws-weights", default=None, help="learned warm-start parameters (.npz)")
+    ap.add_argument("--ws-accept", choices=["energy", "residual"], default="energy",
+                    help="acceptance rule of the learned direction in warm_start_comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (known bytes) for counter calibration")
     ap.add_argument("--hbm-size", type=int, default=2048,
