@@ -1147,79 +1147,6 @@ template <int K>
 __device__ __forceinline__ constexpr int ws_tri(int i, int j) {
     return i <= j ? i * K - i * (i - 1) / 2 + (j - i) : j * K - j * (j - 1) / 2 + (i - j);
 }
-// G is the packed upper triangle (ws_tri); a[i][j] = (G_ij + G_ji) / 2 as
-// ws_small_solve forms it from the full matrix.
-template <int K>
-__device__ __forceinline__ int ws_small_solve_k(const double* G, const double* g, double* c) {
-    double a[K][K], v[K][K];
-#pragma unroll
-    for (int i = 0; i < K; ++i)
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const double gij = G[ws_tri<K>(i, j)];
-            a[i][j] = 0.5 * (gij + gij);
-            v[i][j] = (i == j) ? 1.0 : 0.0;
-        }
-    for (int sweep = 0; sweep < 64; ++sweep) {
-        double off = 0.0, dia = 0.0;
-#pragma unroll
-        for (int p = 0; p < K; ++p) {
-            dia += a[p][p] * a[p][p];
-#pragma unroll
-            for (int q = p + 1; q < K; ++q) off += a[p][q] * a[p][q];
-        }
-        if (!(off > 1e-30 * dia)) break;
-#pragma unroll
-        for (int p = 0; p < K; ++p)
-#pragma unroll
-            for (int q = p + 1; q < K; ++q) {
-                if (a[p][q] == 0.0) continue;
-                const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
-                const double t = fabs(theta) > 1e150 ? 0.5 / theta
-                                                      : (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
-#pragma unroll
-                for (int r = 0; r < K; ++r) {
-                    const double arp = a[r][p], arq = a[r][q];
-                    a[r][p] = cs * arp - sn * arq;
-                    a[r][q] = sn * arp + cs * arq;
-                }
-#pragma unroll
-                for (int r = 0; r < K; ++r) {
-                    const double apr = a[p][r], aqr = a[q][r];
-                    a[p][r] = cs * apr - sn * aqr;
-                    a[q][r] = sn * apr + cs * aqr;
-                }
-#pragma unroll
-                for (int r = 0; r < K; ++r) {
-                    const double vrp = v[r][p], vrq = v[r][q];
-                    v[r][p] = cs * vrp - sn * vrq;
-                    v[r][q] = sn * vrp + cs * vrq;
-                }
-            }
-    }
-    double lmax = 0.0;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-        c[i] = 0.0;
-        lmax = fmax(lmax, a[i][i]);
-    }
-    int kept = 0;
-    if (!(lmax > 0.0)) return 0;
-#pragma unroll
-    for (int m = 0; m < K; ++m) {
-        const double lam = a[m][m];
-        if (!(lam > 1e-12 * lmax)) continue;
-        double pr = 0.0;
-#pragma unroll
-        for (int i = 0; i < K; ++i) pr += v[i][m] * g[i];
-        const double cf = pr / lam;
-#pragma unroll
-        for (int i = 0; i < K; ++i) c[i] += cf * v[i][m];
-        ++kept;
-    }
-    return kept;
-}
 
 int ws_small_solve_host(int k, const double* G, const double* g, double* c) {
     if (k < 1 || k > kWsHistMax) throw std::invalid_argument("ws_small_solve: 1 <= k <= 8");
@@ -1357,11 +1284,98 @@ __global__ __launch_bounds__(256) void ws_chunk_kernel(WsReduce red, int nv, con
     if (threadIdx.x == 0) red.part2[((size_t)inst * red.chunks + c) * nv + v] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-// The projection's decision for K history vectors, with K fixed at compile
-// time so the eigen-solve stays in registers; the Gram matrices are read
-// from the packed sums in LDS where they are needed.
+// The eigen-truncated solve of ws_small_solve by one wavefront: lane
+// (i, j) = (lane >> 3, lane & 7) holds a[i][j] and v[i][j] (K <= 8), and the
+// Jacobi sweeps rotate K' / 2 disjoint pairs at once (K' = K rounded up to
+// even; a round-robin tournament pairs every index with every other once per
+// sweep of K' - 1 rounds).  Rotations of disjoint pairs commute and each
+// takes its angle from its own 2 x 2 block, which the others leave alone, so
+// a round equals those rotations applied one after the other; the angle
+// formula and the stopping test are those of ws_small_solve.  Every lane of
+// the wavefront calls it; c[0..K-1] (shared memory) and the count of kept
+// eigen-directions are the result.
 template <int K>
-__device__ __forceinline__ void ws_finish_projection(const double* __restrict__ tot, WsInst& W) {
+__device__ __forceinline__ int ws_partner(int r, int i) {
+    constexpr int KE = K + (K & 1);  // players; index K is a bye when K is odd
+    if (i == KE - 1) return r;
+    if (i == r) return KE - 1;
+    return (2 * r - i + 2 * (KE - 1)) % (KE - 1);
+}
+__device__ __forceinline__ void ws_rotation(double app, double aqq, double apq, double& cs, double& sn) {
+    if (apq == 0.0) {
+        cs = 1.0;
+        sn = 0.0;
+        return;
+    }
+    const double theta = (aqq - app) / (2.0 * apq);
+    const double t = fabs(theta) > 1e150 ? 0.5 / theta
+                                          : (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+    cs = 1.0 / sqrt(t * t + 1.0);
+    sn = t * cs;
+}
+__device__ __forceinline__ double ws_wave_sum(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+template <int K>
+__device__ int ws_small_solve_wave(const double* G, const double* g, double* c, int lane) {
+    constexpr int KE = K + (K & 1);
+    const int i = lane >> 3, j = lane & 7;
+    const bool act = i < K && j < K;
+    double a = act ? G[ws_tri<K>(i, j)] : 0.0;
+    double v = (act && i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        const double off = ws_wave_sum((act && i < j) ? a * a : 0.0);
+        const double dia = ws_wave_sum((act && i == j) ? a * a : 0.0);
+        if (!(off > 1e-30 * dia)) break;
+        for (int r = 0; r < KE - 1; ++r) {
+            // this round's partners of the lane's row and column (K: a bye)
+            const int pi = act ? ws_partner<K>(r, i) : i, pj = act ? ws_partner<K>(r, j) : j;
+            const bool ri = pi < K, rj = pj < K;
+            // rotation of the row's pair and of the column's pair
+            const int p1 = ri ? min(i, pi) : i, q1 = ri ? max(i, pi) : i;
+            const int p2 = rj ? min(j, pj) : j, q2 = rj ? max(j, pj) : j;
+            const double a11 = __shfl(a, 9 * p1, 64), a12 = __shfl(a, 9 * q1, 64), a13 = __shfl(a, 8 * p1 + q1, 64);
+            const double a21 = __shfl(a, 9 * p2, 64), a22 = __shfl(a, 9 * q2, 64), a23 = __shfl(a, 8 * p2 + q2, 64);
+            double c1 = 1.0, s1 = 0.0, c2 = 1.0, s2 = 0.0;
+            if (ri) ws_rotation(a11, a12, a13, c1, s1);
+            if (rj) ws_rotation(a21, a22, a23, c2, s2);
+            // J of a pair (p, q): J_pp = J_qq = c, J_pq = s, J_qp = -s
+            const double jii = c1, jpi = ri ? (i == p1 ? -s1 : s1) : 0.0;
+            const double jjj = c2, jpj = rj ? (j == p2 ? -s2 : s2) : 0.0;
+            const double a_ipj = __shfl(a, 8 * i + pj, 64), a_pij = __shfl(a, 8 * pi + j, 64),
+                         a_pipj = __shfl(a, 8 * pi + pj, 64), v_ipj = __shfl(v, 8 * i + pj, 64);
+            if (act) {
+                // A' = J^T A J, V' = V J
+                a = jii * (jjj * a + jpj * a_ipj) + jpi * (jjj * a_pij + jpj * a_pipj);
+                v = jjj * v + jpj * v_ipj;
+            }
+        }
+    }
+    // eigenvalue of column j, the largest one, and v_j . g
+    const double lam = __shfl(a, 9 * j, 64);
+    double lmax = act && i == j ? a : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off, 64));
+    double pr = act ? v * g[i] : 0.0;
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1) pr += __shfl_xor(pr, off, 64);
+    const bool keep = act && lmax > 0.0 && lam > 1e-12 * lmax;
+    // c_i = sum over kept columns j of (v_j . g / lambda_j) v_ij
+    double ci = keep ? (pr / lam) * v : 0.0;
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) ci += __shfl_xor(ci, off, 64);
+    if (j == 0 && i < K) c[i] = ci;
+    const double kept = ws_wave_sum((keep && i == 0) ? 1.0 : 0.0);
+    return int(kept);
+}
+
+// The projection's decision for K history vectors from the packed sums in
+// LDS and the coefficients c of the eigen-solve (kept directions: kept).
+template <int K>
+__device__ __forceinline__ void ws_finish_projection(const double* __restrict__ tot, const double* csol, int kept,
+                                                     WsInst& W) {
     constexpr int NT = K * (K + 1) / 2;
     const double* G = tot;       // V^T A V, packed
     const double* Q = tot + NT;  // (AV)^T (AV), packed
@@ -1372,7 +1386,8 @@ __device__ __forceinline__ void ws_finish_projection(const double* __restrict__ 
         bav[i] = tot[2 * NT + K + i];
     }
     const double bb = tot[2 * NT + 2 * K];
-    const int kept = ws_small_solve_k<K>(G, gv, c);
+#pragma unroll
+    for (int i = 0; i < K; ++i) c[i] = csol[i];
     double phi = 0.0, rr = bb;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
@@ -1461,8 +1476,14 @@ __global__ __launch_bounds__(128) void ws_finish_proj_kernel(WsReduce red, int n
         tot[t] = a;
     }
     __syncthreads();
+    if (t >= 64) return;
+    // the eigen-solve by the first wavefront, then the decision by its lane 0
+    __shared__ double csol[K];
+    constexpr int NT = K * (K + 1) / 2;
+    const int kept = ws_small_solve_wave<K>(tot, tot + 2 * NT, csol, t);
+    ws_wave_lds_sync();  // the lanes that wrote csol are in this wavefront
     if (t != 0) return;
-    ws_finish_projection<K>(tot, wi[inst]);
+    ws_finish_projection<K>(tot, csol, kept, wi[inst]);
 }
 
 __global__ __launch_bounds__(256) void ws_proj_apply_kernel(LaunchGeom g, WsHist h, const WsInst* __restrict__ wi,
