@@ -1066,8 +1066,10 @@ __global__ __launch_bounds__(64 * kGroupSlices) __attribute__((amdgpu_num_sgpr(8
     if (s >= g.nslices) return;
     const int grp = s / kGroupSlices;
     const SliceDesc sd = g.slices[s];
-    // a finished cloth (its first workgroup still copies the state)
-    if (g.gactive != nullptr && g.gactive[grp] == 0 && (!ADV || grp != st[sd.inst].group0)) return;
+    // a finished cloth (its first workgroup still copies the state); the
+    // cloth's first group comes with the slice descriptor, so a finished
+    // group's workgroup exits after one round of loads
+    if (g.gactive != nullptr && g.gactive[grp] == 0 && (!ADV || grp != sd.group0)) return;
     const int lane = threadIdx.x & 63;
     const bool valid = lane < sd.nvalid;
     const int row = s * 64 + lane;
