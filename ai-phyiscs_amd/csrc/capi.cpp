@@ -559,6 +559,28 @@ int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts
     });
 }
 
+int chysx_batch_num_cloths(const chysx_batch* b, int* n) {
+    return guard([&] {
+        need(b, "batch");
+        need(n, "n");
+        *n = b->batch.count();
+    });
+}
+
+int chysx_batch_stats_all(const chysx_batch* b, int n, int* iters, int* restarts, int* status, double* relres) {
+    return guard([&] {
+        need(b, "batch");
+        if (n != b->batch.count()) throw std::invalid_argument("batch: n must be the number of cloths");
+        for (int k = 0; k < n; ++k) {
+            const SolveStats s = b->batch.stats(k);
+            if (iters) iters[k] = s.iters;
+            if (restarts) restarts[k] = s.restarts;
+            if (status) status[k] = s.status;
+            if (relres) relres[k] = s.relres;
+        }
+    });
+}
+
 int chysx_cloth_spmv(chysx_cloth* sim, uintptr_t x, uintptr_t y, uintptr_t stream) {
     return guard([&] {
         need(sim, "sim");

@@ -189,8 +189,9 @@ public:
     // writer of the problem's device buffers calls it first.
     void wait_pending(hipStream_t s) const;
 
-    // Per-instance results of the last step (waits for an enqueued step).
-    std::vector<SolveStats> stats() const {
+    // Per-instance results of the last step (waits for an enqueued step); the
+    // reference stays valid until the next step.
+    const std::vector<SolveStats>& stats() const {
         resolve_stats();
         return stats_;
     }

@@ -670,6 +670,12 @@ PYBIND11_MODULE(_chysx_native, m) {
         .def(
             "set_split_advance", [](Batch& b, int mode) { check(chysx_batch_set_split_advance(b.h, mode)); },
             py::arg("mode"))
+        .def("num_cloths",
+             [](const Batch& b) {
+                 int n = 0;
+                 check(chysx_batch_num_cloths(b.h, &n));
+                 return n;
+             })
         .def(
             "stats",
             [](const Batch& b, int k) {
@@ -678,7 +684,27 @@ PYBIND11_MODULE(_chysx_native, m) {
                 check(chysx_batch_stats(b.h, k, &it, &rs, &st, &rr));
                 return stats_dict(it, rs, st, rr);
             },
-            py::arg("index"));
+            py::arg("index"))
+        .def(
+            "stats_all",
+            [](const Batch& b) {
+                int n = 0;
+                check(chysx_batch_num_cloths(b.h, &n));
+                py::array_t<int> it(n), rs(n), st(n);
+                py::array_t<double> rr(n);
+                py::array_t<bool> cv(n);
+                check(chysx_batch_stats_all(b.h, n, it.mutable_data(), rs.mutable_data(), st.mutable_data(),
+                                            rr.mutable_data()));
+                for (int k = 0; k < n; ++k) cv.mutable_data()[k] = st.data()[k] == CHYSX_PCG_CONVERGED;
+                py::dict d;
+                d["iters"] = it;
+                d["restarts"] = rs;
+                d["status"] = st;
+                d["converged"] = cv;
+                d["relres"] = rr;
+                return d;
+            },
+            "Per-cloth iterations, restarts, status and relres of the last step (arrays).");
 
     m.def(
         "topology_query",

@@ -254,6 +254,12 @@ int chysx_batch_set_split_advance(chysx_batch* b, int mode);
 /* Pin projection (as chysx_cloth_set_pin_projection) for every cloth of the batch. */
 int chysx_batch_set_pin_projection(chysx_batch* b, int on);
 int chysx_batch_stats(const chysx_batch* b, int index, int* iters, int* restarts, int* status, double* relres);
+/* Number of cloths added to the batch. */
+int chysx_batch_num_cloths(const chysx_batch* b, int* n);
+/* The same for every cloth at once (arrays of n = the cloth count; any may be
+ * null).  No counterpart in the reference, which steps its cloths one by
+ * one; for callers that check every cloth after every batch step. */
+int chysx_batch_stats_all(const chysx_batch* b, int n, int* iters, int* restarts, int* status, double* relres);
 
 /* ---- kernel-level benchmark hooks -------------------------------------- */
 /* Time `reps` launches of one kernel of the last assembled system on `stream`

@@ -202,10 +202,14 @@ def test_batch_matches_single_simulators_bitwise(cuda):
         b.batch.step(1.0 / 60.0)
         for g in singles:
             g.sim.step(1.0 / 60.0)
+    every = b.batch.stats_all()
+    assert b.batch.num_cloths() == len(sc) and every["iters"].shape == (len(sc),)
     for k, g in enumerate(singles):
         st = b.batch.stats(k)
         assert st["status"] == CONVERGED
         assert st["iters"] == g.sim.last_solve_stats()["iters"]
+        for key in ("iters", "restarts", "status", "relres", "converged"):
+            assert every[key][k] == st[key], (k, key)
         np.testing.assert_array_equal(b.positions(k), g.positions())
 
 
