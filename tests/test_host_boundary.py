@@ -48,6 +48,16 @@ def test_c_abi_error_codes_without_gpu():
     # null handle -> invalid_argument
     assert lib.chysx_cloth_step(None, ctypes.c_float(0.1), ctypes.c_size_t(0)) == 1
     assert lib.chysx_cloth_destroy(h) == 0
+    # a batch's statistics for all cloths: the array length must be the cloth count
+    b = ctypes.c_void_p()
+    assert lib.chysx_batch_create(ctypes.byref(b)) == 0
+    n = ctypes.c_int(-1)
+    assert lib.chysx_batch_num_cloths(b, ctypes.byref(n)) == 0 and n.value == 0
+    iters = (ctypes.c_int * 1)()
+    assert lib.chysx_batch_stats_all(b, 1, iters, None, None, None) == 1
+    assert b"number of cloths" in lib.chysx_last_error()
+    assert lib.chysx_batch_stats_all(b, 0, None, None, None, None) == 0
+    assert lib.chysx_batch_destroy(b) == 0
 
 
 def oracle_topology(scene, springs_active):
