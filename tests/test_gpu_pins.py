@@ -100,3 +100,49 @@ def test_pin_projection_batch_matches_single_cloths(cuda):
         assert b.batch.stats(k)["converged"]
         np.testing.assert_array_equal(b.positions(k), g.positions())
         np.testing.assert_array_equal(b.positions(k)[s.pin_idx], s.pin_targets)
+
+
+@pytest.mark.parametrize("mode", ["projection", "learned", "projection_learned"])
+def test_pin_projection_with_warm_start_modes(cuda, mode):
+    """The warm-start stages run after assembly has started every pinned row
+    at its target displacement and may move it (the projection combines past
+    dx, the learned direction has a component there); the pinned rows' own
+    equations (I dx = d) bring them back within the tolerance, and finalize
+    puts a projected pin exactly on its target.  Every solve converges, the
+    pins sit bitwise on their (moving) targets, and a step matches the
+    oracle's from the same state and initial guess."""
+    from chysx import warmstart
+    s = pinned_scene()
+    g = make_sim(s, tol=1e-6)
+    g.sim.set_pin_projection(True)
+    if mode != "projection":
+        g.sim.ws_set_params(warmstart.load_params(warmstart.weights_for(mode))[0])
+    g.sim.set_warm_start_mode(mode)
+    oc, _, _, im, f = oracle_from_scene(s, precision=64, tol=1e-11, max_iter=100000)
+    oc.set_pin_projection(True)
+    tgt = s.pin_targets.copy()
+    used = []
+    for k in range(12):
+        if k == 6:  # the targets move
+            tgt = (tgt + np.array([0.0, 0.02, -0.01], np.float32)).astype(np.float32)
+            g.sim.update_pin_targets(tgt)
+            oc.set_pins(s.pin_idx, tgt, s.pin_k)
+            oc.set_pin_projection(True)
+        p0 = g.positions().astype(np.float64)
+        v0 = g.velocities().astype(np.float64)
+        dx0 = g.sim.get_dx64() if k > 0 else np.zeros((s.n, 3))
+        g.sim.step(s.dt)
+        st = g.sim.last_solve_stats()
+        assert st["converged"] and st["relres"] <= 1e-6, (mode, k, st)
+        if k >= 2:
+            w = g.sim.ws_stats()
+            used.append(w["projection_used"] if mode == "projection" else w["used"])
+        pos = g.positions()
+        np.testing.assert_array_equal(pos[s.pin_idx], tgt)
+        if k in (5, 6, 11):
+            oc.set_dx(dx0)
+            p, v = p0.copy(), v0.copy()
+            oc.step(p, v, im, f, s.dt, 1)
+            assert rel(pos, p) <= 1e-5, (mode, k)
+    assert np.mean(used) > 0.3, (mode, used)
+
