@@ -116,3 +116,40 @@ def test_resting_on_ground_closed_form(cuda):
     m = 1.0 / s.inv_mass[0]
     expect = -0.05 + H - m * 9.8 / 5e3
     np.testing.assert_allclose(g.positions()[:, 1], expect, atol=2e-6)
+
+
+@pytest.mark.parametrize("mode", ["projection", "learned"])
+def test_contact_with_warm_start_modes(cuda, mode):
+    """The drape falling onto the box with friction, stepped with a warm
+    start: contact lives in the assembled operator, which the warm-start
+    stages multiply by like any other, so every solve converges and a step
+    from the simulator's state matches the oracle's (same bars as without a
+    warm start)."""
+    from chysx import warmstart
+    s = scenes.drape(w=25)
+    g = make_sim(s, tol=1e-6, springs_active=False, force_active=False)
+    oc, _, _, im, f = oracle_from_scene(s, precision=64, springs_active=False, force_active=False, tol=1e-11,
+                                        max_iter=100000)
+    add_shapes(g.sim, oc, 0.4)
+    if mode == "learned":
+        g.sim.ws_set_params(warmstart.load_params()[0])
+    g.sim.set_warm_start_mode(mode)
+    used = []
+    for k in range(50):
+        check = k in (29, 49)
+        if check:
+            p0 = g.positions().astype(np.float64)
+            v0 = g.velocities().astype(np.float64)
+            dx0 = g.sim.get_dx64()
+        g.sim.step(s.dt)
+        st = g.sim.last_solve_stats()
+        assert st["converged"] and st["relres"] <= 1e-6, (mode, k, st)
+        if k >= 2:
+            w = g.sim.ws_stats()
+            used.append(w["projection_used"] if mode == "projection" else w["used"])
+        if check:
+            oc.set_dx(dx0)
+            oc.step(p0, v0, im, f, s.dt, 1)
+            assert rel(g.positions(), p0) <= 5e-5, (mode, k)
+    assert np.mean(used) > 0.3, (mode, np.mean(used))
+
