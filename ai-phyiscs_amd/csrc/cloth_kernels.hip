@@ -906,6 +906,34 @@ __global__ __launch_bounds__(64 * kGroupSlices) void pcg_residual_kernel(LaunchG
     group_partials<3>(part, sl, v.partials, s / kGroupSlices, g.ngroups);
 }
 
+// Start pass from a residual the caller already holds in v.r (FP32, as the
+// residual pass stores it): z = M^-1 r, p = z and the start partials
+// [|r|^2, |b|^2, r . z] as the residual pass forms them, without the product
+// with A.  The learned warm start's line search has r = r_b - alpha A d in
+// hand; the solve's true-residual check before it stops is unchanged.
+__global__ __launch_bounds__(64 * kGroupSlices) void pcg_start_given_r_kernel(LaunchGeom g, DevMatrix A,
+                                                                             DevVectors v,
+                                                                             const InstState* __restrict__ st) {
+    const int s = my_slice<kGroupSlices>(g.nslices);
+    if (s >= g.nslices) return;
+    const SliceDesc sd = g.slices[s];
+    const InstState& S = st[sd.inst];
+    float* pcur = S.parity ? v.p1 : v.p0;
+    const int lane = threadIdx.x & 63, row = s * 64 + lane;
+    double part[3] = {0.0, 0.0, 0.0};
+    if (lane < sd.nvalid) {
+        const F3 rf = ld3(v.r, row), bf = ld3(v.rhs, row);
+        const F3 zf = minv_mul(A, s, lane, rf);
+        st3(v.z, row, zf);
+        st3(pcur, row, zf);
+        part[0] = fma3d(0.0, rf.x, rf.y, rf.z, rf.x, rf.y, rf.z);
+        part[1] = fma3d(0.0, bf.x, bf.y, bf.z, bf.x, bf.y, bf.z);
+        part[2] = fma3d(0.0, rf.x, rf.y, rf.z, zf.x, zf.y, zf.z);
+    }
+    const int sl[3] = {0, 1, 2};
+    group_partials<3>(part, sl, v.partials, s / kGroupSlices, g.ngroups);
+}
+
 // ---------------------------------------------------------------------------
 // PCG SpMV pass.  It first advances the solver state by the previous
 // iteration's partials (advance_in_pass), then works in the mode that state
@@ -1626,6 +1654,14 @@ void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVecto
     if (g.nslices % kGroupSlices != 0 || g.ngroups * kGroupSlices != g.nslices)
         throw std::invalid_argument("PCG passes: slices must form whole 16-slice groups");
     pcg_residual_kernel<<<grid_for(g.nslices, kGroupSlices), 64 * kGroupSlices, 0, s>>>(g, A, v, st, report);
+    CHYSX_LAUNCH_CHECK();
+}
+void launch_pcg_start_given_r(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                              hipStream_t s) {
+    if (g.nslices <= 0) return;
+    if (g.nslices % kGroupSlices != 0 || g.ngroups * kGroupSlices != g.nslices)
+        throw std::invalid_argument("PCG passes: slices must form whole 16-slice groups");
+    pcg_start_given_r_kernel<<<grid_for(g.nslices, kGroupSlices), 64 * kGroupSlices, 0, s>>>(g, A, v, st);
     CHYSX_LAUNCH_CHECK();
 }
 namespace {

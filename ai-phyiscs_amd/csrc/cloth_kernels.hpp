@@ -124,6 +124,10 @@ void launch_rest_angles(const int4* dih, const float* pos, float* rest, int n, h
 // report == 1 only measures the true residual of stopped instances.
 void launch_pcg_residual(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
                          int report, hipStream_t s);
+// Start pass from the residual already in v.r: z, p = z and the same start
+// partials as the residual pass with report == 0, without the product with A.
+void launch_pcg_start_given_r(const LaunchGeom& g, const DevMatrix& A, const DevVectors& v, const InstState* st,
+                              hipStream_t s);
 // SpMV pass (direction update, A p, p . A p; the true-residual check; or the
 // flush of a deferred iterate term).  With st_out (not st): the pass first
 // advances the state in st by the previous iteration's partials (when owed)

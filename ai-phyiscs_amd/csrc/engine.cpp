@@ -606,7 +606,11 @@ void Problem::launch_assemble(float dt, const DevExt& ext, hipStream_t s) {
 }
 void Problem::launch_pcg_start(hipStream_t s) {
     const LaunchGeom g = geom();
-    launch_pcg_residual(g, matrix(), vectors(), state_.data(), 0, s);
+    if (start_r_ready_)
+        launch_pcg_start_given_r(g, matrix(), vectors(), state_.data(), s);
+    else
+        launch_pcg_residual(g, matrix(), vectors(), state_.data(), 0, s);
+    start_r_ready_ = false;
     launch_pcg_reduce(state_.data(), params_.data(), partials_.data(), ninst_, g.ngroups, kPhaseStart, s, g.gactive);
 }
 // One iteration: the SpMV pass (it first folds the previous iteration's
@@ -1011,7 +1015,12 @@ void Problem::warm_start_stage(float dt, const DevExt& ext, hipStream_t s) {
     launch_ws_direction(geom(), matrix(), ws_train_ ? N.out() : nullptr, ws_inst_.data(), rb_.data(), d_.data(),
                         ad_.data(), red, s);
     launch_ws_finish(red, 4, state_.data(), ws_inst_.data(), ninst_, kWsPhaseLine, ws_accept_residual_ ? 1 : 0, s);
-    if (learned) launch_ws_line_apply(geom(), ws_inst_.data(), d_.data(), x_.data(), s);
+    if (learned) {
+        // x <- x_b + alpha d, and the solve starts from r_b - alpha A d
+        launch_ws_line_apply(geom(), ws_inst_.data(), d_.data(), x_.data(), rb_.data(), ad_.data(),
+                             vectors().r, s);
+        start_r_ready_ = true;
+    }
     ws_ran_ = true;
 }
 
@@ -1021,6 +1030,7 @@ void Problem::step(float dt, const DevExt& ext, hipStream_t s) {
     stats_pending_ = false;
     ws_ran_ = false;
     proj_ran_ = false;
+    start_r_ready_ = false;
     const bool ws_on = ws_mode_ != kWarmPrevious || ws_train_;
     // Every instance at a fixed iteration count (tol == 0): the number of
     // rounds is known, so the whole solve is enqueued without a poll.

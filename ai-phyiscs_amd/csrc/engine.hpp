@@ -386,6 +386,7 @@ private:
     double ws_loss_ = 0.0;
     bool ws_ran_ = false;    // network stage ran this step
     bool proj_ran_ = false;  // projection stage ran this step
+    bool start_r_ready_ = false;  // the line search left the start residual in r
     int hist_k_ = 6;         // history length of the projection
     int hist_n_ = 0;         // solutions held (<= hist_k_)
     int hist_head_ = 0;      // slot of the newest

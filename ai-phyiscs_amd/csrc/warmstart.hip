@@ -1821,21 +1821,27 @@ __global__ __launch_bounds__(256) void ws_dir_spmv_kernel(LaunchGeom g, DevMatri
 }
 
 __global__ __launch_bounds__(256) void ws_line_apply_kernel(LaunchGeom g, const WsInst* __restrict__ wi,
-                                                            const float* __restrict__ d, double* __restrict__ x) {
+                                                            const float* __restrict__ d, double* __restrict__ x,
+                                                            const float* __restrict__ rb,
+                                                            const float* __restrict__ ad, float* __restrict__ r) {
     const int s = ws_slice(g.nslices);
     if (s >= g.nslices) return;
     const SliceDesc sd = g.slices[s];
     const int lane = threadIdx.x & 63;
     const WsInst& W = wi[sd.inst];
-    if (!W.use_cand) return;
-    const double al = W.alpha;
+    const bool use = W.use_cand != 0;
+    const double al = use ? W.alpha : 0.0;
     // element-wise over the slice's 3 x 64 components, 64 consecutive ones per access
     const size_t base = (size_t)s * 192;
     const int nv3 = 3 * sd.nvalid;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const int e = lane + 64 * q;
-        if (e < nv3) x[base + e] += al * double(d[base + e]);
+        if (e < nv3) {
+            if (use) x[base + e] += al * double(d[base + e]);
+            // the residual of the new guess, for the solve's start
+            if (r != nullptr) r[base + e] = float(double(rb[base + e]) - al * double(ad[base + e]));
+        }
     }
 }
 
@@ -1979,9 +1985,10 @@ void launch_ws_direction(const LaunchGeom& g, const DevMatrix& A, const float* o
     check_hip(hipGetLastError(), "ws_dir_spmv_kernel");
 }
 
-void launch_ws_line_apply(const LaunchGeom& g, const WsInst* wi, const float* d, double* x, hipStream_t s) {
+void launch_ws_line_apply(const LaunchGeom& g, const WsInst* wi, const float* d, double* x, const float* rb,
+                          const float* ad, float* r, hipStream_t s) {
     if (g.nslices <= 0) return;
-    hipLaunchKernelGGL(ws_line_apply_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, wi, d, x);
+    hipLaunchKernelGGL(ws_line_apply_kernel, dim3(grid_slices(g.nslices)), dim3(256), 0, s, g, wi, d, x, rb, ad, r);
     check_hip(hipGetLastError(), "ws_line_apply_kernel");
 }
 

@@ -225,8 +225,11 @@ void launch_ws_features(const LaunchGeom& g, const DevMatrix& A, const DevVector
 // d = scale * out[:, 0:3] and A d; partials [d.r_b, d.Ad, Ad.r_b, Ad.Ad].
 void launch_ws_direction(const LaunchGeom& g, const DevMatrix& A, const float* out, const WsInst* wi,
                          const float* rb, float* d, float* ad, const WsReduce& red, hipStream_t s);
-// x <- x + alpha d for instances whose direction was accepted.
-void launch_ws_line_apply(const LaunchGeom& g, const WsInst* wi, const float* d, double* x, hipStream_t s);
+// x <- x + alpha d for instances whose direction was accepted; with r, also
+// r <- r_b - alpha A d (alpha 0 where the direction was not applied: r = r_b),
+// the residual of the new guess with the FP64 sums of r_b and A d.
+void launch_ws_line_apply(const LaunchGeom& g, const WsInst* wi, const float* d, double* x, const float* rb,
+                          const float* ad, float* r, hipStream_t s);
 // Training: partials [(x - xb).r_b].
 void launch_ws_energy(const LaunchGeom& g, const double* x, const double* xb, const float* rb, const WsReduce& red,
                       hipStream_t s);
